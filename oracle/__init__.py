@@ -1,0 +1,482 @@
+"""ctypes binding of the CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: tests/, the smoke entry and bench.py's
+cpu_baseline leg import this module as the checker. The product package never
+imports it.
+
+Field elements cross as numpy ``uint64`` arrays of shape ``(..., 4)`` holding
+the reference's in-memory Montgomery limbs; affine G1 points as ``(..., 8)``
+(Montgomery x || y, all-zero = infinity); Jacobian points as ``(..., 12)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+FR, FQ = 0, 1
+
+TAG_COMMITMENT = 3
+TAG_SUMCHECK_ROUND = 4
+TAG_SUMCHECK_CHALLENGE = 5
+TAG_EVALUATION_CLAIM = 6
+TAG_WITNESS_DIGEST = 8
+TAG_SUMCHECK_FINAL_EVALS = 9
+TAG_MERCURY_BATCH = 17
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+P = C.c_void_p
+SZ = C.c_size_t
+U64 = C.c_uint64
+
+
+def _declare(L):
+    def d(name, args, res=None):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+    for n in ("orc_mul", "orc_add", "orc_sub"):
+        d(n, [C.c_int, P, P, P])
+    d("orc_neg", [C.c_int, P, P])
+    d("orc_square", [C.c_int, P, P])
+    d("orc_inverse", [C.c_int, P, P], C.c_int)
+    d("orc_pow", [C.c_int, P, P, P])
+    d("orc_from_u64", [C.c_int, U64, P])
+    d("orc_to_bytes", [C.c_int, P, P])
+    d("orc_from_bytes", [C.c_int, P, P], C.c_int)
+    d("orc_g1_add", [P, P, P])
+    d("orc_g1_add_affine", [P, P, P])
+    d("orc_g1_double", [P, P])
+    d("orc_g1_mul", [P, P, P])
+    d("orc_g1_mul_le_bytes", [P, P, P])
+    d("orc_g1_to_affine", [P, P])
+    d("orc_g1_from_affine", [P, P])
+    d("orc_g1_generator", [P])
+    d("orc_g1_affine_to_wire", [P, P])
+    d("orc_g1_affine_from_wire", [P, P], C.c_int)
+    d("orc_msm_window_bits", [SZ], SZ)
+    d("orc_msm", [P, P, SZ, SZ, SZ, P], C.c_int)
+    d("orc_msm_small_u32", [P, P, SZ, SZ, SZ, P], C.c_int)
+    d("orc_srs_powers", [P, SZ, SZ, P])
+    d("orc_g1_gen_mul_batch", [P, SZ, SZ, P])
+    d("orc_bind", [P, SZ, P])
+    d("orc_eq_table", [P, SZ, P])
+    d("orc_eq_eval", [P, P, SZ, P])
+    d("orc_evaluate", [P, SZ, P, P])
+    d("orc_poseidon2_permute", [P])
+    d("orc_tr_init", [P])
+    d("orc_tr_observe", [P, P])
+    d("orc_tr_sample", [P, P])
+    d("orc_tr_append_scalars", [P, U64, P, SZ])
+    d("orc_tr_append_bytes", [P, U64, P, SZ])
+    d("orc_tr_challenge", [P, U64, P])
+    d("orc_g1_limbs", [P, P])
+    d("orc_tr_append_g1", [P, U64, P, SZ])
+    d("orc_interpolate", [P, SZ, P])
+    d("orc_prove_zerocheck", [P, SZ, P, SZ, SZ, P, P, P])
+    d("orc_prove_sumcheck", [P, SZ, P, SZ, P, SZ, P, SZ, P, P])
+    d("orc_witness_digest", [P, SZ, SZ, P])
+    d("orc_splitmix_next", [P], U64)
+    d("orc_splitmix_next_fr", [P, P])
+    d("orc_chacha_block", [P, C.c_uint32, P, P])
+    d("orc_fill_uniform", [U64, P, SZ])
+    d("orc_fill_witness", [U64, P, SZ])
+
+
+def ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def fe(n=None):
+    return np.zeros((4,) if n is None else (n, 4), dtype=np.uint64)
+
+
+# ---------------------------------------------------------------- fields
+def from_bytes(which: int, b: bytes):
+    out = fe()
+    buf = np.frombuffer(bytes(b), dtype=np.uint8).copy()
+    ok = lib().orc_from_bytes(which, ptr(buf), ptr(out))
+    return out if ok else None
+
+
+def to_bytes(which: int, a) -> bytes:
+    out = np.zeros(32, dtype=np.uint8)
+    lib().orc_to_bytes(which, ptr(np.ascontiguousarray(a, dtype=np.uint64)), ptr(out))
+    return out.tobytes()
+
+
+def from_hex_le(which: int, h: str):
+    return from_bytes(which, bytes.fromhex(h))
+
+
+def to_hex_le(which: int, a) -> str:
+    return to_bytes(which, a).hex()
+
+
+def from_u64(which: int, x: int):
+    out = fe()
+    lib().orc_from_u64(which, x, ptr(out))
+    return out
+
+
+def binop(name, which, a, b):
+    out = fe()
+    getattr(lib(), "orc_" + name)(which, ptr(np.ascontiguousarray(a)), ptr(np.ascontiguousarray(b)), ptr(out))
+    return out
+
+
+def mul(w, a, b):
+    return binop("mul", w, a, b)
+
+
+def add(w, a, b):
+    return binop("add", w, a, b)
+
+
+def sub(w, a, b):
+    return binop("sub", w, a, b)
+
+
+def square(w, a):
+    out = fe()
+    lib().orc_square(w, ptr(np.ascontiguousarray(a)), ptr(out))
+    return out
+
+
+def neg(w, a):
+    out = fe()
+    lib().orc_neg(w, ptr(np.ascontiguousarray(a)), ptr(out))
+    return out
+
+
+def inverse(w, a):
+    out = fe()
+    ok = lib().orc_inverse(w, ptr(np.ascontiguousarray(a)), ptr(out))
+    return out if ok else None
+
+
+def pow_(w, a, e_limbs):
+    out = fe()
+    e = np.ascontiguousarray(e_limbs, dtype=np.uint64)
+    lib().orc_pow(w, ptr(np.ascontiguousarray(a)), ptr(e), ptr(out))
+    return out
+
+
+def one(w=FR):
+    return from_u64(w, 1)
+
+
+# ---------------------------------------------------------------- G1
+def generator_affine():
+    out = np.zeros(8, dtype=np.uint64)
+    lib().orc_g1_generator(ptr(out))
+    return out
+
+
+def jac_from_affine(a):
+    out = np.zeros(12, dtype=np.uint64)
+    lib().orc_g1_from_affine(ptr(np.ascontiguousarray(a)), ptr(out))
+    return out
+
+
+def jac_to_affine(p):
+    out = np.zeros(8, dtype=np.uint64)
+    lib().orc_g1_to_affine(ptr(np.ascontiguousarray(p)), ptr(out))
+    return out
+
+
+def jac_add(p, q):
+    out = np.zeros(12, dtype=np.uint64)
+    lib().orc_g1_add(ptr(np.ascontiguousarray(p)), ptr(np.ascontiguousarray(q)), ptr(out))
+    return out
+
+
+def jac_add_affine(p, q):
+    out = np.zeros(12, dtype=np.uint64)
+    lib().orc_g1_add_affine(ptr(np.ascontiguousarray(p)), ptr(np.ascontiguousarray(q)), ptr(out))
+    return out
+
+
+def jac_double(p):
+    out = np.zeros(12, dtype=np.uint64)
+    lib().orc_g1_double(ptr(np.ascontiguousarray(p)), ptr(out))
+    return out
+
+
+def jac_mul(p, s):
+    out = np.zeros(12, dtype=np.uint64)
+    lib().orc_g1_mul(ptr(np.ascontiguousarray(p)), ptr(np.ascontiguousarray(s)), ptr(out))
+    return out
+
+
+def jac_mul_le_bytes(p, k: bytes):
+    out = np.zeros(12, dtype=np.uint64)
+    kb = np.frombuffer(bytes(k), dtype=np.uint8).copy()
+    lib().orc_g1_mul_le_bytes(ptr(np.ascontiguousarray(p)), ptr(kb), ptr(out))
+    return out
+
+
+def affine_to_wire(a) -> bytes:
+    out = np.zeros(64, dtype=np.uint8)
+    lib().orc_g1_affine_to_wire(ptr(np.ascontiguousarray(a)), ptr(out))
+    return out.tobytes()
+
+
+def affine_from_wire(b: bytes):
+    out = np.zeros(8, dtype=np.uint64)
+    buf = np.frombuffer(bytes(b), dtype=np.uint8).copy()
+    ok = lib().orc_g1_affine_from_wire(ptr(buf), ptr(out))
+    return out if ok else None
+
+
+def jac_to_wire(p) -> bytes:
+    return affine_to_wire(jac_to_affine(p))
+
+
+# ---------------------------------------------------------------- MSM
+def window_bits(n: int) -> int:
+    return lib().orc_msm_window_bits(n)
+
+
+def msm(bases_aff: np.ndarray, scalars: np.ndarray, threads: int = 8, workers: int = 1):
+    n = scalars.shape[0]
+    out = np.zeros(12, dtype=np.uint64)
+    lib().orc_msm(ptr(np.ascontiguousarray(bases_aff)), ptr(np.ascontiguousarray(scalars)), n, threads, workers, ptr(out))
+    return out
+
+
+def msm_small_u32(bases_aff, scalars_u32, threads: int = 8, workers: int = 1):
+    n = scalars_u32.shape[0]
+    out = np.zeros(12, dtype=np.uint64)
+    s = np.ascontiguousarray(scalars_u32, dtype=np.uint32)
+    lib().orc_msm_small_u32(ptr(np.ascontiguousarray(bases_aff)), ptr(s), n, threads, workers, ptr(out))
+    return out
+
+
+def srs_powers(tau, n: int, workers: int = 8):
+    out = np.zeros((n, 8), dtype=np.uint64)
+    lib().orc_srs_powers(ptr(np.ascontiguousarray(tau)), n, workers, ptr(out))
+    return out
+
+
+def gen_mul_batch(scalars, workers: int = 8):
+    n = scalars.shape[0]
+    out = np.zeros((n, 8), dtype=np.uint64)
+    lib().orc_g1_gen_mul_batch(ptr(np.ascontiguousarray(scalars)), n, workers, ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- poly
+def bind(table: np.ndarray, r) -> np.ndarray:
+    t = np.ascontiguousarray(table.copy())
+    lib().orc_bind(ptr(t), t.shape[0], ptr(np.ascontiguousarray(r)))
+    return t[: t.shape[0] // 2].copy()
+
+
+def eq_table(r: np.ndarray) -> np.ndarray:
+    n = r.shape[0]
+    out = fe(1 << n)
+    lib().orc_eq_table(ptr(np.ascontiguousarray(r)), n, ptr(out))
+    return out
+
+
+def eq_eval(r, y):
+    out = fe()
+    lib().orc_eq_eval(ptr(np.ascontiguousarray(r)), ptr(np.ascontiguousarray(y)), r.shape[0], ptr(out))
+    return out
+
+
+def evaluate(table, point):
+    out = fe()
+    lib().orc_evaluate(ptr(np.ascontiguousarray(table)), point.shape[0], ptr(np.ascontiguousarray(point)), ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- transcript
+class OrcTranscript(C.Structure):
+    _fields_ = [
+        ("state", C.c_uint64 * 12),
+        ("input", C.c_uint64 * 8),
+        ("input_len", C.c_uint64),
+        ("output", C.c_uint64 * 8),
+        ("output_len", C.c_uint64),
+    ]
+
+
+class Transcript:
+    """The reference's duplex transcript (crates/transcript/src/lib.rs)."""
+
+    def __init__(self):
+        self.t = OrcTranscript()
+        lib().orc_tr_init(C.byref(self.t))
+
+    def ref(self):
+        return C.byref(self.t)
+
+    def observe(self, x):
+        lib().orc_tr_observe(self.ref(), ptr(np.ascontiguousarray(x)))
+
+    def sample(self):
+        out = fe()
+        lib().orc_tr_sample(self.ref(), ptr(out))
+        return out
+
+    def append_scalars(self, tag, xs):
+        xs = np.ascontiguousarray(np.asarray(xs, dtype=np.uint64).reshape(-1, 4))
+        lib().orc_tr_append_scalars(self.ref(), tag, ptr(xs), xs.shape[0])
+
+    def append_bytes(self, tag, b: bytes):
+        buf = np.frombuffer(bytes(b) + b"\0", dtype=np.uint8).copy()
+        lib().orc_tr_append_bytes(self.ref(), tag, ptr(buf), len(b))
+
+    def challenge(self, tag):
+        out = fe()
+        lib().orc_tr_challenge(self.ref(), tag, ptr(out))
+        return out
+
+    def append_g1(self, tag, wires):
+        buf = np.frombuffer(b"".join(wires) + b"\0", dtype=np.uint8).copy()
+        lib().orc_tr_append_g1(self.ref(), tag, ptr(buf), len(wires))
+
+    def snapshot(self) -> bytes:
+        return bytes(self.t)
+
+
+def poseidon2_permute(state: np.ndarray) -> np.ndarray:
+    s = np.ascontiguousarray(state.copy().reshape(3, 4))
+    lib().orc_poseidon2_permute(ptr(s))
+    return s
+
+
+def g1_limbs(wire: bytes) -> np.ndarray:
+    out = fe(4)
+    buf = np.frombuffer(bytes(wire), dtype=np.uint8).copy()
+    lib().orc_g1_limbs(ptr(buf), ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- sumcheck
+class OrcTerm(C.Structure):
+    _fields_ = [("coef", C.c_uint64 * 4), ("f", C.c_int32 * 3)]
+
+
+def make_terms(terms):
+    """terms: list of (coef_limbs, [factor indices]) with 1..3 factors."""
+    arr = (OrcTerm * max(1, len(terms)))()
+    for i, (coef, fs) in enumerate(terms):
+        for k in range(4):
+            arr[i].coef[k] = int(coef[k])
+        fl = list(fs) + [-1] * (3 - len(fs))
+        for k in range(3):
+            arr[i].f[k] = fl[k]
+    return arr
+
+
+def interpolate(vals: np.ndarray) -> np.ndarray:
+    d = vals.shape[0] - 1
+    out = fe(d + 1)
+    lib().orc_interpolate(ptr(np.ascontiguousarray(vals)), d, ptr(out))
+    return out
+
+
+def _table_ptrs(tables):
+    arr = (C.c_void_p * len(tables))()
+    for i, t in enumerate(tables):
+        arr[i] = t.ctypes.data
+    return arr
+
+
+def gate_degree(terms) -> int:
+    mx = max([len(fs) for _, fs in terms] + [1])
+    return max(3, mx + 1)
+
+
+def prove_sumcheck(eq_point, terms, tables, transcript: Transcript, workers: int = 1):
+    """gkr::prove_sumcheck with a gate summand. Tables are copied (the copies
+    are left bound); returns (rounds[n, d+1, 4], point[n, 4])."""
+    n = eq_point.shape[0]
+    d = gate_degree(terms)
+    work = [np.ascontiguousarray(t.copy()) for t in tables]
+    rounds = np.zeros((n, d + 1, 4), dtype=np.uint64)
+    point = fe(n)
+    tp = make_terms(terms)
+    lib().orc_prove_sumcheck(ptr(np.ascontiguousarray(eq_point)), n, tp, len(terms), _table_ptrs(work), len(work),
+                             transcript.ref(), workers, ptr(rounds), ptr(point))
+    return rounds, point, [w[:1].copy() for w in work]
+
+
+def prove_zerocheck(terms, tables, transcript: Transcript):
+    n = int(tables[0].shape[0]).bit_length() - 1
+    d = gate_degree(terms)
+    work = [np.ascontiguousarray(t.copy()) for t in tables]
+    rounds = np.zeros((n, d + 1, 4), dtype=np.uint64)
+    finals = fe(len(tables))
+    tp = make_terms(terms)
+    lib().orc_prove_zerocheck(tp, len(terms), _table_ptrs(work), len(work), n, transcript.ref(), ptr(rounds), ptr(finals))
+    return rounds, finals
+
+
+def witness_digest(tables):
+    n = int(tables[0].shape[0]).bit_length() - 1
+    work = [np.ascontiguousarray(t) for t in tables]
+    out = fe()
+    lib().orc_witness_digest(_table_ptrs(work), len(work), n, ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- inputs
+class SplitMix:
+    def __init__(self, seed: int):
+        self.s = np.array([seed], dtype=np.uint64)
+
+    def next_u64(self) -> int:
+        return int(lib().orc_splitmix_next(ptr(self.s)))
+
+    def next_fr(self):
+        out = fe()
+        lib().orc_splitmix_next_fr(ptr(self.s), ptr(out))
+        return out
+
+
+def chacha_block(key32: bytes, counter: int, nonce12: bytes) -> bytes:
+    k = np.frombuffer(key32, dtype=np.uint32).copy()
+    nn = np.frombuffer(nonce12, dtype=np.uint32).copy()
+    out = np.zeros(64, dtype=np.uint8)
+    lib().orc_chacha_block(ptr(k), counter, ptr(nn), ptr(out))
+    return out.tobytes()
+
+
+def fill_uniform(seed: int, n: int) -> np.ndarray:
+    out = fe(n)
+    lib().orc_fill_uniform(seed, ptr(out), n)
+    return out
+
+
+def fill_witness(seed: int, n: int) -> np.ndarray:
+    out = fe(n)
+    lib().orc_fill_witness(seed, ptr(out), n)
+    return out

@@ -1,0 +1,151 @@
+"""Oracle sumcheck against the definition (the reference's own pin:
+crates/sumcheck/tests/oracle.rs:45-116, crates/gkr/tests/oracle.rs) and the
+round-sum invariants (crates/sumcheck/src/lib.rs:441-459); the commit path
+against the committed Mercury commitment (crates/pcs/tests/vectors/mercury_proof.txt)."""
+import numpy as np
+import pytest
+
+from conftest import golden_lines
+
+
+def _rand_tables(orc, seed, k, n):
+    rng = orc.SplitMix(seed)
+    return [np.stack([rng.next_fr() for _ in range(1 << n)]) for _ in range(k)]
+
+
+def _horner(orc, coeffs, x):
+    acc = orc.fe()
+    for c in reversed(list(coeffs)):
+        acc = orc.add(orc.FR, orc.mul(orc.FR, acc, x), c)
+    return acc
+
+
+def _gate_value(orc, terms, vals):
+    acc = orc.fe()
+    for coef, fs in terms:
+        v = coef
+        for f in fs:
+            v = orc.mul(orc.FR, v, vals[f])
+        acc = orc.add(orc.FR, acc, v)
+    return acc
+
+
+def _naive_round_value(orc, tables, terms, eq_point, bound, x):
+    n = eq_point.shape[0]
+    rest = n - len(bound) - 1
+    one, zero = orc.from_u64(orc.FR, 1), orc.fe()
+    acc = orc.fe()
+    for y in range(1 << rest):
+        pt = list(bound) + [x] + [one if (y >> j) & 1 else zero for j in range(rest)]
+        pt = np.stack(pt)
+        vals = [orc.evaluate(t, pt) for t in tables]
+        acc = orc.add(orc.FR, acc, orc.mul(orc.FR, orc.eq_eval(eq_point, pt), _gate_value(orc, terms, vals)))
+    return acc
+
+
+def _gates(orc):
+    one = orc.from_u64(orc.FR, 1)
+    minus_one = orc.sub(orc.FR, orc.fe(), one)
+    three = orc.from_u64(orc.FR, 3)
+    return {
+        "product": (2, [(one, [0, 1])]),
+        "square_minus": (2, [(one, [0, 0]), (minus_one, [1])]),
+        "wide": (5, [(one, [0, 1]), (one, [0, 2]), (minus_one, [3, 4])]),
+        "linear": (1, [(three, [0])]),
+        "triple": (3, [(one, [0, 1, 2])]),
+    }
+
+
+@pytest.mark.parametrize("gate", ["product", "square_minus", "wide", "linear", "triple"])
+@pytest.mark.parametrize("n", [1, 2, 3, 4])
+def test_prove_sumcheck_matches_definition(orc, gate, n):
+    k, terms = _gates(orc)[gate]
+    tables = _rand_tables(orc, 100 * n + len(gate), k, n)
+    rng = orc.SplitMix(7 + n)
+    eq_point = np.stack([rng.next_fr() for _ in range(n)])
+    t = orc.Transcript()
+    rounds, point, finals = orc.prove_sumcheck(eq_point, terms, tables, t)
+    d = rounds.shape[1] - 1
+    assert d == (4 if gate == "triple" else 3)
+    claim = None
+    for i in range(n):
+        for node in range(d + 1):
+            x = orc.from_u64(orc.FR, node)
+            want = _naive_round_value(orc, tables, terms, eq_point, point[:i], x)
+            assert np.array_equal(_horner(orc, rounds[i], x), want), (i, node)
+        s01 = orc.add(orc.FR, _horner(orc, rounds[i], orc.fe()), _horner(orc, rounds[i], orc.from_u64(orc.FR, 1)))
+        if claim is not None:
+            assert np.array_equal(s01, claim)
+        claim = _horner(orc, rounds[i], point[i])
+    # final check: claim = eq(eq_point, point) * G(final evals)
+    fe = [f[0] for f in finals]
+    assert np.array_equal(claim, orc.mul(orc.FR, orc.eq_eval(eq_point, point), _gate_value(orc, terms, fe)))
+
+
+def test_zerocheck_on_a_satisfying_witness(orc):
+    """square gate x0*x0 - x1 on a witness with x1 = x0^2: rounds sum to zero and
+    the verifier's equation holds (crates/sumcheck/src/lib.rs:418-465)."""
+    one = orc.from_u64(orc.FR, 1)
+    minus_one = orc.sub(orc.FR, orc.fe(), one)
+    terms = [(one, [0, 0]), (minus_one, [1])]
+    n = 4
+    a = _rand_tables(orc, 31, 1, n)[0]
+    b = np.stack([orc.mul(orc.FR, v, v) for v in a])
+    t = orc.Transcript()
+    rounds, finals = orc.prove_zerocheck(terms, [a, b], t)
+    # verifier replay
+    v = orc.Transcript()
+    r = np.stack([v.challenge(orc.TAG_SUMCHECK_CHALLENGE) for _ in range(n)])
+    claim = orc.fe()
+    pt = []
+    for i in range(n):
+        s01 = orc.add(orc.FR, _horner(orc, rounds[i], orc.fe()), _horner(orc, rounds[i], one))
+        assert np.array_equal(s01, claim)
+        v.append_scalars(orc.TAG_SUMCHECK_ROUND, rounds[i])
+        c = v.challenge(orc.TAG_SUMCHECK_CHALLENGE)
+        claim = _horner(orc, rounds[i], c)
+        pt.append(c)
+    g = _gate_value(orc, terms, [finals[0], finals[1]])
+    assert np.array_equal(claim, orc.mul(orc.FR, orc.eq_eval(r, np.stack(pt)), g))
+    v.append_scalars(orc.TAG_SUMCHECK_FINAL_EVALS, finals)
+    assert v.snapshot() == t.snapshot()
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 5])
+def test_interpolation_hits_every_node(orc, d):
+    rng = orc.SplitMix(d)
+    vals = np.stack([rng.next_fr() for _ in range(d + 1)])
+    c = orc.interpolate(vals)
+    for node in range(d + 1):
+        assert np.array_equal(_horner(orc, c, orc.from_u64(orc.FR, node)), vals[node])
+
+
+def test_interpolate_cubic_closed_form_equals_newton(orc):
+    # the d == 3 closed form (sumcheck/src/lib.rs:294-305) against the general
+    # Newton path, reached through a degree-3 polynomial padded to d = 4
+    rng = orc.SplitMix(44)
+    vals = np.stack([rng.next_fr() for _ in range(4)])
+    c3 = orc.interpolate(vals)
+    v4 = np.concatenate([vals, _horner(orc, c3, orc.from_u64(orc.FR, 4))[None]])
+    c4 = orc.interpolate(v4)
+    assert np.array_equal(c4[:4], c3) and not c4[4].any()
+
+
+def test_witness_digest_is_a_sponge_over_cells(orc):
+    tables = _rand_tables(orc, 5, 2, 3)
+    d1 = orc.witness_digest(tables)
+    tables[1][5, 0] ^= 1
+    assert not np.array_equal(d1, orc.witness_digest(tables))
+
+
+def test_mercury_commitment_kat(orc):
+    """pcs::commit over the toy SRS (tau = 0xabcdef) equals the committed
+    commitment line of mercury_proof.txt (crates/pcs/src/lib.rs:237-247)."""
+    recs = list(golden_lines("mercury_proof.txt"))
+    tau = orc.from_hex_le(orc.FR, next(r for r in recs if r[0] == "tau")[1])
+    nv = int(next(r for r in recs if r[0] == "numvars")[1])
+    vals = {int(r[1]): r[2] for r in recs if r[0] == "value"}
+    f = np.stack([orc.from_hex_le(orc.FR, vals[i]) for i in range(1 << nv)])
+    srs = orc.srs_powers(tau, 1 << nv)
+    cm = orc.jac_to_wire(orc.msm(srs, f)).hex()
+    assert cm == next(r for r in recs if r[0] == "commitment")[1]
