@@ -1,0 +1,654 @@
+"""apogee-vm_amd: the MI355X (gfx950) per-shard proving path of apogee-vm.
+
+Host-side mirror of the reference's interfaces for this path, over the C ABI of
+``libapogee_gpu.so`` (include/apogee_gpu.h):
+
+- ``msm`` / ``msm_small_u32``          crates/curve/src/msm.rs:48 / :79
+- ``Srs``, ``commit``, ``commit_batch``, ``kzg_commit``
+                                        crates/srs/src/lib.rs:96, crates/pcs/src/lib.rs:237,
+                                        crates/prover/src/lib.rs:444, crates/srs/src/kzg.rs:20
+- ``Gate`` / ``GateTerm``, ``prove_zerocheck``
+                                        crates/sumcheck/src/lib.rs:88-151, :359
+- ``prove_sumcheck``                    crates/gkr/src/lib.rs:792
+- ``Transcript``                        crates/transcript/src/lib.rs:197-346
+- ``prove_shard``                       commit + batched sumcheck of one shard
+
+Field elements are numpy ``uint64`` arrays of shape ``(..., 4)`` holding the
+reference's in-memory Montgomery limbs (``Fr::to_memory_bytes``); affine G1
+points are 64 bytes, Montgomery ``x || y``, all-zero for infinity, or the
+canonical wire form where a function says so. There is no CPU fallback: the
+library must be built (``python apogee-vm_amd/build.py``) and a GPU present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libapogee_gpu.so")
+
+# crates/constants/src/lib.rs:579-770
+TAG_COMMITMENT = 3
+TAG_SUMCHECK_ROUND = 4
+TAG_SUMCHECK_CHALLENGE = 5
+TAG_EVALUATION_CLAIM = 6
+TAG_WITNESS_DIGEST = 8
+TAG_SUMCHECK_FINAL_EVALS = 9
+TAG_MERCURY_BATCH = 17
+
+# crates/pcs-verify/src/lib.rs:89-103
+MAX_NUM_VARS = 2 * (28 - 1)
+
+_lib = None
+
+
+class ApgError(RuntimeError):
+    """A negative return code of the C ABI, with apg_last_error()'s message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"apogee_gpu error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run `python apogee-vm_amd/build.py` (there is no CPU fallback)"
+            )
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+VP = C.c_void_p
+SZ = C.c_size_t
+U64 = C.c_uint64
+I32 = C.c_int
+
+
+class ApgTerm(C.Structure):
+    _fields_ = [("coef_mont", C.c_uint8 * 32), ("factor", C.c_int32 * 3)]
+
+
+ROUND_FN = C.CFUNCTYPE(C.c_int, VP, VP, SZ, VP)
+
+
+def _declare(L):
+    def d(name, args, res=I32):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+
+    d("apg_last_error", [], C.c_char_p)
+    d("apg_version", [])
+    d("apg_ctx_create", [I32, C.POINTER(VP)])
+    d("apg_ctx_destroy", [VP], None)
+    d("apg_ctx_synchronize", [VP])
+    d("apg_dev_alloc", [VP, SZ, C.POINTER(VP)])
+    d("apg_dev_free", [VP, VP])
+    d("apg_dev_upload", [VP, VP, VP, SZ])
+    d("apg_dev_download", [VP, VP, VP, SZ])
+    d("apg_srs_upload", [VP, VP, SZ, C.POINTER(VP)])
+    d("apg_srs_generate", [VP, VP, SZ, C.POINTER(VP)])
+    d("apg_srs_download", [VP, VP, SZ, SZ, VP])
+    d("apg_srs_len", [VP], SZ)
+    d("apg_srs_prepare", [VP, VP, SZ])
+    d("apg_srs_device_bytes", [VP], SZ)
+    d("apg_srs_release", [VP], None)
+    d("apg_msm_g1_fr", [VP, VP, VP, SZ, VP])
+    d("apg_msm_g1_u32", [VP, VP, VP, SZ, VP])
+    d("apg_commit_fr", [VP, VP, VP, SZ, SZ, VP])
+    d("apg_commit_u32", [VP, VP, VP, SZ, SZ, VP])
+    d("apg_commit_fr_dev", [VP, VP, VP, SZ, SZ, VP])
+    d("apg_sumcheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
+    d("apg_sumcheck_prove_dev", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
+    d("apg_zerocheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, VP, VP])
+    d("apg_prove_shard_dev", [VP, VP, VP, SZ, SZ, I32, VP, VP, VP, VP])
+    d("apg_transcript_new", [C.POINTER(VP)])
+    d("apg_transcript_free", [VP], None)
+    d("apg_transcript_clone", [VP, C.POINTER(VP)])
+    d("apg_transcript_observe", [VP, VP], None)
+    d("apg_transcript_sample", [VP, VP], None)
+    d("apg_transcript_append_scalars", [VP, U64, VP, SZ], None)
+    d("apg_transcript_append_bytes", [VP, U64, VP, SZ], None)
+    d("apg_transcript_challenge", [VP, U64, VP], None)
+    d("apg_transcript_append_g1", [VP, U64, VP, SZ], None)
+    d("apg_transcript_round", [VP, VP, SZ, VP])
+    d("apg_transcript_state", [VP, VP])
+    d("apg_fr_to_bytes", [VP, VP])
+    d("apg_fr_from_bytes", [VP, VP])
+    d("apg_fq_to_bytes", [VP, VP])
+    d("apg_fq_from_bytes", [VP, VP])
+    d("apg_fr_from_u64", [U64, VP], None)
+    d("apg_g1_to_wire", [VP, VP])
+    d("apg_g1_from_wire", [VP, VP])
+    d("apg_ctx_set_profiling", [VP, I32])
+    d("apg_ctx_stats", [VP, C.POINTER(ApgStats)])
+    d("apg_ctx_reset_stats", [VP])
+    d("apg_synthetic_fill", [U64, I32, SZ, VP])
+
+
+class ApgStats(C.Structure):
+    _fields_ = [
+        ("accumulate_ms", C.c_double),
+        ("accumulate_launches", C.c_uint64),
+        ("accumulate_entries", C.c_uint64),
+        ("msm_ms", C.c_double),
+        ("msm_calls", C.c_uint64),
+        ("msm_points", C.c_uint64),
+        ("sumcheck_kernel_ms", C.c_double),
+        ("sumcheck_launches", C.c_uint64),
+        ("sumcheck_bytes", C.c_uint64),
+        ("sumcheck_ms", C.c_double),
+    ]
+
+
+def synthetic(seed: int, n: int, kind: str = "uniform") -> np.ndarray:
+    """n seeded Montgomery Fr elements: kind "uniform" or "witness" (20% zero,
+    10% one, 50% u32, 20% uniform Fr), from ChaCha20 keyed by seed."""
+    out = np.zeros((n, 4), dtype=np.uint64)
+    _check(lib().apg_synthetic_fill(seed, {"uniform": 0, "witness": 1}[kind], n, _p(out)))
+    return out
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise ApgError(rc, lib().apg_last_error().decode(errors="replace"))
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(VP)
+
+
+def _fe_array(x) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+    return a.reshape(-1, 4)
+
+
+# ------------------------------------------------------------------ encodings
+def fr_from_u64(x: int) -> np.ndarray:
+    out = np.zeros(4, dtype=np.uint64)
+    lib().apg_fr_from_u64(x, _p(out))
+    return out
+
+
+def fr_from_bytes(b: bytes) -> np.ndarray:
+    src = np.frombuffer(bytes(b), dtype=np.uint8).copy()
+    out = np.zeros(4, dtype=np.uint64)
+    _check(lib().apg_fr_from_bytes(_p(src), _p(out)))
+    return out
+
+
+def fr_to_bytes(a) -> bytes:
+    src = np.ascontiguousarray(np.asarray(a, dtype=np.uint64).reshape(4))
+    out = np.zeros(32, dtype=np.uint8)
+    _check(lib().apg_fr_to_bytes(_p(src), _p(out)))
+    return out.tobytes()
+
+
+def g1_to_wire(mont: bytes) -> bytes:
+    src = np.frombuffer(bytes(mont), dtype=np.uint8).copy()
+    out = np.zeros(64, dtype=np.uint8)
+    _check(lib().apg_g1_to_wire(_p(src), _p(out)))
+    return out.tobytes()
+
+
+def g1_from_wire(wire: bytes) -> bytes:
+    src = np.frombuffer(bytes(wire), dtype=np.uint8).copy()
+    out = np.zeros(64, dtype=np.uint8)
+    _check(lib().apg_g1_from_wire(_p(src), _p(out)))
+    return out.tobytes()
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One HIP stream and scratch arena on one device (apg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = VP()
+        _check(lib().apg_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().apg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().apg_ctx_synchronize(self.h))
+
+    def set_profiling(self, on: bool = True):
+        _check(lib().apg_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def stats(self) -> dict:
+        s = ApgStats()
+        _check(lib().apg_ctx_stats(self.h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in ApgStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().apg_ctx_reset_stats(self.h))
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class DeviceBuffer:
+    """Device memory owned by the library's allocator (apg_dev_alloc)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        p = VP()
+        _check(lib().apg_dev_alloc(ctx.h, nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, ctx: Context, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes)
+        _check(lib().apg_dev_upload(ctx.h, b.ptr, _p(a), a.nbytes))
+        return b
+
+    def download(self, dtype=np.uint64) -> np.ndarray:
+        out = np.zeros(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        _check(lib().apg_dev_download(self.ctx.h, _p(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(lib().apg_dev_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ MSM
+class MsmError(Exception):
+    """crates/curve/src/msm.rs:29-33"""
+
+
+class LengthMismatch(MsmError):
+    def __init__(self, bases: int, scalars: int):
+        super().__init__(f"LengthMismatch {{ bases: {bases}, scalars: {scalars} }}")
+        self.bases, self.scalars = bases, scalars
+
+
+def _bases_array(bases) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(bases, dtype=np.uint64).reshape(-1, 8))
+
+
+def msm(bases, scalars, ctx: Optional[Context] = None) -> bytes:
+    """curve::msm::msm: sum_i scalars[i] * bases[i] (bases affine Montgomery
+    (n, 8) uint64; scalars Montgomery Fr (n, 4)). Returns the affine result,
+    64 Montgomery bytes (all-zero = identity)."""
+    b = _bases_array(bases)
+    s = _fe_array(scalars) if len(scalars) else np.zeros((0, 4), dtype=np.uint64)
+    if b.shape[0] != s.shape[0]:
+        raise LengthMismatch(b.shape[0], s.shape[0])
+    ctx = ctx or default_context()
+    out = np.zeros(64, dtype=np.uint8)
+    _check(lib().apg_msm_g1_fr(ctx.h, _p(b), _p(s), s.shape[0], _p(out)))
+    return out.tobytes()
+
+
+def msm_small_u32(bases, scalars, ctx: Optional[Context] = None) -> bytes:
+    """curve::msm::msm_small_u32 over u32 scalars."""
+    b = _bases_array(bases)
+    s = np.ascontiguousarray(np.asarray(scalars, dtype=np.uint32).reshape(-1))
+    if b.shape[0] != s.shape[0]:
+        raise LengthMismatch(b.shape[0], s.shape[0])
+    ctx = ctx or default_context()
+    out = np.zeros(64, dtype=np.uint8)
+    _check(lib().apg_msm_g1_u32(ctx.h, _p(b), _p(s), s.shape[0], _p(out)))
+    return out.tobytes()
+
+
+# ------------------------------------------------------------------ SRS + commit
+class PcsError(Exception):
+    """crates/pcs-verify/src/lib.rs:40-87 (the variants this path raises)."""
+
+
+class UnsupportedNumVars(PcsError):
+    def __init__(self, num_vars: int):
+        super().__init__(f"UnsupportedNumVars {{ num_vars: {num_vars} }}")
+        self.num_vars = num_vars
+
+
+class SrsTooSmall(PcsError):
+    def __init__(self, needed: int, available: int):
+        super().__init__(f"SrsTooSmall {{ needed: {needed}, available: {available} }}")
+        self.needed, self.available = needed, available
+
+
+def check_num_vars(num_vars: int) -> int:
+    """crates/pcs-verify/src/lib.rs:97-103: n = 2^(2t), 1 <= t <= 27."""
+    if not (2 <= num_vars <= MAX_NUM_VARS) or num_vars % 2:
+        raise UnsupportedNumVars(num_vars)
+    return 1 << num_vars
+
+
+class Srs:
+    """The powers [tau^i]_1, resident in HBM (crates/srs/src/lib.rs:96)."""
+
+    def __init__(self, handle, ctx: Context):
+        self.h = handle
+        self.ctx = ctx
+
+    @classmethod
+    def upload(cls, bases, ctx: Optional[Context] = None) -> "Srs":
+        ctx = ctx or default_context()
+        b = _bases_array(bases)
+        h = VP()
+        _check(lib().apg_srs_upload(ctx.h, _p(b), b.shape[0], C.byref(h)))
+        return cls(h, ctx)
+
+    @classmethod
+    def generate(cls, tau, n: int, ctx: Optional[Context] = None) -> "Srs":
+        """[tau^i]G for i < n, built on the device."""
+        ctx = ctx or default_context()
+        t = np.ascontiguousarray(np.asarray(tau, dtype=np.uint64).reshape(4))
+        h = VP()
+        _check(lib().apg_srs_generate(ctx.h, _p(t), n, C.byref(h)))
+        return cls(h, ctx)
+
+    def __len__(self):
+        return lib().apg_srs_len(self.h)
+
+    def g1(self, offset: int = 0, n: Optional[int] = None) -> np.ndarray:
+        n = len(self) - offset if n is None else n
+        out = np.zeros((n, 8), dtype=np.uint64)
+        _check(lib().apg_srs_download(self.ctx.h, self.h, offset, n, _p(out)))
+        return out
+
+    def prepare(self, n: int):
+        _check(lib().apg_srs_prepare(self.ctx.h, self.h, n))
+
+    def device_bytes(self) -> int:
+        return lib().apg_srs_device_bytes(self.h)
+
+    def release(self):
+        if self.h:
+            lib().apg_srs_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+_SMALL = (np.bool_, np.uint8, np.uint16, np.uint32)
+
+
+def _column_kind(col: np.ndarray) -> str:
+    if col.dtype.type in _SMALL:
+        return "u32"
+    if col.dtype == np.uint64 and col.ndim == 2 and col.shape[1] == 4:
+        return "fr"
+    raise TypeError("a column is Fr limbs (n, 4) uint64 or a u1/u8/u16/u32 array")
+
+
+def kzg_commit(srs: Srs, coeffs, ctx: Optional[Context] = None) -> bytes:
+    """srs::kzg::kzg_commit (crates/srs/src/kzg.rs:20-31): one MSM over the
+    first n powers, no size rule. Dispatches on the backing like
+    pcs::column_msm (crates/pcs/src/lib.rs:280-298). Returns affine Montgomery."""
+    return commit_batch(srs, [coeffs], ctx)[0]
+
+
+def commit(srs: Srs, f, ctx: Optional[Context] = None) -> bytes:
+    """pcs::commit (crates/pcs/src/lib.rs:237-247): check_num_vars, then the
+    MSM of the evaluation table against the first n powers."""
+    n = len(f)
+    nv = n.bit_length() - 1
+    if n != 1 << nv:
+        raise ValueError("a multilinear table holds a power of two entries")
+    check_num_vars(nv)
+    if len(srs) < n:
+        raise SrsTooSmall(n, len(srs))
+    return kzg_commit(srs, f, ctx)
+
+
+def commit_batch(srs: Srs, columns: Sequence, ctx: Optional[Context] = None) -> list:
+    """Every column of a shard in one call (prover::commit_all,
+    crates/prover/src/lib.rs:444-455): same-kind, same-length columns go to
+    the device together. Returns affine Montgomery points, one per column."""
+    ctx = ctx or srs.ctx
+    cols = [np.asarray(c) for c in columns]
+    if not cols:
+        return []
+    n = len(cols[0])
+    if any(len(c) != n for c in cols):
+        raise ValueError("commit_batch: columns of one call share a length")
+    if len(srs) < n:
+        raise SrsTooSmall(n, len(srs))
+    kinds = {_column_kind(c) for c in cols}
+    k = len(cols)
+    out = np.zeros(64 * k, dtype=np.uint8)
+    if kinds == {"u32"}:
+        s = np.ascontiguousarray(np.stack([c.astype(np.uint32) for c in cols]))
+        _check(lib().apg_commit_u32(ctx.h, srs.h, _p(s), n, k, _p(out)))
+    elif kinds == {"fr"}:
+        s = np.ascontiguousarray(np.stack(cols))
+        _check(lib().apg_commit_fr(ctx.h, srs.h, _p(s), n, k, _p(out)))
+    else:
+        return [kzg_commit(srs, c, ctx) for c in cols]
+    return [out[64 * i: 64 * (i + 1)].tobytes() for i in range(k)]
+
+
+# ------------------------------------------------------------------ transcript
+class Transcript:
+    """The reference's Poseidon2 duplex transcript, native."""
+
+    def __init__(self, _h=None):
+        if _h is None:
+            _h = VP()
+            _check(lib().apg_transcript_new(C.byref(_h)))
+        self.h = _h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().apg_transcript_free(self.h)
+        except Exception:
+            pass
+
+    def clone(self) -> "Transcript":
+        h = VP()
+        _check(lib().apg_transcript_clone(self.h, C.byref(h)))
+        return Transcript(h)
+
+    def observe(self, x):
+        lib().apg_transcript_observe(self.h, _p(_fe_array(x)))
+
+    def sample(self) -> np.ndarray:
+        out = np.zeros(4, dtype=np.uint64)
+        lib().apg_transcript_sample(self.h, _p(out))
+        return out
+
+    def append_scalar(self, tag: int, x):
+        self.append_scalars(tag, [np.asarray(x, dtype=np.uint64).reshape(4)])
+
+    def append_scalars(self, tag: int, xs):
+        a = _fe_array(xs) if len(xs) else np.zeros((0, 4), dtype=np.uint64)
+        lib().apg_transcript_append_scalars(self.h, tag, _p(a) if a.size else None, a.shape[0])
+
+    def append_bytes(self, tag: int, data: bytes):
+        buf = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8).copy()
+        lib().apg_transcript_append_bytes(self.h, tag, _p(buf), len(data))
+
+    def challenge_scalar(self, tag: int) -> np.ndarray:
+        out = np.zeros(4, dtype=np.uint64)
+        lib().apg_transcript_challenge(self.h, tag, _p(out))
+        return out
+
+    def append_g1_points(self, tag: int, wires: Sequence[bytes]):
+        buf = np.frombuffer(b"".join(wires) + b"\0", dtype=np.uint8).copy()
+        lib().apg_transcript_append_g1(self.h, tag, _p(buf), len(wires))
+
+    def state(self) -> bytes:
+        out = np.zeros(232, dtype=np.uint8)
+        _check(lib().apg_transcript_state(self.h, _p(out)))
+        return out.tobytes()
+
+
+# ------------------------------------------------------------------ sumcheck
+@dataclass
+class GateTerm:
+    """coef * inputs[a] * inputs[b] (* inputs[c]) (crates/sumcheck/src/lib.rs:88-96);
+    the third factor is the degree-4 extension."""
+
+    coef: np.ndarray
+    a: int
+    b: Optional[int] = None
+    c: Optional[int] = None
+
+
+class GateError(Exception):
+    pass
+
+
+class Gate:
+    """crates/sumcheck/src/lib.rs:113-151: a formula over declared inputs."""
+
+    def __init__(self, inputs: Sequence[int], terms: Sequence[GateTerm]):
+        if not inputs:
+            raise GateError("NoInputs")
+        for i in range(len(inputs)):
+            for j in range(i):
+                if inputs[i] == inputs[j]:
+                    raise GateError(f"DuplicateInput {{ input: {i} }}")
+        for t, term in enumerate(terms):
+            for idx in (term.a, term.b, term.c):
+                if idx is not None and idx >= len(inputs):
+                    raise GateError(f"TermIndexOutOfRange {{ term: {t}, index: {idx} }}")
+        self.inputs = list(inputs)
+        self.terms = list(terms)
+
+    @property
+    def arity(self) -> int:
+        return len(self.inputs)
+
+    def degree(self) -> int:
+        mx = max([1 + (t.b is not None) + (t.c is not None) for t in self.terms] + [1])
+        return max(3, mx + 1)
+
+    def c_terms(self):
+        arr = (ApgTerm * max(1, len(self.terms)))()
+        for i, t in enumerate(self.terms):
+            coef = np.ascontiguousarray(np.asarray(t.coef, dtype=np.uint64).reshape(4)).view(np.uint8)
+            for k in range(32):
+                arr[i].coef_mont[k] = int(coef[k])
+            fs = [t.a, -1 if t.b is None else t.b, -1 if t.c is None else t.c]
+            for k in range(3):
+                arr[i].factor[k] = fs[k]
+        return arr
+
+    @classmethod
+    def product(cls, k: int) -> "Gate":
+        """x_0 * x_1 * ... * x_(k-1) with coefficient one (k <= 3)."""
+        one = fr_from_u64(1)
+        fs = list(range(k)) + [None] * (3 - k)
+        return cls(list(range(k)), [GateTerm(one, fs[0], fs[1], fs[2])])
+
+
+@dataclass
+class SumcheckProof:
+    """crates/sumcheck/src/lib.rs:179-186: rounds[i] ascending coefficients."""
+
+    rounds: np.ndarray  # (n, d + 1, 4)
+    final_evals: np.ndarray  # (arity, 4)
+
+
+def _tables_host(tables):
+    arrs = [np.ascontiguousarray(np.asarray(t, dtype=np.uint64).reshape(-1, 4)) for t in tables]
+    ptrs = (VP * len(arrs))(*[a.ctypes.data for a in arrs])
+    return arrs, ptrs
+
+
+def prove_sumcheck(eq_point, gate: Gate, tables, transcript: Transcript, ctx: Optional[Context] = None):
+    """gkr::prove_sumcheck with a gate summand: returns (rounds, point, final_evals).
+    Each round is absorbed as SUMCHECK_ROUND and followed by a
+    SUMCHECK_CHALLENGE on ``transcript`` (crates/gkr/src/lib.rs:886-897)."""
+    ctx = ctx or default_context()
+    arrs, ptrs = _tables_host(tables)
+    n = int(arrs[0].shape[0]).bit_length() - 1
+    if any(a.shape[0] != 1 << n for a in arrs):
+        raise ValueError("prove_sumcheck: tables share one power-of-two length")
+    eqp = _fe_array(eq_point) if n else np.zeros((1, 4), dtype=np.uint64)
+    if n and eqp.shape[0] != n:
+        raise ValueError("prove_sumcheck: the eq point has one coordinate per variable")
+    d = gate.degree()
+    rounds = np.zeros((max(n, 1), d + 1, 4), dtype=np.uint64)
+    point = np.zeros((max(n, 1), 4), dtype=np.uint64)
+    finals = np.zeros((len(arrs), 4), dtype=np.uint64)
+    terms = gate.c_terms()
+    cb = C.cast(lib().apg_transcript_round, ROUND_FN)
+    _check(lib().apg_sumcheck_prove(ctx.h, terms, len(gate.terms), ptrs, len(arrs), n, _p(eqp), cb,
+                                    transcript.h, _p(rounds), _p(point), _p(finals)))
+    return rounds[:n], point[:n], finals
+
+
+def prove_zerocheck(gate: Gate, polys, transcript: Transcript, ctx: Optional[Context] = None) -> SumcheckProof:
+    """sumcheck::prove_zerocheck: the caller has already absorbed the witness
+    digest; draws r, proves, absorbs final_evals."""
+    ctx = ctx or default_context()
+    if len(polys) != gate.arity:
+        raise ValueError(f"prove_zerocheck: {len(polys)} columns for a gate with {gate.arity} inputs")
+    arrs, ptrs = _tables_host(polys)
+    n = int(arrs[0].shape[0]).bit_length() - 1
+    for k, a in enumerate(arrs):
+        if a.shape[0] != 1 << n:
+            raise ValueError(f"prove_zerocheck: column {k} has {a.shape[0]} entries, column 0 has {1 << n}")
+    d = gate.degree()
+    rounds = np.zeros((max(n, 1), d + 1, 4), dtype=np.uint64)
+    finals = np.zeros((len(arrs), 4), dtype=np.uint64)
+    _check(lib().apg_zerocheck_prove(ctx.h, gate.c_terms(), len(gate.terms), ptrs, len(arrs), n, transcript.h,
+                                     _p(rounds), _p(finals)))
+    return SumcheckProof(rounds[:n], finals)
+
+
+@dataclass
+class ShardProof:
+    commitments: list  # wire G1, one per column
+    rounds: np.ndarray  # (num_vars, degree + 1, 4)
+    final_evals: np.ndarray  # (degree - 1, 4)
+
+
+def prove_shard(srs: Srs, columns: DeviceBuffer, num_vars: int, k: int, degree: int,
+                transcript: Transcript, ctx: Optional[Context] = None) -> ShardProof:
+    """Commit k device-resident columns of 2^num_vars Fr, then the batched
+    sumcheck over their rho-combinations (apg_prove_shard_dev)."""
+    ctx = ctx or srs.ctx
+    cms = np.zeros(64 * k, dtype=np.uint8)
+    rounds = np.zeros((num_vars, degree + 1, 4), dtype=np.uint64)
+    finals = np.zeros((degree - 1, 4), dtype=np.uint64)
+    _check(lib().apg_prove_shard_dev(ctx.h, srs.h, columns.ptr, num_vars, k, degree, transcript.h,
+                                     _p(cms), _p(rounds), _p(finals)))
+    return ShardProof([cms[64 * i: 64 * (i + 1)].tobytes() for i in range(k)], rounds, finals)

@@ -1,0 +1,615 @@
+// The C ABI (include/apogee_gpu.h): contexts, SRS residency, MSM/commit,
+// sumcheck, shard composition, transcript and encodings. Every entry catches
+// library exceptions and returns a negative code.
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/apogee_gpu.h"
+#include "msm.h"
+#include "sumcheck.h"
+#include "transcript.h"
+
+using namespace apg;
+
+struct apg_ctx {
+    Context c;
+};
+
+struct apg_srs {
+    int device = 0;
+    Affine *base = nullptr;  // copy 0: [tau^i]G, i < n
+    size_t n = 0;
+    std::mutex mu;
+    std::map<int, std::unique_ptr<BaseTable>> tables;  // window width -> shifted copies
+    ~apg_srs() {
+        tables.clear();
+        if (base) (void)hipFree(base);
+    }
+};
+
+struct apg_transcript {
+    Transcript t;
+};
+
+namespace apg {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace apg
+
+template <class F>
+static int guarded(F &&f) {
+    try {
+        f();
+        return APG_OK;
+    } catch (const ApgError &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host out of memory");
+        return APG_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return APG_ERR_INTERNAL;
+    }
+}
+
+static inline Fe load_fe(const uint8_t *p) {
+    Fe r;
+    std::memcpy(r.l, p, 32);
+    return r;
+}
+static inline void store_fe(uint8_t *p, const Fe &v) { std::memcpy(p, v.l, 32); }
+
+namespace {
+
+__global__ void k_srs_gen(Fe tau, size_t n, Affine *out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Fe p = f_one<Fr>(), b = tau;
+    for (size_t e = i; e; e >>= 1) {
+        if (e & 1) p = f_mul<Fr>(p, b);
+        b = f_sqr<Fr>(b);
+    }
+    const Fe s = f_from_mont<Fr>(p);
+    Affine g;
+    g.x = f_one<Fq>();
+    g.y = f_from_u64<Fq>(2);
+    Xyzz acc = xyzz_identity();
+    for (int bit = 253; bit >= 0; bit--) {
+        acc = xyzz_dbl(acc);
+        if ((s.l[bit >> 5] >> (bit & 31)) & 1) acc = xyzz_add_affine(acc, g);
+    }
+    out[i] = xyzz_to_affine(acc);
+}
+
+BaseTable &srs_table(Context &ctx, apg_srs *srs, size_t n) {
+    if (n > srs->n) throw ApgError(APG_ERR_ARG, "SRS too small for this commit");
+    const int c = msm_pick_c(n, 1);
+    std::lock_guard<std::mutex> lk(srs->mu);
+    auto it = srs->tables.find(c);
+    if (it == srs->tables.end() || it->second->stride < n) {
+        const int copies = msm_window_count(254, c);
+        srs->tables[c] = msm_expand_bases(ctx, srs->base, n, c, copies);
+        ctx.sync();
+        it = srs->tables.find(c);
+    }
+    return *it->second;
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+void commit_device(Context &ctx, apg_srs *srs, const Fe *fr_dev, const uint32_t *u32_dev, size_t n,
+                   size_t k, uint8_t *out_mont) {
+    std::vector<Affine> res(k);
+    if (n == 0) {
+        std::memset(out_mont, 0, 64 * k);
+        return;
+    }
+    BaseTable &tab = srs_table(ctx, srs, n);
+    const double t0 = now_ms();
+    msm_run(ctx, tab, n, k, fr_dev, u32_dev, res.data());
+    if (ctx.profiling) {
+        ctx.stats.msm_ms += now_ms() - t0;
+        ctx.stats.msm_calls += 1;
+        ctx.stats.msm_points += n * k;
+    }
+    for (size_t i = 0; i < k; i++) std::memcpy(out_mont + 64 * i, &res[i], 64);
+}
+
+Gate make_gate(const apg_term *terms, size_t n_terms) {
+    if (n_terms > (size_t)SC_MAX_TERMS) throw ApgError(APG_ERR_ARG, "too many gate terms");
+    Gate g;
+    g.nterms = (int)n_terms;
+    for (size_t q = 0; q < n_terms; q++) {
+        g.terms[q].coef = load_fe(terms[q].coef_mont);
+        for (int f = 0; f < 3; f++) g.terms[q].f[f] = terms[q].factor[f];
+    }
+    return g;
+}
+
+struct CbError {};
+
+RoundFn wrap_round(apg_round_fn fn, void *user) {
+    return [fn, user](const Fe *coeffs, int nc) -> Fe {
+        std::vector<uint8_t> buf(32 * (size_t)nc);
+        for (int i = 0; i < nc; i++) store_fe(&buf[32 * (size_t)i], coeffs[i]);
+        uint8_t ch[32];
+        if (fn(user, buf.data(), (size_t)nc, ch) != 0)
+            throw ApgError(APG_ERR_CALLBACK, "round callback aborted the proof");
+        return load_fe(ch);
+    };
+}
+
+void sumcheck_common(Context &ctx, const Gate &gate, Fe *const *tabs, size_t k, size_t num_vars,
+                     const uint8_t *eq_point_mont, const RoundFn &fn, uint8_t *rounds_out,
+                     uint8_t *point_out, uint8_t *final_evals_out) {
+    const int d = gate.degree();
+    std::vector<Fe> eqp(num_vars + 1), rounds((size_t)num_vars * (d + 1) + 1), point(num_vars + 1),
+        finals(k);
+    for (size_t i = 0; i < num_vars; i++) eqp[i] = load_fe(eq_point_mont + 32 * i);
+    const double t0 = now_ms();
+    sumcheck_prove_device(ctx, gate, tabs, (int)k, (int)num_vars, eqp.data(), fn, rounds.data(),
+                          point.data(), finals.data());
+    if (ctx.profiling) ctx.stats.sumcheck_ms += now_ms() - t0;
+    for (size_t i = 0; i < (size_t)num_vars * (d + 1); i++) store_fe(rounds_out + 32 * i, rounds[i]);
+    if (point_out)
+        for (size_t i = 0; i < num_vars; i++) store_fe(point_out + 32 * i, point[i]);
+    for (size_t i = 0; i < k; i++) store_fe(final_evals_out + 32 * i, finals[i]);
+}
+
+// Upload host tables into the context arena; returns device pointers.
+std::vector<Fe *> upload_tables(Context &ctx, const uint8_t *const *tables, size_t k, size_t len) {
+    Fe *pool = ctx.d<Fe>("sc.inputs", k * len);
+    std::vector<Fe *> p(k);
+    for (size_t t = 0; t < k; t++) {
+        p[t] = pool + t * len;
+        HIP_CHECK(hipMemcpyAsync(p[t], tables[t], len * 32, hipMemcpyHostToDevice, ctx.stream));
+    }
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *apg_last_error(void) { return g_last_error.c_str(); }
+int apg_version(void) { return 1; }
+
+int apg_ctx_create(int device, apg_ctx **out) {
+    return guarded([&] {
+        int ndev = 0;
+        HIP_CHECK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw ApgError(APG_ERR_ARG, "no such device");
+        HIP_CHECK(hipSetDevice(device));
+        auto *c = new apg_ctx();
+        c->c.device = device;
+        HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        c->c.cu_count = prop.multiProcessorCount;
+        *out = c;
+    });
+}
+
+void apg_ctx_destroy(apg_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    ctx->c.dev.clear();
+    ctx->c.host.clear();
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+}
+
+int apg_ctx_synchronize(apg_ctx *ctx) {
+    return guarded([&] { ctx->c.sync(); });
+}
+
+int apg_ctx_set_profiling(apg_ctx *ctx, int on) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.profiling = on != 0;
+    });
+}
+
+int apg_ctx_stats(apg_ctx *ctx, apg_stats *out) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        const Stats &s = ctx->c.stats;
+        out->accumulate_ms = s.accumulate_ms;
+        out->accumulate_launches = s.accumulate_launches;
+        out->accumulate_entries = s.accumulate_entries;
+        out->msm_ms = s.msm_ms;
+        out->msm_calls = s.msm_calls;
+        out->msm_points = s.msm_points;
+        out->sumcheck_kernel_ms = s.sumcheck_kernel_ms;
+        out->sumcheck_launches = s.sumcheck_launches;
+        out->sumcheck_bytes = s.sumcheck_bytes;
+        out->sumcheck_ms = s.sumcheck_ms;
+    });
+}
+
+int apg_ctx_reset_stats(apg_ctx *ctx) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        ctx->c.stats = Stats();
+    });
+}
+
+int apg_dev_alloc(apg_ctx *ctx, size_t bytes, void **out) {
+    return guarded([&] {
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) throw ApgError(APG_ERR_OOM, "device out of memory");
+    });
+}
+int apg_dev_free(apg_ctx *ctx, void *p) {
+    return guarded([&] {
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        HIP_CHECK(hipFree(p));
+    });
+}
+int apg_dev_upload(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+        ctx->c.sync();
+    });
+}
+int apg_dev_download(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+        ctx->c.sync();
+    });
+}
+
+int apg_srs_upload(apg_ctx *ctx, const uint8_t *bases_mont, size_t n, apg_srs **out) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        auto s = std::make_unique<apg_srs>();
+        s->device = ctx->c.device;
+        s->n = n;
+        if (hipMalloc(&s->base, std::max<size_t>(n, 1) * sizeof(Affine)) != hipSuccess)
+            throw ApgError(APG_ERR_OOM, "device out of memory for the SRS");
+        if (n) HIP_CHECK(hipMemcpyAsync(s->base, bases_mont, n * 64, hipMemcpyHostToDevice, ctx->c.stream));
+        ctx->c.sync();
+        *out = s.release();
+    });
+}
+
+int apg_srs_generate(apg_ctx *ctx, const uint8_t tau_mont[32], size_t n, apg_srs **out) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        auto s = std::make_unique<apg_srs>();
+        s->device = ctx->c.device;
+        s->n = n;
+        if (hipMalloc(&s->base, std::max<size_t>(n, 1) * sizeof(Affine)) != hipSuccess)
+            throw ApgError(APG_ERR_OOM, "device out of memory for the SRS");
+        if (n) {
+            hipLaunchKernelGGL(k_srs_gen, dim3(grid_for(n, 128)), dim3(128), 0, ctx->c.stream,
+                               load_fe(tau_mont), n, s->base);
+            HIP_CHECK(hipGetLastError());
+        }
+        ctx->c.sync();
+        *out = s.release();
+    });
+}
+
+int apg_srs_download(apg_ctx *ctx, const apg_srs *srs, size_t offset, size_t n, uint8_t *out) {
+    return guarded([&] {
+        if (offset + n > srs->n) throw ApgError(APG_ERR_ARG, "SRS range out of bounds");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipMemcpyAsync(out, srs->base + offset, n * 64, hipMemcpyDeviceToHost, ctx->c.stream));
+        ctx->c.sync();
+    });
+}
+
+size_t apg_srs_len(const apg_srs *srs) { return srs ? srs->n : 0; }
+
+size_t apg_srs_device_bytes(const apg_srs *srs) {
+    if (!srs) return 0;
+    size_t b = srs->n * 64;
+    for (auto &kv : srs->tables) b += kv.second->stride * kv.second->copies * 64;
+    return b;
+}
+
+int apg_srs_prepare(apg_ctx *ctx, apg_srs *srs, size_t n) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        if (n) srs_table(ctx->c, srs, n);
+    });
+}
+
+void apg_srs_release(apg_srs *srs) { delete srs; }
+
+int apg_msm_g1_fr(apg_ctx *ctx, const uint8_t *bases_mont, const uint8_t *scalars_mont, size_t n,
+                  uint8_t out_mont[64]) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        if (n == 0) {
+            std::memset(out_mont, 0, 64);
+            return;
+        }
+        Context &c = ctx->c;
+        BaseTable tab;
+        tab.points = c.d<Affine>("api.bases", n);
+        tab.stride = n;
+        tab.c = msm_pick_c(n, 1);
+        tab.copies = 1;
+        Fe *sc = c.d<Fe>("api.scalars", n);
+        HIP_CHECK(hipMemcpyAsync(tab.points, bases_mont, n * 64, hipMemcpyHostToDevice, c.stream));
+        HIP_CHECK(hipMemcpyAsync(sc, scalars_mont, n * 32, hipMemcpyHostToDevice, c.stream));
+        Affine res;
+        msm_run(c, tab, n, 1, sc, nullptr, &res);
+        std::memcpy(out_mont, &res, 64);
+    });
+}
+
+int apg_msm_g1_u32(apg_ctx *ctx, const uint8_t *bases_mont, const uint32_t *scalars, size_t n,
+                   uint8_t out_mont[64]) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        if (n == 0) {
+            std::memset(out_mont, 0, 64);
+            return;
+        }
+        Context &c = ctx->c;
+        BaseTable tab;
+        tab.points = c.d<Affine>("api.bases", n);
+        tab.stride = n;
+        tab.c = msm_pick_c(n, 1);
+        tab.copies = 1;
+        uint32_t *sc = c.d<uint32_t>("api.u32", n);
+        HIP_CHECK(hipMemcpyAsync(tab.points, bases_mont, n * 64, hipMemcpyHostToDevice, c.stream));
+        HIP_CHECK(hipMemcpyAsync(sc, scalars, n * 4, hipMemcpyHostToDevice, c.stream));
+        Affine res;
+        msm_run(c, tab, n, 1, nullptr, sc, &res);
+        std::memcpy(out_mont, &res, 64);
+    });
+}
+
+int apg_commit_fr(apg_ctx *ctx, apg_srs *srs, const uint8_t *scalars_mont, size_t n, size_t k,
+                  uint8_t *out_mont) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Context &c = ctx->c;
+        Fe *sc = c.d<Fe>("api.scalars", std::max<size_t>(n * k, 1));
+        if (n * k) HIP_CHECK(hipMemcpyAsync(sc, scalars_mont, n * k * 32, hipMemcpyHostToDevice, c.stream));
+        commit_device(c, srs, sc, nullptr, n, k, out_mont);
+    });
+}
+
+int apg_commit_u32(apg_ctx *ctx, apg_srs *srs, const uint32_t *scalars, size_t n, size_t k,
+                   uint8_t *out_mont) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Context &c = ctx->c;
+        uint32_t *sc = c.d<uint32_t>("api.u32", std::max<size_t>(n * k, 1));
+        if (n * k) HIP_CHECK(hipMemcpyAsync(sc, scalars, n * k * 4, hipMemcpyHostToDevice, c.stream));
+        commit_device(c, srs, nullptr, sc, n, k, out_mont);
+    });
+}
+
+int apg_commit_fr_dev(apg_ctx *ctx, apg_srs *srs, const void *scalars_dev, size_t n, size_t k,
+                      uint8_t *out_mont) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        commit_device(ctx->c, srs, static_cast<const Fe *>(scalars_dev), nullptr, n, k, out_mont);
+    });
+}
+
+int apg_sumcheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
+                       const uint8_t *const *tables_mont, size_t k, size_t num_vars,
+                       const uint8_t *eq_point_mont, apg_round_fn round_fn, void *user,
+                       uint8_t *rounds_out, uint8_t *point_out, uint8_t *final_evals_out) {
+    return guarded([&] {
+        if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Gate g = make_gate(terms, n_terms);
+        auto tabs = upload_tables(ctx->c, tables_mont, k, size_t(1) << num_vars);
+        sumcheck_common(ctx->c, g, tabs.data(), k, num_vars, eq_point_mont, wrap_round(round_fn, user),
+                        rounds_out, point_out, final_evals_out);
+    });
+}
+
+int apg_sumcheck_prove_dev(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
+                           void *const *tables_dev, size_t k, size_t num_vars,
+                           const uint8_t *eq_point_mont, apg_round_fn round_fn, void *user,
+                           uint8_t *rounds_out, uint8_t *point_out, uint8_t *final_evals_out) {
+    return guarded([&] {
+        if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Gate g = make_gate(terms, n_terms);
+        std::vector<Fe *> tabs(k);
+        for (size_t t = 0; t < k; t++) tabs[t] = static_cast<Fe *>(tables_dev[t]);
+        sumcheck_common(ctx->c, g, tabs.data(), k, num_vars, eq_point_mont, wrap_round(round_fn, user),
+                        rounds_out, point_out, final_evals_out);
+    });
+}
+
+int apg_zerocheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
+                        const uint8_t *const *tables_mont, size_t k, size_t num_vars,
+                        apg_transcript *t, uint8_t *rounds_out, uint8_t *final_evals_out) {
+    return guarded([&] {
+        if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Gate g = make_gate(terms, n_terms);
+        // crates/sumcheck/src/lib.rs:381-383: n eq-randomizers first
+        std::vector<uint8_t> r(32 * (num_vars + 1));
+        for (size_t i = 0; i < num_vars; i++) store_fe(&r[32 * i], t->t.challenge(TAG_SUMCHECK_CHALLENGE));
+        auto tabs = upload_tables(ctx->c, tables_mont, k, size_t(1) << num_vars);
+        RoundFn fn = [t](const Fe *coeffs, int nc) -> Fe {
+            t->t.append_scalars(TAG_SUMCHECK_ROUND, coeffs, nc);
+            return t->t.challenge(TAG_SUMCHECK_CHALLENGE);
+        };
+        sumcheck_common(ctx->c, g, tabs.data(), k, num_vars, r.data(), fn, rounds_out, nullptr,
+                        final_evals_out);
+        std::vector<Fe> fin(k);
+        for (size_t i = 0; i < k; i++) fin[i] = load_fe(final_evals_out + 32 * i);
+        t->t.append_scalars(TAG_SUMCHECK_FINAL_EVALS, fin.data(), k);
+    });
+}
+
+int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, size_t num_vars,
+                        size_t k, int degree, apg_transcript *t, uint8_t *commitments_wire,
+                        uint8_t *rounds_out, uint8_t *final_evals_out) {
+    return guarded([&] {
+        if (num_vars > 40 || num_vars < 1) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+        if (degree < 3 || degree > 5) throw ApgError(APG_ERR_ARG, "degree must be 3..5");
+        if (k < 1 || k > (size_t)SC_MAX_TABLES) throw ApgError(APG_ERR_ARG, "column count out of range");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Context &c = ctx->c;
+        const size_t n = size_t(1) << num_vars;
+        const Fe *cols = static_cast<const Fe *>(columns_dev);
+        // 1. commitments
+        std::vector<uint8_t> cm_mont(64 * k);
+        commit_device(c, srs, cols, nullptr, n, k, cm_mont.data());
+        for (size_t i = 0; i < k; i++) apg_g1_to_wire(&cm_mont[64 * i], commitments_wire + 64 * i);
+        // 2. absorb, draw rho, combine
+        t->t.append_g1_wire(TAG_COMMITMENT, commitments_wire, k);
+        const Fe rho = t->t.challenge(TAG_MERCURY_BATCH);
+        const int f = degree - 1;
+        std::vector<Fe> w((size_t)f * k);
+        Fe pw = f_one<Fr>();
+        for (int o = 0; o < f; o++)
+            for (size_t col = 0; col < k; col++) {
+                w[o * k + col] = pw;
+                pw = f_mul<Fr>(pw, rho);
+            }
+        Fe *comb = c.d<Fe>("shard.comb", (size_t)f * n);
+        std::vector<Fe *> outs(f);
+        for (int o = 0; o < f; o++) outs[o] = comb + (size_t)o * n;
+        combine_columns_device(c, cols, n, (int)k, w.data(), f, outs.data());
+        // 3. eq point, rounds, finals
+        std::vector<uint8_t> r(32 * num_vars);
+        for (size_t i = 0; i < num_vars; i++) store_fe(&r[32 * i], t->t.challenge(TAG_SUMCHECK_CHALLENGE));
+        Gate g;
+        g.nterms = 1;
+        g.terms[0].coef = f_one<Fr>();
+        for (int q = 0; q < 3; q++) g.terms[0].f[q] = q < f ? q : -1;
+        RoundFn fn = [t](const Fe *coeffs, int nc) -> Fe {
+            t->t.append_scalars(TAG_SUMCHECK_ROUND, coeffs, nc);
+            return t->t.challenge(TAG_SUMCHECK_CHALLENGE);
+        };
+        sumcheck_common(c, g, outs.data(), (size_t)f, num_vars, r.data(), fn, rounds_out, nullptr,
+                        final_evals_out);
+        std::vector<Fe> fin(f);
+        for (int i = 0; i < f; i++) fin[i] = load_fe(final_evals_out + 32 * i);
+        t->t.append_scalars(TAG_SUMCHECK_FINAL_EVALS, fin.data(), f);
+    });
+}
+
+int apg_transcript_new(apg_transcript **out) {
+    return guarded([&] { *out = new apg_transcript(); });
+}
+void apg_transcript_free(apg_transcript *t) { delete t; }
+int apg_transcript_clone(const apg_transcript *t, apg_transcript **out) {
+    return guarded([&] { *out = new apg_transcript(*t); });
+}
+void apg_transcript_observe(apg_transcript *t, const uint8_t x[32]) { t->t.observe(load_fe(x)); }
+void apg_transcript_sample(apg_transcript *t, uint8_t out[32]) { store_fe(out, t->t.sample()); }
+void apg_transcript_append_scalars(apg_transcript *t, uint64_t tag, const uint8_t *xs, size_t n) {
+    std::vector<Fe> v(n + 1);
+    for (size_t i = 0; i < n; i++) v[i] = load_fe(xs + 32 * i);
+    t->t.append_scalars(tag, v.data(), n);
+}
+void apg_transcript_append_bytes(apg_transcript *t, uint64_t tag, const uint8_t *b, size_t len) {
+    t->t.append_bytes(tag, b, len);
+}
+void apg_transcript_challenge(apg_transcript *t, uint64_t tag, uint8_t out[32]) {
+    store_fe(out, t->t.challenge(tag));
+}
+void apg_transcript_append_g1(apg_transcript *t, uint64_t tag, const uint8_t *w, size_t k) {
+    t->t.append_g1_wire(tag, w, k);
+}
+int apg_transcript_round(void *tr, const uint8_t *coeffs, size_t n, uint8_t ch[32]) {
+    auto *t = static_cast<apg_transcript *>(tr);
+    apg_transcript_append_scalars(t, TAG_SUMCHECK_ROUND, coeffs, n);
+    apg_transcript_challenge(t, TAG_SUMCHECK_CHALLENGE, ch);
+    return 0;
+}
+int apg_transcript_state(const apg_transcript *t, uint8_t out[232]) {
+    for (int i = 0; i < 3; i++) store_fe(out + 32 * i, t->t.state[i]);
+    for (int i = 0; i < 2; i++) store_fe(out + 96 + 32 * i, t->t.input[i]);
+    for (int i = 0; i < 2; i++) store_fe(out + 160 + 32 * i, t->t.output[i]);
+    std::memcpy(out + 224, &t->t.input_len, 4);
+    std::memcpy(out + 228, &t->t.output_len, 4);
+    return APG_OK;
+}
+
+int apg_fr_to_bytes(const uint8_t mont[32], uint8_t canonical[32]) {
+    Fe a = load_fe(mont);
+    if (!f_is_canonical<Fr>(a)) return APG_ERR_ARG;
+    fe_to_bytes_fr(a, canonical);
+    return APG_OK;
+}
+int apg_fr_from_bytes(const uint8_t canonical[32], uint8_t mont[32]) {
+    Fe a;
+    if (!fe_from_bytes_fr(canonical, a)) return APG_ERR_ARG;
+    store_fe(mont, a);
+    return APG_OK;
+}
+int apg_fq_to_bytes(const uint8_t mont[32], uint8_t canonical[32]) {
+    Fe a = load_fe(mont);
+    if (!f_is_canonical<Fq>(a)) return APG_ERR_ARG;
+    fe_to_bytes_fq(a, canonical);
+    return APG_OK;
+}
+int apg_fq_from_bytes(const uint8_t canonical[32], uint8_t mont[32]) {
+    Fe a;
+    if (!fe_from_bytes_fq(canonical, a)) return APG_ERR_ARG;
+    store_fe(mont, a);
+    return APG_OK;
+}
+void apg_fr_from_u64(uint64_t x, uint8_t mont[32]) { store_fe(mont, f_from_u64<Fr>(x)); }
+
+int apg_g1_to_wire(const uint8_t mont[64], uint8_t wire[64]) {
+    Affine a;
+    std::memcpy(&a, mont, 64);
+    if (aff_is_inf(a)) {
+        std::memset(wire, 0, 64);
+        return APG_OK;
+    }
+    if (!f_is_canonical<Fq>(a.x) || !f_is_canonical<Fq>(a.y)) return APG_ERR_ARG;
+    fe_to_bytes_fq(a.x, wire);
+    fe_to_bytes_fq(a.y, wire + 32);
+    return APG_OK;
+}
+
+int apg_g1_from_wire(const uint8_t wire[64], uint8_t mont[64]) {
+    bool zero = true;
+    for (int i = 0; i < 64; i++) zero &= wire[i] == 0;
+    if (zero) {
+        std::memset(mont, 0, 64);
+        return APG_OK;
+    }
+    Affine a;
+    if (!fe_from_bytes_fq(wire, a.x) || !fe_from_bytes_fq(wire + 32, a.y)) return APG_ERR_ARG;
+    Fe y2 = f_sqr<Fq>(a.y);
+    Fe rhs = f_add<Fq>(f_mul<Fq>(f_sqr<Fq>(a.x), a.x), f_from_u64<Fq>(3));
+    if (!f_eq(y2, rhs)) return APG_ERR_ARG;
+    std::memcpy(mont, &a, 64);
+    return APG_OK;
+}
+
+}  // extern "C"

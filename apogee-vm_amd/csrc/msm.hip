@@ -1,0 +1,480 @@
+// Multi-scalar multiplication over BN254 G1 for gfx950.
+//
+// Replaces curve::msm::{msm, msm_small_u32} (crates/curve/src/msm.rs:48-100).
+// The value is the group element sum_i s_i B_i, so any correct bucket schedule
+// returns the reference's exact affine bytes; this one is built for the MI355X:
+//
+//  1. digits  : one thread per scalar converts Montgomery -> canonical, recodes
+//               into signed c-bit digits exactly as msm.rs:180-224 does (same
+//               carry rule), and counts each nonzero digit into its bucket key
+//               (column, group, |digit|-1).
+//  2. sort    : exclusive scan of the counts, then a scatter of
+//               (table index | sign) entries, i.e. a counting sort by bucket.
+//  3. chunks  : every bucket is cut into chunks of at most K1 entries, so the
+//               work per thread is bounded whatever the digit skew (the
+//               witness-like mix piles 10% of all points into one bucket).
+//  4. accumulate: one thread per chunk sums its points with mixed XYZZ adds;
+//               the gather of 64-byte affine bases is the only HBM traffic.
+//  5. buckets : chunk sums are combined per bucket (a thread per light bucket,
+//               an LDS tree per heavy one).
+//  6. reduce  : sum_b (b+1) S_b per group by a two-level split
+//               (b = 256 r + t: 256 * sum_r r Z_r + sum_t t U_t + sum_r Z_r),
+//               each level an LDS tree or suffix scan: ~25 dependent adds
+//               instead of the 2 * 2^(c-1) serial running sum of msm.rs:304-310.
+//  7. host    : groups combine with 2^(c P) doublings (none when the table holds
+//               every window shifted), one batched inversion gives affine.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdio>
+
+#include "msm.h"
+
+namespace apg {
+
+namespace {
+
+constexpr uint32_t SIGN = 0x80000000u;
+constexpr int TPB = 256;
+
+struct Chunk {
+    uint32_t key, start, len;
+};
+
+__device__ __forceinline__ uint32_t window_of32(const uint32_t l[8], int off, int c) {
+    const int limb = off >> 5, sh = off & 31;
+    uint32_t v = l[limb] >> sh;
+    if (sh && limb + 1 < 8) v |= l[limb + 1] << (32 - sh);
+    return v & ((1u << c) - 1u);
+}
+
+__device__ __forceinline__ void load_scalar(const Fe *fr, const uint32_t *u32, size_t idx,
+                                            uint32_t l[8]) {
+    if (fr) {
+        Fe c = f_from_mont<Fr>(fr[idx]);
+#pragma unroll
+        for (int i = 0; i < 8; i++) l[i] = c.l[i];
+    } else {
+        l[0] = u32[idx];
+#pragma unroll
+        for (int i = 1; i < 8; i++) l[i] = 0;
+    }
+}
+
+// Recode (crates/curve/src/msm.rs:180-198) and either count or scatter.
+template <bool SCATTER>
+__global__ void k_digits(const Fe *__restrict__ fr, const uint32_t *__restrict__ u32,
+                         uint32_t n, uint32_t K, int c, int windows, int P, int G, uint32_t B,
+                         uint32_t tab_stride, uint32_t *__restrict__ counts,
+                         uint32_t *__restrict__ entries) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n * K) return;
+    const uint32_t col = (uint32_t)(idx / n);
+    const uint32_t i = (uint32_t)(idx - (size_t)col * n);
+    uint32_t l[8];
+    load_scalar(fr, u32, idx, l);
+    const uint32_t half = 1u << (c - 1);
+    uint32_t carry = 0;
+    for (int j = 0; j < windows; j++) {
+        uint32_t coef = window_of32(l, j * c, c) + carry;
+        uint32_t mag;
+        bool neg = false;
+        if (j == windows - 1) {
+            mag = coef;  // the top window swallows the carry (coef <= half)
+        } else if (coef >= half) {
+            mag = (1u << c) - coef;  // digit coef - 2^c, possibly 0 when coef == 2^c
+            neg = mag != 0;
+            carry = 1;
+        } else {
+            mag = coef;
+            carry = 0;
+        }
+        if (mag == 0) continue;
+        const uint32_t key = ((col * (uint32_t)G) + (uint32_t)(j / P)) * B + (mag - 1);
+        if (SCATTER) {
+            const uint32_t pos = atomicAdd(&counts[key], 1u);
+            entries[pos] = ((uint32_t)(j % P) * tab_stride + i) | (neg ? SIGN : 0u);
+        } else {
+            atomicAdd(&counts[key], 1u);
+        }
+    }
+}
+
+__global__ void k_chunk_count(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t K1,
+                              uint32_t *__restrict__ q) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nkeys) return;
+    if (k == nkeys) {
+        q[k] = 0;
+        return;
+    }
+    const uint32_t m = off[k + 1] - off[k];
+    q[k] = (m + K1 - 1) / K1;
+}
+
+__global__ void k_chunk_fill(const uint32_t *__restrict__ off, const uint32_t *__restrict__ qoff,
+                             uint32_t nkeys, uint32_t K1, Chunk *__restrict__ desc) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const uint32_t m = off[k + 1] - off[k];
+    const uint32_t base = qoff[k];
+    const uint32_t q = qoff[k + 1] - base;
+    for (uint32_t s = 0; s < q; s++) {
+        Chunk ch;
+        ch.key = k;
+        ch.start = off[k] + s * K1;
+        ch.len = min(K1, m - s * K1);
+        desc[base + s] = ch;
+    }
+}
+
+__device__ __forceinline__ Affine load_affine(const Affine *__restrict__ p, uint32_t idx) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p + idx);
+    uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+    Affine r;
+    r.x.l[0] = a.x; r.x.l[1] = a.y; r.x.l[2] = a.z; r.x.l[3] = a.w;
+    r.x.l[4] = b.x; r.x.l[5] = b.y; r.x.l[6] = b.z; r.x.l[7] = b.w;
+    r.y.l[0] = c.x; r.y.l[1] = c.y; r.y.l[2] = c.z; r.y.l[3] = c.w;
+    r.y.l[4] = d.x; r.y.l[5] = d.y; r.y.l[6] = d.z; r.y.l[7] = d.w;
+    return r;
+}
+
+__global__ void __launch_bounds__(TPB) k_accumulate(const Chunk *__restrict__ desc, uint32_t nchunks,
+                                                  const uint32_t *__restrict__ entries,
+                                                  const Affine *__restrict__ bases,
+                                                  Xyzz *__restrict__ chunk_sums) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nchunks) return;
+    const Chunk ch = desc[t];
+    Xyzz acc = xyzz_identity();
+    for (uint32_t e = 0; e < ch.len; e++) {
+        const uint32_t ent = entries[ch.start + e];
+        Affine p = load_affine(bases, ent & ~SIGN);
+        if (ent & SIGN) p.y = f_neg<Fq>(p.y);
+        acc = xyzz_add_affine(acc, p);
+    }
+    chunk_sums[t] = acc;
+}
+
+constexpr uint32_t LIGHT_CHUNKS = 8;
+
+__global__ void k_bucket_light(const uint32_t *__restrict__ qoff, uint32_t nkeys,
+                               const Xyzz *__restrict__ chunk_sums, Xyzz *__restrict__ buckets,
+                               uint32_t *__restrict__ heavy, uint32_t *__restrict__ nheavy) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const uint32_t a = qoff[k], q = qoff[k + 1] - a;
+    if (q > LIGHT_CHUNKS) {
+        heavy[atomicAdd(nheavy, 1u)] = k;
+        return;
+    }
+    Xyzz acc = xyzz_identity();
+    for (uint32_t s = 0; s < q; s++) acc = xyzz_add(acc, chunk_sums[a + s]);
+    buckets[k] = acc;
+}
+
+// sum of count points src[base + k * stride], reduced through an LDS tree.
+__device__ Xyzz block_sum(const Xyzz *__restrict__ src, size_t base, size_t stride,
+                          uint32_t count, Xyzz *lds) {
+    const uint32_t tid = threadIdx.x;
+    Xyzz acc = xyzz_identity();
+    for (uint32_t k = tid; k < count; k += TPB) acc = xyzz_add(acc, src[base + (size_t)k * stride]);
+    lds[tid] = acc;
+    __syncthreads();
+    for (uint32_t s = TPB / 2; s > 0; s >>= 1) {
+        if (tid < s && tid + s < count) lds[tid] = xyzz_add(lds[tid], lds[tid + s]);
+        __syncthreads();
+    }
+    Xyzz r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(TPB) k_bucket_heavy(const uint32_t *__restrict__ qoff,
+                                                    const Xyzz *__restrict__ chunk_sums,
+                                                    Xyzz *__restrict__ buckets,
+                                                    const uint32_t *__restrict__ heavy,
+                                                    const uint32_t *__restrict__ nheavy) {
+    __shared__ Xyzz lds[TPB];
+    const uint32_t nh = *nheavy;
+    for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+        const uint32_t k = heavy[h];
+        const uint32_t a = qoff[k], q = qoff[k + 1] - a;
+        Xyzz r = block_sum(chunk_sums, a, 1, q, lds);
+        if (threadIdx.x == 0) buckets[k] = r;
+    }
+}
+
+// Level 1 of the bucket reduction for B > 256 buckets per problem:
+// Z_r = sum_t S[256 r + t] (R = B / 256 of them), U_t = sum_r S[256 r + t].
+__global__ void __launch_bounds__(TPB) k_reduce_split(const Xyzz *__restrict__ S, uint32_t B,
+                                                    Xyzz *__restrict__ Z, Xyzz *__restrict__ U) {
+    __shared__ Xyzz lds[TPB];
+    const uint32_t R = B / TPB;
+    const uint32_t per = R + TPB;
+    const uint32_t p = blockIdx.x / per, j = blockIdx.x % per;
+    const size_t base = (size_t)p * B;
+    if (j < R) {
+        Xyzz r = block_sum(S, base + (size_t)j * TPB, 1, TPB, lds);
+        if (threadIdx.x == 0) Z[(size_t)p * R + j] = r;
+    } else {
+        const uint32_t t = j - R;
+        Xyzz r = block_sum(S, base + t, TPB, R, lds);
+        if (threadIdx.x == 0) U[(size_t)p * TPB + t] = r;
+    }
+}
+
+// For m <= 256 items X_0..X_{m-1}: WS = sum_i i X_i and TOT = sum_i X_i, by an
+// LDS suffix scan (Suf_i = sum_{j >= i} X_j; WS = sum_{i >= 1} Suf_i).
+// Block b reads problem b: items src[b * m ... + m), writes out[2b] = WS,
+// out[2b + 1] = TOT.
+__global__ void __launch_bounds__(TPB) k_weighted(const Xyzz *__restrict__ src, uint32_t m,
+                                                Xyzz *__restrict__ out) {
+    __shared__ Xyzz lds[TPB];
+    const uint32_t tid = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * m;
+    lds[tid] = tid < m ? src[base + tid] : xyzz_identity();
+    __syncthreads();
+    for (uint32_t s = 1; s < m; s <<= 1) {
+        Xyzz v = lds[tid];
+        const bool take = tid + s < m;
+        Xyzz o = take ? lds[tid + s] : v;
+        __syncthreads();
+        if (take) lds[tid] = xyzz_add(v, o);
+        __syncthreads();
+    }
+    const Xyzz tot = lds[0];
+    __syncthreads();
+    if (tid == 0) lds[0] = xyzz_identity();
+    __syncthreads();
+    for (uint32_t s = TPB / 2; s > 0; s >>= 1) {
+        if (tid < s && tid + s < m) lds[tid] = xyzz_add(lds[tid], lds[tid + s]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[2 * blockIdx.x] = lds[0];
+        out[2 * blockIdx.x + 1] = tot;
+    }
+}
+
+// Window-shifted copies: copy w of base i is 2^(c w) B_i, affine.
+__global__ void k_expand(const Affine *__restrict__ src, size_t n, int c, int copies,
+                         Affine *__restrict__ dst, size_t stride) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Affine a = src[i];
+    dst[i] = a;
+    Xyzz p = xyzz_from_affine(a);
+    for (int w = 1; w < copies; w++) {
+        for (int d = 0; d < c; d++) p = xyzz_dbl(p);
+        dst[(size_t)w * stride + i] = xyzz_to_affine(p);
+    }
+}
+
+}  // namespace
+
+Xyzz host_xyzz_mul_pow2(Xyzz p, int k) {
+    for (int i = 0; i < k; i++) p = xyzz_dbl(p);
+    return p;
+}
+
+void host_batch_to_affine(const Xyzz *in, Affine *out, size_t k) {
+    // Montgomery's trick over the nonzero ZZ * ZZZ products, one inversion.
+    std::vector<Fe> t(k), pre(k + 1);
+    pre[0] = f_one<Fq>();
+    for (size_t i = 0; i < k; i++) {
+        t[i] = xyzz_is_id(in[i]) ? f_one<Fq>() : f_mul<Fq>(in[i].zz, in[i].zzz);
+        pre[i + 1] = f_mul<Fq>(pre[i], t[i]);
+    }
+    Fe inv = f_inv<Fq>(pre[k]);
+    for (size_t i = k; i-- > 0;) {
+        Fe ti = f_mul<Fq>(inv, pre[i]);  // 1 / t[i]
+        inv = f_mul<Fq>(inv, t[i]);
+        if (xyzz_is_id(in[i])) {
+            out[i].x = f_zero<Fq>();
+            out[i].y = f_zero<Fq>();
+            continue;
+        }
+        Fe zz_inv = f_mul<Fq>(ti, in[i].zzz);
+        Fe zzz_inv = f_mul<Fq>(ti, in[i].zz);
+        out[i].x = f_mul<Fq>(in[i].x, zz_inv);
+        out[i].y = f_mul<Fq>(in[i].y, zzz_inv);
+    }
+}
+
+int msm_pick_c(size_t n, size_t columns) {
+    (void)columns;
+    if (n < 32) return 3;
+    int lg = 0;
+    while ((size_t(1) << (lg + 1)) <= n) lg++;
+    int c = lg - 2;
+    if (c < 4) c = 4;
+    if (c > 17) c = 17;
+    return c;
+}
+
+std::unique_ptr<BaseTable> msm_expand_bases(Context &ctx, const Affine *src, size_t n, int c,
+                                            int copies) {
+    auto t = std::make_unique<BaseTable>();
+    t->c = c;
+    t->copies = copies;
+    t->stride = n;
+    t->owned = true;
+    HIP_CHECK(hipMalloc(&t->points, std::max<size_t>(1, (size_t)copies * n) * sizeof(Affine)));
+    if (n) {
+        hipLaunchKernelGGL(k_expand, dim3(grid_for(n, 128)), dim3(128), 0, ctx.stream, src, n, c,
+                           copies, t->points, n);
+        HIP_CHECK(hipGetLastError());
+    }
+    return t;
+}
+
+void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *fr_dev,
+             const uint32_t *u32_dev, Affine *out_host) {
+    if (K == 0) return;
+    if (n == 0) {
+        for (size_t k = 0; k < K; k++) {
+            out_host[k].x = f_zero<Fq>();
+            out_host[k].y = f_zero<Fq>();
+        }
+        return;
+    }
+    if (n > tab.stride) throw ApgError(-2, "msm: more scalars than table points");
+    if (n * K >= (size_t(1) << 31)) throw ApgError(-2, "msm: batch too large");
+    MsmShape sh;
+    sh.c = tab.c;
+    sh.windows = msm_window_count(fr_dev ? 254 : 32, sh.c);
+    sh.P = std::min(tab.copies, sh.windows);
+    sh.G = (sh.windows + sh.P - 1) / sh.P;
+    sh.B = 1u << (sh.c - 1);
+    if ((size_t)tab.copies * tab.stride >= (size_t(1) << 31))
+        throw ApgError(-2, "msm: table too large for 31-bit entry indices");
+    const size_t nkeys64 = K * (size_t)sh.G * sh.B;
+    if (nkeys64 >= (size_t(1) << 31)) throw ApgError(-2, "msm: too many buckets");
+    const uint32_t nkeys = (uint32_t)nkeys64;
+    hipStream_t st = ctx.stream;
+
+    // 1. counts
+    uint32_t *counts = ctx.d<uint32_t>("msm.counts", nkeys + 1);
+    uint32_t *off = ctx.d<uint32_t>("msm.off", nkeys + 1);
+    HIP_CHECK(hipMemsetAsync(counts, 0, (nkeys + 1) * sizeof(uint32_t), st));
+    const size_t nthreads = n * K;
+    hipLaunchKernelGGL(k_digits<false>, dim3(grid_for(nthreads, TPB)), dim3(TPB), 0, st, fr_dev,
+                       u32_dev, (uint32_t)n, (uint32_t)K, sh.c, sh.windows, sh.P, sh.G, sh.B,
+                       (uint32_t)tab.stride, counts, nullptr);
+    HIP_CHECK(hipGetLastError());
+
+    // 2. scan + scatter
+    size_t tmp_bytes = 0;
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, off, nkeys + 1, st));
+    void *tmp = ctx.d<uint8_t>("msm.cubtmp", tmp_bytes);
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, off, nkeys + 1, st));
+    uint32_t total = 0;
+    uint32_t *h_total = ctx.h<uint32_t>("msm.total", 1);
+    HIP_CHECK(hipMemcpyAsync(h_total, off + nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(counts, off, (nkeys + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    ctx.sync();
+    total = *h_total;
+    uint32_t *entries = ctx.d<uint32_t>("msm.entries", std::max<uint32_t>(total, 1));
+    hipLaunchKernelGGL(k_digits<true>, dim3(grid_for(nthreads, TPB)), dim3(TPB), 0, st, fr_dev,
+                       u32_dev, (uint32_t)n, (uint32_t)K, sh.c, sh.windows, sh.P, sh.G, sh.B,
+                       (uint32_t)tab.stride, counts, entries);
+    HIP_CHECK(hipGetLastError());
+
+    // 3. chunks
+    const uint32_t avg = total / std::max<uint32_t>(nkeys, 1);
+    uint32_t K1 = std::max<uint32_t>(16, avg / 4);
+    if (K1 > 256) K1 = 256;
+    uint32_t *q = ctx.d<uint32_t>("msm.q", nkeys + 1);
+    uint32_t *qoff = ctx.d<uint32_t>("msm.qoff", nkeys + 1);
+    hipLaunchKernelGGL(k_chunk_count, dim3(grid_for(nkeys + 1, TPB)), dim3(TPB), 0, st, off,
+                       nkeys, K1, q);
+    size_t tmp2 = 0;
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, q, qoff, nkeys + 1, st));
+    tmp = ctx.d<uint8_t>("msm.cubtmp", std::max(tmp_bytes, tmp2));
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp2, q, qoff, nkeys + 1, st));
+    HIP_CHECK(hipMemcpyAsync(h_total, qoff + nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    ctx.sync();
+    const uint32_t nchunks = *h_total;
+    Chunk *desc = ctx.d<Chunk>("msm.desc", std::max<uint32_t>(nchunks, 1));
+    hipLaunchKernelGGL(k_chunk_fill, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, off, qoff,
+                       nkeys, K1, desc);
+
+    // 4. accumulate
+    Xyzz *csum = ctx.d<Xyzz>("msm.csum", std::max<uint32_t>(nchunks, 1));
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (ctx.profiling) {
+        HIP_CHECK(hipEventCreate(&ev0));
+        HIP_CHECK(hipEventCreate(&ev1));
+        HIP_CHECK(hipEventRecord(ev0, st));
+    }
+    if (nchunks)
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(nchunks, TPB)), dim3(TPB), 0, st, desc,
+                           nchunks, entries, tab.points, csum);
+    HIP_CHECK(hipGetLastError());
+    if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
+
+    // 5. buckets
+    Xyzz *buckets = ctx.d<Xyzz>("msm.buckets", nkeys);
+    uint32_t *heavy = ctx.d<uint32_t>("msm.heavy", nkeys + 1);
+    uint32_t *nheavy = heavy + nkeys;
+    HIP_CHECK(hipMemsetAsync(nheavy, 0, sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_bucket_light, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys,
+                       csum, buckets, heavy, nheavy);
+    hipLaunchKernelGGL(k_bucket_heavy, dim3(1024), dim3(TPB), 0, st, qoff, csum, buckets, heavy,
+                       nheavy);
+    HIP_CHECK(hipGetLastError());
+
+    // 6. reduce: per problem (column, group): W = sum_b (b + 1) S_b
+    const uint32_t nprob = (uint32_t)(K * sh.G);
+    const uint32_t B = sh.B;
+    Xyzz *wout = ctx.d<Xyzz>("msm.wout", (size_t)nprob * 4);
+    Xyzz *h_w = ctx.h<Xyzz>("msm.hw", (size_t)nprob * 4);
+    if (B > (uint32_t)TPB) {
+        const uint32_t R = B / TPB;
+        Xyzz *Z = ctx.d<Xyzz>("msm.Z", (size_t)nprob * R);
+        Xyzz *U = ctx.d<Xyzz>("msm.U", (size_t)nprob * TPB);
+        hipLaunchKernelGGL(k_reduce_split, dim3(nprob * (R + TPB)), dim3(TPB), 0, st, buckets, B,
+                           Z, U);
+        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, Z, R, wout);
+        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, U, (uint32_t)TPB,
+                           wout + 2 * nprob);
+    } else {
+        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, buckets, B, wout);
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(h_w, wout, (size_t)nprob * 4 * sizeof(Xyzz), hipMemcpyDeviceToHost, st));
+    ctx.sync();
+
+    // 7. host tail
+    std::vector<Xyzz> res(K);
+    for (size_t col = 0; col < K; col++) {
+        Xyzz acc = xyzz_identity();
+        for (int g = sh.G - 1; g >= 0; g--) {
+            acc = host_xyzz_mul_pow2(acc, sh.c * sh.P);
+            const uint32_t p = (uint32_t)(col * sh.G + g);
+            Xyzz W;
+            if (B > (uint32_t)TPB) {
+                // 256 * WS(Z) + WS(U) + TOT(Z)
+                W = xyzz_add(host_xyzz_mul_pow2(h_w[2 * p], 8), h_w[2 * nprob + 2 * p]);
+                W = xyzz_add(W, h_w[2 * p + 1]);
+            } else {
+                W = xyzz_add(h_w[2 * p], h_w[2 * p + 1]);
+            }
+            acc = xyzz_add(acc, W);
+        }
+        res[col] = acc;
+    }
+    host_batch_to_affine(res.data(), out_host, K);
+    if (ctx.profiling) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        ctx.stats.accumulate_ms += ms;
+        ctx.stats.accumulate_launches += nchunks ? 1 : 0;
+        ctx.stats.accumulate_entries += total;
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    }
+}
+
+}  // namespace apg

@@ -1,0 +1,111 @@
+// Library runtime: per-context HIP stream, a grow-only device arena of named
+// scratch buffers, pinned host staging, and the error channel of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace apg {
+
+// Raised inside the library; every extern "C" entry catches it and returns a
+// negative code. Nothing is thrown across the C ABI.
+struct ApgError : std::runtime_error {
+    int code;
+    ApgError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define HIP_CHECK(x)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            throw ::apg::ApgError(-3, std::string("HIP error at " __FILE__ ":") +           \
+                                          std::to_string(__LINE__) + ": " + #x + ": " +      \
+                                          hipGetErrorString(e_));                            \
+    } while (0)
+
+// A device buffer that only grows. Reused across calls on one context.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes) {
+        if (bytes > cap) {
+            if (ptr) HIP_CHECK(hipFree(ptr));
+            ptr = nullptr;
+            size_t c = bytes + bytes / 4 + 256;
+            HIP_CHECK(hipMalloc(&ptr, c));
+            cap = c;
+        }
+        return ptr;
+    }
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+struct HostBuf {
+    void *ptr = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes) {
+        if (bytes > cap) {
+            if (ptr) HIP_CHECK(hipHostFree(ptr));
+            ptr = nullptr;
+            size_t c = bytes + bytes / 4 + 256;
+            HIP_CHECK(hipHostMalloc(&ptr, c, hipHostMallocDefault));
+            cap = c;
+        }
+        return ptr;
+    }
+    ~HostBuf() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+};
+
+// Kernel timings recorded with HIP events on the context's stream when
+// profiling is on (apg_ctx_set_profiling).
+struct Stats {
+    double accumulate_ms = 0;        // k_accumulate (MSM bucket accumulation)
+    uint64_t accumulate_launches = 0;
+    uint64_t accumulate_entries = 0;  // nonzero signed digits processed
+    double msm_ms = 0;                // whole msm_run, host wall clock
+    uint64_t msm_calls = 0;
+    uint64_t msm_points = 0;          // scalars (points x columns)
+    double sumcheck_kernel_ms = 0;    // round kernels (fold + evaluate)
+    uint64_t sumcheck_launches = 0;
+    uint64_t sumcheck_bytes = 0;      // algorithmic HBM bytes of those launches
+    double sumcheck_ms = 0;           // whole sumcheck, host wall clock
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    Stats stats;
+    std::mutex mu;  // one call at a time per context
+    std::unordered_map<std::string, DevBuf> dev;
+    std::unordered_map<std::string, HostBuf> host;
+    int cu_count = 256;
+
+    template <class T>
+    T *d(const char *name, size_t count) {
+        return static_cast<T *>(dev[name].get(count * sizeof(T)));
+    }
+    template <class T>
+    T *h(const char *name, size_t count) {
+        return static_cast<T *>(host[name].get(count * sizeof(T)));
+    }
+    void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+inline unsigned grid_for(size_t n, unsigned block) {
+    return (unsigned)((n + block - 1) / block);
+}
+
+}  // namespace apg

@@ -1,0 +1,56 @@
+// Sumcheck round evaluation and fold over multilinear Fr tables on gfx950.
+#pragma once
+#include <functional>
+#include <vector>
+
+#include "field.cuh"
+#include "runtime.h"
+
+namespace apg {
+
+constexpr int SC_MAX_TABLES = 24;
+constexpr int SC_MAX_TERMS = 32;
+constexpr int SC_MAX_NODES = 6;  // degree <= 5
+
+// One gate term: coef * x[f0] * x[f1] * x[f2]; an index < 0 is absent. The
+// reference's Gate (crates/sumcheck/src/lib.rs:88-112) has at most two
+// factors; a third factor is the degree-4 extension.
+struct Term {
+    Fe coef;
+    int32_t f[3];
+};
+
+struct Gate {
+    int nterms = 0;
+    Term terms[SC_MAX_TERMS];
+    // degree of a round polynomial: eq times the largest term, never below a
+    // cubic (SumcheckProof.rounds are always 4 coefficients for the reference).
+    int degree() const;
+    bool is_plain_product(int k) const;  // one term, coef 1, factors 0..k-1
+};
+
+// Per-round hook: receives the d+1 ascending coefficients, returns the challenge.
+using RoundFn = std::function<Fe(const Fe *coeffs, int ncoeffs)>;
+
+// gkr::prove_sumcheck (crates/gkr/src/lib.rs:792-899) with a gate summand.
+// Tables are device buffers of 2^num_vars elements each; they are consumed
+// (ping-pong scratch is internal). rounds_out receives num_vars * (d + 1)
+// coefficients, point_out num_vars challenges, finals_out the k fully bound
+// values.
+void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables_dev, int k,
+                           int num_vars, const Fe *eq_point_host, const RoundFn &round_fn,
+                           Fe *rounds_out, Fe *point_out, Fe *finals_out);
+
+// eq(r, x) for every x in {0,1}^n (crates/poly/src/lib.rs:233-248), on device.
+void eq_table_device(Context &ctx, const Fe *r_host, int n, Fe *out_dev);
+
+// Host interpolation: the d+1 values at nodes 0..d to ascending coefficients
+// (crates/sumcheck/src/lib.rs:294-305 for d = 3, Newton form beyond).
+void interpolate_host(const Fe *vals, int d, Fe *coeffs);
+
+// out[i] = sum_c w[c] * cols[c][i] for each of `outs` weight rows; cols are K
+// device columns of n elements, w is outs x K host weights.
+void combine_columns_device(Context &ctx, const Fe *cols_dev, size_t n, int K, const Fe *w_host,
+                            int outs, Fe *const *out_dev);
+
+}  // namespace apg

@@ -1,0 +1,197 @@
+/*
+ * apogee_gpu.h — C ABI of the MI355X (gfx950) per-shard proving path:
+ * the commitment MSM over BN254 G1 and the sumcheck round evaluation + fold.
+ *
+ * Everything crosses as plain bytes:
+ *   - Fr / Fq: 32 bytes, the reference's in-memory Montgomery image
+ *     (Fr::to_memory_bytes, crates/field/src/lib.rs:321): eight little-endian
+ *     u32 words of x * 2^256 mod m, fully reduced.
+ *   - G1 affine: 64 bytes, Montgomery x || y; all-zero is the point at
+ *     infinity ((0,0) is off the curve, crates/curve/src/g1.rs:106-111).
+ *   - "wire" G1: canonical x || y (G1Affine::to_bytes, g1.rs:92-100).
+ * Inputs are borrowed for the call, outputs go to caller-owned buffers.
+ * Entries return APG_OK (0) or a negative code and never abort or throw;
+ * apg_last_error() describes the last failure on the calling thread.
+ * Calls on one apg_ctx are serialised; use one context per worker thread
+ * for concurrency. Results never depend on concurrency.
+ */
+#ifndef APOGEE_GPU_H
+#define APOGEE_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APG_OK 0
+#define APG_ERR_ARG (-2)
+#define APG_ERR_HIP (-3)
+#define APG_ERR_OOM (-4)
+#define APG_ERR_INTERNAL (-5)
+#define APG_ERR_CALLBACK (-6)
+
+typedef struct apg_ctx apg_ctx;
+typedef struct apg_srs apg_srs;
+typedef struct apg_transcript apg_transcript;
+
+const char *apg_last_error(void);
+int apg_version(void);
+
+/* ---- context: one HIP stream + scratch arena on one device ---- */
+int apg_ctx_create(int device, apg_ctx **out);
+void apg_ctx_destroy(apg_ctx *ctx);
+int apg_ctx_synchronize(apg_ctx *ctx);
+
+/* ---- kernel timing, recorded with HIP events on the context's stream.
+ * No counterpart in the reference (its timing is tools/bench's wall clock,
+ * tools/bench/src/timing.rs:9-47); for benchmarks and roofline reports. */
+typedef struct {
+    double accumulate_ms;          /* MSM bucket-accumulation kernel */
+    uint64_t accumulate_launches;
+    uint64_t accumulate_entries;   /* nonzero signed digits it processed */
+    double msm_ms;                 /* whole commits, host wall clock */
+    uint64_t msm_calls;
+    uint64_t msm_points;           /* scalars committed (points x columns) */
+    double sumcheck_kernel_ms;     /* fused fold + round-evaluation kernels */
+    uint64_t sumcheck_launches;
+    uint64_t sumcheck_bytes;       /* algorithmic HBM bytes of those launches */
+    double sumcheck_ms;            /* whole sumchecks, host wall clock */
+} apg_stats;
+int apg_ctx_set_profiling(apg_ctx *ctx, int on);
+int apg_ctx_stats(apg_ctx *ctx, apg_stats *out);
+int apg_ctx_reset_stats(apg_ctx *ctx);
+
+/* ---- seeded synthetic inputs (host): ChaCha20 keyed by seed; kind 0 =
+ * uniform Fr (32 bytes, top two bits cleared, reject >= p, the draw rule of
+ * crates/pcs/tests/common/mod.rs:114-123), kind 1 = witness-like mix (20%
+ * zero, 10% one, 50% u32, 20% uniform Fr). Writes n Montgomery elements. No
+ * counterpart in the reference (its benches draw from splitmix64). */
+int apg_synthetic_fill(uint64_t seed, int kind, size_t n, uint8_t *out_mont);
+
+/* ---- device memory for callers without their own allocator ---- */
+int apg_dev_alloc(apg_ctx *ctx, size_t bytes, void **out);
+int apg_dev_free(apg_ctx *ctx, void *p);
+int apg_dev_upload(apg_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int apg_dev_download(apg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+
+/* ---- SRS residency.
+ * Replaces the &Srs every pcs::commit borrows (crates/srs/src/lib.rs:96-276,
+ * Srs::g1 :244): the powers [tau^i]_1 stay in HBM for the life of the handle.
+ * apg_srs_generate builds [tau^i]G on the device (the toy SRS construction of
+ * crates/pcs/tests/common/mod.rs:43-60, for any tau).
+ * apg_srs_prepare builds the window-shifted base tables for commits of up to
+ * n points ahead of time (otherwise the first commit of a size class does). */
+int apg_srs_upload(apg_ctx *ctx, const uint8_t *bases_mont, size_t n, apg_srs **out);
+int apg_srs_generate(apg_ctx *ctx, const uint8_t tau_mont[32], size_t n, apg_srs **out);
+int apg_srs_download(apg_ctx *ctx, const apg_srs *srs, size_t offset, size_t n, uint8_t *out);
+size_t apg_srs_len(const apg_srs *srs);
+int apg_srs_prepare(apg_ctx *ctx, apg_srs *srs, size_t n);
+size_t apg_srs_device_bytes(const apg_srs *srs);
+void apg_srs_release(apg_srs *srs);
+
+/* ---- MSM over explicit bases.
+ * apg_msm_g1_fr replaces curve::msm::msm (crates/curve/src/msm.rs:48);
+ * apg_msm_g1_u32 replaces curve::msm::msm_small_u32 (msm.rs:79). n == 0
+ * gives the identity (msm.rs:55-57); the reference's only error,
+ * LengthMismatch (msm.rs:29-33), cannot arise with one length. */
+int apg_msm_g1_fr(apg_ctx *ctx, const uint8_t *bases_mont, const uint8_t *scalars_mont, size_t n,
+                  uint8_t out_mont[64]);
+int apg_msm_g1_u32(apg_ctx *ctx, const uint8_t *bases_mont, const uint32_t *scalars, size_t n,
+                   uint8_t out_mont[64]);
+
+/* ---- commitments over the resident SRS.
+ * k columns of n scalars each, column-major, against srs powers [0, n):
+ * pcs::commit / column_msm (crates/pcs/src/lib.rs:237-247, :280-298) for each
+ * column; one call covers a whole commit_all (crates/prover/src/lib.rs:444).
+ * No size rule is applied here (that is kzg_commit, crates/srs/src/kzg.rs:20);
+ * pcs::commit's check_num_vars stays with the caller.
+ * The _dev forms take device pointers (scalars resident in HBM). */
+int apg_commit_fr(apg_ctx *ctx, apg_srs *srs, const uint8_t *scalars_mont, size_t n, size_t k,
+                  uint8_t *out_mont);
+int apg_commit_u32(apg_ctx *ctx, apg_srs *srs, const uint32_t *scalars, size_t n, size_t k,
+                   uint8_t *out_mont);
+int apg_commit_fr_dev(apg_ctx *ctx, apg_srs *srs, const void *scalars_dev, size_t n, size_t k,
+                      uint8_t *out_mont);
+
+/* ---- sumcheck.
+ * A gate term is coef * x[factor[0]] * x[factor[1]] * x[factor[2]], factor
+ * -1 = absent (crates/sumcheck/src/lib.rs:88-112 allows two factors; a third
+ * is the degree-4 extension). Round polynomials have d + 1 coefficients,
+ * d = max(3, 1 + max factors). */
+typedef struct {
+    uint8_t coef_mont[32];
+    int32_t factor[3];
+} apg_term;
+
+/* Per-round hook: receives the round's d + 1 ascending coefficients and
+ * writes the challenge. Return 0, or nonzero to abort the proof. */
+typedef int (*apg_round_fn)(void *user, const uint8_t *coeffs_mont, size_t n_coeffs,
+                            uint8_t challenge_mont[32]);
+
+/* gkr::prove_sumcheck (crates/gkr/src/lib.rs:792) with a gate summand: the
+ * rounds of sum_x eq(eq_point, x) * G(tables(x)). tables: k host pointers
+ * to 2^num_vars elements each (not modified). Outputs: rounds_out
+ * num_vars * (d + 1) elements, point_out num_vars, final_evals_out k (the
+ * tables bound to the point). */
+int apg_sumcheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
+                       const uint8_t *const *tables_mont, size_t k, size_t num_vars,
+                       const uint8_t *eq_point_mont, apg_round_fn round_fn, void *user,
+                       uint8_t *rounds_out, uint8_t *point_out, uint8_t *final_evals_out);
+/* the same with device-resident tables (consumed: contents are left undefined) */
+int apg_sumcheck_prove_dev(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
+                           void *const *tables_dev, size_t k, size_t num_vars,
+                           const uint8_t *eq_point_mont, apg_round_fn round_fn, void *user,
+                           uint8_t *rounds_out, uint8_t *point_out, uint8_t *final_evals_out);
+
+/* sumcheck::prove_zerocheck (crates/sumcheck/src/lib.rs:359): n
+ * SUMCHECK_CHALLENGE draws for r, the rounds, then final_evals absorbed under
+ * SUMCHECK_FINAL_EVALS, all on the given transcript. */
+int apg_zerocheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
+                        const uint8_t *const *tables_mont, size_t k, size_t num_vars,
+                        apg_transcript *t, uint8_t *rounds_out, uint8_t *final_evals_out);
+
+/* ---- one shard: commit k columns, then the batched sumcheck.
+ * The build-defined composition of reference primitives used by the
+ * benchmarks: commitments absorbed as one COMMITMENT message (pcs
+ * append_g1_list, crates/pcs/src/lib.rs:220); rho = MERCURY_BATCH challenge;
+ * f = degree - 1 combined tables t_o = sum_c rho^(o k + c) col_c; eq point =
+ * num_vars SUMCHECK_CHALLENGE draws; rounds of eq * t_0 * ... * t_(f-1)
+ * (SUMCHECK_ROUND / SUMCHECK_CHALLENGE per round); final evals absorbed under
+ * SUMCHECK_FINAL_EVALS. columns_dev: k * 2^num_vars elements, device. */
+int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, size_t num_vars,
+                        size_t k, int degree, apg_transcript *t, uint8_t *commitments_wire,
+                        uint8_t *rounds_out, uint8_t *final_evals_out);
+
+/* ---- the Poseidon2 transcript (crates/transcript/src/lib.rs) ---- */
+int apg_transcript_new(apg_transcript **out);
+void apg_transcript_free(apg_transcript *t);
+int apg_transcript_clone(const apg_transcript *t, apg_transcript **out);
+void apg_transcript_observe(apg_transcript *t, const uint8_t x_mont[32]);
+void apg_transcript_sample(apg_transcript *t, uint8_t out_mont[32]);
+void apg_transcript_append_scalars(apg_transcript *t, uint64_t tag, const uint8_t *xs_mont,
+                                   size_t n);
+void apg_transcript_append_bytes(apg_transcript *t, uint64_t tag, const uint8_t *bytes, size_t len);
+void apg_transcript_challenge(apg_transcript *t, uint64_t tag, uint8_t out_mont[32]);
+void apg_transcript_append_g1(apg_transcript *t, uint64_t tag, const uint8_t *wire64s, size_t k);
+/* an apg_round_fn: append_scalars(SUMCHECK_ROUND, coeffs), then
+ * challenge_scalar(SUMCHECK_CHALLENGE) (crates/sumcheck/src/lib.rs:392-395) */
+int apg_transcript_round(void *transcript, const uint8_t *coeffs_mont, size_t n_coeffs,
+                         uint8_t challenge_mont[32]);
+/* sponge state: state[3], input[2], output[2] (Montgomery, 32 B each), then
+ * input_len and output_len as u32: 232 bytes */
+int apg_transcript_state(const apg_transcript *t, uint8_t out[232]);
+
+/* ---- encodings ---- */
+int apg_fr_to_bytes(const uint8_t mont[32], uint8_t canonical[32]);
+int apg_fr_from_bytes(const uint8_t canonical[32], uint8_t mont[32]); /* APG_ERR_ARG if >= p */
+int apg_fq_to_bytes(const uint8_t mont[32], uint8_t canonical[32]);
+int apg_fq_from_bytes(const uint8_t canonical[32], uint8_t mont[32]);
+void apg_fr_from_u64(uint64_t x, uint8_t mont[32]);
+int apg_g1_to_wire(const uint8_t mont[64], uint8_t wire[64]);
+int apg_g1_from_wire(const uint8_t wire[64], uint8_t mont[64]); /* checks canonicity + curve */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -99,6 +99,7 @@ def _declare(L):
     d("orc_prove_zerocheck", [P, SZ, P, SZ, SZ, P, P, P])
     d("orc_prove_sumcheck", [P, SZ, P, SZ, P, SZ, P, SZ, P, P])
     d("orc_witness_digest", [P, SZ, SZ, P])
+    d("orc_combine", [P, SZ, SZ, P, SZ, P])
     d("orc_splitmix_next", [P], U64)
     d("orc_splitmix_next_fr", [P, P])
     d("orc_chacha_block", [P, C.c_uint32, P, P])
@@ -446,6 +447,40 @@ def witness_digest(tables):
     out = fe()
     lib().orc_witness_digest(_table_ptrs(work), len(work), n, ptr(out))
     return out
+
+
+def combine(cols: np.ndarray, weights: np.ndarray, outs: int) -> np.ndarray:
+    """cols (k, n, 4), weights (outs * k, 4) -> (outs, n, 4)."""
+    k, n = cols.shape[0], cols.shape[1]
+    out = np.zeros((outs, n, 4), dtype=np.uint64)
+    lib().orc_combine(ptr(np.ascontiguousarray(cols)), n, k, ptr(np.ascontiguousarray(weights)), outs, ptr(out))
+    return out
+
+
+def shard_script(srs_aff, cols: np.ndarray, degree: int, transcript: "Transcript", workers: int = 8):
+    """The shard composition (commit k columns, absorb as one COMMITMENT
+    message, rho = MERCURY_BATCH, f = degree - 1 combinations with weights
+    rho^(o k + c), eq point = num_vars SUMCHECK_CHALLENGE draws, the rounds of
+    eq * t_0 * ... * t_(f-1), finals under SUMCHECK_FINAL_EVALS), built from
+    the restated primitives. Returns (commitment wires, rounds, finals)."""
+    k, n = cols.shape[0], cols.shape[1]
+    nv = n.bit_length() - 1
+    cms = [jac_to_wire(msm(srs_aff[:n], cols[c], threads=8, workers=workers)) for c in range(k)]
+    transcript.append_g1(TAG_COMMITMENT, cms)
+    rho = transcript.challenge(TAG_MERCURY_BATCH)
+    f = degree - 1
+    w = fe(f * k)
+    acc = from_u64(FR, 1)
+    for i in range(f * k):
+        w[i] = acc
+        acc = mul(FR, acc, rho)
+    tables = combine(cols, w, f)
+    r = np.stack([transcript.challenge(TAG_SUMCHECK_CHALLENGE) for _ in range(nv)])
+    terms = [(from_u64(FR, 1), list(range(f)))]
+    rounds, point, finals = prove_sumcheck(r, terms, [tables[o] for o in range(f)], transcript, workers=workers)
+    fin = np.stack([x[0] for x in finals])
+    transcript.append_scalars(TAG_SUMCHECK_FINAL_EVALS, fin)
+    return cms, rounds, fin
 
 
 # ---------------------------------------------------------------- inputs

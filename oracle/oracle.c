@@ -1133,6 +1133,22 @@ void orc_prove_zerocheck(const orc_term *terms, size_t nt, u64 **tables, size_t 
     free(point);
 }
 
+/* The random-linear-combination fold of k same-size columns:
+ * out_o[i] = sum_c w[o k + c] * col_c[i] (pcs::batch_open_stacked builds its
+ * combined column this way, crates/pcs/src/lib.rs:768-780, with weights
+ * rho^i from batch_preamble, crates/pcs-verify/src/lib.rs:333-345). */
+void orc_combine(const u64 *cols, size_t n, size_t k, const u64 *w, size_t outs, u64 *out) {
+    for (size_t o = 0; o < outs; o++)
+        for (size_t i = 0; i < n; i++) {
+            u64 acc[4] = {0, 0, 0, 0}, t[4];
+            for (size_t c = 0; c < k; c++) {
+                f_mul(FR, w + 4 * (o * k + c), cols + 4 * (c * n + i), t);
+                f_add(FR, acc, t, acc);
+            }
+            memcpy(out + 4 * (o * n + i), acc, 32);
+        }
+}
+
 /* sumcheck/src/lib.rs:228-255 witness_digest */
 void orc_witness_digest(u64 **tables, size_t k, size_t n, u64 *out) {
     orc_transcript s;

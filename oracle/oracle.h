@@ -102,6 +102,9 @@ void orc_prove_sumcheck(const uint64_t *eq_point, size_t num_vars, const orc_ter
                         size_t n_terms, uint64_t **tables, size_t k, orc_transcript *t,
                         size_t workers, uint64_t *rounds_out, uint64_t *point_out);
 void orc_witness_digest(uint64_t **tables, size_t k, size_t num_vars, uint64_t *out);
+/* out_o[i] = sum_c w[o k + c] * cols[c n + i] for o < outs */
+void orc_combine(const uint64_t *cols, size_t n, size_t k, const uint64_t *w, size_t outs,
+                 uint64_t *out);
 
 /* ---- seeded inputs ---- */
 uint64_t orc_splitmix_next(uint64_t *state);

@@ -1,0 +1,136 @@
+"""GPU parity for the sumcheck rounds and the shard composition, through the C
+ABI, against the oracle (bit-exact round polynomials, points, final
+evaluations and transcript state)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def apg():
+    import importlib
+
+    m = importlib.import_module("apogee-vm_amd")
+    m.default_context()
+    return m
+
+
+def _gates(apg, orc):
+    one = orc.from_u64(orc.FR, 1)
+    m1 = orc.sub(orc.FR, orc.fe(), one)
+    seven = orc.from_u64(orc.FR, 7)
+    spec = {
+        "product": (2, [(one, [0, 1])]),
+        "product3": (3, [(one, [0, 1, 2])]),
+        "square_minus": (2, [(one, [0, 0]), (m1, [1])]),
+        "wide": (5, [(one, [0, 1]), (one, [0, 2]), (m1, [3, 4])]),
+        "linear": (1, [(seven, [0])]),
+        "mixed3": (4, [(seven, [0, 1, 2]), (m1, [3]), (one, [1, 3])]),
+    }
+    out = {}
+    for name, (k, terms) in spec.items():
+        g = apg.Gate(list(range(k)), [apg.GateTerm(c, *(fs + [None] * (3 - len(fs)))) for c, fs in terms])
+        out[name] = (k, terms, g)
+    return out
+
+
+def _tables(orc, k, n, seed, witness=False):
+    return [(orc.fill_witness if witness else orc.fill_uniform)(seed + t, 1 << n) for t in range(k)]
+
+
+@pytest.mark.parametrize("gate", ["product", "product3", "square_minus", "wide", "linear", "mixed3"])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8])
+def test_prove_sumcheck_vs_oracle(apg, orc, gate, n):
+    k, terms, g = _gates(apg, orc)[gate]
+    tabs = _tables(orc, k, n, 1000 * n + len(gate))
+    eqp = orc.fill_uniform(77 + n, n)
+    t_gpu, t_orc = apg.Transcript(), orc.Transcript()
+    r1, p1, f1 = apg.prove_sumcheck(eqp, g, tabs, t_gpu)
+    r2, p2, f2 = orc.prove_sumcheck(eqp, terms, tabs, t_orc)
+    assert np.array_equal(r1, r2)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(f1, np.stack([x[0] for x in f2]))
+    assert np.array_equal(t_gpu.challenge_scalar(5), t_orc.challenge(5))
+
+
+@pytest.mark.parametrize("n,k,seed", [(12, 2, 5), (14, 3, 6)])
+def test_prove_sumcheck_product_large(apg, orc, n, k, seed):
+    one = orc.from_u64(orc.FR, 1)
+    terms = [(one, list(range(k)))]
+    g = apg.Gate.product(k)
+    tabs = _tables(orc, k, n, seed, witness=True)
+    eqp = orc.fill_uniform(seed, n)
+    t_gpu, t_orc = apg.Transcript(), orc.Transcript()
+    r1, p1, f1 = apg.prove_sumcheck(eqp, g, tabs, t_gpu)
+    r2, p2, f2 = orc.prove_sumcheck(eqp, terms, tabs, t_orc, workers=8)
+    assert np.array_equal(r1, r2)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(f1, np.stack([x[0] for x in f2]))
+
+
+def test_zerocheck_vs_oracle(apg, orc):
+    one = orc.from_u64(orc.FR, 1)
+    m1 = orc.sub(orc.FR, orc.fe(), one)
+    n = 7
+    a = orc.fill_uniform(3, 1 << n)
+    b = np.stack([orc.mul(orc.FR, v, v) for v in a])  # satisfies a*a - b = 0
+    terms = [(one, [0, 0]), (m1, [1])]
+    g = apg.Gate([0, 1], [apg.GateTerm(one, 0, 0), apg.GateTerm(m1, 1)])
+    t_gpu, t_orc = apg.Transcript(), orc.Transcript()
+    digest = orc.witness_digest([a, b])
+    t_gpu.append_scalar(orc.TAG_WITNESS_DIGEST, digest)
+    t_orc.append_scalars(orc.TAG_WITNESS_DIGEST, digest)
+    proof = apg.prove_zerocheck(g, [a, b], t_gpu)
+    r2, f2 = orc.prove_zerocheck(terms, [a, b], t_orc)
+    assert np.array_equal(proof.rounds, r2)
+    assert np.array_equal(proof.final_evals, f2)
+    assert t_gpu.state()[:96] == t_orc.snapshot()[:96]
+    # round 0 sums to zero on a satisfying witness
+    s = orc.add(orc.FR, r2[0][0], orc.add(orc.FR, orc.add(orc.FR, r2[0][0], r2[0][1]), orc.add(orc.FR, r2[0][2], r2[0][3])))
+    assert not s.any()
+
+
+@pytest.mark.parametrize("nv,k,degree", [(10, 4, 3), (9, 3, 4), (12, 8, 3)])
+def test_prove_shard_vs_oracle(apg, orc, nv, k, degree):
+    n = 1 << nv
+    tau = orc.fill_uniform(555, 1)[0]
+    srs = apg.Srs.generate(tau, n)
+    srs_aff = srs.g1()
+    cols = np.stack([orc.fill_witness(7000 + c, n) for c in range(k)])
+    ctx = apg.default_context()
+    dev = apg.DeviceBuffer.from_array(ctx, cols)
+    t_gpu, t_orc = apg.Transcript(), orc.Transcript()
+    proof = apg.prove_shard(srs, dev, nv, k, degree, t_gpu)
+    cms, rounds, fin = orc.shard_script(srs_aff, cols, degree, t_orc)
+    assert proof.commitments == cms
+    assert np.array_equal(proof.rounds, rounds)
+    assert np.array_equal(proof.final_evals, fin)
+    assert t_gpu.state()[:96] == t_orc.snapshot()[:96]
+
+
+def test_config1_end_to_end(apg, orc):
+    """One 2^16 column committed, then a 16-variable degree-3 sumcheck on
+    eq(r, x) a(x) b(x) with r seeded: commitment + 16 rounds bit-exact."""
+    n, nv = 1 << 16, 16
+    seed = 2026
+    tau = orc.fill_uniform(seed, 1)[0]
+    srs = apg.Srs.generate(tau, n)
+    srs_aff = srs.g1()
+    f, a, b = (orc.fill_uniform(seed + 1 + i, n) for i in range(3))
+    r = orc.fill_uniform(seed + 4, nv)
+    one = orc.from_u64(orc.FR, 1)
+    # oracle
+    t_orc = orc.Transcript()
+    cm_o = orc.jac_to_wire(orc.msm(srs_aff, f, workers=8))
+    t_orc.append_g1(orc.TAG_COMMITMENT, [cm_o])
+    r_o, p_o, f_o = orc.prove_sumcheck(r, [(one, [0, 1])], [a, b], t_orc, workers=8)
+    # gpu
+    t_gpu = apg.Transcript()
+    cm_g = apg.g1_to_wire(apg.commit(srs, f))
+    t_gpu.append_g1_points(orc.TAG_COMMITMENT, [cm_g])
+    r_g, p_g, f_g = apg.prove_sumcheck(r, apg.Gate.product(2), [a, b], t_gpu)
+    assert cm_g == cm_o
+    assert np.array_equal(r_g, r_o)
+    assert np.array_equal(p_g, p_o)
+    assert np.array_equal(f_g, np.stack([x[0] for x in f_o]))
