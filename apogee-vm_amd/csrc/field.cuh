@@ -41,6 +41,7 @@ struct FrP {
         return r[i];
     }
     static constexpr uint32_t inv = 0xefffffffu;
+    static constexpr uint64_t inv64 = 0xc2e1f593efffffffull;
 };
 
 // Fq: crates/constants/src/lib.rs:97-143. -q^-1 mod 2^32 = 0xe4866389.
@@ -61,6 +62,7 @@ struct FqP {
         return r[i];
     }
     static constexpr uint32_t inv = 0xe4866389u;
+    static constexpr uint64_t inv64 = 0x87d20782e4866389ull;
 };
 
 template <class P>
@@ -109,8 +111,70 @@ APG_HD void f_reduce_once(Fe &a) {
     for (int i = 0; i < 8; i++) a.l[i] = br ? a.l[i] : s[i];
 }
 
+#if !defined(__HIP_DEVICE_COMPILE__)
+template <class P>
+inline void host_sub_mod_if_ge(uint64_t t[4]) {
+    uint64_t d[4], br = 0;
+    for (int i = 0; i < 4; i++) {
+        const uint64_t m = (uint64_t)P::mod(2 * i) | ((uint64_t)P::mod(2 * i + 1) << 32);
+        unsigned __int128 x = (unsigned __int128)t[i] - m - br;
+        d[i] = (uint64_t)x;
+        br = (uint64_t)(x >> 64) & 1;
+    }
+    if (!br)
+        for (int i = 0; i < 4; i++) t[i] = d[i];
+}
+template <class P>
+inline Fe f_add_host64(const Fe &a, const Fe &b) {
+    uint64_t t[4], c = 0;
+    for (int i = 0; i < 4; i++) {
+        const uint64_t x = (uint64_t)a.l[2 * i] | ((uint64_t)a.l[2 * i + 1] << 32);
+        const uint64_t y = (uint64_t)b.l[2 * i] | ((uint64_t)b.l[2 * i + 1] << 32);
+        unsigned __int128 s = (unsigned __int128)x + y + c;
+        t[i] = (uint64_t)s;
+        c = (uint64_t)(s >> 64);
+    }
+    host_sub_mod_if_ge<P>(t);
+    Fe r;
+    for (int i = 0; i < 4; i++) {
+        r.l[2 * i] = (uint32_t)t[i];
+        r.l[2 * i + 1] = (uint32_t)(t[i] >> 32);
+    }
+    return r;
+}
+template <class P>
+inline Fe f_sub_host64(const Fe &a, const Fe &b) {
+    uint64_t t[4], br = 0;
+    for (int i = 0; i < 4; i++) {
+        const uint64_t x = (uint64_t)a.l[2 * i] | ((uint64_t)a.l[2 * i + 1] << 32);
+        const uint64_t y = (uint64_t)b.l[2 * i] | ((uint64_t)b.l[2 * i + 1] << 32);
+        unsigned __int128 s = (unsigned __int128)x - y - br;
+        t[i] = (uint64_t)s;
+        br = (uint64_t)(s >> 64) & 1;
+    }
+    if (br) {
+        uint64_t c = 0;
+        for (int i = 0; i < 4; i++) {
+            const uint64_t m = (uint64_t)P::mod(2 * i) | ((uint64_t)P::mod(2 * i + 1) << 32);
+            unsigned __int128 s = (unsigned __int128)t[i] + m + c;
+            t[i] = (uint64_t)s;
+            c = (uint64_t)(s >> 64);
+        }
+    }
+    Fe r;
+    for (int i = 0; i < 4; i++) {
+        r.l[2 * i] = (uint32_t)t[i];
+        r.l[2 * i + 1] = (uint32_t)(t[i] >> 32);
+    }
+    return r;
+}
+#endif
+
 template <class P>
 APG_HD Fe f_add(const Fe &a, const Fe &b) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+    return f_add_host64<P>(a, b);
+#endif
     Fe r;
     uint64_t c = 0;
 #pragma unroll
@@ -126,6 +190,9 @@ APG_HD Fe f_add(const Fe &a, const Fe &b) {
 
 template <class P>
 APG_HD Fe f_sub(const Fe &a, const Fe &b) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+    return f_sub_host64<P>(a, b);
+#endif
     Fe r;
     uint64_t br = 0;
 #pragma unroll
@@ -160,9 +227,64 @@ APG_HD Fe f_dbl(const Fe &a) {
     return f_add<P>(a, a);
 }
 
+#if !defined(__HIP_DEVICE_COMPILE__)
+// Host build: the same CIOS over four 64-bit words with 128-bit intermediates
+// (the reference's own limb width), so host tails and the host transcript run
+// at CPU speed. Same R = 2^256, so the memory images agree.
+template <class P>
+inline Fe f_mul_host64(const Fe &a, const Fe &b) {
+    typedef unsigned __int128 u128;
+    uint64_t A[4], Bv[4], M[4], t[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+        A[i] = (uint64_t)a.l[2 * i] | ((uint64_t)a.l[2 * i + 1] << 32);
+        Bv[i] = (uint64_t)b.l[2 * i] | ((uint64_t)b.l[2 * i + 1] << 32);
+        M[i] = (uint64_t)P::mod(2 * i) | ((uint64_t)P::mod(2 * i + 1) << 32);
+    }
+    for (int i = 0; i < 4; i++) {
+        uint64_t c = 0;
+        for (int j = 0; j < 4; j++) {
+            u128 x = (u128)A[j] * Bv[i] + t[j] + c;
+            t[j] = (uint64_t)x;
+            c = (uint64_t)(x >> 64);
+        }
+        u128 s = (u128)t[4] + c;
+        t[4] = (uint64_t)s;
+        t[5] = (uint64_t)(s >> 64);
+        const uint64_t m = t[0] * P::inv64;
+        u128 x = (u128)m * M[0] + t[0];
+        c = (uint64_t)(x >> 64);
+        for (int j = 1; j < 4; j++) {
+            x = (u128)m * M[j] + t[j] + c;
+            t[j - 1] = (uint64_t)x;
+            c = (uint64_t)(x >> 64);
+        }
+        s = (u128)t[4] + c;
+        t[3] = (uint64_t)s;
+        t[4] = t[5] + (uint64_t)(s >> 64);
+    }
+    // conditional subtraction of m
+    uint64_t d[4], br = 0;
+    for (int i = 0; i < 4; i++) {
+        u128 x = (u128)t[i] - M[i] - br;
+        d[i] = (uint64_t)x;
+        br = (uint64_t)(x >> 64) & 1;
+    }
+    Fe r;
+    for (int i = 0; i < 4; i++) {
+        const uint64_t v = br ? t[i] : d[i];
+        r.l[2 * i] = (uint32_t)v;
+        r.l[2 * i + 1] = (uint32_t)(v >> 32);
+    }
+    return r;
+}
+#endif
+
 // a * b * 2^-256 mod m, CIOS over 32-bit words.
 template <class P>
 APG_HD Fe f_mul(const Fe &a, const Fe &b) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+    return f_mul_host64<P>(a, b);
+#endif
     uint32_t t[9];
 #pragma unroll
     for (int j = 0; j < 9; j++) t[j] = 0;
