@@ -30,7 +30,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libapogee_gpu.so")
+# APG_LIB selects another build of the library (A/B timing of variants)
+LIB_PATH = os.environ.get("APG_LIB") or os.path.join(HERE, "libapogee_gpu.so")
 
 # crates/constants/src/lib.rs:579-770
 TAG_COMMITMENT = 3
@@ -134,6 +135,7 @@ def _declare(L):
     d("apg_ctx_stats", [VP, C.POINTER(ApgStats)])
     d("apg_ctx_reset_stats", [VP])
     d("apg_synthetic_fill", [U64, I32, SZ, VP])
+    d("apg_debug_mul32_batch", [I32, VP, VP, SZ, VP])
 
 
 class ApgStats(C.Structure):

@@ -190,22 +190,14 @@ int apg_ctx_create(int device, apg_ctx **out) {
         if (device < 0 || device >= ndev) throw ApgError(APG_ERR_ARG, "no such device");
         HIP_CHECK(hipSetDevice(device));
         auto *c = new apg_ctx();
-        c->c.device = device;
-        HIP_CHECK(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
-        hipDeviceProp_t prop;
-        HIP_CHECK(hipGetDeviceProperties(&prop, device));
-        c->c.cu_count = prop.multiProcessorCount;
+        context_init(c->c, device);
         *out = c;
     });
 }
 
 void apg_ctx_destroy(apg_ctx *ctx) {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->c.device);
-    (void)hipStreamSynchronize(ctx->c.stream);
-    ctx->c.dev.clear();
-    ctx->c.host.clear();
-    (void)hipStreamDestroy(ctx->c.stream);
+    context_release(ctx->c);
     delete ctx;
 }
 
@@ -582,6 +574,15 @@ int apg_fq_from_bytes(const uint8_t canonical[32], uint8_t mont[32]) {
     return APG_OK;
 }
 void apg_fr_from_u64(uint64_t x, uint8_t mont[32]) { store_fe(mont, f_from_u64<Fr>(x)); }
+
+int apg_debug_mul32_batch(int which, const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
+    if (which != 0 && which != 1) return APG_ERR_ARG;
+    for (size_t i = 0; i < n; i++) {
+        const Fe x = load_fe(a + 32 * i), y = load_fe(b + 32 * i);
+        store_fe(out + 32 * i, which ? f_mul32<Fq>(x, y) : f_mul32<Fr>(x, y));
+    }
+    return APG_OK;
+}
 
 int apg_g1_to_wire(const uint8_t mont[64], uint8_t wire[64]) {
     Affine a;

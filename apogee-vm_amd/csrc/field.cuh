@@ -279,46 +279,58 @@ inline Fe f_mul_host64(const Fe &a, const Fe &b) {
 }
 #endif
 
-// a * b * 2^-256 mod m, CIOS over 32-bit words.
+// lo + 2^32 hi = a * b + x + y for 32-bit a, b, x, y (never overflows 64
+// bits): one add-with-carry builds the 64-bit addend, one v_mad_u64_u32 does
+// the rest, with no register shuffling of zero-extended operands.
+APG_HD uint32_t mac2(uint32_t a, uint32_t b, uint32_t x, uint32_t y, uint32_t &hi) {
+    unsigned co;
+    const uint32_t lo = __builtin_addc(x, y, 0u, &co);
+    const uint64_t p = (uint64_t)a * b + (((uint64_t)co << 32) | lo);
+    hi = (uint32_t)(p >> 32);
+    return (uint32_t)p;
+}
+
+// a * b * 2^-256 mod m over eight 32-bit words: CIOS with the product and
+// reduction rows interleaved and no words above the eighth. That is exact
+// when the modulus' top word is below 2^31 - 1 (0x30644e72 for both BN254
+// moduli): every row's accumulator stays below 2m < 2^255, so the two carries
+// of a row fit in the top word (the "no-carry" CIOS bound). Inputs reduced,
+// output reduced by one conditional subtraction.
 template <class P>
-APG_HD Fe f_mul(const Fe &a, const Fe &b) {
-#if !defined(__HIP_DEVICE_COMPILE__)
-    return f_mul_host64<P>(a, b);
-#endif
-    uint32_t t[9];
+APG_HD Fe f_mul32(const Fe &a, const Fe &b) {
+    static_assert(P::mod(7) < 0x7fffffffu, "no-carry CIOS needs a spare top bit");
+    uint32_t t[8];
 #pragma unroll
-    for (int j = 0; j < 9; j++) t[j] = 0;
+    for (int j = 0; j < 8; j++) t[j] = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         const uint32_t bi = b.l[i];
-        uint64_t c = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            uint64_t p = (uint64_t)a.l[j] * bi + t[j] + c;
-            t[j] = (uint32_t)p;
-            c = p >> 32;
-        }
-        uint64_t s = (uint64_t)t[8] + c;
-        t[8] = (uint32_t)s;
-        const uint32_t t9 = (uint32_t)(s >> 32);
-        const uint32_t mq = t[0] * P::inv;
-        uint64_t p = (uint64_t)mq * P::mod(0) + t[0];
-        c = p >> 32;
+        const uint64_t p0 = (uint64_t)a.l[0] * bi + t[0];
+        t[0] = (uint32_t)p0;
+        uint32_t A = (uint32_t)(p0 >> 32);
+        const uint32_t m = t[0] * P::inv;
+        uint32_t C = (uint32_t)(((uint64_t)m * P::mod(0) + t[0]) >> 32);
 #pragma unroll
         for (int j = 1; j < 8; j++) {
-            p = (uint64_t)mq * P::mod(j) + t[j] + c;
-            t[j - 1] = (uint32_t)p;
-            c = p >> 32;
+            t[j] = mac2(a.l[j], bi, t[j], A, A);
+            t[j - 1] = mac2(m, P::mod(j), t[j], C, C);
         }
-        s = (uint64_t)t[8] + c;
-        t[7] = (uint32_t)s;
-        t[8] = t9 + (uint32_t)(s >> 32);
+        t[7] = C + A;
     }
     Fe r;
 #pragma unroll
     for (int j = 0; j < 8; j++) r.l[j] = t[j];
     f_reduce_once<P>(r);
     return r;
+}
+
+template <class P>
+APG_HD Fe f_mul(const Fe &a, const Fe &b) {
+#if !defined(__HIP_DEVICE_COMPILE__)
+    return f_mul_host64<P>(a, b);
+#else
+    return f_mul32<P>(a, b);
+#endif
 }
 
 template <class P>

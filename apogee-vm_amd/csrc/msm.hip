@@ -6,17 +6,17 @@
 //
 //  1. digits  : one thread per scalar converts Montgomery -> canonical, recodes
 //               into signed c-bit digits exactly as msm.rs:180-224 does (same
-//               carry rule), and counts each nonzero digit into its bucket key
-//               (column, group, |digit|-1).
-//  2. sort    : exclusive scan of the counts, then a scatter of
-//               (table index | sign) entries, i.e. a counting sort by bucket.
+//               carry rule) and emits one (bucket key, table index | sign)
+//               pair per nonzero digit, packed densely per block.
+//  2. sort    : an LSD radix sort of the pairs by bucket key (hipCUB), then a
+//               binary search per key for the bucket boundaries.
 //  3. chunks  : every bucket is cut into chunks of at most K1 entries, so the
 //               work per thread is bounded whatever the digit skew (the
 //               witness-like mix piles 10% of all points into one bucket).
 //  4. accumulate: one thread per chunk sums its points with mixed XYZZ adds;
 //               the gather of 64-byte affine bases is the only HBM traffic.
-//  5. buckets : chunk sums are combined per bucket (a thread per light bucket,
-//               an LDS tree per heavy one).
+//  5. buckets : chunk sums are combined per bucket: a thread per light bucket,
+//               an LDS tree per heavy one, the two on forked streams.
 //  6. reduce  : sum_b (b+1) S_b per group by a two-level split
 //               (b = 256 r + t: 256 * sum_r r Z_r + sum_t t U_t + sum_r Z_r),
 //               each level an LDS tree or suffix scan: ~25 dependent adds
@@ -61,43 +61,88 @@ __device__ __forceinline__ void load_scalar(const Fe *fr, const uint32_t *u32, s
     }
 }
 
-// Recode (crates/curve/src/msm.rs:180-198) and either count or scatter.
-template <bool SCATTER>
-__global__ void k_digits(const Fe *__restrict__ fr, const uint32_t *__restrict__ u32,
-                         uint32_t n, uint32_t K, int c, int windows, int P, int G, uint32_t B,
-                         uint32_t tab_stride, uint32_t *__restrict__ counts,
-                         uint32_t *__restrict__ entries) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)n * K) return;
-    const uint32_t col = (uint32_t)(idx / n);
-    const uint32_t i = (uint32_t)(idx - (size_t)col * n);
-    uint32_t l[8];
-    load_scalar(fr, u32, idx, l);
+// Signed-digit recode of one scalar (crates/curve/src/msm.rs:180-198): digit
+// j in [-2^(c-1), 2^(c-1)], carry 1 out of any window whose value reaches
+// 2^(c-1), the top window swallowing the last carry. Calls f(j, |d|, d < 0)
+// for every nonzero digit.
+template <class F>
+__device__ __forceinline__ void recode_digits(const uint32_t l[8], int c, int windows, F &&f) {
     const uint32_t half = 1u << (c - 1);
     uint32_t carry = 0;
     for (int j = 0; j < windows; j++) {
-        uint32_t coef = window_of32(l, j * c, c) + carry;
+        const uint32_t coef = window_of32(l, j * c, c) + carry;
         uint32_t mag;
         bool neg = false;
         if (j == windows - 1) {
-            mag = coef;  // the top window swallows the carry (coef <= half)
+            mag = coef;  // coef <= half here
         } else if (coef >= half) {
-            mag = (1u << c) - coef;  // digit coef - 2^c, possibly 0 when coef == 2^c
+            mag = (1u << c) - coef;  // digit coef - 2^c; 0 when coef == 2^c
             neg = mag != 0;
             carry = 1;
         } else {
             mag = coef;
             carry = 0;
         }
-        if (mag == 0) continue;
-        const uint32_t key = ((col * (uint32_t)G) + (uint32_t)(j / P)) * B + (mag - 1);
-        if (SCATTER) {
-            const uint32_t pos = atomicAdd(&counts[key], 1u);
-            entries[pos] = ((uint32_t)(j % P) * tab_stride + i) | (neg ? SIGN : 0u);
-        } else {
-            atomicAdd(&counts[key], 1u);
-        }
+        if (mag) f(j, mag, neg);
     }
+}
+
+// Emits one (bucket key, table index | sign) pair per nonzero digit. Each
+// block packs its pairs contiguously at an offset claimed with one atomic, so
+// the output is dense (zero digits, 20-70% of all digits for witness-like
+// scalars, never reach the sort).
+__global__ void __launch_bounds__(TPB) k_emit(const Fe *__restrict__ fr,
+                                            const uint32_t *__restrict__ u32, uint32_t n,
+                                            uint32_t K, int c, int windows, int P, int G,
+                                            uint32_t B, uint32_t tab_stride,
+                                            uint32_t *__restrict__ counter,
+                                            uint32_t *__restrict__ keys,
+                                            uint32_t *__restrict__ vals) {
+    __shared__ uint32_t scan[TPB];
+    __shared__ uint32_t base;
+    const uint32_t tid = threadIdx.x;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + tid;
+    const bool live = idx < (size_t)n * K;
+    uint32_t l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (live) load_scalar(fr, u32, idx, l);
+    uint32_t cnt = 0;
+    if (live) recode_digits(l, c, windows, [&](int, uint32_t, bool) { cnt++; });
+    // inclusive block scan of the counts
+    scan[tid] = cnt;
+    __syncthreads();
+    for (uint32_t s = 1; s < TPB; s <<= 1) {
+        const uint32_t v = tid >= s ? scan[tid - s] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    if (tid == TPB - 1) base = atomicAdd(counter, scan[TPB - 1]);
+    __syncthreads();
+    if (!live || !cnt) return;
+    uint32_t pos = base + scan[tid] - cnt;
+    const uint32_t col = (uint32_t)(idx / n);
+    const uint32_t i = (uint32_t)(idx - (size_t)col * n);
+    recode_digits(l, c, windows, [&](int j, uint32_t mag, bool neg) {
+        keys[pos] = ((col * (uint32_t)G) + (uint32_t)(j / P)) * B + (mag - 1);
+        vals[pos] = ((uint32_t)(j % P) * tab_stride + i) | (neg ? SIGN : 0u);
+        pos++;
+    });
+}
+
+// off[k] = first position of key k in the sorted key array (lower bound).
+__global__ void k_bounds(const uint32_t *__restrict__ skeys, uint32_t total, uint32_t nkeys,
+                         uint32_t *__restrict__ off) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nkeys) return;
+    uint32_t lo = 0, hi = total;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (skeys[mid] < k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    off[k] = lo;
 }
 
 __global__ void k_chunk_count(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t K1,
@@ -112,20 +157,27 @@ __global__ void k_chunk_count(const uint32_t *__restrict__ off, uint32_t nkeys, 
     q[k] = (m + K1 - 1) / K1;
 }
 
+// One thread per chunk: its bucket is the last key whose first chunk is at or
+// before it (binary search over the chunk offsets).
 __global__ void k_chunk_fill(const uint32_t *__restrict__ off, const uint32_t *__restrict__ qoff,
-                             uint32_t nkeys, uint32_t K1, Chunk *__restrict__ desc) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nkeys) return;
-    const uint32_t m = off[k + 1] - off[k];
-    const uint32_t base = qoff[k];
-    const uint32_t q = qoff[k + 1] - base;
-    for (uint32_t s = 0; s < q; s++) {
-        Chunk ch;
-        ch.key = k;
-        ch.start = off[k] + s * K1;
-        ch.len = min(K1, m - s * K1);
-        desc[base + s] = ch;
+                             uint32_t nkeys, uint32_t nchunks, uint32_t K1,
+                             Chunk *__restrict__ desc) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nchunks) return;
+    uint32_t lo = 0, hi = nkeys;  // largest k < nkeys with qoff[k] <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (qoff[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
     }
+    const uint32_t k = lo, s = t - qoff[k];
+    Chunk ch;
+    ch.key = k;
+    ch.start = off[k] + s * K1;
+    ch.len = min(K1, off[k + 1] - ch.start);
+    desc[t] = ch;
 }
 
 __device__ __forceinline__ Affine load_affine(const Affine *__restrict__ p, uint32_t idx) {
@@ -158,16 +210,20 @@ __global__ void __launch_bounds__(TPB) k_accumulate(const Chunk *__restrict__ de
 
 constexpr uint32_t LIGHT_CHUNKS = 8;
 
-__global__ void k_bucket_light(const uint32_t *__restrict__ qoff, uint32_t nkeys,
-                               const Xyzz *__restrict__ chunk_sums, Xyzz *__restrict__ buckets,
-                               uint32_t *__restrict__ heavy, uint32_t *__restrict__ nheavy) {
+// Lists the buckets with more than LIGHT_CHUNKS chunks (summed by LDS trees).
+__global__ void k_classify(const uint32_t *__restrict__ qoff, uint32_t nkeys,
+                           uint32_t *__restrict__ heavy, uint32_t *__restrict__ nheavy) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    if (qoff[k + 1] - qoff[k] > LIGHT_CHUNKS) heavy[atomicAdd(nheavy, 1u)] = k;
+}
+
+__global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restrict__ qoff, uint32_t nkeys,
+                               const Xyzz *__restrict__ chunk_sums, Xyzz *__restrict__ buckets) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
     const uint32_t a = qoff[k], q = qoff[k + 1] - a;
-    if (q > LIGHT_CHUNKS) {
-        heavy[atomicAdd(nheavy, 1u)] = k;
-        return;
-    }
+    if (q > LIGHT_CHUNKS) return;
     Xyzz acc = xyzz_identity();
     for (uint32_t s = 0; s < q; s++) acc = xyzz_add(acc, chunk_sums[a + s]);
     buckets[k] = acc;
@@ -178,11 +234,11 @@ __device__ Xyzz block_sum(const Xyzz *__restrict__ src, size_t base, size_t stri
                           uint32_t count, Xyzz *lds) {
     const uint32_t tid = threadIdx.x;
     Xyzz acc = xyzz_identity();
-    for (uint32_t k = tid; k < count; k += TPB) acc = xyzz_add(acc, src[base + (size_t)k * stride]);
+    for (uint32_t k = tid; k < count; k += TPB) acc = xyzz_add_ool(acc, src[base + (size_t)k * stride]);
     lds[tid] = acc;
     __syncthreads();
     for (uint32_t s = TPB / 2; s > 0; s >>= 1) {
-        if (tid < s && tid + s < count) lds[tid] = xyzz_add(lds[tid], lds[tid + s]);
+        if (tid < s && tid + s < count) lds[tid] = xyzz_add_ool(lds[tid], lds[tid + s]);
         __syncthreads();
     }
     Xyzz r = lds[0];
@@ -240,7 +296,7 @@ __global__ void __launch_bounds__(TPB) k_weighted(const Xyzz *__restrict__ src, 
         const bool take = tid + s < m;
         Xyzz o = take ? lds[tid + s] : v;
         __syncthreads();
-        if (take) lds[tid] = xyzz_add(v, o);
+        if (take) lds[tid] = xyzz_add_ool(v, o);
         __syncthreads();
     }
     const Xyzz tot = lds[0];
@@ -248,7 +304,7 @@ __global__ void __launch_bounds__(TPB) k_weighted(const Xyzz *__restrict__ src, 
     if (tid == 0) lds[0] = xyzz_identity();
     __syncthreads();
     for (uint32_t s = TPB / 2; s > 0; s >>= 1) {
-        if (tid < s && tid + s < m) lds[tid] = xyzz_add(lds[tid], lds[tid + s]);
+        if (tid < s && tid + s < m) lds[tid] = xyzz_add_ool(lds[tid], lds[tid + s]);
         __syncthreads();
     }
     if (tid == 0) {
@@ -354,51 +410,58 @@ void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *f
     const uint32_t nkeys = (uint32_t)nkeys64;
     hipStream_t st = ctx.stream;
 
-    // 1. counts
-    uint32_t *counts = ctx.d<uint32_t>("msm.counts", nkeys + 1);
-    uint32_t *off = ctx.d<uint32_t>("msm.off", nkeys + 1);
-    HIP_CHECK(hipMemsetAsync(counts, 0, (nkeys + 1) * sizeof(uint32_t), st));
+    // 1. digits -> dense (key, entry) pairs
     const size_t nthreads = n * K;
-    hipLaunchKernelGGL(k_digits<false>, dim3(grid_for(nthreads, TPB)), dim3(TPB), 0, st, fr_dev,
-                       u32_dev, (uint32_t)n, (uint32_t)K, sh.c, sh.windows, sh.P, sh.G, sh.B,
-                       (uint32_t)tab.stride, counts, nullptr);
+    const size_t max_pairs = nthreads * (size_t)sh.windows;
+    if (max_pairs >= (size_t(1) << 32)) throw ApgError(-2, "msm: too many digits");
+    uint32_t *counter = ctx.d<uint32_t>("msm.counter", 1);
+    uint32_t *keys_in = ctx.d<uint32_t>("msm.keys_in", max_pairs);
+    uint32_t *vals_in = ctx.d<uint32_t>("msm.vals_in", max_pairs);
+    HIP_CHECK(hipMemsetAsync(counter, 0, sizeof(uint32_t), st));
+    hipLaunchKernelGGL(k_emit, dim3(grid_for(nthreads, TPB)), dim3(TPB), 0, st, fr_dev, u32_dev,
+                       (uint32_t)n, (uint32_t)K, sh.c, sh.windows, sh.P, sh.G, sh.B,
+                       (uint32_t)tab.stride, counter, keys_in, vals_in);
     HIP_CHECK(hipGetLastError());
-
-    // 2. scan + scatter
-    size_t tmp_bytes = 0;
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, counts, off, nkeys + 1, st));
-    void *tmp = ctx.d<uint8_t>("msm.cubtmp", tmp_bytes);
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, counts, off, nkeys + 1, st));
-    uint32_t total = 0;
     uint32_t *h_total = ctx.h<uint32_t>("msm.total", 1);
-    HIP_CHECK(hipMemcpyAsync(h_total, off + nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(hipMemcpyAsync(counts, off, (nkeys + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(h_total, counter, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     ctx.sync();
-    total = *h_total;
+    const uint32_t total = *h_total;
+
+    // 2. radix sort by bucket key, then bucket boundaries
+    int key_bits = 1;
+    while ((1ull << key_bits) < (uint64_t)nkeys) key_bits++;
+    uint32_t *skeys = ctx.d<uint32_t>("msm.skeys", std::max<uint32_t>(total, 1));
     uint32_t *entries = ctx.d<uint32_t>("msm.entries", std::max<uint32_t>(total, 1));
-    hipLaunchKernelGGL(k_digits<true>, dim3(grid_for(nthreads, TPB)), dim3(TPB), 0, st, fr_dev,
-                       u32_dev, (uint32_t)n, (uint32_t)K, sh.c, sh.windows, sh.P, sh.G, sh.B,
-                       (uint32_t)tab.stride, counts, entries);
+    size_t tmp_bytes = 0;
+    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_in, skeys, vals_in, entries,
+                                                 (int)total, 0, key_bits, st));
+    size_t tmp2 = 0;
+    uint32_t *q = ctx.d<uint32_t>("msm.q", nkeys + 1);
+    uint32_t *qoff = ctx.d<uint32_t>("msm.qoff", nkeys + 1);
+    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, q, qoff, nkeys + 1, st));
+    void *tmp = ctx.d<uint8_t>("msm.cubtmp", std::max(tmp_bytes, tmp2));
+    if (total)
+        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, skeys, vals_in, entries,
+                                                     (int)total, 0, key_bits, st));
+    uint32_t *off = ctx.d<uint32_t>("msm.off", nkeys + 1);
+    hipLaunchKernelGGL(k_bounds, dim3(grid_for(nkeys + 1, TPB)), dim3(TPB), 0, st, skeys, total,
+                       nkeys, off);
     HIP_CHECK(hipGetLastError());
 
     // 3. chunks
     const uint32_t avg = total / std::max<uint32_t>(nkeys, 1);
     uint32_t K1 = std::max<uint32_t>(16, avg / 4);
     if (K1 > 256) K1 = 256;
-    uint32_t *q = ctx.d<uint32_t>("msm.q", nkeys + 1);
-    uint32_t *qoff = ctx.d<uint32_t>("msm.qoff", nkeys + 1);
     hipLaunchKernelGGL(k_chunk_count, dim3(grid_for(nkeys + 1, TPB)), dim3(TPB), 0, st, off,
                        nkeys, K1, q);
-    size_t tmp2 = 0;
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, q, qoff, nkeys + 1, st));
-    tmp = ctx.d<uint8_t>("msm.cubtmp", std::max(tmp_bytes, tmp2));
     HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp2, q, qoff, nkeys + 1, st));
     HIP_CHECK(hipMemcpyAsync(h_total, qoff + nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     ctx.sync();
     const uint32_t nchunks = *h_total;
     Chunk *desc = ctx.d<Chunk>("msm.desc", std::max<uint32_t>(nchunks, 1));
-    hipLaunchKernelGGL(k_chunk_fill, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, off, qoff,
-                       nkeys, K1, desc);
+    if (nchunks)
+        hipLaunchKernelGGL(k_chunk_fill, dim3(grid_for(nchunks, TPB)), dim3(TPB), 0, st, off, qoff,
+                           nkeys, nchunks, K1, desc);
 
     // 4. accumulate
     Xyzz *csum = ctx.d<Xyzz>("msm.csum", std::max<uint32_t>(nchunks, 1));
@@ -419,11 +482,19 @@ void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *f
     uint32_t *heavy = ctx.d<uint32_t>("msm.heavy", nkeys + 1);
     uint32_t *nheavy = heavy + nkeys;
     HIP_CHECK(hipMemsetAsync(nheavy, 0, sizeof(uint32_t), st));
-    hipLaunchKernelGGL(k_bucket_light, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys,
-                       csum, buckets, heavy, nheavy);
-    hipLaunchKernelGGL(k_bucket_heavy, dim3(1024), dim3(TPB), 0, st, qoff, csum, buckets, heavy,
+    hipLaunchKernelGGL(k_classify, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys, heavy,
                        nheavy);
+    // heavy buckets (latency-bound LDS trees) run on the side stream while
+    // the light ones (throughput-bound) run here
+    HIP_CHECK(hipEventRecord(ctx.ev_fork, st));
+    HIP_CHECK(hipStreamWaitEvent(ctx.side, ctx.ev_fork, 0));
+    hipLaunchKernelGGL(k_bucket_heavy, dim3(1024), dim3(TPB), 0, ctx.side, qoff, csum, buckets,
+                       heavy, nheavy);
+    HIP_CHECK(hipEventRecord(ctx.ev_join, ctx.side));
+    hipLaunchKernelGGL(k_bucket_light, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys,
+                       csum, buckets);
     HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamWaitEvent(st, ctx.ev_join, 0));
 
     // 6. reduce: per problem (column, group): W = sum_b (b + 1) S_b
     const uint32_t nprob = (uint32_t)(K * sh.G);

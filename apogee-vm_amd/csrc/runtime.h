@@ -86,6 +86,8 @@ struct Stats {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // fork/join partner of `stream` within one call
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool profiling = false;
     Stats stats;
     std::mutex mu;  // one call at a time per context
@@ -103,6 +105,33 @@ struct Context {
     }
     void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
 };
+
+// Streams (default priority, non-blocking) and fork/join events of a context.
+inline void context_init(Context &c, int device) {
+    c.device = device;
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    c.cu_count = prop.multiProcessorCount;
+}
+
+inline void context_release(Context &c) {
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    if (c.side) (void)hipStreamSynchronize(c.side);
+    c.dev.clear();
+    c.host.clear();
+    if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+    if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+    if (c.side) (void)hipStreamDestroy(c.side);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.stream = c.side = nullptr;
+    c.ev_fork = c.ev_join = nullptr;
+}
 
 inline unsigned grid_for(size_t n, unsigned block) {
     return (unsigned)((n + block - 1) / block);

@@ -75,6 +75,7 @@ __device__ void reduce_nodes(Fe (&acc)[ND], Fe *partials, unsigned *counter, Fe 
     __shared__ Fe lds[RTPB];
     __shared__ bool is_last;
     const int tid = threadIdx.x;
+#pragma unroll
     for (int x = 0; x < ND; x++) {
         lds[tid] = acc[x];
         __syncthreads();
@@ -217,16 +218,20 @@ struct CombineArgs {
     Fe *out[4];
 };
 
-__global__ void k_combine(const Fe *__restrict__ cols, size_t n, int K, int outs, CombineArgs a) {
+template <int OUTS>
+__global__ void k_combine(const Fe *__restrict__ cols, size_t n, int K, CombineArgs a) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Fe s[4];
-    for (int o = 0; o < outs; o++) s[o] = f_zero<Fr>();
+    Fe s[OUTS];
+#pragma unroll
+    for (int o = 0; o < OUTS; o++) s[o] = f_zero<Fr>();
     for (int c = 0; c < K; c++) {
         const Fe v = ldfe(cols, (size_t)c * n + i);
-        for (int o = 0; o < outs; o++) s[o] = f_add<Fr>(s[o], f_mul<Fr>(a.w[o][c], v));
+#pragma unroll
+        for (int o = 0; o < OUTS; o++) s[o] = f_add<Fr>(s[o], f_mul<Fr>(a.w[o][c], v));
     }
-    for (int o = 0; o < outs; o++) stfe(a.out[o], i, s[o]);
+#pragma unroll
+    for (int o = 0; o < OUTS; o++) stfe(a.out[o], i, s[o]);
 }
 
 }  // namespace
@@ -324,8 +329,13 @@ void combine_columns_device(Context &ctx, const Fe *cols, size_t n, int K, const
         for (int c = 0; c < K; c++) a.w[o][c] = w[o * K + c];
         a.out[o] = out[o];
     }
-    hipLaunchKernelGGL(k_combine, dim3(grid_for(n, 256)), dim3(256), 0, ctx.stream, cols, n, K, outs,
-                       a);
+    const dim3 g(grid_for(n, 256)), b(256);
+    switch (outs) {
+        case 1: hipLaunchKernelGGL(k_combine<1>, g, b, 0, ctx.stream, cols, n, K, a); break;
+        case 2: hipLaunchKernelGGL(k_combine<2>, g, b, 0, ctx.stream, cols, n, K, a); break;
+        case 3: hipLaunchKernelGGL(k_combine<3>, g, b, 0, ctx.stream, cols, n, K, a); break;
+        default: hipLaunchKernelGGL(k_combine<4>, g, b, 0, ctx.stream, cols, n, K, a); break;
+    }
     HIP_CHECK(hipGetLastError());
 }
 

@@ -69,6 +69,13 @@ int apg_ctx_reset_stats(apg_ctx *ctx);
  * counterpart in the reference (its benches draw from splitmix64). */
 int apg_synthetic_fill(uint64_t seed, int kind, size_t n, uint8_t *out_mont);
 
+/* ---- the device's Montgomery multiplication (the 32-bit-word CIOS the
+ * kernels run), compiled for the host so a CPU-only test can check it on
+ * many inputs: out[i] = a[i] * b[i] * 2^-256 mod m, which = 0 for Fr, 1 for
+ * Fq. No counterpart in the reference. */
+int apg_debug_mul32_batch(int which, const uint8_t *a_mont, const uint8_t *b_mont, size_t n,
+                          uint8_t *out_mont);
+
 /* ---- device memory for callers without their own allocator ---- */
 int apg_dev_alloc(apg_ctx *ctx, size_t bytes, void **out);
 int apg_dev_free(apg_ctx *ctx, void *p);

@@ -104,3 +104,42 @@ def test_interface_errors(apg):
     assert apg.Gate.product(2).degree() == 3
     assert apg.Gate.product(3).degree() == 4
     assert apg.Gate([0], [apg.GateTerm(one, 0)]).degree() == 3
+
+
+def _mul32(apg, which, a, b):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.zeros_like(a)
+    rc = apg.lib().apg_debug_mul32_batch(which, a.ctypes.data, b.ctypes.data, a.shape[0], out.ctypes.data)
+    assert rc == 0
+    return out
+
+
+@pytest.mark.parametrize("which,modulus", [
+    (0, 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001),
+    (1, 0x30644E72E131A029B85045B68181585D97816A916871CA8D3C208C16D87CFD47),
+])
+def test_device_montgomery_mul_matches_oracle(apg, orc, which, modulus):
+    """The kernels' 32-bit-word no-carry CIOS (compiled for the host) equals
+    the reference's u64 CIOS on random inputs and on the edge grid where a
+    carry bound would break first (m - 1, m - 2^k, 2^k, R, R^2, ...)."""
+    rng = np.random.default_rng(1000 + which)
+    n = 100_000
+    vals = [int.from_bytes(rng.bytes(32), "little") % modulus for _ in range(n)]
+    edges = [0, 1, 2, modulus - 1, modulus - 2, (modulus - 1) // 2, (modulus + 1) // 2]
+    edges += [(1 << k) % modulus for k in range(0, 256, 7)]
+    edges += [modulus - (1 << k) for k in range(0, 250, 5)]
+    edges += [(1 << 256) % modulus, (1 << 512) % modulus, modulus - ((1 << 256) % modulus)]
+    pairs_a = vals + [e for e in edges for _ in edges]
+    pairs_b = vals[::-1] + [f for _ in edges for f in edges]
+
+    def limbs(xs):
+        return np.array([[(x >> (64 * i)) & ((1 << 64) - 1) for i in range(4)] for x in xs], dtype=np.uint64)
+
+    a, b = limbs(pairs_a), limbs(pairs_b)
+    got = _mul32(apg, which, a, b)
+    want = np.zeros_like(a)
+    L = orc.lib()
+    for i in range(a.shape[0]):
+        L.orc_mul(which, orc.ptr(a[i]), orc.ptr(b[i]), orc.ptr(want[i]))
+    assert np.array_equal(got, want)
