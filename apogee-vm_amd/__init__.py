@@ -136,6 +136,23 @@ def _declare(L):
     d("apg_ctx_reset_stats", [VP])
     d("apg_synthetic_fill", [U64, I32, SZ, VP])
     d("apg_debug_mul32_batch", [I32, VP, VP, SZ, VP])
+    d("apg_synthetic_fill_dev", [VP, U64, I32, SZ, VP])
+    d("apg_engine_create", [I32, I32, SZ, C.POINTER(VP)])
+    d("apg_engine_destroy", [VP], None)
+    d("apg_engine_submit_shard", [VP, VP, VP, SZ, SZ, I32, VP, C.POINTER(C.c_uint64)])
+    d("apg_engine_poll", [VP, C.c_uint64])
+    d("apg_engine_wait", [VP, C.c_uint64, VP, VP, VP, VP, C.POINTER(ApgJobStats)])
+    d("apg_engine_stats", [VP, C.POINTER(ApgEngineInfo)])
+
+
+class ApgJobStats(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double),
+                ("worker", C.c_int)]
+
+
+class ApgEngineInfo(C.Structure):
+    _fields_ = [(f, C.c_size_t) for f in ("queued", "running", "completed", "hbm_in_use", "hbm_peak",
+                                          "hbm_budget", "arena_bytes", "workers")]
 
 
 class ApgStats(C.Structure):
@@ -654,3 +671,66 @@ def prove_shard(srs: Srs, columns: DeviceBuffer, num_vars: int, k: int, degree: 
     _check(lib().apg_prove_shard_dev(ctx.h, srs.h, columns.ptr, num_vars, k, degree, transcript.h,
                                      _p(cms), _p(rounds), _p(finals)))
     return ShardProof([cms[64 * i: 64 * (i + 1)].tobytes() for i in range(k)], rounds, finals)
+
+
+def synthetic_device(ctx: Context, seed: int, n: int, kind: str = "witness", out: Optional[DeviceBuffer] = None,
+                     offset_elems: int = 0) -> DeviceBuffer:
+    """n counter-mode ChaCha20 elements (the values `synthetic` returns) written
+    straight into device memory."""
+    buf = out or DeviceBuffer(ctx, 32 * n)
+    ptr = C.c_void_p(buf.ptr.value + 32 * offset_elems)
+    _check(lib().apg_synthetic_fill_dev(ctx.h, seed, {"uniform": 0, "witness": 1}[kind], n, ptr))
+    return buf
+
+
+# ------------------------------------------------------------------ engine
+class Engine:
+    """Continuous batching of shard jobs on one device (apg_engine): `workers`
+    concurrent HIP streams, FIFO admission against an HBM budget."""
+
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 200 << 30):
+        h = VP()
+        _check(lib().apg_engine_create(device, workers, hbm_budget, C.byref(h)))
+        self.h = h
+        self._jobs = {}
+
+    def close(self):
+        if self.h:
+            lib().apg_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_shard(self, srs: Srs, columns: DeviceBuffer, num_vars: int, k: int, degree: int,
+                     seed: Optional[Transcript] = None, column_offset_elems: int = 0) -> int:
+        jid = C.c_uint64()
+        ptr = C.c_void_p(columns.ptr.value + 32 * column_offset_elems)
+        _check(lib().apg_engine_submit_shard(self.h, srs.h, ptr, num_vars, k, degree,
+                                             seed.h if seed is not None else None, C.byref(jid)))
+        self._jobs[jid.value] = (num_vars, k, degree)
+        return jid.value
+
+    def done(self, job: int) -> bool:
+        return lib().apg_engine_poll(self.h, job) == 1
+
+    def wait(self, job: int):
+        """-> (ShardProof, final Transcript, stats dict)"""
+        num_vars, k, degree = self._jobs.pop(job)
+        cms = np.zeros(64 * k, dtype=np.uint8)
+        rounds = np.zeros((num_vars, degree + 1, 4), dtype=np.uint64)
+        finals = np.zeros((degree - 1, 4), dtype=np.uint64)
+        t = Transcript()
+        st = ApgJobStats()
+        _check(lib().apg_engine_wait(self.h, job, _p(cms), _p(rounds), _p(finals), t.h, C.byref(st)))
+        proof = ShardProof([cms[64 * i: 64 * (i + 1)].tobytes() for i in range(k)], rounds, finals)
+        return proof, t, {"queue_ms": st.queue_ms, "run_ms": st.run_ms, "total_ms": st.total_ms,
+                          "worker": st.worker}
+
+    def stats(self) -> dict:
+        s = ApgEngineInfo()
+        _check(lib().apg_engine_stats(self.h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in ApgEngineInfo._fields_}

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/apogee_gpu.h"
+#include "engine.h"
 #include "msm.h"
 #include "sumcheck.h"
 #include "transcript.h"
@@ -176,7 +177,74 @@ std::vector<Fe *> upload_tables(Context &ctx, const uint8_t *const *tables, size
     return p;
 }
 
+void check_shard_args(size_t num_vars, size_t k, int degree) {
+    if (num_vars > 30 || num_vars < 1) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+    if (degree < 3 || degree > 5) throw ApgError(APG_ERR_ARG, "degree must be 3..5");
+    if (k < 1 || k > (size_t)SC_MAX_TABLES) throw ApgError(APG_ERR_ARG, "column count out of range");
+}
+
+// HBM a shard holds while it runs: the MSM's digit pairs and their sorted
+// copies (16 B per digit slot, in column groups of at most 2^28 slots), chunk
+// and bucket sums, and the sumcheck's combined tables, eq and fold buffers.
+size_t shard_hbm_estimate(size_t num_vars, size_t k, int degree) {
+    const size_t n = size_t(1) << num_vars;
+    const int c = msm_pick_c(n, k);
+    const size_t windows = (size_t)msm_window_count(254, c);
+    const size_t slots = std::min(n * k * windows, size_t(1) << 28);
+    const size_t msm = slots * 16 + slots / 4 * 128 + k * (size_t(1) << (c - 1)) * 128;
+    const size_t f = (size_t)degree - 1;
+    const size_t sc = f * n * 32 + (f + 1) * n * 32 * 2;
+    return msm + sc + (size_t(64) << 20);
+}
+
+// The shard composition: commit the k columns, absorb the commitments as one
+// COMMITMENT message (pcs::append_g1_list, crates/pcs/src/lib.rs:220), draw
+// rho (MERCURY_BATCH), fold the columns into degree - 1 tables with weights
+// rho^(o k + c), draw the eq point (num_vars SUMCHECK_CHALLENGE), prove the
+// rounds of eq * t_0 * ... * t_(degree-2), absorb the final evaluations.
+void prove_shard_impl(Context &c, apg_srs *srs, const Fe *cols, size_t num_vars, size_t k, int degree,
+                      Transcript &t, uint8_t *commitments_wire, uint8_t *rounds_out,
+                      uint8_t *final_evals_out) {
+    const size_t n = size_t(1) << num_vars;
+    std::vector<uint8_t> cm_mont(64 * k);
+    commit_device(c, srs, cols, nullptr, n, k, cm_mont.data());
+    for (size_t i = 0; i < k; i++) apg_g1_to_wire(&cm_mont[64 * i], commitments_wire + 64 * i);
+    t.append_g1_wire(TAG_COMMITMENT, commitments_wire, k);
+    const Fe rho = t.challenge(TAG_MERCURY_BATCH);
+    const int f = degree - 1;
+    std::vector<Fe> w((size_t)f * k);
+    Fe pw = f_one<Fr>();
+    for (int o = 0; o < f; o++)
+        for (size_t col = 0; col < k; col++) {
+            w[o * k + col] = pw;
+            pw = f_mul<Fr>(pw, rho);
+        }
+    Fe *comb = c.d<Fe>("shard.comb", (size_t)f * n);
+    std::vector<Fe *> outs(f);
+    for (int o = 0; o < f; o++) outs[o] = comb + (size_t)o * n;
+    combine_columns_device(c, cols, n, (int)k, w.data(), f, outs.data());
+    std::vector<uint8_t> r(32 * num_vars);
+    for (size_t i = 0; i < num_vars; i++) store_fe(&r[32 * i], t.challenge(TAG_SUMCHECK_CHALLENGE));
+    Gate g;
+    g.nterms = 1;
+    g.terms[0].coef = f_one<Fr>();
+    for (int q = 0; q < 3; q++) g.terms[0].f[q] = q < f ? q : -1;
+    RoundFn fn = [&t](const Fe *coeffs, int nc) -> Fe {
+        t.append_scalars(TAG_SUMCHECK_ROUND, coeffs, nc);
+        return t.challenge(TAG_SUMCHECK_CHALLENGE);
+    };
+    sumcheck_common(c, g, outs.data(), (size_t)f, num_vars, r.data(), fn, rounds_out, nullptr,
+                    final_evals_out);
+    std::vector<Fe> fin(f);
+    for (int i = 0; i < f; i++) fin[i] = load_fe(final_evals_out + 32 * i);
+    t.append_scalars(TAG_SUMCHECK_FINAL_EVALS, fin.data(), f);
+}
+
 }  // namespace
+
+struct apg_engine {
+    std::unique_ptr<Engine> e;
+};
 
 extern "C" {
 
@@ -233,6 +301,18 @@ int apg_ctx_reset_stats(apg_ctx *ctx) {
     return guarded([&] {
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.stats = Stats();
+    });
+}
+
+extern "C" int apg_synthetic_fill_dev_impl(void *stream, uint64_t seed, int kind, size_t n, void *out_dev);
+
+int apg_synthetic_fill_dev(apg_ctx *ctx, uint64_t seed, int kind, size_t n, void *out_dev) {
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        const int rc = apg_synthetic_fill_dev_impl(ctx->c.stream, seed, kind, n, out_dev);
+        if (rc != APG_OK) throw ApgError(rc, "synthetic fill failed");
+        ctx->c.sync();
     });
 }
 
@@ -465,49 +545,82 @@ int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, siz
                         size_t k, int degree, apg_transcript *t, uint8_t *commitments_wire,
                         uint8_t *rounds_out, uint8_t *final_evals_out) {
     return guarded([&] {
-        if (num_vars > 40 || num_vars < 1) throw ApgError(APG_ERR_ARG, "num_vars out of range");
-        if (degree < 3 || degree > 5) throw ApgError(APG_ERR_ARG, "degree must be 3..5");
-        if (k < 1 || k > (size_t)SC_MAX_TABLES) throw ApgError(APG_ERR_ARG, "column count out of range");
+        check_shard_args(num_vars, k, degree);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
-        Context &c = ctx->c;
-        const size_t n = size_t(1) << num_vars;
+        prove_shard_impl(ctx->c, srs, static_cast<const Fe *>(columns_dev), num_vars, k, degree, t->t,
+                         commitments_wire, rounds_out, final_evals_out);
+    });
+}
+
+int apg_engine_create(int device, int workers, size_t hbm_budget_bytes, apg_engine **out) {
+    return guarded([&] {
+        int ndev = 0;
+        HIP_CHECK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw ApgError(APG_ERR_ARG, "no such device");
+        if (workers < 1 || workers > 64) throw ApgError(APG_ERR_ARG, "workers must be 1..64");
+        auto *e = new apg_engine();
+        e->e = std::make_unique<Engine>(device, workers, hbm_budget_bytes);
+        *out = e;
+    });
+}
+
+void apg_engine_destroy(apg_engine *e) { delete e; }
+
+int apg_engine_submit_shard(apg_engine *e, apg_srs *srs, const void *columns_dev, size_t num_vars,
+                            size_t k, int degree, const apg_transcript *seed, uint64_t *job_id) {
+    return guarded([&] {
+        check_shard_args(num_vars, k, degree);
+        auto job = std::make_unique<Job>();
+        job->mem_bytes = shard_hbm_estimate(num_vars, k, degree);
+        const Transcript t0 = seed ? seed->t : Transcript();
         const Fe *cols = static_cast<const Fe *>(columns_dev);
-        // 1. commitments
-        std::vector<uint8_t> cm_mont(64 * k);
-        commit_device(c, srs, cols, nullptr, n, k, cm_mont.data());
-        for (size_t i = 0; i < k; i++) apg_g1_to_wire(&cm_mont[64 * i], commitments_wire + 64 * i);
-        // 2. absorb, draw rho, combine
-        t->t.append_g1_wire(TAG_COMMITMENT, commitments_wire, k);
-        const Fe rho = t->t.challenge(TAG_MERCURY_BATCH);
-        const int f = degree - 1;
-        std::vector<Fe> w((size_t)f * k);
-        Fe pw = f_one<Fr>();
-        for (int o = 0; o < f; o++)
-            for (size_t col = 0; col < k; col++) {
-                w[o * k + col] = pw;
-                pw = f_mul<Fr>(pw, rho);
-            }
-        Fe *comb = c.d<Fe>("shard.comb", (size_t)f * n);
-        std::vector<Fe *> outs(f);
-        for (int o = 0; o < f; o++) outs[o] = comb + (size_t)o * n;
-        combine_columns_device(c, cols, n, (int)k, w.data(), f, outs.data());
-        // 3. eq point, rounds, finals
-        std::vector<uint8_t> r(32 * num_vars);
-        for (size_t i = 0; i < num_vars; i++) store_fe(&r[32 * i], t->t.challenge(TAG_SUMCHECK_CHALLENGE));
-        Gate g;
-        g.nterms = 1;
-        g.terms[0].coef = f_one<Fr>();
-        for (int q = 0; q < 3; q++) g.terms[0].f[q] = q < f ? q : -1;
-        RoundFn fn = [t](const Fe *coeffs, int nc) -> Fe {
-            t->t.append_scalars(TAG_SUMCHECK_ROUND, coeffs, nc);
-            return t->t.challenge(TAG_SUMCHECK_CHALLENGE);
+        job->fn = [=](Context &c, Job &j) {
+            Transcript t = t0;
+            const int d = degree;
+            j.commitments.assign(64 * k, 0);
+            j.rounds.assign(32 * num_vars * (d + 1), 0);
+            j.finals.assign(32 * (d - 1), 0);
+            prove_shard_impl(c, srs, cols, num_vars, k, degree, t, j.commitments.data(), j.rounds.data(),
+                             j.finals.data());
+            j.transcript_blob.assign(reinterpret_cast<const uint8_t *>(&t),
+                                     reinterpret_cast<const uint8_t *>(&t) + sizeof(Transcript));
         };
-        sumcheck_common(c, g, outs.data(), (size_t)f, num_vars, r.data(), fn, rounds_out, nullptr,
-                        final_evals_out);
-        std::vector<Fe> fin(f);
-        for (int i = 0; i < f; i++) fin[i] = load_fe(final_evals_out + 32 * i);
-        t->t.append_scalars(TAG_SUMCHECK_FINAL_EVALS, fin.data(), f);
+        *job_id = e->e->submit(std::move(job));
+    });
+}
+
+int apg_engine_poll(apg_engine *e, uint64_t job_id) { return e->e->poll(job_id) ? 1 : 0; }
+
+int apg_engine_wait(apg_engine *e, uint64_t job_id, uint8_t *commitments_wire, uint8_t *rounds_out,
+                    uint8_t *final_evals_out, apg_transcript *transcript_out, apg_job_stats *stats) {
+    return guarded([&] {
+        auto j = e->e->wait(job_id);
+        if (stats) {
+            stats->queue_ms = j->t_start - j->t_submit;
+            stats->run_ms = j->t_end - j->t_start;
+            stats->total_ms = j->t_end - j->t_submit;
+            stats->worker = j->worker;
+        }
+        if (j->status != 0) throw ApgError(j->status < 0 ? j->status : APG_ERR_INTERNAL, j->error);
+        if (commitments_wire) std::memcpy(commitments_wire, j->commitments.data(), j->commitments.size());
+        if (rounds_out) std::memcpy(rounds_out, j->rounds.data(), j->rounds.size());
+        if (final_evals_out) std::memcpy(final_evals_out, j->finals.data(), j->finals.size());
+        if (transcript_out) std::memcpy(&transcript_out->t, j->transcript_blob.data(), sizeof(Transcript));
+    });
+}
+
+int apg_engine_stats(apg_engine *e, apg_engine_info *out) {
+    return guarded([&] {
+        EngineStats s = e->e->stats();
+        out->queued = s.queued;
+        out->running = s.running;
+        out->completed = s.completed;
+        out->hbm_in_use = s.hbm_in_use;
+        out->hbm_peak = s.hbm_peak;
+        out->hbm_budget = s.hbm_budget;
+        out->arena_bytes = s.arena_bytes;
+        out->workers = (size_t)e->e->workers();
     });
 }
 

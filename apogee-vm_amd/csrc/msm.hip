@@ -385,6 +385,11 @@ std::unique_ptr<BaseTable> msm_expand_bases(Context &ctx, const Affine *src, siz
     return t;
 }
 
+static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *fr_dev,
+                          const uint32_t *u32_dev, Affine *out_host);
+
+// Columns are processed in groups of at most MAX_PAIRS digit slots, which
+// bounds the scratch one call holds (keys, entries and their sorted copies).
 void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *fr_dev,
              const uint32_t *u32_dev, Affine *out_host) {
     if (K == 0) return;
@@ -395,6 +400,18 @@ void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *f
         }
         return;
     }
+    constexpr size_t MAX_PAIRS = size_t(1) << 28;
+    const size_t windows = (size_t)msm_window_count(fr_dev ? 254 : 32, tab.c);
+    size_t per = std::max<size_t>(1, MAX_PAIRS / (n * windows));
+    for (size_t k0 = 0; k0 < K; k0 += per) {
+        const size_t kg = std::min(per, K - k0);
+        msm_run_group(ctx, tab, n, kg, fr_dev ? fr_dev + k0 * n : nullptr,
+                      u32_dev ? u32_dev + k0 * n : nullptr, out_host + k0);
+    }
+}
+
+static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *fr_dev,
+                          const uint32_t *u32_dev, Affine *out_host) {
     if (n > tab.stride) throw ApgError(-2, "msm: more scalars than table points");
     if (n * K >= (size_t(1) << 31)) throw ApgError(-2, "msm: batch too large");
     MsmShape sh;

@@ -62,12 +62,15 @@ int apg_ctx_set_profiling(apg_ctx *ctx, int on);
 int apg_ctx_stats(apg_ctx *ctx, apg_stats *out);
 int apg_ctx_reset_stats(apg_ctx *ctx);
 
-/* ---- seeded synthetic inputs (host): ChaCha20 keyed by seed; kind 0 =
- * uniform Fr (32 bytes, top two bits cleared, reject >= p, the draw rule of
+/* ---- seeded synthetic inputs: counter-mode ChaCha20 keyed by seed, element
+ * i from block counter i (nonce = attempt, domain); kind 0 = uniform Fr
+ * (32-byte candidates, top two bits cleared, reject >= p: the draw rule of
  * crates/pcs/tests/common/mod.rs:114-123), kind 1 = witness-like mix (20%
- * zero, 10% one, 50% u32, 20% uniform Fr). Writes n Montgomery elements. No
- * counterpart in the reference (its benches draw from splitmix64). */
+ * zero, 10% one, 50% u32, 20% uniform Fr). n Montgomery elements, on the host
+ * or into device memory (same values). No counterpart in the reference (its
+ * benches draw from splitmix64). */
 int apg_synthetic_fill(uint64_t seed, int kind, size_t n, uint8_t *out_mont);
+int apg_synthetic_fill_dev(apg_ctx *ctx, uint64_t seed, int kind, size_t n, void *out_dev);
 
 /* ---- the device's Montgomery multiplication (the 32-bit-word CIOS the
  * kernels run), compiled for the host so a CPU-only test can check it on
@@ -169,6 +172,38 @@ int apg_zerocheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
 int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, size_t num_vars,
                         size_t k, int degree, apg_transcript *t, uint8_t *commitments_wire,
                         uint8_t *rounds_out, uint8_t *final_evals_out);
+
+/* ---- the shard engine: continuous batching of shard jobs on one device.
+ * Mirrors the streaming prover's worker model (max_in_flight scoped workers
+ * claiming shards, crates/prover/src/streaming.rs:601-680): `workers` HIP
+ * streams, each with its own scratch; jobs wait in one FIFO queue and start
+ * when a worker is free and their HBM estimate fits hbm_budget_bytes beside
+ * the running ones. Each job runs apg_prove_shard_dev on its own copy of the
+ * seed transcript; its bytes do not depend on what else runs. Columns stay
+ * caller-owned device memory until the job has been waited for. */
+typedef struct apg_engine apg_engine;
+typedef struct {
+    double queue_ms; /* submit -> start */
+    double run_ms;   /* start -> end */
+    double total_ms; /* submit -> end */
+    int worker;
+} apg_job_stats;
+typedef struct {
+    size_t queued, running, completed;
+    size_t hbm_in_use, hbm_peak, hbm_budget; /* admitted job estimates */
+    size_t arena_bytes;                      /* scratch the workers hold */
+    size_t workers;
+} apg_engine_info;
+int apg_engine_create(int device, int workers, size_t hbm_budget_bytes, apg_engine **out);
+void apg_engine_destroy(apg_engine *e);
+int apg_engine_submit_shard(apg_engine *e, apg_srs *srs, const void *columns_dev, size_t num_vars,
+                            size_t k, int degree, const apg_transcript *seed, uint64_t *job_id);
+int apg_engine_poll(apg_engine *e, uint64_t job_id); /* 1 done, 0 pending */
+/* blocks until the job is done; outputs as apg_prove_shard_dev, plus the final
+ * transcript (nullable) and the job's timing (nullable) */
+int apg_engine_wait(apg_engine *e, uint64_t job_id, uint8_t *commitments_wire, uint8_t *rounds_out,
+                    uint8_t *final_evals_out, apg_transcript *transcript_out, apg_job_stats *stats);
+int apg_engine_stats(apg_engine *e, apg_engine_info *out);
 
 /* ---- the Poseidon2 transcript (crates/transcript/src/lib.rs) ---- */
 int apg_transcript_new(apg_transcript **out);

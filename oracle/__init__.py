@@ -105,6 +105,7 @@ def _declare(L):
     d("orc_chacha_block", [P, C.c_uint32, P, P])
     d("orc_fill_uniform", [U64, P, SZ])
     d("orc_fill_witness", [U64, P, SZ])
+    d("orc_fill_ctr", [U64, C.c_int, P, SZ])
 
 
 def ptr(a: np.ndarray):
@@ -508,6 +509,13 @@ def chacha_block(key32: bytes, counter: int, nonce12: bytes) -> bytes:
 def fill_uniform(seed: int, n: int) -> np.ndarray:
     out = fe(n)
     lib().orc_fill_uniform(seed, ptr(out), n)
+    return out
+
+
+def fill_ctr(seed: int, n: int, kind: str = "witness") -> np.ndarray:
+    """Counter-mode ChaCha20 draws (the benchmark inputs)."""
+    out = fe(n)
+    lib().orc_fill_ctr(seed, {"uniform": 0, "witness": 1}[kind], ptr(out), n)
     return out
 
 

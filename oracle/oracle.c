@@ -1260,6 +1260,56 @@ void orc_chacha_witness(orc_chacha *c, u64 *out) {
         orc_chacha_fr(c, out);
     }
 }
+/* Counter-mode draws: element i from ChaCha20 blocks with counter i and nonce
+ * (attempt, domain, 0); domain "fr" = 0x7266 for uniform Fr, "wi" = 0x6977
+ * for the witness mix. A block offers 32-byte Fr candidates at bytes 0-31
+ * and 32-63 (top two bits cleared, reject >= p); the witness category is the
+ * u32 at bytes 0-3 of attempt 0 mod 10, its u32 value bytes 4-7, its Fr
+ * candidates bytes 32-63 of attempt 0 then both halves of attempts 1, 2... */
+static int ctr_candidate(const uint8_t *b32, u64 *out) {
+    uint8_t b[32];
+    memcpy(b, b32, 32);
+    b[31] &= 0x3f;
+    return f_from_bytes(FR, b, out);
+}
+static void ctr_block(u64 seed, uint32_t i, uint32_t attempt, uint32_t dom, uint8_t out[64]) {
+    uint32_t key[8] = {(uint32_t)seed, (uint32_t)(seed >> 32), 0, 0, 0, 0, 0, 0};
+    uint32_t nonce[3] = {attempt, dom, 0};
+    orc_chacha_block(key, i, nonce, out);
+}
+static void ctr_fr(u64 seed, uint32_t i, uint32_t dom, uint32_t first, u64 *out) {
+    for (uint32_t a = first;; a++) {
+        uint8_t blk[64];
+        ctr_block(seed, i, a, dom, blk);
+        if (ctr_candidate(blk, out) || ctr_candidate(blk + 32, out)) return;
+    }
+}
+void orc_fill_ctr(u64 seed, int kind, u64 *out, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        u64 *o = out + 4 * i;
+        if (kind == 0) {
+            ctr_fr(seed, (uint32_t)i, 0x7266, 0, o);
+            continue;
+        }
+        uint8_t blk[64];
+        ctr_block(seed, (uint32_t)i, 0, 0x6977, blk);
+        uint32_t cat = (uint32_t)blk[0] | ((uint32_t)blk[1] << 8) | ((uint32_t)blk[2] << 16) |
+                       ((uint32_t)blk[3] << 24);
+        uint32_t v = (uint32_t)blk[4] | ((uint32_t)blk[5] << 8) | ((uint32_t)blk[6] << 16) |
+                     ((uint32_t)blk[7] << 24);
+        cat %= 10;
+        if (cat < 2) {
+            memset(o, 0, 32);
+        } else if (cat == 2) {
+            memcpy(o, FR_P.r, 32);
+        } else if (cat < 8) {
+            f_from_u64(FR, v, o);
+        } else if (!ctr_candidate(blk + 32, o)) {
+            ctr_fr(seed, (uint32_t)i, 0x6977, 1, o);
+        }
+    }
+}
+
 void orc_fill_uniform(u64 seed, u64 *out, size_t n) {
     orc_chacha c;
     orc_chacha_init(&c, seed);

@@ -124,6 +124,9 @@ void orc_chacha_fr(orc_chacha *c, uint64_t *out);
 void orc_chacha_witness(orc_chacha *c, uint64_t *out);
 void orc_fill_uniform(uint64_t seed, uint64_t *out, size_t n);
 void orc_fill_witness(uint64_t seed, uint64_t *out, size_t n);
+/* counter-mode draws (element i independent of the others); kind 0 uniform
+ * Fr, 1 witness mix */
+void orc_fill_ctr(uint64_t seed, int kind, uint64_t *out, size_t n);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,89 @@
+"""The shard engine (continuous batching) through the C ABI: each shard's
+bytes are the plain one-shard path's, alone and in the mix, and equal the
+oracle's; admission refuses a job above the HBM budget."""
+import importlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def apg():
+    m = importlib.import_module("apogee-vm_amd")
+    m.default_context()
+    return m
+
+
+@pytest.fixture(scope="module")
+def setup(apg):
+    wl = importlib.import_module("apogee-vm_amd.workload")
+    ctx = apg.default_context()
+    specs = wl.mixed_batch(seed=7, shards=14, families=5, min_cols=1, max_cols=6,
+                           classes=((0.5, 3, 7), (0.5, 8, 11)))
+    tau = apg.synthetic(99, 1, "uniform")[0]
+    nmax = max(s.rows for s in specs)
+    srs = apg.Srs.generate(tau, nmax, ctx)
+    bufs = []
+    for s in specs:
+        b = apg.DeviceBuffer(ctx, 32 * s.elements)
+        for c in range(s.columns):
+            apg.synthetic_device(ctx, s.seed + c, s.rows, "witness", out=b, offset_elems=c * s.rows)
+        bufs.append(b)
+    return specs, srs, bufs
+
+
+def _plain(apg, srs, spec, buf):
+    t = apg.Transcript()
+    p = apg.prove_shard(srs, buf, spec.num_vars, spec.columns, spec.degree, t)
+    return p, t.state()
+
+
+def test_engine_matches_plain_path_in_the_mix(apg, setup):
+    specs, srs, bufs = setup
+    plain = [_plain(apg, srs, s, b) for s, b in zip(specs, bufs)]
+    eng = apg.Engine(0, workers=4, hbm_budget=8 << 30)
+    ids = [eng.submit_shard(srs, b, s.num_vars, s.columns, s.degree) for s, b in zip(specs, bufs)]
+    for (p, st), jid in zip(plain, ids):
+        q, t, stats = eng.wait(jid)
+        assert q.commitments == p.commitments
+        assert np.array_equal(q.rounds, p.rounds) and np.array_equal(q.final_evals, p.final_evals)
+        assert t.state() == st
+        assert stats["total_ms"] >= stats["run_ms"] >= 0
+    # alone: one shard through a one-worker engine gives the same bytes again
+    solo = apg.Engine(0, workers=1, hbm_budget=8 << 30)
+    q, t, _ = solo.wait(solo.submit_shard(srs, bufs[3], specs[3].num_vars, specs[3].columns, specs[3].degree))
+    assert q.commitments == plain[3][0].commitments and t.state() == plain[3][1]
+    info = eng.stats()
+    assert info["completed"] == len(specs) and info["hbm_in_use"] == 0 and info["hbm_peak"] > 0
+
+
+def test_engine_matches_oracle(apg, orc, setup):
+    specs, srs, bufs = setup
+    s, b = specs[-1], bufs[-1]
+    eng = apg.Engine(0, workers=2, hbm_budget=8 << 30)
+    q, _, _ = eng.wait(eng.submit_shard(srs, b, s.num_vars, s.columns, s.degree))
+    cols = b.download().reshape(s.columns, s.rows, 4)
+    cms, rounds, fin = orc.shard_script(srs.g1(0, s.rows), cols, s.degree, orc.Transcript())
+    assert q.commitments == cms and np.array_equal(q.rounds, rounds) and np.array_equal(q.final_evals, fin)
+
+
+def test_engine_seeded_transcript_and_rejection(apg, setup):
+    specs, srs, bufs = setup
+    s, b = specs[0], bufs[0]
+    seed = apg.Transcript()
+    seed.append_bytes(2, b"shard seed")
+    t = seed.clone()
+    p = apg.prove_shard(srs, b, s.num_vars, s.columns, s.degree, t)
+    eng = apg.Engine(0, workers=2, hbm_budget=8 << 30)
+    q, t2, _ = eng.wait(eng.submit_shard(srs, b, s.num_vars, s.columns, s.degree, seed=seed))
+    assert q.commitments == p.commitments and t2.state() == t.state()
+    tiny = apg.Engine(0, workers=1, hbm_budget=1 << 20)
+    jid = tiny.submit_shard(srs, b, s.num_vars, s.columns, s.degree)
+    with pytest.raises(apg.ApgError):
+        tiny.wait(jid)
+    # the refusal leaves the engine serving
+    ok = apg.Engine(0, workers=1, hbm_budget=8 << 30)
+    q2, _, _ = ok.wait(ok.submit_shard(srs, b, s.num_vars, s.columns, s.degree))
+    assert q2.commitments == p.commitments

@@ -143,3 +143,16 @@ def test_device_montgomery_mul_matches_oracle(apg, orc, which, modulus):
     for i in range(a.shape[0]):
         L.orc_mul(which, orc.ptr(a[i]), orc.ptr(b[i]), orc.ptr(want[i]))
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "witness"])
+def test_synthetic_inputs_match_oracle(apg, orc, kind):
+    """The benchmark's counter-mode ChaCha20 draws (host path of the library)
+    equal the oracle's restatement; the witness mix has its stated shares."""
+    a = apg.synthetic(12345, 20000, kind)
+    b = orc.fill_ctr(12345, 20000, kind)
+    assert np.array_equal(a, b)
+    if kind == "witness":
+        zero = (a == 0).all(1).mean()
+        one = (a == orc.from_u64(orc.FR, 1)).all(1).mean()
+        assert abs(zero - 0.2) < 0.02 and abs(one - 0.1) < 0.02
