@@ -97,6 +97,7 @@ def _declare(L):
     d("apg_dev_free", [VP, VP])
     d("apg_dev_upload", [VP, VP, VP, SZ])
     d("apg_dev_download", [VP, VP, VP, SZ])
+    d("apg_dev_copy", [VP, VP, VP, SZ])
     d("apg_srs_upload", [VP, VP, SZ, C.POINTER(VP)])
     d("apg_srs_generate", [VP, VP, SZ, C.POINTER(VP)])
     d("apg_srs_download", [VP, VP, SZ, SZ, VP])
@@ -143,6 +144,10 @@ def _declare(L):
     d("apg_engine_poll", [VP, C.c_uint64])
     d("apg_engine_wait", [VP, C.c_uint64, VP, VP, VP, VP, C.POINTER(ApgJobStats)])
     d("apg_engine_stats", [VP, C.POINTER(ApgEngineInfo)])
+    d("apg_engine_set_policy", [VP, I32, I32, C.c_double])
+    d("apg_engine_set_profiling", [VP, I32])
+    d("apg_engine_kernel_stats", [VP, C.POINTER(ApgStats)])
+    d("apg_engine_reset_kernel_stats", [VP])
 
 
 class ApgJobStats(C.Structure):
@@ -294,6 +299,13 @@ class DeviceBuffer:
         out = np.zeros(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
         _check(lib().apg_dev_download(self.ctx.h, _p(out), self.ptr, self.nbytes))
         return out
+
+    def copy_from(self, src: "DeviceBuffer", nbytes: Optional[int] = None):
+        """Device-to-device copy by the library's streaming kernel (apg_dev_copy)."""
+        nbytes = min(self.nbytes, src.nbytes) if nbytes is None else nbytes
+        if nbytes > min(self.nbytes, src.nbytes):
+            raise ValueError("copy_from: nbytes exceeds a buffer")
+        _check(lib().apg_dev_copy(self.ctx.h, self.ptr, src.ptr, nbytes))
 
     def free(self):
         if self.ptr:
@@ -686,7 +698,8 @@ def synthetic_device(ctx: Context, seed: int, n: int, kind: str = "witness", out
 # ------------------------------------------------------------------ engine
 class Engine:
     """Continuous batching of shard jobs on one device (apg_engine): `workers`
-    concurrent HIP streams, FIFO admission against an HBM budget."""
+    concurrent HIP streams, admission against an HBM budget in arrival order
+    or shortest first (set_policy)."""
 
     def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 200 << 30):
         h = VP()
@@ -734,3 +747,23 @@ class Engine:
         s = ApgEngineInfo()
         _check(lib().apg_engine_stats(self.h, C.byref(s)))
         return {name: getattr(s, name) for name, _ in ApgEngineInfo._fields_}
+
+    POLICIES = {"fifo": 0, "shortest": 1}
+
+    def set_policy(self, policy: str = "fifo", reserve: int = 0, small_scalars: float = 0.0):
+        """Queue order: "fifo" or "shortest" (rows x columns); with reserve > 0,
+        jobs above small_scalars hold at most workers - reserve workers."""
+        if policy not in self.POLICIES:
+            raise ValueError(f"unknown policy {policy!r}")
+        _check(lib().apg_engine_set_policy(self.h, self.POLICIES[policy], reserve, float(small_scalars)))
+
+    def set_profiling(self, on: bool = True):
+        _check(lib().apg_engine_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_stats(self) -> dict:
+        s = ApgStats()
+        _check(lib().apg_engine_kernel_stats(self.h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in ApgStats._fields_}
+
+    def reset_kernel_stats(self):
+        _check(lib().apg_engine_reset_kernel_stats(self.h))

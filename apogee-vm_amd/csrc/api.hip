@@ -68,6 +68,12 @@ static inline void store_fe(uint8_t *p, const Fe &v) { std::memcpy(p, v.l, 32); 
 
 namespace {
 
+// Grid-stride 16-byte copy: each byte read once and written once.
+__global__ void __launch_bounds__(256) k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_srs_gen(Fe tau, size_t n, Affine *out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -280,20 +286,23 @@ int apg_ctx_set_profiling(apg_ctx *ctx, int on) {
     });
 }
 
+static void export_stats(const Stats &s, apg_stats *out) {
+    out->accumulate_ms = s.accumulate_ms;
+    out->accumulate_launches = s.accumulate_launches;
+    out->accumulate_entries = s.accumulate_entries;
+    out->msm_ms = s.msm_ms;
+    out->msm_calls = s.msm_calls;
+    out->msm_points = s.msm_points;
+    out->sumcheck_kernel_ms = s.sumcheck_kernel_ms;
+    out->sumcheck_launches = s.sumcheck_launches;
+    out->sumcheck_bytes = s.sumcheck_bytes;
+    out->sumcheck_ms = s.sumcheck_ms;
+}
+
 int apg_ctx_stats(apg_ctx *ctx, apg_stats *out) {
     return guarded([&] {
         std::lock_guard<std::mutex> lk(ctx->c.mu);
-        const Stats &s = ctx->c.stats;
-        out->accumulate_ms = s.accumulate_ms;
-        out->accumulate_launches = s.accumulate_launches;
-        out->accumulate_entries = s.accumulate_entries;
-        out->msm_ms = s.msm_ms;
-        out->msm_calls = s.msm_calls;
-        out->msm_points = s.msm_points;
-        out->sumcheck_kernel_ms = s.sumcheck_kernel_ms;
-        out->sumcheck_launches = s.sumcheck_launches;
-        out->sumcheck_bytes = s.sumcheck_bytes;
-        out->sumcheck_ms = s.sumcheck_ms;
+        export_stats(ctx->c.stats, out);
     });
 }
 
@@ -332,6 +341,20 @@ int apg_dev_upload(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return guarded([&] {
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+        ctx->c.sync();
+    });
+}
+int apg_dev_copy(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guarded([&] {
+        if (bytes % 16) throw ApgError(APG_ERR_ARG, "apg_dev_copy: bytes must be a multiple of 16");
+        if (!bytes) return;
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        const size_t n = bytes / 16;
+        const unsigned grid = (unsigned)std::min<size_t>(grid_for(n, 256), 8u * (unsigned)ctx->c.cu_count);
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, ctx->c.stream, static_cast<uint4 *>(dst),
+                           static_cast<const uint4 *>(src), n);
+        HIP_CHECK(hipGetLastError());
         ctx->c.sync();
     });
 }
@@ -573,6 +596,7 @@ int apg_engine_submit_shard(apg_engine *e, apg_srs *srs, const void *columns_dev
         check_shard_args(num_vars, k, degree);
         auto job = std::make_unique<Job>();
         job->mem_bytes = shard_hbm_estimate(num_vars, k, degree);
+        job->work = std::ldexp((double)k, (int)num_vars);
         const Transcript t0 = seed ? seed->t : Transcript();
         const Fe *cols = static_cast<const Fe *>(columns_dev);
         job->fn = [=](Context &c, Job &j) {
@@ -622,6 +646,27 @@ int apg_engine_stats(apg_engine *e, apg_engine_info *out) {
         out->arena_bytes = s.arena_bytes;
         out->workers = (size_t)e->e->workers();
     });
+}
+
+int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_scalars) {
+    return guarded([&] {
+        if (policy != APG_POLICY_FIFO && policy != APG_POLICY_SHORTEST)
+            throw ApgError(APG_ERR_ARG, "unknown engine policy");
+        if (reserve < 0) throw ApgError(APG_ERR_ARG, "reserve must be >= 0");
+        e->e->set_policy(static_cast<Policy>(policy), reserve, small_scalars);
+    });
+}
+
+int apg_engine_set_profiling(apg_engine *e, int on) {
+    return guarded([&] { e->e->set_profiling(on != 0); });
+}
+
+int apg_engine_kernel_stats(apg_engine *e, apg_stats *out) {
+    return guarded([&] { export_stats(e->e->kernel_stats(), out); });
+}
+
+int apg_engine_reset_kernel_stats(apg_engine *e) {
+    return guarded([&] { e->e->reset_kernel_stats(); });
 }
 
 int apg_transcript_new(apg_transcript **out) {

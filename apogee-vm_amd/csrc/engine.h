@@ -1,5 +1,5 @@
-// Shard-proving engine: a worker pool with per-worker HIP streams, FIFO
-// admission against an HBM budget, and per-job timing.
+// Shard-proving engine: a worker pool with per-worker HIP streams, admission
+// against an HBM budget (FIFO or shortest-first), and per-job timing.
 #pragma once
 #include <condition_variable>
 #include <deque>
@@ -18,6 +18,7 @@ namespace apg {
 struct Job {
     uint64_t id = 0;
     size_t mem_bytes = 0;  // HBM the job holds while it runs (admission)
+    double work = 0;       // size estimate (scalars), orders the queue
     std::function<void(Context &, Job &)> fn;
     int status = 1;  // 1 pending, 0 done, < 0 error code
     std::string error;
@@ -33,6 +34,14 @@ struct EngineStats {
     size_t hbm_in_use = 0, hbm_peak = 0, hbm_budget = 0, arena_bytes = 0;
 };
 
+// Which queued job a free worker takes.
+//   kFifo:     arrival order; the head waits until its HBM estimate fits.
+//   kShortest: the smallest job (by work) whose estimate fits, so short shards
+//              do not queue behind long ones. With `reserve` > 0, jobs above
+//              `small_work` may hold at most workers - reserve workers, which
+//              keeps that many free for short shards arriving later.
+enum class Policy { kFifo = 0, kShortest = 1 };
+
 class Engine {
    public:
     Engine(int device, int workers, size_t hbm_budget);
@@ -42,11 +51,24 @@ class Engine {
     bool poll(uint64_t id);
     EngineStats stats();
     int workers() const { return (int)ctxs_.size(); }
+    void set_policy(Policy p, int reserve, double small_work);
+    // Kernel timing on every worker's stream (see Stats); summed over workers
+    // as their jobs finish.
+    void set_profiling(bool on);
+    Stats kernel_stats();
+    void reset_kernel_stats();
 
    private:
     void run(int w);
+    // index into queue_ of the job a free worker takes now, or -1
+    long pick_locked() const;
     int device_;
     size_t budget_;
+    Policy policy_ = Policy::kFifo;
+    int reserve_ = 0;
+    double small_work_ = 0;
+    size_t running_large_ = 0;
+    Stats kstats_;
     std::vector<std::unique_ptr<Context>> ctxs_;
     std::vector<std::thread> threads_;
     std::mutex mu_;

@@ -4,13 +4,15 @@
 // claim shards as the streaming pipeline emits them
 // (crates/prover/src/streaming.rs:601-680, default 8 in
 // tools/bench/src/block.rs:129). Here each worker owns a HIP stream and a
-// scratch arena (a Context); jobs wait in one FIFO queue and are admitted when
-// a worker is free and the job's HBM estimate fits the budget beside the jobs
-// already running. Workers run concurrently, so one shard's latency-bound
+// scratch arena (a Context); jobs wait in one queue and are admitted when a
+// worker is free and the job's HBM estimate fits the budget beside the jobs
+// already running, in arrival order or shortest first (Policy). Workers run
+// concurrently, so one shard's latency-bound
 // phases (bucket-reduction trees, sumcheck rounds waiting on the host
 // transcript) overlap other shards' throughput-bound kernels. A job's result
 // depends only on its own inputs and transcript: the same bytes alone or in
 // any mix.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -99,24 +101,66 @@ EngineStats Engine::stats() {
     return s;
 }
 
+void Engine::set_policy(Policy p, int reserve, double small_work) {
+    std::lock_guard<std::mutex> lk(mu_);
+    policy_ = p;
+    reserve_ = std::max(0, std::min(reserve, (int)ctxs_.size() - 1));
+    small_work_ = small_work;
+    cv_.notify_all();
+}
+
+void Engine::set_profiling(bool on) {
+    for (auto &c : ctxs_) c->profiling = on;
+}
+
+Stats Engine::kernel_stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return kstats_;
+}
+
+void Engine::reset_kernel_stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    kstats_ = Stats();
+}
+
+long Engine::pick_locked() const {
+    if (queue_.empty()) return -1;
+    if (policy_ == Policy::kFifo)
+        return in_use_ + queue_.front()->mem_bytes <= budget_ ? 0 : -1;
+    const bool large_ok = running_large_ + reserve_ < ctxs_.size();
+    long best = -1;
+    for (size_t i = 0; i < queue_.size(); i++) {
+        const Job &j = *queue_[i];
+        if (in_use_ + j.mem_bytes > budget_) continue;
+        if (!large_ok && j.work > small_work_) continue;
+        if (best < 0 || j.work < queue_[best]->work) best = (long)i;
+    }
+    return best;
+}
+
 void Engine::run(int w) {
     Context &ctx = *ctxs_[w];
     HIP_CHECK(hipSetDevice(device_));
     for (;;) {
         std::unique_ptr<Job> job;
+        bool large = false;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            // FIFO admission: the head job starts when its HBM estimate fits
+            long at = -1;
+            // queued jobs still run after stop_; every queued job fits an
+            // idle engine (submit refuses the rest), so the queue drains
             cv_.wait(lk, [&] {
-                return stop_ || (!queue_.empty() && in_use_ + queue_.front()->mem_bytes <= budget_);
+                at = pick_locked();
+                return at >= 0 || (stop_ && queue_.empty());
             });
-            if (stop_ && queue_.empty()) return;
-            if (queue_.empty()) continue;
-            job = std::move(queue_.front());
-            queue_.pop_front();
+            if (at < 0) return;
+            job = std::move(queue_[at]);
+            queue_.erase(queue_.begin() + at);
             in_use_ += job->mem_bytes;
             peak_ = std::max(peak_, in_use_);
             running_++;
+            large = job->work > small_work_;
+            if (large) running_large_++;
         }
         job->t_start = now_ms();
         job->worker = w;
@@ -133,8 +177,11 @@ void Engine::run(int w) {
         job->t_end = now_ms();
         {
             std::lock_guard<std::mutex> lk(mu_);
+            kstats_ += ctx.stats;
+            ctx.stats = Stats();
             in_use_ -= job->mem_bytes;
             running_--;
+            if (large) running_large_--;
             completed_++;
             done_[job->id] = std::move(job);
         }

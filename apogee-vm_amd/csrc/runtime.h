@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -81,6 +82,20 @@ struct Stats {
     uint64_t sumcheck_launches = 0;
     uint64_t sumcheck_bytes = 0;      // algorithmic HBM bytes of those launches
     double sumcheck_ms = 0;           // whole sumcheck, host wall clock
+
+    Stats &operator+=(const Stats &o) {
+        accumulate_ms += o.accumulate_ms;
+        accumulate_launches += o.accumulate_launches;
+        accumulate_entries += o.accumulate_entries;
+        msm_ms += o.msm_ms;
+        msm_calls += o.msm_calls;
+        msm_points += o.msm_points;
+        sumcheck_kernel_ms += o.sumcheck_kernel_ms;
+        sumcheck_launches += o.sumcheck_launches;
+        sumcheck_bytes += o.sumcheck_bytes;
+        sumcheck_ms += o.sumcheck_ms;
+        return *this;
+    }
 };
 
 struct Context {
@@ -88,7 +103,7 @@ struct Context {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // fork/join partner of `stream` within one call
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool profiling = false;
+    std::atomic<bool> profiling{false};  // set from other threads (engine)
     Stats stats;
     std::mutex mu;  // one call at a time per context
     std::unordered_map<std::string, DevBuf> dev;

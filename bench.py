@@ -14,14 +14,18 @@ Workloads (--workload):
   serve  the batch's shards arriving as a seeded Poisson process at 0.5x,
          1x and 1.5x the measured batched capacity: shards/s, p50/p99/p999
          latency (and latency over the shard's isolated time), HBM occupancy.
+  copy   1 GiB device copies by the library's k_copy16 kernel: the
+         known-size reference for rocprofv3 FETCH_SIZE / WRITE_SIZE passes.
 Inputs are generated into HBM before any timed region. One JSON line on
-stdout; progress on stderr. The mixed run also carries the single large
-shard's numbers, including the bucket-accumulation kernel's roofline (timed
-with HIP events on its stream), and a CPU baseline: the oracle's restatement
-of the reference algorithms timed on a sample of the batch's shards on the
-host cores, whose outputs are checked against the GPU's.
+stdout; progress on stderr. The mixed run reports the bucket-accumulation
+kernel's roofline over the engine's timed passes (HIP events on each launch's
+stream; also its average over every launch of the process, and alone in a
+single 2^20 x 8 shard), the single large shard's numbers, and a CPU baseline:
+the oracle's restatement of the reference algorithms timed on a sample of the
+batch's shards on the host cores, whose outputs are checked against the GPU's.
 
-    python bench.py [--workload mixed|shard|serve] [--steps K] [--warmup W]
+    python bench.py [--workload mixed|shard|serve|copy] [--steps K] [--warmup W]
+                    [--workers N] [--policy fifo|shortest] [--reserve R]
 """
 from __future__ import annotations
 
@@ -72,8 +76,8 @@ def run_single(ctx, srs, args, nv=20, k=8, deg=3, steps=10, warmup=2):
     for _ in range(warmup):
         apg.prove_shard(srs, buf, nv, k, deg, apg.Transcript(), ctx)
     ctx.synchronize()
-    ctx.reset_stats()
     ctx.set_profiling(True)
+    before = ctx.stats()
     lat = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -82,11 +86,8 @@ def run_single(ctx, srs, args, nv=20, k=8, deg=3, steps=10, warmup=2):
         lat.append(time.perf_counter() - ts)
     ctx.synchronize()
     total = time.perf_counter() - t0
-    ctx.set_profiling(False)
-    st = ctx.stats()
-    acc_ms = st["accumulate_ms"] / max(1, st["accumulate_launches"])
-    entries = st["accumulate_entries"] / max(1, st["accumulate_launches"])
-    achieved = IMUL_PER_MADD * entries / (acc_ms * 1e-3)
+    st = stats_delta(ctx.stats(), before)
+    roof = roofline(st, f"k_accumulate (MSM bucket accumulation), one 2^{nv} x {k} shard alone")
     sc_gbs = st["sumcheck_bytes"] / (st["sumcheck_kernel_ms"] * 1e-3) / 1e9 if st["sumcheck_kernel_ms"] else 0.0
     res = {
         "workload": f"one shard: 2^{nv} rows x {k} witness columns, degree-{deg} sumcheck",
@@ -100,19 +101,54 @@ def run_single(ctx, srs, args, nv=20, k=8, deg=3, steps=10, warmup=2):
         "latency_ms": {"p50": round(float(np.percentile(np.array(lat) * 1e3, 50)), 3),
                        "p99": round(float(np.percentile(np.array(lat) * 1e3, 99)), 3)},
     }
-    roof = {
-        "kernel": "k_accumulate (MSM bucket accumulation, single 2^20 x 8 shard)",
+    return res, roof, (spec, buf, proof)
+
+
+def stats_delta(after, before):
+    return {k: after[k] - before[k] for k in after}
+
+
+def stats_sum(a, b):
+    return {k: a[k] + b[k] for k in a}
+
+
+def traffic_per_entry():
+    """HBM bytes per accumulated digit entry of k_accumulate, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the same
+    counters on the k_copy16 kernel); None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", "k_accumulate_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("bytes_per_entry:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def roofline(st, label):
+    """Roofline of k_accumulate over the launches recorded in `st`: the
+    algorithmic multiplies (1496 per digit entry: one mixed addition) per
+    launch over the average launch time (HIP events on the launch stream)."""
+    launches = max(1, st["accumulate_launches"])
+    avg_ms = st["accumulate_ms"] / launches
+    entries = st["accumulate_entries"] / launches
+    achieved = IMUL_PER_MADD * entries / (avg_ms * 1e-3) if avg_ms else 0.0
+    tpe = traffic_per_entry()
+    return {
+        "kernel": label,
         "bound": "valu-imul",
         "achieved": round(achieved / 1e12, 3),
         "peak": round(PEAK_IMUL / 1e12, 3),
         "unit": "TIMUL/s",
         "frac": round(achieved / PEAK_IMUL, 4),
-        "traffic": None,
-        "avg_launch_ms": round(acc_ms, 4),
-        "entries_per_launch": int(entries),
-        "algorithmic_imul_per_launch": int(IMUL_PER_MADD * entries),
+        "traffic": round(tpe * entries) if tpe is not None else None,
+        "launches": int(st["accumulate_launches"]),
+        "avg_launch_ms": round(avg_ms, 4),
+        "entries_per_launch": round(entries, 1),
+        "algorithmic_imul_per_launch": round(IMUL_PER_MADD * entries),
     }
-    return res, roof, (spec, buf, proof)
 
 
 # ------------------------------------------------------------------ mixed batch
@@ -194,14 +230,35 @@ def cpu_baseline(srs, specs, bufs, gpu_proofs, budget_s=20.0):
     }, ok, len(done)
 
 
+def copy_calibration(ctx, args, nbytes=1 << 30):
+    """The known-size reference for the HBM traffic counters: k_copy16 reads
+    and writes nbytes per launch; a FETCH_SIZE / WRITE_SIZE pass over this
+    run gives the counter's reading per byte moved."""
+    src = apg.DeviceBuffer(ctx, nbytes)
+    dst = apg.DeviceBuffer(ctx, nbytes)
+    apg.synthetic_device(ctx, args.seed, nbytes // 32, "uniform", out=src)
+    for _ in range(max(1, args.warmup)):
+        dst.copy_from(src)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_from(src)
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"metric": "k_copy16 device copy", "value": round(2 * nbytes / dt / 1e9, 1), "unit": "GB/s",
+            "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes, "steps": args.steps,
+            "ms_per_step": round(1e3 * dt, 4), "frac_of_hbm_peak": round(2 * nbytes / dt / PEAK_HBM, 4)}
+
+
 def main():
     global apg, wl
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve", "copy"])
     ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--policy", default="fifo", choices=["fifo", "shortest"])
+    ap.add_argument("--reserve", type=int, default=0, help="workers kept for shards below --small-scalars")
+    ap.add_argument("--small-scalars", type=float, default=float(1 << 18))
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work (0 = skip)")
     args = ap.parse_args()
@@ -212,6 +269,10 @@ def main():
     apg = importlib.import_module("apogee-vm_amd")
     wl = importlib.import_module("apogee-vm_amd.workload")
     ctx = apg.Context(0)
+
+    if args.workload == "copy":
+        print(json.dumps(copy_calibration(ctx, args)), flush=True)
+        return
 
     if args.workload == "shard":
         tau = apg.synthetic(args.seed, 1, "uniform")[0]
@@ -228,10 +289,15 @@ def main():
         return
 
     specs, srs, bufs = setup_batch(ctx, args)
+    # every k_accumulate launch of the process is timed (HIP events on its
+    # stream), so the process-wide average compares with a rocprofv3 trace
+    ctx.set_profiling(True)
     log("plain path: one shard per call, sequential")
     t_plain, plain, iso = run_plain(ctx, srs, specs, bufs)
     log(f"plain: {len(specs) / t_plain:.1f} shards/s ({t_plain:.3f} s)")
     eng = apg.Engine(0, workers=args.workers, hbm_budget=200 << 30)
+    eng.set_policy(args.policy, args.reserve, args.small_scalars)
+    eng.set_profiling(True)
 
     if args.workload == "serve":
         out = serve(eng, srs, specs, bufs, plain, iso, args)
@@ -240,17 +306,25 @@ def main():
 
     for _ in range(args.warmup):
         run_engine_pass(eng, srs, specs, bufs)
+    warm = eng.kernel_stats()
+    eng.reset_kernel_stats()
     times, exact = [], True
     for step in range(args.steps):
         dt, res = run_engine_pass(eng, srs, specs, bufs)
         times.append(dt)
         exact = exact and all(same(p, r[0]) for p, r in zip(plain, res))
         log(f"engine pass {step}: {len(specs) / dt:.1f} shards/s ({dt:.3f} s)")
+    timed = eng.kernel_stats()
     t_eng = float(np.mean(times))
     info = eng.stats()
     value = len(specs) / t_eng
     log("single large shard (2^20 x 8) with kernel timing")
-    single, roof, _ = run_single(ctx, srs, args)
+    single, roof_single, _ = run_single(ctx, srs, args)
+    roof = roofline(timed, "k_accumulate (MSM bucket accumulation), the engine's timed passes")
+    everything = roofline(stats_sum(stats_sum(ctx.stats(), warm), timed), "all launches of the process")
+    roof["process"] = {"launches": everything["launches"], "avg_launch_ms": everything["avg_launch_ms"]}
+    roof["alone"] = {k: roof_single[k] for k in ("kernel", "achieved", "frac", "traffic", "launches",
+                                                 "avg_launch_ms", "entries_per_launch")}
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -268,7 +342,7 @@ def main():
             "workload": f"mixed batch: {len(specs)} shards, 23 families of 4-24 columns, log2 rows 60% 8-14, "
                         f"30% 15-19, 10% 20-22; commit MSMs + degree-3 batched sumcheck per shard",
             "shards": len(specs), "scalars": int(sum(s.elements for s in specs)),
-            "engine_workers": args.workers, "parallelism": "1 GPU",
+            "engine_workers": args.workers, "engine_policy": args.policy, "parallelism": "1 GPU",
         },
         "plain_shards_per_s": round(len(specs) / t_plain, 3),
         "batched_over_plain": round(value / (len(specs) / t_plain), 3),
@@ -335,7 +409,8 @@ def serve(eng, srs, specs, bufs, plain, iso, args):
             "vs_baseline": None, "dtype": "u32x8 (BN254 Fr/Fq Montgomery, exact)",
             "data": "synthetic, seeded Poisson arrivals",
             "config": {"workload": "served stream of the mixed batch's shards", "parallelism": "1 GPU",
-                       "engine_workers": args.workers, "hbm_budget_gib": 200},
+                       "engine_workers": args.workers, "engine_policy": args.policy,
+                       "reserve": args.reserve, "small_scalars": args.small_scalars, "hbm_budget_gib": 200},
             "capacity_shards_per_s": round(capacity, 2), "serve": rows, "bit_exact_vs_plain": bool(exact),
             "hbm": {"srs_gib": round(srs.device_bytes() / 2**30, 3),
                     "engine_arena_gib": round(info["arena_bytes"] / 2**30, 3)}}

@@ -84,6 +84,10 @@ int apg_dev_alloc(apg_ctx *ctx, size_t bytes, void **out);
 int apg_dev_free(apg_ctx *ctx, void *p);
 int apg_dev_upload(apg_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
 int apg_dev_download(apg_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+/* device-to-device copy by the library's own streaming kernel (k_copy16:
+ * 16-byte loads and stores, bytes a multiple of 16); it reads and writes
+ * exactly `bytes` each, the known-size reference for HBM traffic counters */
+int apg_dev_copy(apg_ctx *ctx, void *dst_dev, const void *src_dev, size_t bytes);
 
 /* ---- SRS residency.
  * Replaces the &Srs every pcs::commit borrows (crates/srs/src/lib.rs:96-276,
@@ -204,6 +208,18 @@ int apg_engine_poll(apg_engine *e, uint64_t job_id); /* 1 done, 0 pending */
 int apg_engine_wait(apg_engine *e, uint64_t job_id, uint8_t *commitments_wire, uint8_t *rounds_out,
                     uint8_t *final_evals_out, apg_transcript *transcript_out, apg_job_stats *stats);
 int apg_engine_stats(apg_engine *e, apg_engine_info *out);
+/* Queue policy: APG_POLICY_FIFO (arrival order, the default) or
+ * APG_POLICY_SHORTEST (smallest rows x columns first among jobs whose HBM
+ * estimate fits). With reserve > 0, jobs above small_scalars rows x columns
+ * hold at most workers - reserve workers at once. No counterpart in the
+ * reference, whose workers claim shards in emission order
+ * (crates/prover/src/streaming.rs:601-680). */
+enum { APG_POLICY_FIFO = 0, APG_POLICY_SHORTEST = 1 };
+int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_scalars);
+/* kernel timing (apg_stats) on every worker's stream, summed as jobs finish */
+int apg_engine_set_profiling(apg_engine *e, int on);
+int apg_engine_kernel_stats(apg_engine *e, apg_stats *out);
+int apg_engine_reset_kernel_stats(apg_engine *e);
 
 /* ---- the Poseidon2 transcript (crates/transcript/src/lib.rs) ---- */
 int apg_transcript_new(apg_transcript **out);
