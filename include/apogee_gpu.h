@@ -180,9 +180,10 @@ int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, siz
 /* ---- the shard engine: continuous batching of shard jobs on one device.
  * Mirrors the streaming prover's worker model (max_in_flight scoped workers
  * claiming shards, crates/prover/src/streaming.rs:601-680): `workers` HIP
- * streams, each with its own scratch; jobs wait in one FIFO queue and start
- * when a worker is free and their HBM estimate fits hbm_budget_bytes beside
- * the running ones. Each job runs apg_prove_shard_dev on its own copy of the
+ * streams, each with its own scratch; jobs wait in one queue (arrival order
+ * unless apg_engine_set_policy says otherwise) and start when a worker is
+ * free and their HBM estimate fits hbm_budget_bytes beside the running ones.
+ * Each job runs apg_prove_shard_dev on its own copy of the
  * seed transcript; its bytes do not depend on what else runs. Columns stay
  * caller-owned device memory until the job has been waited for. */
 typedef struct apg_engine apg_engine;
