@@ -26,6 +26,8 @@ struct Gate {
     // degree of a round polynomial: eq times the largest term, never below a
     // cubic (SumcheckProof.rounds are always 4 coefficients for the reference).
     int degree() const;
+    // largest number of factors in a term: G's degree along one variable
+    int x_degree() const;
     bool is_plain_product(int k) const;  // one term, coef 1, factors 0..k-1
 };
 
@@ -40,9 +42,6 @@ using RoundFn = std::function<Fe(const Fe *coeffs, int ncoeffs)>;
 void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables_dev, int k,
                            int num_vars, const Fe *eq_point_host, const RoundFn &round_fn,
                            Fe *rounds_out, Fe *point_out, Fe *finals_out);
-
-// eq(r, x) for every x in {0,1}^n (crates/poly/src/lib.rs:233-248), on device.
-void eq_table_device(Context &ctx, const Fe *r_host, int n, Fe *out_dev);
 
 // Host interpolation: the d+1 values at nodes 0..d to ascending coefficients
 // (crates/sumcheck/src/lib.rs:294-305 for d = 3, Newton form beyond).

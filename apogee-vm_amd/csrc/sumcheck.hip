@@ -2,18 +2,30 @@
 //
 // Replaces the round loop of gkr::prove_sumcheck (crates/gkr/src/lib.rs:792-899)
 // and sumcheck::prove_zerocheck (crates/sumcheck/src/lib.rs:359-408): per
-// round, the values of g(X) = sum_pairs eq_X * G(lines_X) at X = 0..d, then the
-// bind f'(i) = f(2i) + r (f(2i+1) - f(2i)) (crates/poly/src/lib.rs:156-185) of
-// every table and of eq.
+// round j, the values of g_j(X) = sum_pairs eq_X * G(lines_X) at X = 0..d,
+// then the bind f'(i) = f(2i) + r (f(2i+1) - f(2i)) (crates/poly/src/lib.rs:
+// 156-185) of every table. The round polynomials are exact field elements, so
+// any evaluation order gives the reference's coefficients; this one is built
+// for HBM streaming:
 //
-// The fold of round j-1 is fused into the evaluation of round j: one pass per
-// round reads each live table once (4 entries per thread = one pair of the
-// folded table), writes the half-size fold, and accumulates the node sums, so
-// each round streams every table from HBM exactly once. Node 1 is taken from
-// the previous round's claim (g_j(1) = g_{j-1}(r_{j-1}) - g_j(0)) from round 1
-// on, which is the identity the verifier checks (crates/sumcheck/src/lib.rs:441)
-// and saves one product per pair. All arithmetic is exact, so the round
-// polynomials are the reference's, coefficient for coefficient.
+//  * eq is never materialised or folded. With variable j bound by round j,
+//    eq(r, (rho_0..rho_{j-1}, X, i)) = P_j * eq(r_j, X) * E_{j+1}(i), where
+//    P_j = prod_{l<j} eq(r_l, rho_l) is a host scalar and E_s(i) =
+//    eq(r_s..r_{n-1}, i) is a product of two small tables (the variables
+//    below and above a split point h0, built once per proof), so a round reads
+//    only the gate's tables. The kernel sums S_j(X) = sum_i E(i) G(lines_X(i));
+//    the host multiplies by P_j eq(r_j, X).
+//  * One node of S_j comes from the claim: g_j(0) + g_j(1) = g_{j-1}(rho_{j-1})
+//    (the verifier's identity, crates/sumcheck/src/lib.rs:441) gives
+//    P_j S_j(1) = (claim - g_j(0)) / r_j, so a gate of degree D in X needs D
+//    device nodes (X = 0, 2..D) and the rest follow by extrapolation. Round 0
+//    (no claim yet) and any r_j = 0 evaluate all D + 1 nodes instead.
+//  * The fold of round j-1 is fused into the evaluation of round j: one pass
+//    reads each table once (4 entries per pair of the folded table), writes
+//    the half-size fold and accumulates the node sums.
+//  * Node sums reduce by wave shuffles, then across the block's waves in LDS,
+//    then (more than one block) through per-block partials summed by the last
+//    block to finish.
 #include <algorithm>
 #include <cstring>
 
@@ -24,16 +36,25 @@ namespace apg {
 namespace {
 
 constexpr int RTPB = 256;
+constexpr int RWAVES = RTPB / 64;
 constexpr int MAX_RBLOCKS = 2048;
 
 struct TablePtrs {
-    const Fe *src[SC_MAX_TABLES + 1];  // last slot: eq
-    Fe *dst[SC_MAX_TABLES + 1];
+    const Fe *src[SC_MAX_TABLES];
+    Fe *dst[SC_MAX_TABLES];
 };
 
 struct GateArgs {
     int nterms;
     Term terms[SC_MAX_TERMS];
+};
+
+// E_s(i) for the pairs of one round: lo[i mod 2^lo_bits] * hi[i >> lo_bits],
+// or hi[i] when lo is null.
+struct EqArgs {
+    const Fe *lo;
+    const Fe *hi;
+    uint32_t lo_bits;
 };
 
 __device__ __forceinline__ Fe ldfe(const Fe *p, size_t i) {
@@ -49,6 +70,11 @@ __device__ __forceinline__ void stfe(Fe *p, size_t i, const Fe &v) {
     uint4 *q = reinterpret_cast<uint4 *>(p + i);
     q[0] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
     q[1] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
+}
+
+__device__ __forceinline__ Fe eq_factor(const EqArgs &q, size_t i) {
+    if (!q.lo) return ldfe(q.hi, i);
+    return f_mul<Fr>(ldfe(q.lo, i & ((size_t(1) << q.lo_bits) - 1)), ldfe(q.hi, i >> q.lo_bits));
 }
 
 // lo/hi of pair i of table t, folding the previous round's table if asked.
@@ -68,61 +94,81 @@ __device__ __forceinline__ void pair_of(const TablePtrs &tp, int t, size_t i, bo
     }
 }
 
-// Block-level sum of ND Fr accumulators, then the last block to finish sums
-// all block partials into out[0..ND).
-template <int ND>
-__device__ void reduce_nodes(Fe (&acc)[ND], Fe *partials, unsigned *counter, Fe *out) {
-    __shared__ Fe lds[RTPB];
-    __shared__ bool is_last;
-    const int tid = threadIdx.x;
+__device__ __forceinline__ Fe wave_sum(Fe v) {
 #pragma unroll
-    for (int x = 0; x < ND; x++) {
-        lds[tid] = acc[x];
-        __syncthreads();
-        for (int s = RTPB / 2; s > 0; s >>= 1) {
-            if (tid < s) lds[tid] = f_add<Fr>(lds[tid], lds[tid + s]);
-            __syncthreads();
-        }
-        if (tid == 0) partials[(size_t)blockIdx.x * ND + x] = lds[0];
-        __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        Fe o;
+#pragma unroll
+        for (int q = 0; q < 8; q++) o.l[q] = __shfl_xor(v.l[q], off, 64);
+        v = f_add<Fr>(v, o);
     }
+    return v;
+}
+
+// Sum of the first nn of NN per-thread accumulators over the grid into out[].
+template <int NN>
+__device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *counter, Fe *out) {
+    __shared__ Fe lds[RWAVES][NN];
+    __shared__ bool is_last;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int x = 0; x < NN; x++) {
+        if (x < nn) {
+            const Fe v = wave_sum(acc[x]);
+            if (lane == 0) lds[w][x] = v;
+        }
+    }
+    __syncthreads();
+    const bool single = gridDim.x == 1;
+    if (tid < nn) {
+        Fe s = lds[0][tid];
+        for (int q = 1; q < RWAVES; q++) s = f_add<Fr>(s, lds[q][tid]);
+        if (single)
+            out[tid] = s;
+        else
+            partials[(size_t)blockIdx.x * NN + tid] = s;
+    }
+    if (single) return;
     __threadfence();
+    __syncthreads();
     if (tid == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
     __syncthreads();
     if (!is_last) return;
     __threadfence();
-    for (int x = 0; x < ND; x++) {
+    for (int x = 0; x < nn; x++) {
         Fe a = f_zero<Fr>();
         for (unsigned b = tid; b < gridDim.x; b += RTPB) {
             // partials written by other blocks: read through the L2
-            const Fe *pp = partials + (size_t)b * ND + x;
+            const Fe *pp = partials + (size_t)b * NN + x;
             Fe v;
             for (int q = 0; q < 8; q++) v.l[q] = __atomic_load_n(&pp->l[q], __ATOMIC_RELAXED);
             a = f_add<Fr>(a, v);
         }
-        lds[tid] = a;
-        __syncthreads();
-        for (int s = RTPB / 2; s > 0; s >>= 1) {
-            if (tid < s) lds[tid] = f_add<Fr>(lds[tid], lds[tid + s]);
-            __syncthreads();
-        }
-        if (tid == 0) out[x] = lds[0];
-        __syncthreads();
+        a = wave_sum(a);
+        if (lane == 0) lds[w][x] = a;
+    }
+    __syncthreads();
+    if (tid < nn) {
+        Fe s = lds[0][tid];
+        for (int q = 1; q < RWAVES; q++) s = f_add<Fr>(s, lds[q][tid]);
+        out[tid] = s;
     }
     if (tid == 0) *counter = 0;
 }
 
-// Product summand over KT tables (coef 1): the configs' eq * a * b (* c).
-template <int KT, int ND>
+// Product summand over KT tables (coef 1): the shards' eq * a * b (* c).
+// S(X) = sum_i E(i) prod_t line_t(i, X) at X = 0..KT (FULL) or X = 0, 2..KT.
+template <int KT, bool FULL>
 __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, int fold, Fe r,
-                                                      int skip1, Fe *partials,
-                                                      unsigned *counter, Fe *out) {
-    Fe acc[ND];
+                                                      EqArgs eq, Fe *partials, unsigned *counter,
+                                                      Fe *out) {
+    constexpr int NN = FULL ? KT + 1 : KT;
+    Fe acc[NN];
 #pragma unroll
-    for (int x = 0; x < ND; x++) acc[x] = f_zero<Fr>();
+    for (int x = 0; x < NN; x++) acc[x] = f_zero<Fr>();
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
-        Fe cur[KT], step[KT], e_lo, e_hi;
+        Fe cur[KT], step[KT];
 #pragma unroll
         for (int t = 0; t < KT; t++) {
             Fe lo, hi;
@@ -130,50 +176,48 @@ __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, 
             cur[t] = lo;
             step[t] = f_sub<Fr>(hi, lo);
         }
-        pair_of(tp, SC_MAX_TABLES, i, fold, r, e_lo, e_hi);
-        const Fe e_step = f_sub<Fr>(e_hi, e_lo);
-        Fe e = e_lo;
+        const Fe e = eq_factor(eq, i);
 #pragma unroll
-        for (int x = 0; x < ND; x++) {
-            if (!(skip1 && x == 1)) {
-                Fe g = cur[0];
+        for (int x = 0; x < NN; x++) {
+            Fe g = cur[0];
 #pragma unroll
-                for (int t = 1; t < KT; t++) g = f_mul<Fr>(g, cur[t]);
-                acc[x] = f_add<Fr>(acc[x], f_mul<Fr>(e, g));
-            }
-            if (x + 1 < ND) {
-                e = f_add<Fr>(e, e_step);
+            for (int t = 1; t < KT; t++) g = f_mul<Fr>(g, cur[t]);
+            acc[x] = f_add<Fr>(acc[x], f_mul<Fr>(e, g));
+            if (x + 1 < NN) {
 #pragma unroll
-                for (int t = 0; t < KT; t++) cur[t] = f_add<Fr>(cur[t], step[t]);
+                for (int t = 0; t < KT; t++) {
+                    cur[t] = f_add<Fr>(cur[t], step[t]);
+                    if (!FULL && x == 0) cur[t] = f_add<Fr>(cur[t], step[t]);  // X: 0 -> 2
+                }
             }
         }
     }
-    reduce_nodes<ND>(acc, partials, counter, out);
+    reduce_nodes<NN>(acc, NN, partials, counter, out);
 }
 
 // Any gate: terms read from the kernel argument, table lines kept in a local
-// array (dynamic factor indices).
-template <int ND>
+// array (dynamic factor indices). nn of NN nodes: X = 0..nn-1 (full) or
+// X = 0, 2..nn.
+template <int NN>
 __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateArgs gate, size_t m,
-                                                   int fold, Fe r, int skip1, Fe *partials,
-                                                   unsigned *counter, Fe *out) {
-    Fe acc[ND];
+                                                   int fold, Fe r, EqArgs eq, int full, int nn,
+                                                   Fe *partials, unsigned *counter, Fe *out) {
+    Fe acc[NN];
 #pragma unroll
-    for (int x = 0; x < ND; x++) acc[x] = f_zero<Fr>();
+    for (int x = 0; x < NN; x++) acc[x] = f_zero<Fr>();
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
-        Fe cur[SC_MAX_TABLES], step[SC_MAX_TABLES], e_lo, e_hi;
+        Fe cur[SC_MAX_TABLES], step[SC_MAX_TABLES];
         for (int t = 0; t < k; t++) {
             Fe lo, hi;
             pair_of(tp, t, i, fold, r, lo, hi);
             cur[t] = lo;
             step[t] = f_sub<Fr>(hi, lo);
         }
-        pair_of(tp, SC_MAX_TABLES, i, fold, r, e_lo, e_hi);
-        const Fe e_step = f_sub<Fr>(e_hi, e_lo);
-        Fe e = e_lo;
-        for (int x = 0; x < ND; x++) {
-            if (!(skip1 && x == 1)) {
+        const Fe e = eq_factor(eq, i);
+#pragma unroll
+        for (int x = 0; x < NN; x++) {
+            if (x < nn) {
                 Fe g = f_zero<Fr>();
                 for (int q = 0; q < gate.nterms; q++) {
                     const Term &tm = gate.terms[q];
@@ -183,34 +227,53 @@ __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateAr
                     g = f_add<Fr>(g, v);
                 }
                 acc[x] = f_add<Fr>(acc[x], f_mul<Fr>(e, g));
-            }
-            if (x + 1 < ND) {
-                e = f_add<Fr>(e, e_step);
-                for (int t = 0; t < k; t++) cur[t] = f_add<Fr>(cur[t], step[t]);
+                if (x + 1 < nn) {
+                    for (int t = 0; t < k; t++) {
+                        cur[t] = f_add<Fr>(cur[t], step[t]);
+                        if (!full && x == 0) cur[t] = f_add<Fr>(cur[t], step[t]);
+                    }
+                }
             }
         }
     }
-    reduce_nodes<ND>(acc, partials, counter, out);
+    reduce_nodes<NN>(acc, nn, partials, counter, out);
 }
 
-// eq tables: lo half by bits of x & (2^h - 1), hi half by x >> h.
-__global__ void k_eq_half(const Fe *__restrict__ r, int nbits, Fe *__restrict__ out) {
-    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= (size_t(1) << nbits)) return;
-    Fe acc = f_one<Fr>();
-    const Fe one = f_one<Fr>();
-    for (int j = 0; j < nbits; j++) {
-        const Fe f = ((x >> j) & 1) ? r[j] : f_sub<Fr>(one, r[j]);
-        acc = f_mul<Fr>(acc, f);
+// The suffix-eq tables of one proof, as products over the variables they
+// cover (crates/poly/src/lib.rs:233-248: eq(r, x) = prod_j (r_j x_j +
+// (1 - r_j)(1 - x_j)), variable j = bit j):
+//   A_s (1 <= s < h0): 2^(h0-s) entries over variables s..h0-1, at A + (s-1) 2^(h0-1);
+//   B_s (h0 <= s <= n): 2^(n-s) entries over variables s..n-1, at B + (s-h0) 2^(n-h0).
+__global__ void k_eq_suffix(const Fe *__restrict__ r, int n, int h0, Fe *__restrict__ A,
+                            Fe *__restrict__ B) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t astride = size_t(1) << (h0 - 1), bstride = size_t(1) << (n - h0);
+    const size_t asz = (size_t)(h0 - 1) * astride, bsz = (size_t)(n - h0 + 1) * bstride;
+    int s, top;
+    size_t x;
+    Fe *dst;
+    if (tid < asz) {
+        s = 1 + (int)(tid / astride);
+        x = tid % astride;
+        top = h0;
+        dst = A + tid;
+    } else if (tid < asz + bsz) {
+        const size_t u = tid - asz;
+        s = h0 + (int)(u / bstride);
+        x = u % bstride;
+        top = n;
+        dst = B + u;
+    } else {
+        return;
     }
-    out[x] = acc;
-}
-
-__global__ void k_eq_full(const Fe *__restrict__ lo, const Fe *__restrict__ hi, int h, size_t len,
-                          Fe *__restrict__ out) {
-    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= len) return;
-    stfe(out, x, f_mul<Fr>(ldfe(lo, x & ((size_t(1) << h) - 1)), ldfe(hi, x >> h)));
+    if (x >= (size_t(1) << (top - s))) return;
+    const Fe one = f_one<Fr>();
+    Fe acc = one;
+    for (int l = s; l < top; l++) {
+        const Fe rl = ldfe(r, l);
+        acc = f_mul<Fr>(acc, ((x >> (l - s)) & 1) ? rl : f_sub<Fr>(one, rl));
+    }
+    stfe(dst, 0, acc);
 }
 
 struct CombineArgs {
@@ -236,14 +299,16 @@ __global__ void k_combine(const Fe *__restrict__ cols, size_t n, int K, CombineA
 
 }  // namespace
 
-int Gate::degree() const {
+int Gate::x_degree() const {
     int mx = 1;
     for (int q = 0; q < nterms; q++) {
         int f = 1 + (terms[q].f[1] >= 0) + (terms[q].f[2] >= 0);
         mx = std::max(mx, f);
     }
-    return std::max(3, mx + 1);
+    return mx;
 }
+
+int Gate::degree() const { return std::max(3, x_degree() + 1); }
 
 bool Gate::is_plain_product(int k) const {
     if (nterms != 1) return false;
@@ -303,23 +368,31 @@ static Fe horner(const Fe *c, int d, const Fe &x) {
     return acc;
 }
 
-void eq_table_device(Context &ctx, const Fe *r_host, int n, Fe *out_dev) {
-    const int h = n / 2, hn = n - h;
-    Fe *rd = ctx.d<Fe>("eq.r", std::max(n, 1));
-    Fe *lo = ctx.d<Fe>("eq.lo", size_t(1) << h);
-    Fe *hi = ctx.d<Fe>("eq.hi", size_t(1) << hn);
-    if (n) HIP_CHECK(hipMemcpyAsync(rd, r_host, n * sizeof(Fe), hipMemcpyHostToDevice, ctx.stream));
-    hipLaunchKernelGGL(k_eq_half, dim3(grid_for(size_t(1) << h, 256)), dim3(256), 0, ctx.stream, rd, h,
-                       lo);
-    hipLaunchKernelGGL(k_eq_half, dim3(grid_for(size_t(1) << hn, 256)), dim3(256), 0, ctx.stream,
-                       rd + h, hn, hi);
-    const size_t len = size_t(1) << n;
-    hipLaunchKernelGGL(k_eq_full, dim3(grid_for(len, 256)), dim3(256), 0, ctx.stream, lo, hi, h, len,
-                       out_dev);
-    HIP_CHECK(hipGetLastError());
-    // the host copy of r must outlive the async copy
-    ctx.sync();
+// Values of a degree-D polynomial at 0..D extended to 0..d (d >= D) by
+// forward differences (the D-th difference is constant).
+static void extrapolate(Fe *v, int D, int d) {
+    Fe last[SC_MAX_NODES];  // last[k] = k-th difference ending at v[D]
+    Fe row[SC_MAX_NODES];
+    for (int i = 0; i <= D; i++) row[i] = v[i];
+    for (int k = 0; k <= D; k++) {
+        last[k] = row[D - k];
+        for (int i = 0; i + k < D; i++) row[i] = f_sub<Fr>(row[i + 1], row[i]);
+    }
+    for (int x = D + 1; x <= d; x++) {
+        for (int k = D - 1; k >= 0; k--) last[k] = f_add<Fr>(last[k], last[k + 1]);
+        v[x] = last[0];
+    }
 }
+
+// eq(r, x) = (1 - r) + x (2r - 1), the one-variable factor of eq.
+static Fe eq_fe(const Fe &r, const Fe &x) {
+    const Fe one = f_one<Fr>();
+    const Fe base = f_sub<Fr>(one, r);
+    const Fe slope = f_sub<Fr>(f_add<Fr>(r, r), one);
+    return f_add<Fr>(base, f_mul<Fr>(x, slope));
+}
+
+static Fe eq_at(const Fe &r, int x) { return eq_fe(r, f_from_u64<Fr>((uint64_t)x)); }
 
 void combine_columns_device(Context &ctx, const Fe *cols, size_t n, int K, const Fe *w, int outs,
                             Fe *const *out) {
@@ -339,32 +412,50 @@ void combine_columns_device(Context &ctx, const Fe *cols, size_t n, int K, const
     HIP_CHECK(hipGetLastError());
 }
 
-template <int ND>
-static void launch_round(Context &ctx, const Gate &gate, int k, bool product, const TablePtrs &tp,
-                         size_t m, bool fold, const Fe &r, bool skip1, Fe *partials,
-                         unsigned *counter, Fe *out) {
-    const unsigned blocks = (unsigned)std::min<size_t>(MAX_RBLOCKS, std::max<size_t>(1, (m + RTPB - 1) / RTPB));
+// Blocks for m pairs: one pair per thread up to 4 resident blocks per CU,
+// beyond that a grid-stride loop.
+static unsigned round_blocks(const Context &ctx, size_t m) {
+    const size_t cap = std::min<size_t>(MAX_RBLOCKS, (size_t)4 * ctx.cu_count);
+    return (unsigned)std::min(cap, std::max<size_t>(1, (m + RTPB - 1) / RTPB));
+}
+
+template <int KT>
+static void launch_product(Context &ctx, bool full, const TablePtrs &tp, size_t m, bool fold,
+                           const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out) {
+    const dim3 g(round_blocks(ctx, m)), b(RTPB);
+    if (full)
+        hipLaunchKernelGGL((k_round_product<KT, true>), g, b, 0, ctx.stream, tp, m, (int)fold, r, eq,
+                           partials, counter, out);
+    else
+        hipLaunchKernelGGL((k_round_product<KT, false>), g, b, 0, ctx.stream, tp, m, (int)fold, r, eq,
+                           partials, counter, out);
+}
+
+static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bool full, int D,
+                         const TablePtrs &tp, size_t m, bool fold, const Fe &r, const EqArgs &eq,
+                         Fe *partials, unsigned *counter, Fe *out) {
     if (product) {
         switch (k) {
-#define APG_PROD(KT)                                                                            \
-    case KT:                                                                                    \
-        hipLaunchKernelGGL((k_round_product<KT, ND>), dim3(blocks), dim3(RTPB), 0, ctx.stream, tp, \
-                           m, (int)fold, r, (int)skip1, partials, counter, out);               \
-        return;
-            APG_PROD(1)
-            APG_PROD(2)
-            APG_PROD(3)
-            APG_PROD(4)
-#undef APG_PROD
-            default:
-                break;
+            case 1: launch_product<1>(ctx, full, tp, m, fold, r, eq, partials, counter, out); return;
+            case 2: launch_product<2>(ctx, full, tp, m, fold, r, eq, partials, counter, out); return;
+            case 3: launch_product<3>(ctx, full, tp, m, fold, r, eq, partials, counter, out); return;
+            default: break;
         }
     }
     GateArgs ga;
     ga.nterms = gate.nterms;
     for (int q = 0; q < gate.nterms; q++) ga.terms[q] = gate.terms[q];
-    hipLaunchKernelGGL((k_round_gate<ND>), dim3(blocks), dim3(RTPB), 0, ctx.stream, tp, k, ga, m,
-                       (int)fold, r, (int)skip1, partials, counter, out);
+    const int nn = full ? D + 1 : D;
+    const dim3 g(round_blocks(ctx, m)), b(RTPB);
+#define APG_GATE(NN)                                                                                  \
+    hipLaunchKernelGGL((k_round_gate<NN>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r, eq,      \
+                       (int)full, nn, partials, counter, out)
+    switch (D + 1) {
+        case 2: APG_GATE(2); break;
+        case 3: APG_GATE(3); break;
+        default: APG_GATE(4); break;
+    }
+#undef APG_GATE
 }
 
 void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, int k, int n,
@@ -376,26 +467,59 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
         for (int f = 0; f < 3; f++)
             if (gate.terms[q].f[f] >= k || (f == 0 && gate.terms[q].f[0] < 0))
                 throw ApgError(-2, "sumcheck: term factor out of range");
-    const int d = gate.degree();
-    if (d + 1 > SC_MAX_NODES) throw ApgError(-2, "sumcheck: degree too high");
+    const int D = gate.x_degree();  // degree of G along one variable
+    const int d = gate.degree();    // round polynomial degree
+    if (d + 1 > SC_MAX_NODES || D > 3) throw ApgError(-2, "sumcheck: degree too high");
     const bool product = gate.is_plain_product(k);
     hipStream_t st = ctx.stream;
     const size_t len0 = size_t(1) << n;
 
-    // eq and two ping-pong buffers per table (the inputs are read in round 0
-    // and folded into scratch from round 1 on).
-    Fe *eq = ctx.d<Fe>("sc.eq", std::max<size_t>(len0, 2));
-    eq_table_device(ctx, eq_point, n, eq);
-    std::vector<Fe *> bufA(k + 1), bufB(k + 1);
-    Fe *poolA = ctx.d<Fe>("sc.poolA", (size_t)(k + 1) * std::max<size_t>(len0 / 2, 1));
-    Fe *poolB = ctx.d<Fe>("sc.poolB", (size_t)(k + 1) * std::max<size_t>(len0 / 4, 1));
-    for (int t = 0; t <= k; t++) {
+    if (n == 0) {  // nothing to bind: the finals are the single entries
+        for (int t = 0; t < k; t++)
+            HIP_CHECK(hipMemcpyAsync(&finals_out[t], tables[t], sizeof(Fe), hipMemcpyDeviceToHost, st));
+        ctx.sync();
+        return;
+    }
+
+    // suffix-eq tables (k_eq_suffix), split at h0
+    const int h0 = std::max(1, std::min(n, (n + 1) / 2));
+    const size_t astride = size_t(1) << (h0 - 1), bstride = size_t(1) << (n - h0);
+    const size_t asz = (size_t)(h0 - 1) * astride, bsz = (size_t)(n - h0 + 1) * bstride;
+    Fe *rd = ctx.d<Fe>("sc.r", n);
+    Fe *eqA = ctx.d<Fe>("sc.eqA", std::max<size_t>(asz, 1));
+    Fe *eqB = ctx.d<Fe>("sc.eqB", bsz);
+    HIP_CHECK(hipMemcpyAsync(rd, eq_point, n * sizeof(Fe), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_eq_suffix, dim3(grid_for(asz + bsz, 256)), dim3(256), 0, st, rd, n, h0, eqA, eqB);
+    HIP_CHECK(hipGetLastError());
+
+    // 1 / r_j by one batched inversion (zeros skipped: such rounds use all nodes)
+    std::vector<Fe> inv_r(n), pre(n + 1);
+    pre[0] = f_one<Fr>();
+    for (int j = 0; j < n; j++)
+        pre[j + 1] = f_is_zero(eq_point[j]) ? pre[j] : f_mul<Fr>(pre[j], eq_point[j]);
+    {
+        Fe inv = f_inv<Fr>(pre[n]);
+        for (int j = n - 1; j >= 0; j--) {
+            if (f_is_zero(eq_point[j])) {
+                inv_r[j] = f_zero<Fr>();
+                continue;
+            }
+            inv_r[j] = f_mul<Fr>(inv, pre[j]);
+            inv = f_mul<Fr>(inv, eq_point[j]);
+        }
+    }
+
+    // two ping-pong buffers per table (the inputs are read in round 0 and
+    // folded into scratch from round 1 on)
+    std::vector<Fe *> bufA(k), bufB(k);
+    Fe *poolA = ctx.d<Fe>("sc.poolA", (size_t)k * std::max<size_t>(len0 / 2, 1));
+    Fe *poolB = ctx.d<Fe>("sc.poolB", (size_t)k * std::max<size_t>(len0 / 4, 1));
+    for (int t = 0; t < k; t++) {
         bufA[t] = poolA + (size_t)t * std::max<size_t>(len0 / 2, 1);
         bufB[t] = poolB + (size_t)t * std::max<size_t>(len0 / 4, 1);
     }
-    std::vector<const Fe *> cur(k + 1);
+    std::vector<const Fe *> cur(k);
     for (int t = 0; t < k; t++) cur[t] = tables[t];
-    cur[k] = eq;
 
     Fe *partials = ctx.d<Fe>("sc.partials", (size_t)MAX_RBLOCKS * SC_MAX_NODES);
     Fe *dout = ctx.d<Fe>("sc.out", SC_MAX_NODES);
@@ -407,15 +531,6 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
     unsigned *counter = static_cast<unsigned *>(ctx.dev["sc.counter"].ptr);
     Fe *hvals = ctx.h<Fe>("sc.hvals", SC_MAX_NODES);
 
-    // Round 0 reads the inputs T^0 (2^n entries). Round j >= 1 folds T^(j-1)
-    // with r_(j-1) into T^j (2^(n-j) entries: odd j into bufA, even j into
-    // bufB, so source and destination never alias) and evaluates T^j's pairs.
-    if (n == 0) {  // nothing to bind: the finals are the single entries
-        for (int t = 0; t < k; t++)
-            HIP_CHECK(hipMemcpyAsync(&finals_out[t], tables[t], sizeof(Fe), hipMemcpyDeviceToHost, st));
-        ctx.sync();
-        return;
-    }
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (ctx.profiling) {
         HIP_CHECK(hipEventCreate(&ev0));
@@ -428,61 +543,82 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
             if (b) (void)hipEventDestroy(b);
         }
     } evg{ev0, ev1};
-    Fe r = f_zero<Fr>();
+
+    // Round 0 reads the inputs T^0 (2^n entries). Round j >= 1 folds T^(j-1)
+    // with rho_(j-1) into T^j (2^(n-j) entries: odd j into bufA, even j into
+    // bufB, so source and destination never alias) and evaluates T^j's pairs.
+    Fe rho = f_zero<Fr>();
     Fe claim = f_zero<Fr>();
+    Fe P = f_one<Fr>();  // prod_{l<j} eq(r_l, rho_l)
     for (int j = 0; j < n; j++) {
         const bool fold = j > 0;
         const size_t m = len0 >> (j + 1);  // pairs of T^j
+        const Fe &rj = eq_point[j];
+        const bool full = j == 0 || f_is_zero(rj);
         TablePtrs tp;
-        std::vector<Fe *> nxt(k + 1);
-        for (int t = 0; t <= k; t++) nxt[t] = (j % 2 == 1) ? bufA[t] : bufB[t];
-        const int slot = SC_MAX_TABLES;
+        std::vector<Fe *> nxt(k);
         for (int t = 0; t < k; t++) {
+            nxt[t] = (j % 2 == 1) ? bufA[t] : bufB[t];
             tp.src[t] = cur[t];
             tp.dst[t] = nxt[t];
         }
-        tp.src[slot] = cur[k];
-        tp.dst[slot] = nxt[k];
-        const bool skip1 = fold;
-        if (ctx.profiling) HIP_CHECK(hipEventRecord(ev0, st));
-        switch (d + 1) {
-            case 4: launch_round<4>(ctx, gate, k, product, tp, m, fold, r, skip1, partials, counter, dout); break;
-            case 5: launch_round<5>(ctx, gate, k, product, tp, m, fold, r, skip1, partials, counter, dout); break;
-            case 6: launch_round<6>(ctx, gate, k, product, tp, m, fold, r, skip1, partials, counter, dout); break;
-            default: throw ApgError(-2, "sumcheck: unsupported degree");
+        const int s = j + 1;
+        EqArgs eq;
+        if (s < h0) {
+            eq.lo = eqA + (size_t)(s - 1) * astride;
+            eq.lo_bits = (uint32_t)(h0 - s);
+            eq.hi = eqB;
+        } else {
+            eq.lo = nullptr;
+            eq.lo_bits = 0;
+            eq.hi = eqB + (size_t)(s - h0) * bstride;
         }
+        if (ctx.profiling) HIP_CHECK(hipEventRecord(ev0, st));
+        launch_round(ctx, gate, k, product, full, D, tp, m, fold, rho, eq, partials, counter, dout);
         HIP_CHECK(hipGetLastError());
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
-        HIP_CHECK(hipMemcpyAsync(hvals, dout, (d + 1) * sizeof(Fe), hipMemcpyDeviceToHost, st));
+        const int nn = full ? D + 1 : D;
+        HIP_CHECK(hipMemcpyAsync(hvals, dout, nn * sizeof(Fe), hipMemcpyDeviceToHost, st));
         ctx.sync();
         if (ctx.profiling) {
             float ms = 0;
             HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
             ctx.stats.sumcheck_kernel_ms += ms;
             ctx.stats.sumcheck_launches += 1;
-            // reads every live table (eq included) once; a folding round also
-            // writes the half-size fold
-            ctx.stats.sumcheck_bytes += (uint64_t)(k + 1) * 32 * (fold ? 6 * m : 2 * m);
+            // reads every table once; a folding round also writes the
+            // half-size fold
+            ctx.stats.sumcheck_bytes += (uint64_t)k * 32 * (fold ? 6 * m : 2 * m);
         }
-        Fe vals[SC_MAX_NODES];
-        for (int x = 0; x <= d; x++) vals[x] = hvals[x];
-        if (skip1) vals[1] = f_sub<Fr>(claim, vals[0]);
+        // T(X) = P S(X) at X = 0..D, then X = D+1..d by extrapolation
+        Fe T[SC_MAX_NODES], vals[SC_MAX_NODES];
+        if (full) {
+            for (int x = 0; x <= D; x++) T[x] = f_mul<Fr>(P, hvals[x]);
+        } else {
+            T[0] = f_mul<Fr>(P, hvals[0]);
+            for (int x = 2; x <= D; x++) T[x] = f_mul<Fr>(P, hvals[x - 1]);
+            const Fe g0 = f_mul<Fr>(eq_at(rj, 0), T[0]);
+            T[1] = f_mul<Fr>(f_sub<Fr>(claim, g0), inv_r[j]);  // g(1) = r_j T(1)
+        }
+        extrapolate(T, D, d);
+        for (int x = 0; x <= d; x++) vals[x] = f_mul<Fr>(eq_at(rj, x), T[x]);
+        if (!full) vals[1] = f_sub<Fr>(claim, vals[0]);
         Fe *coeffs = rounds_out + (size_t)j * (d + 1);
         interpolate_host(vals, d, coeffs);
-        r = round_fn(coeffs, d + 1);
-        point_out[j] = r;
-        claim = horner(coeffs, d, r);
+        rho = round_fn(coeffs, d + 1);
+        point_out[j] = rho;
+        claim = horner(coeffs, d, rho);
+        P = f_mul<Fr>(P, eq_fe(rj, rho));
         if (fold)
-            for (int t = 0; t <= k; t++) cur[t] = nxt[t];
+            for (int t = 0; t < k; t++) cur[t] = nxt[t];
     }
     // final fold of the two-entry tables with the last challenge
-    std::vector<Fe> last(2 * (k + 1));
+    std::vector<Fe> last(2 * k);
     for (int t = 0; t < k; t++)
         HIP_CHECK(hipMemcpyAsync(&last[2 * t], cur[t], 2 * sizeof(Fe), hipMemcpyDeviceToHost, st));
     ctx.sync();
     for (int t = 0; t < k; t++) {
         const Fe lo = last[2 * t], hi = last[2 * t + 1];
-        finals_out[t] = f_add<Fr>(lo, f_mul<Fr>(r, f_sub<Fr>(hi, lo)));
+        finals_out[t] = f_add<Fr>(lo, f_mul<Fr>(rho, f_sub<Fr>(hi, lo)));
     }
 }
 

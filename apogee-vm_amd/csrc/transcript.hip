@@ -19,15 +19,95 @@ Fe from_hex_be(const char *s) {
     return r;
 }
 
+// The permutation runs on four 64-bit Montgomery words (the same R = 2^256,
+// so the memory image is unchanged): a CIOS multiply unrolled for the Fr
+// modulus, whose top word is below 2^62, so the running sum never needs a
+// fifth carry word.
+typedef unsigned __int128 u128;
+struct W4 {
+    uint64_t v[4];
+};
+
+constexpr uint64_t RP0 = 0x43e1f593f0000001ull, RP1 = 0x2833e84879b97091ull,
+                   RP2 = 0xb85045b68181585dull, RP3 = 0x30644e72e131a029ull;
+static_assert(RP0 == ((uint64_t)FrP::mod(0) | ((uint64_t)FrP::mod(1) << 32)), "Fr modulus word 0");
+static_assert(RP3 == ((uint64_t)FrP::mod(6) | ((uint64_t)FrP::mod(7) << 32)), "Fr modulus word 3");
+
+inline W4 w4_of(const Fe &a) {
+    W4 r;
+    for (int i = 0; i < 4; i++) r.v[i] = (uint64_t)a.l[2 * i] | ((uint64_t)a.l[2 * i + 1] << 32);
+    return r;
+}
+
+inline Fe fe_of(const W4 &a) {
+    Fe r;
+    for (int i = 0; i < 4; i++) {
+        r.l[2 * i] = (uint32_t)a.v[i];
+        r.l[2 * i + 1] = (uint32_t)(a.v[i] >> 32);
+    }
+    return r;
+}
+
+// t - p if t >= p
+inline W4 w4_reduce(uint64_t t0, uint64_t t1, uint64_t t2, uint64_t t3) {
+    u128 x = (u128)t0 - RP0;
+    const uint64_t d0 = (uint64_t)x;
+    x = (u128)t1 - RP1 - (uint64_t)((x >> 64) & 1);
+    const uint64_t d1 = (uint64_t)x;
+    x = (u128)t2 - RP2 - (uint64_t)((x >> 64) & 1);
+    const uint64_t d2 = (uint64_t)x;
+    x = (u128)t3 - RP3 - (uint64_t)((x >> 64) & 1);
+    const uint64_t d3 = (uint64_t)x;
+    const bool lt = (x >> 64) & 1;
+    return lt ? W4{{t0, t1, t2, t3}} : W4{{d0, d1, d2, d3}};
+}
+
+inline W4 w4_mul(const W4 &a, const W4 &b) {
+    uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    for (int i = 0; i < 4; i++) {
+        const uint64_t bi = b.v[i];
+        u128 x = (u128)a.v[0] * bi + t0;
+        t0 = (uint64_t)x;
+        x = (u128)a.v[1] * bi + t1 + (uint64_t)(x >> 64);
+        t1 = (uint64_t)x;
+        x = (u128)a.v[2] * bi + t2 + (uint64_t)(x >> 64);
+        t2 = (uint64_t)x;
+        x = (u128)a.v[3] * bi + t3 + (uint64_t)(x >> 64);
+        t3 = (uint64_t)x;
+        const uint64_t t4 = (uint64_t)(x >> 64);
+        const uint64_t m = t0 * FrP::inv64;
+        x = (u128)m * RP0 + t0;
+        x = (u128)m * RP1 + t1 + (uint64_t)(x >> 64);
+        t0 = (uint64_t)x;
+        x = (u128)m * RP2 + t2 + (uint64_t)(x >> 64);
+        t1 = (uint64_t)x;
+        x = (u128)m * RP3 + t3 + (uint64_t)(x >> 64);
+        t2 = (uint64_t)x;
+        t3 = t4 + (uint64_t)(x >> 64);
+    }
+    return w4_reduce(t0, t1, t2, t3);
+}
+
+inline W4 w4_add(const W4 &a, const W4 &b) {
+    u128 x = (u128)a.v[0] + b.v[0];
+    const uint64_t s0 = (uint64_t)x;
+    x = (u128)a.v[1] + b.v[1] + (uint64_t)(x >> 64);
+    const uint64_t s1 = (uint64_t)x;
+    x = (u128)a.v[2] + b.v[2] + (uint64_t)(x >> 64);
+    const uint64_t s2 = (uint64_t)x;
+    x = (u128)a.v[3] + b.v[3] + (uint64_t)(x >> 64);
+    return w4_reduce(s0, s1, s2, (uint64_t)x);  // a + b < 2p < 2^256
+}
+
 struct RoundConstants {
-    Fe ini[4][3], mid[56], ter[4][3];
+    W4 ini[4][3], mid[56], ter[4][3];
     RoundConstants() {
         for (int r = 0; r < 4; r++)
             for (int l = 0; l < 3; l++) {
-                ini[r][l] = from_hex_be(APG_RC3_INITIAL[r][l]);
-                ter[r][l] = from_hex_be(APG_RC3_TERMINAL[r][l]);
+                ini[r][l] = w4_of(from_hex_be(APG_RC3_INITIAL[r][l]));
+                ter[r][l] = w4_of(from_hex_be(APG_RC3_TERMINAL[r][l]));
             }
-        for (int i = 0; i < 56; i++) mid[i] = from_hex_be(APG_RC3_INTERNAL[i]);
+        for (int i = 0; i < 56; i++) mid[i] = w4_of(from_hex_be(APG_RC3_INTERNAL[i]));
     }
 };
 
@@ -36,22 +116,22 @@ const RoundConstants &rc() {
     return k;
 }
 
-inline Fe sbox(const Fe &x) {
-    Fe x2 = f_sqr<Fr>(x);
-    Fe x4 = f_sqr<Fr>(x2);
-    return f_mul<Fr>(x4, x);
+inline W4 sbox(const W4 &x) {
+    const W4 x2 = w4_mul(x, x);
+    const W4 x4 = w4_mul(x2, x2);
+    return w4_mul(x4, x);
 }
 
-inline void ext_matrix(Fe s[3]) {
-    Fe sum = f_add<Fr>(f_add<Fr>(s[0], s[1]), s[2]);
-    for (int i = 0; i < 3; i++) s[i] = f_add<Fr>(s[i], sum);
+inline void ext_matrix(W4 s[3]) {
+    const W4 sum = w4_add(w4_add(s[0], s[1]), s[2]);
+    for (int i = 0; i < 3; i++) s[i] = w4_add(s[i], sum);
 }
 
-inline void int_matrix(Fe s[3]) {
-    Fe sum = f_add<Fr>(f_add<Fr>(s[0], s[1]), s[2]);
-    s[0] = f_add<Fr>(s[0], sum);
-    s[1] = f_add<Fr>(s[1], sum);
-    s[2] = f_add<Fr>(f_dbl<Fr>(s[2]), sum);
+inline void int_matrix(W4 s[3]) {
+    const W4 sum = w4_add(w4_add(s[0], s[1]), s[2]);
+    s[0] = w4_add(s[0], sum);
+    s[1] = w4_add(s[1], sum);
+    s[2] = w4_add(w4_add(s[2], s[2]), sum);
 }
 
 void to_bytes(const Fe &a, uint8_t out[32], bool fq) {
@@ -79,21 +159,23 @@ void fe_to_bytes_fq(const Fe &a, uint8_t out[32]) { to_bytes(a, out, true); }
 bool fe_from_bytes_fr(const uint8_t in[32], Fe &out) { return from_bytes<Fr>(in, out); }
 bool fe_from_bytes_fq(const uint8_t in[32], Fe &out) { return from_bytes<Fq>(in, out); }
 
-void poseidon2_permute_host(Fe s[3]) {
+void poseidon2_permute_host(Fe st[3]) {
     const RoundConstants &k = rc();
+    W4 s[3] = {w4_of(st[0]), w4_of(st[1]), w4_of(st[2])};
     ext_matrix(s);
     for (int r = 0; r < 4; r++) {
-        for (int l = 0; l < 3; l++) s[l] = sbox(f_add<Fr>(s[l], k.ini[r][l]));
+        for (int l = 0; l < 3; l++) s[l] = sbox(w4_add(s[l], k.ini[r][l]));
         ext_matrix(s);
     }
     for (int i = 0; i < 56; i++) {
-        s[0] = sbox(f_add<Fr>(s[0], k.mid[i]));
+        s[0] = sbox(w4_add(s[0], k.mid[i]));
         int_matrix(s);
     }
     for (int r = 0; r < 4; r++) {
-        for (int l = 0; l < 3; l++) s[l] = sbox(f_add<Fr>(s[l], k.ter[r][l]));
+        for (int l = 0; l < 3; l++) s[l] = sbox(w4_add(s[l], k.ter[r][l]));
         ext_matrix(s);
     }
+    for (int l = 0; l < 3; l++) st[l] = fe_of(s[l]);
 }
 
 Transcript::Transcript() {
