@@ -104,6 +104,8 @@ def _declare(L):
     d("apg_srs_len", [VP], SZ)
     d("apg_srs_prepare", [VP, VP, SZ])
     d("apg_srs_device_bytes", [VP], SZ)
+    d("apg_msm_set_window_bits", [I32, I32])
+    d("apg_msm_window_bits", [SZ])
     d("apg_srs_release", [VP], None)
     d("apg_msm_g1_fr", [VP, VP, VP, SZ, VP])
     d("apg_msm_g1_u32", [VP, VP, VP, SZ, VP])
@@ -360,6 +362,16 @@ def msm_small_u32(bases, scalars, ctx: Optional[Context] = None) -> bytes:
     return out.tobytes()
 
 
+def set_window_bits(log2n: int, c: int):
+    """Pippenger window width for the size class floor(log2 n) = log2n
+    (0 restores the built-in table). Results never depend on it."""
+    _check(lib().apg_msm_set_window_bits(log2n, c))
+
+
+def window_bits(n: int) -> int:
+    return int(lib().apg_msm_window_bits(n))
+
+
 # ------------------------------------------------------------------ SRS + commit
 class PcsError(Exception):
     """crates/pcs-verify/src/lib.rs:40-87 (the variants this path raises)."""
@@ -490,6 +502,21 @@ def commit_batch(srs: Srs, columns: Sequence, ctx: Optional[Context] = None) -> 
         _check(lib().apg_commit_fr(ctx.h, srs.h, _p(s), n, k, _p(out)))
     else:
         return [kzg_commit(srs, c, ctx) for c in cols]
+    return [out[64 * i: 64 * (i + 1)].tobytes() for i in range(k)]
+
+
+def commit_device(srs: Srs, columns: DeviceBuffer, n: int, k: int, ctx: Optional[Context] = None,
+                  offset_elems: int = 0) -> list:
+    """k Fr columns of n scalars resident in device memory (column-major),
+    committed in one call (apg_commit_fr_dev). Affine Montgomery points."""
+    ctx = ctx or srs.ctx
+    if len(srs) < n:
+        raise SrsTooSmall(n, len(srs))
+    if 32 * (offset_elems + n * k) > columns.nbytes:
+        raise ValueError("commit_device: columns exceed the buffer")
+    out = np.zeros(64 * k, dtype=np.uint8)
+    ptr = C.c_void_p(columns.ptr.value + 32 * offset_elems)
+    _check(lib().apg_commit_fr_dev(ctx.h, srs.h, ptr, n, k, _p(out)))
     return [out[64 * i: 64 * (i + 1)].tobytes() for i in range(k)]
 
 

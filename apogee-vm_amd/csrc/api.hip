@@ -418,6 +418,12 @@ size_t apg_srs_device_bytes(const apg_srs *srs) {
     return b;
 }
 
+int apg_msm_set_window_bits(int log2n, int c) {
+    return guarded([&] { msm_set_window_bits(log2n, c); });
+}
+
+int apg_msm_window_bits(size_t n) { return msm_pick_c(n, 1); }
+
 int apg_srs_prepare(apg_ctx *ctx, apg_srs *srs, size_t n) {
     return guarded([&] {
         std::lock_guard<std::mutex> lk(ctx->c.mu);

@@ -34,8 +34,14 @@ struct MsmShape {
 // Windows needed for a bits-bit scalar at width c (crates/curve/src/msm.rs:140-142).
 inline int msm_window_count(int bits, int c) { return (bits + 1 + c - 1) / c; }
 
-// Window width for n points under this library's size classes.
+// Window width for n points under this library's size classes (by
+// floor(log2 n)); msm_set_window_bits overrides one class (c = 0 restores the
+// default rule), which is how the autotuner tries widths.
+// c <= 17 keeps B / 256 <= 256 row sums per problem, the most k_weighted
+// takes in one block.
+constexpr int MSM_MIN_C = 3, MSM_MAX_C = 17;
 int msm_pick_c(size_t n, size_t columns);
+void msm_set_window_bits(int log2n, int c);
 
 // Builds copies = windows(254, c) shifted copies of the first n bases (device
 // pointer `src`, affine Montgomery) into a new table.

@@ -10,22 +10,27 @@
 //               pair per nonzero digit, packed densely per block.
 //  2. sort    : an LSD radix sort of the pairs by bucket key (hipCUB), then a
 //               binary search per key for the bucket boundaries.
-//  3. chunks  : every bucket is cut into chunks of at most K1 entries, so the
-//               work per thread is bounded whatever the digit skew (the
-//               witness-like mix piles 10% of all points into one bucket).
+//  3. chunks  : every bucket is cut into near-equal chunks of at most K1
+//               entries, so the work per thread is bounded whatever the digit
+//               skew (the witness-like mix piles 10% of all points into one
+//               bucket) and the lanes of a wave finish together.
 //  4. accumulate: one thread per chunk sums its points with mixed XYZZ adds;
 //               the gather of 64-byte affine bases is the only HBM traffic.
-//  5. buckets : chunk sums are combined per bucket: a thread per light bucket,
-//               an LDS tree per heavy one, the two on forked streams.
+//  5. buckets : chunk sums are combined per bucket: a thread per light bucket;
+//               heavy buckets cut into pieces over the whole grid (block
+//               slices + LDS tree, then the few piece sums), on a forked stream.
 //  6. reduce  : sum_b (b+1) S_b per group by a two-level split
-//               (b = 256 r + t: 256 * sum_r r Z_r + sum_t t U_t + sum_r Z_r),
-//               each level an LDS tree or suffix scan: ~25 dependent adds
-//               instead of the 2 * 2^(c-1) serial running sum of msm.rs:304-310.
+//               (b = 256 r + t: 256 * sum_r r Z_r + sum_t t U_t + sum_r Z_r):
+//               row and column sums by lane groups (serial slices, then wave
+//               shuffles), then an LDS suffix scan per 256-item problem:
+//               ~30 dependent adds instead of the 2 * 2^(c-1) serial running
+//               sum of msm.rs:304-310.
 //  7. host    : groups combine with 2^(c P) doublings (none when the table holds
 //               every window shifted), one batched inversion gives affine.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 
 #include "msm.h"
@@ -160,8 +165,7 @@ __global__ void k_chunk_count(const uint32_t *__restrict__ off, uint32_t nkeys, 
 // One thread per chunk: its bucket is the last key whose first chunk is at or
 // before it (binary search over the chunk offsets).
 __global__ void k_chunk_fill(const uint32_t *__restrict__ off, const uint32_t *__restrict__ qoff,
-                             uint32_t nkeys, uint32_t nchunks, uint32_t K1,
-                             Chunk *__restrict__ desc) {
+                             uint32_t nkeys, uint32_t nchunks, Chunk *__restrict__ desc) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nchunks) return;
     uint32_t lo = 0, hi = nkeys;  // largest k < nkeys with qoff[k] <= t
@@ -173,10 +177,14 @@ __global__ void k_chunk_fill(const uint32_t *__restrict__ off, const uint32_t *_
             hi = mid;
     }
     const uint32_t k = lo, s = t - qoff[k];
+    // the bucket's len entries in q near-equal chunks: the first len % q get
+    // one entry more, so the lanes of a wave finish together
+    const uint32_t len = off[k + 1] - off[k], q = qoff[k + 1] - qoff[k];
+    const uint32_t base = len / q, rem = len % q;
     Chunk ch;
     ch.key = k;
-    ch.start = off[k] + s * K1;
-    ch.len = min(K1, off[k + 1] - ch.start);
+    ch.start = off[k] + s * base + min(s, rem);
+    ch.len = base + (s < rem ? 1u : 0u);
     desc[t] = ch;
 }
 
@@ -208,15 +216,7 @@ __global__ void __launch_bounds__(TPB) k_accumulate(const Chunk *__restrict__ de
     chunk_sums[t] = acc;
 }
 
-constexpr uint32_t LIGHT_CHUNKS = 8;
-
-// Lists the buckets with more than LIGHT_CHUNKS chunks (summed by LDS trees).
-__global__ void k_classify(const uint32_t *__restrict__ qoff, uint32_t nkeys,
-                           uint32_t *__restrict__ heavy, uint32_t *__restrict__ nheavy) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nkeys) return;
-    if (qoff[k + 1] - qoff[k] > LIGHT_CHUNKS) heavy[atomicAdd(nheavy, 1u)] = k;
-}
+constexpr uint32_t LIGHT_CHUNKS = 8;  // more chunks than this: a heavy bucket
 
 __global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restrict__ qoff, uint32_t nkeys,
                                const Xyzz *__restrict__ chunk_sums, Xyzz *__restrict__ buckets) {
@@ -229,66 +229,136 @@ __global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restr
     buckets[k] = acc;
 }
 
-// sum of count points src[base + k * stride], reduced through an LDS tree.
-__device__ Xyzz block_sum(const Xyzz *__restrict__ src, size_t base, size_t stride,
-                          uint32_t count, Xyzz *lds) {
-    const uint32_t tid = threadIdx.x;
-    Xyzz acc = xyzz_identity();
-    for (uint32_t k = tid; k < count; k += TPB) acc = xyzz_add_ool(acc, src[base + (size_t)k * stride]);
-    lds[tid] = acc;
-    __syncthreads();
-    for (uint32_t s = TPB / 2; s > 0; s >>= 1) {
-        if (tid < s && tid + s < count) lds[tid] = xyzz_add_ool(lds[tid], lds[tid + s]);
-        __syncthreads();
+// Xyzz exchange between lanes of one wave.
+__device__ __forceinline__ Xyzz shfl_xor_xyzz(const Xyzz &p, int mask) {
+    Xyzz r;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        r.x.l[q] = __shfl_xor(p.x.l[q], mask, 64);
+        r.y.l[q] = __shfl_xor(p.y.l[q], mask, 64);
+        r.zz.l[q] = __shfl_xor(p.zz.l[q], mask, 64);
+        r.zzz.l[q] = __shfl_xor(p.zzz.l[q], mask, 64);
     }
+    return r;
+}
+
+// Heavy buckets (more than LIGHT_CHUNKS chunks; the witness-like mix puts
+// every scalar equal to 1 into one bucket per column) are cut into pieces of
+// PIECE chunks spread over the whole grid: each piece is one block's serial
+// slices plus a shuffle/LDS tree, and a last pass adds each bucket's few
+// piece sums. k_classify allocates the pieces as it finds the buckets.
+constexpr uint32_t PIECE = 4 * TPB;
+
+struct HeavyPlan {
+    uint32_t nheavy, npieces;
+};
+
+// heavy[h] = (bucket, first piece, pieces) of heavy bucket h; piece_of[p] =
+// (heavy index, piece within the bucket).
+__global__ void k_classify(const uint32_t *__restrict__ qoff, uint32_t nkeys, HeavyPlan *plan,
+                           uint4 *__restrict__ heavy, uint2 *__restrict__ piece_of) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const uint32_t q = qoff[k + 1] - qoff[k];
+    if (q <= LIGHT_CHUNKS) return;
+    const uint32_t np = (q + PIECE - 1) / PIECE;
+    const uint32_t h = atomicAdd(&plan->nheavy, 1u);
+    const uint32_t p0 = atomicAdd(&plan->npieces, np);
+    heavy[h] = make_uint4(k, p0, np, 0);
+    for (uint32_t j = 0; j < np; j++) piece_of[p0 + j] = make_uint2(h, j);
+}
+
+// Sum of the 256 lanes' points of a block (all lanes call it).
+__device__ __forceinline__ Xyzz block_sum256(Xyzz acc, Xyzz *lds) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) acc = xyzz_add(acc, shfl_xor_xyzz(acc, m));
+    const uint32_t tid = threadIdx.x;
+    if ((tid & 63) == 0) lds[tid >> 6] = acc;
+    __syncthreads();
     Xyzz r = lds[0];
+    for (int w = 1; w < TPB / 64; w++) r = xyzz_add(r, lds[w]);
     __syncthreads();
     return r;
 }
 
-__global__ void __launch_bounds__(TPB) k_bucket_heavy(const uint32_t *__restrict__ qoff,
-                                                    const Xyzz *__restrict__ chunk_sums,
-                                                    Xyzz *__restrict__ buckets,
-                                                    const uint32_t *__restrict__ heavy,
-                                                    const uint32_t *__restrict__ nheavy) {
-    __shared__ Xyzz lds[TPB];
-    const uint32_t nh = *nheavy;
-    for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
-        const uint32_t k = heavy[h];
-        const uint32_t a = qoff[k], q = qoff[k + 1] - a;
-        Xyzz r = block_sum(chunk_sums, a, 1, q, lds);
-        if (threadIdx.x == 0) buckets[k] = r;
+__global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restrict__ qoff,
+                                                       const Xyzz *__restrict__ chunk_sums,
+                                                       const HeavyPlan *plan,
+                                                       const uint4 *__restrict__ heavy,
+                                                       const uint2 *__restrict__ piece_of,
+                                                       Xyzz *__restrict__ piece_sums) {
+    __shared__ Xyzz lds[TPB / 64];
+    const uint32_t np = plan->npieces, tid = threadIdx.x;
+    for (uint32_t pc = blockIdx.x; pc < np; pc += gridDim.x) {
+        const uint2 po = piece_of[pc];
+        const uint32_t k = heavy[po.x].x;
+        const uint32_t a = qoff[k] + po.y * PIECE;
+        const uint32_t end = min(qoff[k + 1], a + PIECE);
+        Xyzz acc = xyzz_identity();
+        for (uint32_t c = a + tid; c < end; c += TPB) acc = xyzz_add(acc, chunk_sums[c]);
+        const Xyzz r = block_sum256(acc, lds);
+        if (tid == 0) piece_sums[pc] = r;
     }
 }
 
-// Level 1 of the bucket reduction for B > 256 buckets per problem:
-// Z_r = sum_t S[256 r + t] (R = B / 256 of them), U_t = sum_r S[256 r + t].
-__global__ void __launch_bounds__(TPB) k_reduce_split(const Xyzz *__restrict__ S, uint32_t B,
-                                                    Xyzz *__restrict__ Z, Xyzz *__restrict__ U) {
-    __shared__ Xyzz lds[TPB];
-    const uint32_t R = B / TPB;
-    const uint32_t per = R + TPB;
-    const uint32_t p = blockIdx.x / per, j = blockIdx.x % per;
-    const size_t base = (size_t)p * B;
-    if (j < R) {
-        Xyzz r = block_sum(S, base + (size_t)j * TPB, 1, TPB, lds);
-        if (threadIdx.x == 0) Z[(size_t)p * R + j] = r;
+__global__ void __launch_bounds__(TPB, 2) k_heavy_final(const HeavyPlan *plan,
+                                                      const uint4 *__restrict__ heavy,
+                                                      const Xyzz *__restrict__ piece_sums,
+                                                      Xyzz *__restrict__ buckets) {
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= plan->nheavy) return;
+    const uint4 e = heavy[h];
+    Xyzz acc = xyzz_identity();
+    for (uint32_t j = 0; j < e.z; j++) acc = xyzz_add(acc, piece_sums[e.y + j]);
+    buckets[e.x] = acc;
+}
+
+constexpr int RGS = 32;  // threads per row / column sum
+
+// Level 1 of the bucket reduction for B = 256 R buckets per problem, read as
+// an R x 256 matrix S[r][t] = S[256 r + t]: row sums Z_r and column sums U_t.
+// RGS consecutive lanes share one output: each sums a strided slice of it
+// serially (full waves), then a shuffle tree across the RGS lanes.
+__global__ void __launch_bounds__(TPB, 2) k_reduce_rows(const Xyzz *__restrict__ S, uint32_t R,
+                                                      uint32_t nprob, Xyzz *__restrict__ Z,
+                                                      Xyzz *__restrict__ U) {
+    const size_t gt = (size_t)blockIdx.x * TPB + threadIdx.x;
+    const uint32_t per = R + 256;
+    const size_t o = gt / RGS;
+    const uint32_t j = (uint32_t)(gt % RGS);
+    if (o >= (size_t)nprob * per) return;  // whole groups only: RGS divides TPB
+    const uint32_t p = (uint32_t)(o / per), w = (uint32_t)(o % per);
+    const Xyzz *base = S + (size_t)p * R * 256;
+    Xyzz acc = xyzz_identity();
+    if (w < R) {
+        for (uint32_t t = j; t < 256; t += RGS) acc = xyzz_add(acc, base[(size_t)w * 256 + t]);
     } else {
-        const uint32_t t = j - R;
-        Xyzz r = block_sum(S, base + t, TPB, R, lds);
-        if (threadIdx.x == 0) U[(size_t)p * TPB + t] = r;
+        const uint32_t t = w - R;
+        for (uint32_t r = j; r < R; r += RGS) acc = xyzz_add(acc, base[(size_t)r * 256 + t]);
+    }
+#pragma unroll
+    for (int m = 1; m < RGS; m <<= 1) acc = xyzz_add(acc, shfl_xor_xyzz(acc, m));
+    if (j == 0) {
+        if (w < R)
+            Z[(size_t)p * R + w] = acc;
+        else
+            U[(size_t)p * 256 + (w - R)] = acc;
     }
 }
 
 // For m <= 256 items X_0..X_{m-1}: WS = sum_i i X_i and TOT = sum_i X_i, by an
 // LDS suffix scan (Suf_i = sum_{j >= i} X_j; WS = sum_{i >= 1} Suf_i).
-// Block b reads problem b: items src[b * m ... + m), writes out[2b] = WS,
-// out[2b + 1] = TOT.
-__global__ void __launch_bounds__(TPB) k_weighted(const Xyzz *__restrict__ src, uint32_t m,
-                                                Xyzz *__restrict__ out) {
+// Block b < na reads problem b of a (ma items each), block na + b problem b of
+// b_src (mb items each); block b writes out[2b] = WS, out[2b + 1] = TOT.
+__global__ void __launch_bounds__(TPB) k_weighted(const Xyzz *__restrict__ a_src, uint32_t ma,
+                                                uint32_t na, const Xyzz *__restrict__ b_src,
+                                                uint32_t mb, Xyzz *__restrict__ out) {
     __shared__ Xyzz lds[TPB];
     const uint32_t tid = threadIdx.x;
-    const size_t base = (size_t)blockIdx.x * m;
+    const bool first = blockIdx.x < na;
+    const uint32_t m = first ? ma : mb;
+    const Xyzz *src = first ? a_src : b_src;
+    const size_t base = (size_t)(first ? blockIdx.x : blockIdx.x - na) * m;
     lds[tid] = tid < m ? src[base + tid] : xyzz_identity();
     __syncthreads();
     for (uint32_t s = 1; s < m; s <<= 1) {
@@ -358,11 +428,22 @@ void host_batch_to_affine(const Xyzz *in, Affine *out, size_t k) {
     }
 }
 
+namespace {
+std::atomic<int> g_window_override[64];  // by floor(log2 n); 0 = the default rule
+}
+
+void msm_set_window_bits(int log2n, int c) {
+    if (log2n < 0 || log2n >= 64) throw ApgError(-2, "window table: log2 n out of range");
+    if (c != 0 && (c < MSM_MIN_C || c > MSM_MAX_C)) throw ApgError(-2, "window table: width out of range");
+    g_window_override[log2n] = c;
+}
+
 int msm_pick_c(size_t n, size_t columns) {
     (void)columns;
-    if (n < 32) return 3;
     int lg = 0;
-    while ((size_t(1) << (lg + 1)) <= n) lg++;
+    while (lg < 63 && (size_t(1) << (lg + 1)) <= n) lg++;
+    if (const int o = g_window_override[lg]) return o;
+    if (n < 32) return 3;
     int c = lg - 2;
     if (c < 4) c = 4;
     if (c > 17) c = 17;
@@ -444,7 +525,9 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     ctx.sync();
     const uint32_t total = *h_total;
 
-    // 2. radix sort by bucket key, then bucket boundaries
+    // 2. one radix sort of all pairs by bucket key (per-column sorts of the
+    // low key bits measured slower: small onesweep passes lose more than the
+    // saved pass), then bucket boundaries
     int key_bits = 1;
     while ((1ull << key_bits) < (uint64_t)nkeys) key_bits++;
     uint32_t *skeys = ctx.d<uint32_t>("msm.skeys", std::max<uint32_t>(total, 1));
@@ -478,7 +561,7 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     Chunk *desc = ctx.d<Chunk>("msm.desc", std::max<uint32_t>(nchunks, 1));
     if (nchunks)
         hipLaunchKernelGGL(k_chunk_fill, dim3(grid_for(nchunks, TPB)), dim3(TPB), 0, st, off, qoff,
-                           nkeys, nchunks, K1, desc);
+                           nkeys, nchunks, desc);
 
     // 4. accumulate
     Xyzz *csum = ctx.d<Xyzz>("msm.csum", std::max<uint32_t>(nchunks, 1));
@@ -496,17 +579,25 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
 
     // 5. buckets
     Xyzz *buckets = ctx.d<Xyzz>("msm.buckets", nkeys);
-    uint32_t *heavy = ctx.d<uint32_t>("msm.heavy", nkeys + 1);
-    uint32_t *nheavy = heavy + nkeys;
-    HIP_CHECK(hipMemsetAsync(nheavy, 0, sizeof(uint32_t), st));
-    hipLaunchKernelGGL(k_classify, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys, heavy,
-                       nheavy);
-    // heavy buckets (latency-bound LDS trees) run on the side stream while
-    // the light ones (throughput-bound) run here
+    // heavy buckets have > LIGHT_CHUNKS chunks each, so at most max_heavy of
+    // them, cut into at most max_pieces pieces
+    const uint32_t max_heavy = std::min<uint32_t>(nkeys, nchunks / (LIGHT_CHUNKS + 1) + 1);
+    const uint32_t max_pieces = nchunks / PIECE + max_heavy + 1;
+    HeavyPlan *plan = ctx.d<HeavyPlan>("msm.plan", 1);
+    uint4 *heavy = ctx.d<uint4>("msm.heavy", max_heavy);
+    uint2 *piece_of = ctx.d<uint2>("msm.piece_of", max_pieces);
+    Xyzz *piece_sums = ctx.d<Xyzz>("msm.piece_sums", max_pieces);
+    HIP_CHECK(hipMemsetAsync(plan, 0, sizeof(HeavyPlan), st));
+    hipLaunchKernelGGL(k_classify, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys, plan, heavy,
+                       piece_of);
+    // heavy pieces (a few latency-bound blocks) on the side stream, queued
+    // first, beside the light buckets (throughput-bound) here
     HIP_CHECK(hipEventRecord(ctx.ev_fork, st));
     HIP_CHECK(hipStreamWaitEvent(ctx.side, ctx.ev_fork, 0));
-    hipLaunchKernelGGL(k_bucket_heavy, dim3(1024), dim3(TPB), 0, ctx.side, qoff, csum, buckets,
-                       heavy, nheavy);
+    hipLaunchKernelGGL(k_heavy_pieces, dim3(std::min<uint32_t>(max_pieces, 4u * ctx.cu_count)), dim3(TPB), 0,
+                       ctx.side, qoff, csum, plan, heavy, piece_of, piece_sums);
+    hipLaunchKernelGGL(k_heavy_final, dim3(grid_for(max_heavy, TPB)), dim3(TPB), 0, ctx.side, plan, heavy,
+                       piece_sums, buckets);
     HIP_CHECK(hipEventRecord(ctx.ev_join, ctx.side));
     hipLaunchKernelGGL(k_bucket_light, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys,
                        csum, buckets);
@@ -522,13 +613,15 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
         const uint32_t R = B / TPB;
         Xyzz *Z = ctx.d<Xyzz>("msm.Z", (size_t)nprob * R);
         Xyzz *U = ctx.d<Xyzz>("msm.U", (size_t)nprob * TPB);
-        hipLaunchKernelGGL(k_reduce_split, dim3(nprob * (R + TPB)), dim3(TPB), 0, st, buckets, B,
-                           Z, U);
-        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, Z, R, wout);
-        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, U, (uint32_t)TPB,
-                           wout + 2 * nprob);
+        const size_t threads = (size_t)nprob * (R + TPB) * RGS;
+        hipLaunchKernelGGL(k_reduce_rows, dim3(grid_for(threads, TPB)), dim3(TPB), 0, st, buckets, R,
+                           nprob, Z, U);
+        // WS/TOT of the Z_r (blocks 0..nprob) and of the U_t (nprob..2 nprob)
+        hipLaunchKernelGGL(k_weighted, dim3(2 * nprob), dim3(TPB), 0, st, Z, R, nprob, U,
+                           (uint32_t)TPB, wout);
     } else {
-        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, buckets, B, wout);
+        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, buckets, B, nprob, buckets, B,
+                           wout);
     }
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(h_w, wout, (size_t)nprob * 4 * sizeof(Xyzz), hipMemcpyDeviceToHost, st));

@@ -102,6 +102,14 @@ int apg_srs_download(apg_ctx *ctx, const apg_srs *srs, size_t offset, size_t n, 
 size_t apg_srs_len(const apg_srs *srs);
 int apg_srs_prepare(apg_ctx *ctx, apg_srs *srs, size_t n);
 size_t apg_srs_device_bytes(const apg_srs *srs);
+/* Pippenger window width per size class (floor(log2 n)); results never
+ * depend on it. apg_msm_set_window_bits overrides one class (c = 0 restores
+ * the built-in table; 3 <= c <= 17), for the autotuner
+ * (apogee-vm_amd/autotune.py). Replaces the fixed rule of
+ * crates/curve/src/msm.rs:112-118. A new width builds its own SRS table on
+ * the next commit of that class. */
+int apg_msm_set_window_bits(int log2n, int c);
+int apg_msm_window_bits(size_t n);
 void apg_srs_release(apg_srs *srs);
 
 /* ---- MSM over explicit bases.
