@@ -21,9 +21,11 @@ library must be built (``python apogee-vm_amd/build.py``) and a GPU present.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import enum
 import os
+import weakref
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -32,6 +34,21 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 # APG_LIB selects another build of the library (A/B timing of variants)
 LIB_PATH = os.environ.get("APG_LIB") or os.path.join(HERE, "libapogee_gpu.so")
+
+# Live native handles, released at interpreter exit in dependency order
+# (engines, then buffers and SRS tables, then the contexts they belong to)
+# rather than in whatever order the interpreter tears objects down.
+_live = {kind: weakref.WeakSet() for kind in ("engine", "buffer", "srs", "ctx")}
+
+
+@atexit.register
+def _release_all():
+    for kind, method in (("engine", "close"), ("buffer", "free"), ("srs", "release"), ("ctx", "close")):
+        for obj in list(_live[kind]):
+            try:
+                getattr(obj, method)()
+            except Exception:
+                pass
 
 # crates/constants/src/lib.rs:579-770
 TAG_COMMITMENT = 3
@@ -141,6 +158,7 @@ def _declare(L):
     d("apg_debug_mul32_batch", [I32, VP, VP, SZ, VP])
     d("apg_synthetic_fill_dev", [VP, U64, I32, SZ, VP])
     d("apg_engine_create", [I32, I32, SZ, C.POINTER(VP)])
+    d("apg_engine_create_lanes", [I32, I32, SZ, I32, I32, C.c_double, C.POINTER(VP)])
     d("apg_engine_destroy", [VP], None)
     d("apg_engine_submit_shard", [VP, VP, VP, SZ, SZ, I32, VP, C.POINTER(C.c_uint64)])
     d("apg_engine_poll", [VP, C.c_uint64])
@@ -243,6 +261,7 @@ class Context:
         h = VP()
         _check(lib().apg_ctx_create(device, C.byref(h)))
         self.h = h
+        _live["ctx"].add(self)
 
     def close(self):
         if self.h:
@@ -289,6 +308,7 @@ class DeviceBuffer:
         p = VP()
         _check(lib().apg_dev_alloc(ctx.h, nbytes, C.byref(p)))
         self.ptr = p
+        _live["buffer"].add(self)
 
     @classmethod
     def from_array(cls, ctx: Context, a: np.ndarray) -> "DeviceBuffer":
@@ -310,7 +330,7 @@ class DeviceBuffer:
         _check(lib().apg_dev_copy(self.ctx.h, self.ptr, src.ptr, nbytes))
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             _check(lib().apg_dev_free(self.ctx.h, self.ptr))
             self.ptr = None
 
@@ -402,6 +422,7 @@ class Srs:
     def __init__(self, handle, ctx: Context):
         self.h = handle
         self.ctx = ctx
+        _live["srs"].add(self)
 
     @classmethod
     def upload(cls, bases, ctx: Optional[Context] = None) -> "Srs":
@@ -728,11 +749,22 @@ class Engine:
     concurrent HIP streams, admission against an HBM budget in arrival order
     or shortest first (set_policy)."""
 
-    def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 200 << 30):
+    def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 200 << 30, lane_workers: int = 0,
+                 lane_cus: int = 0, small_scalars: float = float(1 << 18)):
+        """lane_workers > 0: those workers are a fast lane for jobs of at most
+        small_scalars rows x columns (shortest-first), on CUs [0, lane_cus)
+        apart from the rest (apg_engine_create_lanes). Engine streams run
+        concurrently only with enough hardware queues: GPU_MAX_HW_QUEUES >=
+        2 x workers in the environment before HIP starts."""
         h = VP()
-        _check(lib().apg_engine_create(device, workers, hbm_budget, C.byref(h)))
+        if lane_workers > 0:
+            _check(lib().apg_engine_create_lanes(device, workers, hbm_budget, lane_workers, lane_cus,
+                                                 float(small_scalars), C.byref(h)))
+        else:
+            _check(lib().apg_engine_create(device, workers, hbm_budget, C.byref(h)))
         self.h = h
         self._jobs = {}
+        _live["engine"].add(self)
 
     def close(self):
         if self.h:

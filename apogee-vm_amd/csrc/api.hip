@@ -59,6 +59,11 @@ static int guarded(F &&f) {
     }
 }
 
+// A null handle is an argument error, never a dereference.
+static inline void need(const void *handle) {
+    if (!handle) throw ApgError(APG_ERR_ARG, "null handle");
+}
+
 static inline Fe load_fe(const uint8_t *p) {
     Fe r;
     std::memcpy(r.l, p, 32);
@@ -277,10 +282,12 @@ void apg_ctx_destroy(apg_ctx *ctx) {
 
 int apg_ctx_synchronize(apg_ctx *ctx) {
     return guarded([&] { ctx->c.sync(); });
+        need(ctx);
 }
 
 int apg_ctx_set_profiling(apg_ctx *ctx, int on) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.profiling = on != 0;
     });
@@ -301,6 +308,7 @@ static void export_stats(const Stats &s, apg_stats *out) {
 
 int apg_ctx_stats(apg_ctx *ctx, apg_stats *out) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         export_stats(ctx->c.stats, out);
     });
@@ -308,6 +316,7 @@ int apg_ctx_stats(apg_ctx *ctx, apg_stats *out) {
 
 int apg_ctx_reset_stats(apg_ctx *ctx) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         ctx->c.stats = Stats();
     });
@@ -317,6 +326,7 @@ extern "C" int apg_synthetic_fill_dev_impl(void *stream, uint64_t seed, int kind
 
 int apg_synthetic_fill_dev(apg_ctx *ctx, uint64_t seed, int kind, size_t n, void *out_dev) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         const int rc = apg_synthetic_fill_dev_impl(ctx->c.stream, seed, kind, n, out_dev);
@@ -327,18 +337,21 @@ int apg_synthetic_fill_dev(apg_ctx *ctx, uint64_t seed, int kind, size_t n, void
 
 int apg_dev_alloc(apg_ctx *ctx, size_t bytes, void **out) {
     return guarded([&] {
+        need(ctx);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess) throw ApgError(APG_ERR_OOM, "device out of memory");
     });
 }
 int apg_dev_free(apg_ctx *ctx, void *p) {
     return guarded([&] {
+        need(ctx);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         HIP_CHECK(hipFree(p));
     });
 }
 int apg_dev_upload(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
         ctx->c.sync();
@@ -346,6 +359,7 @@ int apg_dev_upload(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }
 int apg_dev_copy(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return guarded([&] {
+        need(ctx);
         if (bytes % 16) throw ApgError(APG_ERR_ARG, "apg_dev_copy: bytes must be a multiple of 16");
         if (!bytes) return;
         std::lock_guard<std::mutex> lk(ctx->c.mu);
@@ -360,6 +374,7 @@ int apg_dev_copy(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
 }
 int apg_dev_download(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
         ctx->c.sync();
@@ -368,6 +383,7 @@ int apg_dev_download(apg_ctx *ctx, void *dst, const void *src, size_t bytes) {
 
 int apg_srs_upload(apg_ctx *ctx, const uint8_t *bases_mont, size_t n, apg_srs **out) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         auto s = std::make_unique<apg_srs>();
@@ -383,6 +399,7 @@ int apg_srs_upload(apg_ctx *ctx, const uint8_t *bases_mont, size_t n, apg_srs **
 
 int apg_srs_generate(apg_ctx *ctx, const uint8_t tau_mont[32], size_t n, apg_srs **out) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         auto s = std::make_unique<apg_srs>();
@@ -402,6 +419,7 @@ int apg_srs_generate(apg_ctx *ctx, const uint8_t tau_mont[32], size_t n, apg_srs
 
 int apg_srs_download(apg_ctx *ctx, const apg_srs *srs, size_t offset, size_t n, uint8_t *out) {
     return guarded([&] {
+        need(ctx);
         if (offset + n > srs->n) throw ApgError(APG_ERR_ARG, "SRS range out of bounds");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipMemcpyAsync(out, srs->base + offset, n * 64, hipMemcpyDeviceToHost, ctx->c.stream));
@@ -426,6 +444,7 @@ int apg_msm_window_bits(size_t n) { return msm_pick_c(n, 1); }
 
 int apg_srs_prepare(apg_ctx *ctx, apg_srs *srs, size_t n) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         if (n) srs_table(ctx->c, srs, n);
@@ -437,6 +456,7 @@ void apg_srs_release(apg_srs *srs) { delete srs; }
 int apg_msm_g1_fr(apg_ctx *ctx, const uint8_t *bases_mont, const uint8_t *scalars_mont, size_t n,
                   uint8_t out_mont[64]) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         if (n == 0) {
@@ -461,6 +481,7 @@ int apg_msm_g1_fr(apg_ctx *ctx, const uint8_t *bases_mont, const uint8_t *scalar
 int apg_msm_g1_u32(apg_ctx *ctx, const uint8_t *bases_mont, const uint32_t *scalars, size_t n,
                    uint8_t out_mont[64]) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         if (n == 0) {
@@ -485,6 +506,7 @@ int apg_msm_g1_u32(apg_ctx *ctx, const uint8_t *bases_mont, const uint32_t *scal
 int apg_commit_fr(apg_ctx *ctx, apg_srs *srs, const uint8_t *scalars_mont, size_t n, size_t k,
                   uint8_t *out_mont) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         Context &c = ctx->c;
@@ -497,6 +519,7 @@ int apg_commit_fr(apg_ctx *ctx, apg_srs *srs, const uint8_t *scalars_mont, size_
 int apg_commit_u32(apg_ctx *ctx, apg_srs *srs, const uint32_t *scalars, size_t n, size_t k,
                    uint8_t *out_mont) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         Context &c = ctx->c;
@@ -509,6 +532,7 @@ int apg_commit_u32(apg_ctx *ctx, apg_srs *srs, const uint32_t *scalars, size_t n
 int apg_commit_fr_dev(apg_ctx *ctx, apg_srs *srs, const void *scalars_dev, size_t n, size_t k,
                       uint8_t *out_mont) {
     return guarded([&] {
+        need(ctx);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         commit_device(ctx->c, srs, static_cast<const Fe *>(scalars_dev), nullptr, n, k, out_mont);
@@ -520,6 +544,7 @@ int apg_sumcheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
                        const uint8_t *eq_point_mont, apg_round_fn round_fn, void *user,
                        uint8_t *rounds_out, uint8_t *point_out, uint8_t *final_evals_out) {
     return guarded([&] {
+        need(ctx);
         if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
@@ -535,6 +560,7 @@ int apg_sumcheck_prove_dev(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
                            const uint8_t *eq_point_mont, apg_round_fn round_fn, void *user,
                            uint8_t *rounds_out, uint8_t *point_out, uint8_t *final_evals_out) {
     return guarded([&] {
+        need(ctx);
         if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
@@ -550,6 +576,7 @@ int apg_zerocheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
                         const uint8_t *const *tables_mont, size_t k, size_t num_vars,
                         apg_transcript *t, uint8_t *rounds_out, uint8_t *final_evals_out) {
     return guarded([&] {
+        need(ctx);
         if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
@@ -574,6 +601,7 @@ int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, siz
                         size_t k, int degree, apg_transcript *t, uint8_t *commitments_wire,
                         uint8_t *rounds_out, uint8_t *final_evals_out) {
     return guarded([&] {
+        need(ctx);
         check_shard_args(num_vars, k, degree);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
@@ -590,6 +618,22 @@ int apg_engine_create(int device, int workers, size_t hbm_budget_bytes, apg_engi
         if (workers < 1 || workers > 64) throw ApgError(APG_ERR_ARG, "workers must be 1..64");
         auto *e = new apg_engine();
         e->e = std::make_unique<Engine>(device, workers, hbm_budget_bytes);
+        *out = e;
+    });
+}
+
+int apg_engine_create_lanes(int device, int workers, size_t hbm_budget_bytes, int lane_workers, int lane_cus,
+                            double small_scalars, apg_engine **out) {
+    return guarded([&] {
+        int ndev = 0;
+        HIP_CHECK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw ApgError(APG_ERR_ARG, "no such device");
+        if (workers < 2 || workers > 64) throw ApgError(APG_ERR_ARG, "workers must be 2..64");
+        if (lane_workers < 1 || lane_workers >= workers) throw ApgError(APG_ERR_ARG, "lane_workers must be 1..workers-1");
+        if (lane_cus < 0) throw ApgError(APG_ERR_ARG, "lane_cus must be >= 0");
+        auto *e = new apg_engine();
+        e->e = std::make_unique<Engine>(device, workers, hbm_budget_bytes, lane_workers, lane_cus);
+        e->e->set_policy(Policy::kShortest, lane_workers, small_scalars);
         *out = e;
     });
 }

@@ -37,14 +37,17 @@ struct EngineStats {
 // Which queued job a free worker takes.
 //   kFifo:     arrival order; the head waits until its HBM estimate fits.
 //   kShortest: the smallest job (by work) whose estimate fits, so short shards
-//              do not queue behind long ones. With `reserve` > 0, jobs above
-//              `small_work` may hold at most workers - reserve workers, which
-//              keeps that many free for short shards arriving later.
+//              do not queue behind long ones.
+// With `reserve` > 0 the first `reserve` workers are a fast lane: they take
+// only jobs of at most `small_work`, and the other workers leave such jobs to
+// an idle fast-lane worker. Built with lane_cus > 0, the fast lane's streams
+// run on CUs [0, lane_cus) and the rest on the other CUs, so a short shard's
+// kernels never wait for a slot behind a long shard's.
 enum class Policy { kFifo = 0, kShortest = 1 };
 
 class Engine {
    public:
-    Engine(int device, int workers, size_t hbm_budget);
+    Engine(int device, int workers, size_t hbm_budget, int lane_workers = 0, int lane_cus = 0);
     ~Engine();
     uint64_t submit(std::unique_ptr<Job> job);
     std::unique_ptr<Job> wait(uint64_t id);
@@ -60,14 +63,14 @@ class Engine {
 
    private:
     void run(int w);
-    // index into queue_ of the job a free worker takes now, or -1
-    long pick_locked() const;
+    // index into queue_ of the job free worker w takes now, or -1
+    long pick_locked(int w) const;
     int device_;
     size_t budget_;
     Policy policy_ = Policy::kFifo;
     int reserve_ = 0;
     double small_work_ = 0;
-    size_t running_large_ = 0;
+    int idle_fast_ = 0;  // fast-lane workers waiting for a job
     Stats kstats_;
     std::vector<std::unique_ptr<Context>> ctxs_;
     std::vector<std::thread> threads_;

@@ -3,15 +3,20 @@
 // The reference proves shards with `max_in_flight` scoped worker threads that
 // claim shards as the streaming pipeline emits them
 // (crates/prover/src/streaming.rs:601-680, default 8 in
-// tools/bench/src/block.rs:129). Here each worker owns a HIP stream and a
-// scratch arena (a Context); jobs wait in one queue and are admitted when a
-// worker is free and the job's HBM estimate fits the budget beside the jobs
-// already running, in arrival order or shortest first (Policy). Workers run
-// concurrently, so one shard's latency-bound
-// phases (bucket-reduction trees, sumcheck rounds waiting on the host
-// transcript) overlap other shards' throughput-bound kernels. A job's result
-// depends only on its own inputs and transcript: the same bytes alone or in
-// any mix.
+// tools/bench/src/block.rs:129). Here each worker owns one HIP stream (one
+// hardware queue) and a scratch arena (a Context); jobs wait in one queue and
+// are admitted when a worker is free and the job's HBM estimate fits the
+// budget beside the jobs already running, in arrival order or shortest first
+// (Policy). Workers run concurrently, so one shard's latency-bound phases
+// (bucket-reduction trees, sumcheck rounds waiting on the host transcript)
+// overlap other shards' throughput-bound kernels.
+//
+// Latency: the device services about five of a process's busy hardware queues
+// at once and time-slices the rest in ~1 ms slices, so a serving engine keeps
+// workers + other busy streams within that (four workers measured best), and
+// a fast lane (workers on their own CUs, short jobs only) keeps short shards
+// from waiting for execution slots behind long ones. A job's result depends
+// only on its own inputs and transcript: the same bytes alone or in any mix.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -34,13 +39,26 @@ double now_ms() {
 }
 }  // namespace
 
-Engine::Engine(int device, int workers, size_t hbm_budget) : device_(device), budget_(hbm_budget) {
+Engine::Engine(int device, int workers, size_t hbm_budget, int lane_workers, int lane_cus)
+    : device_(device), budget_(hbm_budget) {
     if (workers < 1) workers = 1;
+    lane_workers = std::max(0, std::min(lane_workers, workers - 1));
+    std::vector<uint32_t> fast_mask, slow_mask;
+    if (lane_workers > 0 && lane_cus > 0) {
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        const int ncu = prop.multiProcessorCount;
+        lane_cus = std::min(lane_cus, ncu - 1);
+        fast_mask.assign((ncu + 31) / 32, 0u);
+        slow_mask.assign((ncu + 31) / 32, 0u);
+        for (int cu = 0; cu < ncu; cu++) (cu < lane_cus ? fast_mask : slow_mask)[cu / 32] |= 1u << (cu % 32);
+    }
     ctxs_.resize(workers);
     for (int w = 0; w < workers; w++) {
         ctxs_[w] = std::make_unique<Context>();
-        context_init(*ctxs_[w], device);
+        context_init(*ctxs_[w], device, w < lane_workers ? fast_mask : slow_mask, /*side_stream=*/false);
     }
+    reserve_ = lane_workers;
     for (int w = 0; w < workers; w++) threads_.emplace_back([this, w] { run(w); });
 }
 
@@ -123,16 +141,17 @@ void Engine::reset_kernel_stats() {
     kstats_ = Stats();
 }
 
-long Engine::pick_locked() const {
-    if (queue_.empty()) return -1;
-    if (policy_ == Policy::kFifo)
-        return in_use_ + queue_.front()->mem_bytes <= budget_ ? 0 : -1;
-    const bool large_ok = running_large_ + reserve_ < ctxs_.size();
+long Engine::pick_locked(int w) const {
+    const bool fast = w < reserve_;
     long best = -1;
     for (size_t i = 0; i < queue_.size(); i++) {
         const Job &j = *queue_[i];
+        const bool small = j.work <= small_work_;
+        if (fast && !small) continue;                      // the fast lane runs short jobs only
+        if (!fast && small && reserve_ > 0 && idle_fast_ > 0) continue;  // left to the fast lane
+        if (policy_ == Policy::kFifo)  // this lane's oldest job, once it fits
+            return in_use_ + j.mem_bytes <= budget_ ? (long)i : -1;
         if (in_use_ + j.mem_bytes > budget_) continue;
-        if (!large_ok && j.work > small_work_) continue;
         if (best < 0 || j.work < queue_[best]->work) best = (long)i;
     }
     return best;
@@ -143,25 +162,27 @@ void Engine::run(int w) {
     HIP_CHECK(hipSetDevice(device_));
     for (;;) {
         std::unique_ptr<Job> job;
-        bool large = false;
         {
             std::unique_lock<std::mutex> lk(mu_);
             long at = -1;
+            const bool fast = w < reserve_;
+            if (fast) idle_fast_++;
             // queued jobs still run after stop_; every queued job fits an
-            // idle engine (submit refuses the rest), so the queue drains
+            // idle engine (submit refuses the rest) and the non-fast workers
+            // take any job, so the queue drains
             cv_.wait(lk, [&] {
-                at = pick_locked();
+                at = pick_locked(w);
                 return at >= 0 || (stop_ && queue_.empty());
             });
+            if (fast) idle_fast_--;
             if (at < 0) return;
             job = std::move(queue_[at]);
             queue_.erase(queue_.begin() + at);
             in_use_ += job->mem_bytes;
             peak_ = std::max(peak_, in_use_);
             running_++;
-            large = job->work > small_work_;
-            if (large) running_large_++;
         }
+        cv_.notify_all();  // idle_fast_ changed: others may now take a short job
         job->t_start = now_ms();
         job->worker = w;
         try {
@@ -181,7 +202,6 @@ void Engine::run(int w) {
             ctx.stats = Stats();
             in_use_ -= job->mem_bytes;
             running_--;
-            if (large) running_large_--;
             completed_++;
             done_[job->id] = std::move(job);
         }

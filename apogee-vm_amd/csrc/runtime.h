@@ -32,13 +32,17 @@ void set_last_error(const std::string &msg);
                                           hipGetErrorString(e_));                            \
     } while (0)
 
-// A device buffer that only grows. Reused across calls on one context.
+// A device buffer that only grows. Reused across calls on one context. A
+// buffer it outgrows is retired, not freed: hipFree waits for the whole
+// device, which would stall this context behind every other stream's work
+// (other engine workers' long kernels). Retired buffers are freed when the
+// context is released.
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
-    void *get(size_t bytes) {
+    void *get(size_t bytes, std::vector<void *> &retired) {
         if (bytes > cap) {
-            if (ptr) HIP_CHECK(hipFree(ptr));
+            if (ptr) retired.push_back(ptr);
             ptr = nullptr;
             size_t c = bytes + bytes / 4 + 256;
             HIP_CHECK(hipMalloc(&ptr, c));
@@ -54,9 +58,9 @@ struct DevBuf {
 struct HostBuf {
     void *ptr = nullptr;
     size_t cap = 0;
-    void *get(size_t bytes) {
+    void *get(size_t bytes, std::vector<void *> &retired) {
         if (bytes > cap) {
-            if (ptr) HIP_CHECK(hipHostFree(ptr));
+            if (ptr) retired.push_back(ptr);
             ptr = nullptr;
             size_t c = bytes + bytes / 4 + 256;
             HIP_CHECK(hipHostMalloc(&ptr, c, hipHostMallocDefault));
@@ -108,25 +112,39 @@ struct Context {
     std::mutex mu;  // one call at a time per context
     std::unordered_map<std::string, DevBuf> dev;
     std::unordered_map<std::string, HostBuf> host;
+    std::vector<void *> retired_dev, retired_host;  // outgrown buffers (see DevBuf)
     int cu_count = 256;
 
     template <class T>
     T *d(const char *name, size_t count) {
-        return static_cast<T *>(dev[name].get(count * sizeof(T)));
+        return static_cast<T *>(dev[name].get(count * sizeof(T), retired_dev));
     }
     template <class T>
     T *h(const char *name, size_t count) {
-        return static_cast<T *>(host[name].get(count * sizeof(T)));
+        return static_cast<T *>(host[name].get(count * sizeof(T), retired_host));
     }
     void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
 };
 
-// Streams (default priority, non-blocking) and fork/join events of a context.
-inline void context_init(Context &c, int device) {
+// Streams (default priority; non-blocking, or restricted to the CUs set in
+// cu_mask, 32 CUs per word) and fork/join events of a context. With
+// side_stream = false the side work runs in order on the one stream: engine
+// workers use one stream each, because the device time-slices a process's
+// hardware queues once more than about five are busy, and a slice is ~1 ms.
+inline void context_init(Context &c, int device, const std::vector<uint32_t> &cu_mask = {},
+                         bool side_stream = true) {
     c.device = device;
     HIP_CHECK(hipSetDevice(device));
-    HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-    HIP_CHECK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    if (cu_mask.empty())
+        HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    else
+        HIP_CHECK(hipExtStreamCreateWithCUMask(&c.stream, (uint32_t)cu_mask.size(), cu_mask.data()));
+    if (!side_stream)
+        c.side = c.stream;
+    else if (cu_mask.empty())
+        HIP_CHECK(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    else
+        HIP_CHECK(hipExtStreamCreateWithCUMask(&c.side, (uint32_t)cu_mask.size(), cu_mask.data()));
     HIP_CHECK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
     hipDeviceProp_t prop;
@@ -140,9 +158,13 @@ inline void context_release(Context &c) {
     if (c.side) (void)hipStreamSynchronize(c.side);
     c.dev.clear();
     c.host.clear();
+    for (void *p : c.retired_dev) (void)hipFree(p);
+    for (void *p : c.retired_host) (void)hipHostFree(p);
+    c.retired_dev.clear();
+    c.retired_host.clear();
     if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
     if (c.ev_join) (void)hipEventDestroy(c.ev_join);
-    if (c.side) (void)hipStreamDestroy(c.side);
+    if (c.side && c.side != c.stream) (void)hipStreamDestroy(c.side);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = c.side = nullptr;
     c.ev_fork = c.ev_join = nullptr;

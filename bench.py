@@ -25,7 +25,7 @@ the oracle's restatement of the reference algorithms timed on a sample of the
 batch's shards on the host cores, whose outputs are checked against the GPU's.
 
     python bench.py [--workload mixed|shard|serve|copy] [--steps K] [--warmup W]
-                    [--workers N] [--policy fifo|shortest] [--reserve R]
+                    [--workers N] [--policy fifo|shortest] [--reserve R] [--lane-cus C]
 """
 from __future__ import annotations
 
@@ -40,6 +40,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# One hardware queue per engine worker stream (plus this process's context):
+# with HIP's default of 4, streams share in-order queues and a short shard's
+# kernels wait behind a long shard's. Read once, when HIP starts.
+os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "shards/s (commit MSM+sumcheck) on mixed 2^8–2^22 batch; G1 MSM pts/s; p99 lat"
 # v_mad_u64_u32 issues at half rate on gfx950: 256 CUs x 4 SIMDs x 16 lanes / 2
@@ -255,10 +259,15 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve", "copy"])
-    ap.add_argument("--workers", type=int, default=8)
-    ap.add_argument("--policy", default="fifo", choices=["fifo", "shortest"])
-    ap.add_argument("--reserve", type=int, default=0, help="workers kept for shards below --small-scalars")
-    ap.add_argument("--small-scalars", type=float, default=float(1 << 18))
+    # engine defaults: throughput (mixed) 8 workers in arrival order; serving
+    # 4 workers (within the hardware queues the device services at once), one
+    # of them a fast lane on 32 CUs, shortest job first
+    ap.add_argument("--workers", type=int, default=None)
+    ap.add_argument("--policy", default=None, choices=["fifo", "shortest"])
+    ap.add_argument("--reserve", type=int, default=None,
+                    help="fast-lane workers, kept for shards of at most --small-scalars rows x columns")
+    ap.add_argument("--lane-cus", type=int, default=None, help="CUs of the fast lane (0: no CU split)")
+    ap.add_argument("--small-scalars", type=float, default=float(1 << 20))
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work (0 = skip)")
     args = ap.parse_args()
@@ -266,6 +275,11 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(the path would shard across GPUs as described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    serving = args.workload == "serve"
+    for name, mixed_default, serve_default in (("workers", 8, 4), ("policy", "fifo", "shortest"),
+                                               ("reserve", 0, 1), ("lane_cus", 0, 32)):
+        if getattr(args, name) is None:
+            setattr(args, name, serve_default if serving else mixed_default)
     apg = importlib.import_module("apogee-vm_amd")
     wl = importlib.import_module("apogee-vm_amd.workload")
     ctx = apg.Context(0)
@@ -295,7 +309,8 @@ def main():
     log("plain path: one shard per call, sequential")
     t_plain, plain, iso = run_plain(ctx, srs, specs, bufs)
     log(f"plain: {len(specs) / t_plain:.1f} shards/s ({t_plain:.3f} s)")
-    eng = apg.Engine(0, workers=args.workers, hbm_budget=200 << 30)
+    eng = apg.Engine(0, workers=args.workers, hbm_budget=200 << 30, lane_workers=args.reserve,
+                     lane_cus=args.lane_cus, small_scalars=args.small_scalars)
     eng.set_policy(args.policy, args.reserve, args.small_scalars)
     eng.set_profiling(True)
 
@@ -367,7 +382,10 @@ def main():
 
 def serve(eng, srs, specs, bufs, plain, iso, args):
     """Poisson arrivals at 0.5x, 1x, 1.5x the batched capacity."""
-    dt, _ = run_engine_pass(eng, srs, specs, bufs)  # capacity probe (and warmup)
+    eng.set_profiling(False)
+    for _ in range(max(0, args.warmup)):  # workers' scratch grows to its steady state
+        run_engine_pass(eng, srs, specs, bufs)
+    dt, _ = run_engine_pass(eng, srs, specs, bufs)  # capacity probe
     capacity = len(specs) / dt
     log(f"batched capacity {capacity:.1f} shards/s")
     iso_ms = iso * 1e3
@@ -386,12 +404,13 @@ def serve(eng, srs, specs, bufs, plain, iso, args):
             s = specs[i]
             ids.append((i, eng.submit_shard(srs, bufs[i], s.num_vars, s.columns, s.degree)))
             peak = max(peak, eng.stats()["hbm_in_use"])
-        lats, ratios = [], []
+        lats, ratios, detail = [], [], []
         for i, jid in ids:
             p, _, st = eng.wait(jid)
             exact = exact and same(plain[i], p)
             lats.append(st["total_ms"])
             ratios.append(st["total_ms"] / iso_ms[i])
+            detail.append((ratios[-1], i, st))
         wall = time.perf_counter() - t0
         lats, ratios = np.array(lats), np.array(ratios)
         rows.append({
@@ -401,6 +420,10 @@ def serve(eng, srs, specs, bufs, plain, iso, args):
             "latency_over_isolated": {q: round(float(np.percentile(ratios, p)), 3) for q, p in
                                       (("p50", 50), ("p99", 99))},
             "hbm_admitted_peak_gib": round(peak / 2**30, 3),
+            "worst": [{"rows_log2": specs[i].num_vars, "columns": specs[i].columns,
+                       "isolated_ms": round(float(iso_ms[i]), 3), "queue_ms": round(st["queue_ms"], 3),
+                       "run_ms": round(st["run_ms"], 3), "over_isolated": round(r, 2)}
+                      for r, i, st in sorted(detail, key=lambda t: -t[0])[:5]],
         })
         log(f"load {mult}: {rows[-1]['shards_per_s']} shards/s, p99 {rows[-1]['latency_ms']['p99']} ms")
     info = eng.stats()
@@ -410,7 +433,7 @@ def serve(eng, srs, specs, bufs, plain, iso, args):
             "data": "synthetic, seeded Poisson arrivals",
             "config": {"workload": "served stream of the mixed batch's shards", "parallelism": "1 GPU",
                        "engine_workers": args.workers, "engine_policy": args.policy,
-                       "reserve": args.reserve, "small_scalars": args.small_scalars, "hbm_budget_gib": 200},
+                       "reserve": args.reserve, "lane_cus": args.lane_cus, "small_scalars": args.small_scalars, "hbm_budget_gib": 200},
             "capacity_shards_per_s": round(capacity, 2), "serve": rows, "bit_exact_vs_plain": bool(exact),
             "hbm": {"srs_gib": round(srs.device_bytes() / 2**30, 3),
                     "engine_arena_gib": round(info["arena_bytes"] / 2**30, 3)}}

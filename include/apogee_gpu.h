@@ -219,12 +219,20 @@ int apg_engine_wait(apg_engine *e, uint64_t job_id, uint8_t *commitments_wire, u
 int apg_engine_stats(apg_engine *e, apg_engine_info *out);
 /* Queue policy: APG_POLICY_FIFO (arrival order, the default) or
  * APG_POLICY_SHORTEST (smallest rows x columns first among jobs whose HBM
- * estimate fits). With reserve > 0, jobs above small_scalars rows x columns
- * hold at most workers - reserve workers at once. No counterpart in the
- * reference, whose workers claim shards in emission order
+ * estimate fits). With reserve > 0 the first `reserve` workers are a fast
+ * lane that takes only jobs of at most small_scalars rows x columns; the
+ * other workers leave such jobs to an idle fast-lane worker. No counterpart
+ * in the reference, whose workers claim shards in emission order
  * (crates/prover/src/streaming.rs:601-680). */
 enum { APG_POLICY_FIFO = 0, APG_POLICY_SHORTEST = 1 };
 int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_scalars);
+/* An engine whose first lane_workers workers are a fast lane for jobs of at
+ * most small_scalars rows x columns (shortest-first policy), with their
+ * streams on CUs [0, lane_cus) and every other worker's streams on the
+ * remaining CUs (lane_cus = 0: no CU split), so a short shard's kernels never
+ * wait for execution slots behind a long shard's. */
+int apg_engine_create_lanes(int device, int workers, size_t hbm_budget_bytes, int lane_workers, int lane_cus,
+                            double small_scalars, apg_engine **out);
 /* kernel timing (apg_stats) on every worker's stream, summed as jobs finish */
 int apg_engine_set_profiling(apg_engine *e, int on);
 int apg_engine_kernel_stats(apg_engine *e, apg_stats *out);

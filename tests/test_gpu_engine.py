@@ -59,6 +59,27 @@ def test_engine_matches_plain_path_in_the_mix(apg, setup):
     assert info["completed"] == len(specs) and info["hbm_in_use"] == 0 and info["hbm_peak"] > 0
 
 
+def test_engine_lanes_match_plain_path(apg, setup):
+    """A fast lane on its own CUs (short jobs only, shortest first) beside
+    the other workers: every shard's bytes are the plain path's, whichever
+    lane and queue order ran it."""
+    specs, srs, bufs = setup
+    plain = [_plain(apg, srs, s, b) for s, b in zip(specs, bufs)]
+    small = float(np.median([s.elements for s in specs]))
+    eng = apg.Engine(0, workers=3, hbm_budget=8 << 30, lane_workers=1, lane_cus=32, small_scalars=small)
+    ids = [eng.submit_shard(srs, b, s.num_vars, s.columns, s.degree) for s, b in zip(specs, bufs)]
+    workers = set()
+    for (p, st), jid in zip(plain, ids):
+        q, t, stats = eng.wait(jid)
+        assert q.commitments == p.commitments and np.array_equal(q.rounds, p.rounds)
+        assert t.state() == st
+        workers.add(stats["worker"])
+    assert 0 in workers  # the fast lane ran some of them
+    eng.set_policy("fifo", 1, small)
+    q, _, _ = eng.wait(eng.submit_shard(srs, bufs[0], specs[0].num_vars, specs[0].columns, specs[0].degree))
+    assert q.commitments == plain[0][0].commitments
+
+
 def test_engine_matches_oracle(apg, orc, setup):
     specs, srs, bufs = setup
     s, b = specs[-1], bufs[-1]
