@@ -132,6 +132,8 @@ def _declare(L):
     d("apg_sumcheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_sumcheck_prove_dev", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_zerocheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, VP, VP])
+    d("apg_witness_digest", [VP, SZ, SZ, VP])
+    d("apg_witness_digest_dev", [VP, VP, SZ, SZ, VP])
     d("apg_prove_shard_dev", [VP, VP, VP, SZ, SZ, I32, VP, VP, VP, VP])
     d("apg_transcript_new", [C.POINTER(VP)])
     d("apg_transcript_free", [VP], None)
@@ -692,6 +694,26 @@ def prove_sumcheck(eq_point, gate: Gate, tables, transcript: Transcript, ctx: Op
     _check(lib().apg_sumcheck_prove(ctx.h, terms, len(gate.terms), ptrs, len(arrs), n, _p(eqp), cb,
                                     transcript.h, _p(rounds), _p(point), _p(finals)))
     return rounds[:n], point[:n], finals
+
+
+def witness_digest(columns) -> np.ndarray:
+    """sumcheck::witness_digest (crates/sumcheck/src/lib.rs:228-256): the
+    Poseidon2 sponge over every cell (host, sequential). Returns one Fr."""
+    if len(columns) == 0:
+        raise ValueError("witness_digest: a witness has at least one column")
+    arrs, ptrs = _tables_host(columns)
+    n = int(arrs[0].shape[0]).bit_length() - 1
+    for k, a in enumerate(arrs):
+        if a.shape[0] != 1 << n:
+            raise ValueError(f"witness_digest: column {k} has {a.shape[0]} cells, column 0 has {1 << n}")
+    out = np.zeros(4, dtype=np.uint64)
+    _check(lib().apg_witness_digest(ptrs, len(arrs), n, _p(out)))
+    return out
+
+
+def absorb_witness_digest(t: Transcript, digest) -> None:
+    """crates/sumcheck/src/lib.rs:262: append_scalars(WITNESS_DIGEST, [digest])."""
+    t.append_scalars(TAG_WITNESS_DIGEST, np.asarray(digest, dtype=np.uint64).reshape(1, 4))
 
 
 def prove_zerocheck(gate: Gate, polys, transcript: Transcript, ctx: Optional[Context] = None) -> SumcheckProof:

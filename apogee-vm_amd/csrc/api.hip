@@ -572,6 +572,43 @@ int apg_sumcheck_prove_dev(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
     });
 }
 
+static void witness_digest_host(const Fe *const *cols, size_t k, size_t num_vars, uint8_t out[32]) {
+    if (k == 0) throw ApgError(APG_ERR_ARG, "witness_digest: a witness has at least one column");
+    if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+    Transcript s;
+    const Fe hdr[2] = {f_from_u64<Fr>(k), f_from_u64<Fr>(num_vars)};
+    s.append_scalars(TAG_WITNESS_DIGEST, hdr, 2);
+    for (size_t c = 0; c < k; c++) s.append_scalars(TAG_WITNESS_DIGEST, cols[c], size_t(1) << num_vars);
+    store_fe(out, s.sample());
+}
+
+int apg_witness_digest(const uint8_t *const *columns_mont, size_t k, size_t num_vars, uint8_t out_mont[32]) {
+    return guarded([&] {
+        std::vector<const Fe *> cols(k);
+        for (size_t c = 0; c < k; c++) cols[c] = reinterpret_cast<const Fe *>(columns_mont[c]);
+        witness_digest_host(cols.data(), k, num_vars, out_mont);
+    });
+}
+
+int apg_witness_digest_dev(apg_ctx *ctx, const void *columns_dev, size_t k, size_t num_vars,
+                           uint8_t out_mont[32]) {
+    return guarded([&] {
+        need(ctx);
+        if (k == 0) throw ApgError(APG_ERR_ARG, "witness_digest: a witness has at least one column");
+        if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        const size_t n = size_t(1) << num_vars;
+        std::vector<Fe> host(k * n);
+        HIP_CHECK(hipMemcpyAsync(host.data(), columns_dev, k * n * sizeof(Fe), hipMemcpyDeviceToHost,
+                                 ctx->c.stream));
+        ctx->c.sync();
+        std::vector<const Fe *> cols(k);
+        for (size_t c = 0; c < k; c++) cols[c] = host.data() + c * n;
+        witness_digest_host(cols.data(), k, num_vars, out_mont);
+    });
+}
+
 int apg_zerocheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
                         const uint8_t *const *tables_mont, size_t k, size_t num_vars,
                         apg_transcript *t, uint8_t *rounds_out, uint8_t *final_evals_out) {

@@ -173,6 +173,18 @@ int apg_zerocheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
                         const uint8_t *const *tables_mont, size_t k, size_t num_vars,
                         apg_transcript *t, uint8_t *rounds_out, uint8_t *final_evals_out);
 
+/* sumcheck::witness_digest (crates/sumcheck/src/lib.rs:228-256): a Poseidon2
+ * sponge: append_scalars(WITNESS_DIGEST, [k, num_vars]), then every column's
+ * cells as one append_scalars(WITNESS_DIGEST, ...) each, then sample. A
+ * strictly sequential chain of permutations, so it runs on the host CPU
+ * (the _dev form copies device columns down first; k * 2^num_vars elements,
+ * column-major). absorb_witness_digest (:262) is
+ * apg_transcript_append_scalars(t, WITNESS_DIGEST, digest, 1). k == 0 is
+ * APG_ERR_ARG (the reference panics). */
+int apg_witness_digest(const uint8_t *const *columns_mont, size_t k, size_t num_vars, uint8_t out_mont[32]);
+int apg_witness_digest_dev(apg_ctx *ctx, const void *columns_dev, size_t k, size_t num_vars,
+                           uint8_t out_mont[32]);
+
 /* ---- one shard: commit k columns, then the batched sumcheck.
  * The build-defined composition of reference primitives used by the
  * benchmarks: commitments absorbed as one COMMITMENT message (pcs

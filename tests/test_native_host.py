@@ -156,3 +156,24 @@ def test_synthetic_inputs_match_oracle(apg, orc, kind):
         zero = (a == 0).all(1).mean()
         one = (a == orc.from_u64(orc.FR, 1)).all(1).mean()
         assert abs(zero - 0.2) < 0.02 and abs(one - 0.1) < 0.02
+
+
+@pytest.mark.parametrize("k,nv", [(1, 0), (2, 3), (3, 7)])
+def test_witness_digest_matches_oracle(apg, orc, k, nv):
+    """sumcheck::witness_digest (host path of the library) equals the
+    oracle's sponge, and a one-cell change moves it."""
+    cols = [orc.fill_uniform(500 + c, 1 << nv) for c in range(k)]
+    d = apg.witness_digest(cols)
+    assert np.array_equal(d, orc.witness_digest(cols))
+    cols[-1][-1] = orc.from_u64(orc.FR, 7)
+    assert not np.array_equal(apg.witness_digest(cols), d)
+    t = apg.Transcript()
+    apg.absorb_witness_digest(t, d)
+    ot = orc.Transcript()
+    ot.append_scalars(orc.TAG_WITNESS_DIGEST, d.reshape(1, 4))
+    assert t.state()[:96] == ot.snapshot()[:96]
+
+
+def test_witness_digest_rejects_no_columns(apg):
+    with pytest.raises(ValueError):
+        apg.witness_digest([])
