@@ -438,12 +438,18 @@ void msm_set_window_bits(int log2n, int c) {
     g_window_override[log2n] = c;
 }
 
+// Built-in widths for floor(log2 n) = 8..22, the fastest of c in default +- 3
+// for 8 witness-like columns on MI355X (apogee-vm_amd/autotune.py); outside
+// that range, lg n - 2 clamped to [4, 17].
+static const int kTunedC[23] = {0, 0, 0, 0, 0, 0, 0, 0, 7, 8, 8, 9, 9, 11, 12, 14, 14, 14, 17, 17, 17, 17, 17};
+
 int msm_pick_c(size_t n, size_t columns) {
     (void)columns;
     int lg = 0;
     while (lg < 63 && (size_t(1) << (lg + 1)) <= n) lg++;
     if (const int o = g_window_override[lg]) return o;
     if (n < 32) return 3;
+    if (lg < 23 && kTunedC[lg]) return kTunedC[lg];
     int c = lg - 2;
     if (c < 4) c = 4;
     if (c > 17) c = 17;
