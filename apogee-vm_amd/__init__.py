@@ -158,6 +158,8 @@ def _declare(L):
     d("apg_ctx_reset_stats", [VP])
     d("apg_synthetic_fill", [U64, I32, SZ, VP])
     d("apg_debug_mul32_batch", [I32, VP, VP, SZ, VP])
+    d("apg_debug_fq29_batch", [VP, I32, VP, VP, SZ, VP])
+    d("apg_debug_accumulate29", [VP, I32, VP, VP, VP, SZ, VP])
     d("apg_synthetic_fill_dev", [VP, U64, I32, SZ, VP])
     d("apg_engine_create", [I32, I32, SZ, C.POINTER(VP)])
     d("apg_engine_create_lanes", [I32, I32, SZ, I32, I32, C.c_double, C.POINTER(VP)])

@@ -11,6 +11,7 @@
 
 #include "../../include/apogee_gpu.h"
 #include "engine.h"
+#include "curve29.cuh"
 #include "msm.h"
 #include "sumcheck.h"
 #include "transcript.h"
@@ -819,6 +820,119 @@ int apg_fq_from_bytes(const uint8_t canonical[32], uint8_t mont[32]) {
     return APG_OK;
 }
 void apg_fr_from_u64(uint64_t x, uint8_t mont[32]) { store_fe(mont, f_from_u64<Fr>(x)); }
+
+namespace {
+
+__global__ void k_debug_fq29(int op, const Fe *a, const Fe *b, size_t n, Fe *out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f29_test_chain<Fq>(a[i], b[i], op);
+}
+
+// one chain per thread: points [off[c], off[c + 1]) summed in the lazy 29-bit
+// form (form 0) or the 32-bit form (form 1)
+__global__ void k_debug_accumulate(int form, const Affine *pts, const uint8_t *neg, const uint64_t *off,
+                                   size_t ncases, Xyzz *out) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncases) return;
+    if (form == 0) {
+        Xyzz29 acc{};
+        bool id = true;
+        for (uint64_t i = off[c]; i < off[c + 1]; i++) xyzz29_accumulate(acc, id, pts[i], neg[i] != 0);
+        out[c] = xyzz29_to32(acc, id);
+    } else {
+        Xyzz acc = xyzz_identity();
+        for (uint64_t i = off[c]; i < off[c + 1]; i++) {
+            Affine p = pts[i];
+            if (neg[i]) p.y = f_neg<Fq>(p.y);
+            acc = xyzz_add_affine(acc, p);
+        }
+        out[c] = acc;
+    }
+}
+
+// Device scratch for the debug entries: copies in, one launch, copies out.
+struct DebugBufs {
+    std::vector<void *> ptrs;
+    void *get(size_t bytes) {
+        void *p = nullptr;
+        HIP_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+        ptrs.push_back(p);
+        return p;
+    }
+    ~DebugBufs() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
+    if (op < 0 || op > 3) return APG_ERR_ARG;
+    if (!ctx) {
+        for (size_t i = 0; i < n; i++)
+            store_fe(out + 32 * i, f29_test_chain<Fq>(load_fe(a + 32 * i), load_fe(b + 32 * i), op));
+        return APG_OK;
+    }
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        DebugBufs d;
+        Fe *da = (Fe *)d.get(32 * n), *db = (Fe *)d.get(32 * n), *dout = (Fe *)d.get(32 * n);
+        HIP_CHECK(hipMemcpy(da, a, 32 * n, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(db, b, 32 * n, hipMemcpyHostToDevice));
+        if (n) hipLaunchKernelGGL(k_debug_fq29, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->c.stream, op, da, db, n, dout);
+        HIP_CHECK(hipGetLastError());
+        ctx->c.sync();
+        HIP_CHECK(hipMemcpy(out, dout, 32 * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int apg_debug_accumulate29(apg_ctx *ctx, int form, const uint8_t *points, const uint8_t *neg,
+                           const uint64_t *offsets, size_t ncases, uint8_t *out_affine) {
+    if (form != 0 && form != 1) return APG_ERR_ARG;
+    for (size_t c = 0; c < ncases; c++)
+        if (offsets[c + 1] < offsets[c]) return APG_ERR_ARG;
+    const size_t npts = ncases ? (size_t)offsets[ncases] : 0;
+    std::vector<Xyzz> sums(ncases);
+    if (!ctx) {
+        if (form != 0) return APG_ERR_ARG;
+        for (size_t c = 0; c < ncases; c++) {
+            Xyzz29 acc{};
+            bool id = true;
+            for (uint64_t i = offsets[c]; i < offsets[c + 1]; i++) {
+                Affine a;
+                std::memcpy(&a, points + 64 * i, 64);
+                xyzz29_accumulate(acc, id, a, neg[i] != 0);
+            }
+            sums[c] = xyzz29_to32(acc, id);
+        }
+    } else {
+        const int rc = guarded([&] {
+            std::lock_guard<std::mutex> lk(ctx->c.mu);
+            HIP_CHECK(hipSetDevice(ctx->c.device));
+            DebugBufs d;
+            Affine *dp = (Affine *)d.get(64 * npts);
+            uint8_t *dn = (uint8_t *)d.get(npts);
+            uint64_t *doff = (uint64_t *)d.get(8 * (ncases + 1));
+            Xyzz *dout = (Xyzz *)d.get(sizeof(Xyzz) * ncases);
+            HIP_CHECK(hipMemcpy(dp, points, 64 * npts, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(dn, neg, npts, hipMemcpyHostToDevice));
+            HIP_CHECK(hipMemcpy(doff, offsets, 8 * (ncases + 1), hipMemcpyHostToDevice));
+            if (ncases)
+                hipLaunchKernelGGL(k_debug_accumulate, dim3((unsigned)((ncases + 63) / 64)), dim3(64), 0,
+                                   ctx->c.stream, form, dp, dn, doff, ncases, dout);
+            HIP_CHECK(hipGetLastError());
+            ctx->c.sync();
+            HIP_CHECK(hipMemcpy(sums.data(), dout, sizeof(Xyzz) * ncases, hipMemcpyDeviceToHost));
+        });
+        if (rc != APG_OK) return rc;
+    }
+    for (size_t c = 0; c < ncases; c++) {
+        const Affine r = xyzz_to_affine(sums[c]);
+        std::memcpy(out_affine + 64 * c, &r, 64);
+    }
+    return APG_OK;
+}
 
 int apg_debug_mul32_batch(int which, const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
     if (which != 0 && which != 1) return APG_ERR_ARG;

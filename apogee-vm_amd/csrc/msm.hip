@@ -33,6 +33,7 @@
 #include <atomic>
 #include <cstdio>
 
+#include "curve29.cuh"
 #include "msm.h"
 
 namespace apg {
@@ -206,14 +207,15 @@ __global__ void __launch_bounds__(TPB) k_accumulate(const Chunk *__restrict__ de
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nchunks) return;
     const Chunk ch = desc[t];
-    Xyzz acc = xyzz_identity();
+    // lazy 29-bit Fq (curve29.cuh): no carry chains in the multiplies, one
+    // full reduction at the write-back
+    Xyzz29 acc{};
+    bool id = true;
     for (uint32_t e = 0; e < ch.len; e++) {
         const uint32_t ent = entries[ch.start + e];
-        Affine p = load_affine(bases, ent & ~SIGN);
-        if (ent & SIGN) p.y = f_neg<Fq>(p.y);
-        acc = xyzz_add_affine(acc, p);
+        xyzz29_accumulate(acc, id, load_affine(bases, ent & ~SIGN), (ent & SIGN) != 0);
     }
-    chunk_sums[t] = acc;
+    chunk_sums[t] = xyzz29_to32(acc, id);
 }
 
 constexpr uint32_t LIGHT_CHUNKS = 8;  // more chunks than this: a heavy bucket

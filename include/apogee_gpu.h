@@ -78,6 +78,19 @@ int apg_synthetic_fill_dev(apg_ctx *ctx, uint64_t seed, int kind, size_t n, void
  * Fq. No counterpart in the reference. */
 int apg_debug_mul32_batch(int which, const uint8_t *a_mont, const uint8_t *b_mont, size_t n,
                           uint8_t *out_mont);
+/* The MSM accumulation's Fq arithmetic in nine 29-bit words with lazy
+ * reduction (field29.cuh), on the host (ctx NULL) or in a kernel on ctx's
+ * device (host buffers in and out): op 0 = a b, 1 = a + b, 2 = a - b,
+ * 3 = ((a + b)(a - b) + a^2) - b; outputs fully reduced Montgomery Fq. */
+int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a_mont, const uint8_t *b_mont, size_t n,
+                         uint8_t *out_mont);
+/* The MSM bucket loop's chains: for each case c, the sum over points
+ * i in [offsets[c], offsets[c+1]) of (neg[i] ? -P_i : P_i), P_i affine (64 B
+ * Montgomery x || y, all-zero = infinity), as an affine point (all-zero for
+ * the identity). form 0 = lazy 29-bit arithmetic, 1 = 32-bit (device only);
+ * ctx NULL runs on the host. */
+int apg_debug_accumulate29(apg_ctx *ctx, int form, const uint8_t *points, const uint8_t *neg,
+                           const uint64_t *offsets, size_t ncases, uint8_t *out_affine);
 
 /* ---- device memory for callers without their own allocator ---- */
 int apg_dev_alloc(apg_ctx *ctx, size_t bytes, void **out);

@@ -145,6 +145,146 @@ def test_device_montgomery_mul_matches_oracle(apg, orc, which, modulus):
     assert np.array_equal(got, want)
 
 
+FQ_MODULUS = 0x30644E72E131A029B85045B68181585D97816A916871CA8D3C208C16D87CFD47
+
+
+def _limbs(xs):
+    return np.array([[(x >> (64 * i)) & ((1 << 64) - 1) for i in range(4)] for x in xs], dtype=np.uint64)
+
+
+def fq29_cases(orc, n=40_000):
+    """Operand pairs for the lazy 29-bit Fq and the oracle's results for ops
+    0..3: random values, the edge grid (0, 1, p - 1, 2^k, p - 2^k, R mod p),
+    and non-canonical inputs x + p (below 2p), which the lazy form accepts."""
+    p = FQ_MODULUS
+    rng = np.random.default_rng(2029)
+    vals = [int.from_bytes(rng.bytes(32), "little") % p for _ in range(n)]
+    edges = [0, 1, 2, p - 1, p - 2, (p - 1) // 2, (p + 1) // 2, (1 << 256) % p, p - ((1 << 256) % p)]
+    edges += [(1 << k) % p for k in range(0, 254, 9)] + [p - (1 << k) for k in range(0, 250, 11)]
+    xa = vals + [e for e in edges for _ in edges]
+    xb = vals[::-1] + [f for _ in edges for f in edges]
+    ra = [x + p if i % 3 == 0 else x for i, x in enumerate(xa)]
+    rb = [x + p if i % 3 == 1 else x for i, x in enumerate(xb)]
+    assert max(ra + rb) < 2 * p
+    a, b = _limbs(xa), _limbs(xb)
+    L = orc.lib()
+    want = {op: np.zeros_like(a) for op in range(4)}
+    s, d, q = (np.zeros(4, dtype=np.uint64) for _ in range(3))
+    for i in range(a.shape[0]):
+        pa, pb = orc.ptr(a[i]), orc.ptr(b[i])
+        L.orc_mul(1, pa, pb, orc.ptr(want[0][i]))
+        L.orc_add(1, pa, pb, orc.ptr(want[1][i]))
+        L.orc_sub(1, pa, pb, orc.ptr(want[2][i]))
+        L.orc_mul(1, orc.ptr(want[1][i]), orc.ptr(want[2][i]), orc.ptr(d))
+        L.orc_mul(1, pa, pa, orc.ptr(q))
+        L.orc_add(1, orc.ptr(d), orc.ptr(q), orc.ptr(s))
+        L.orc_sub(1, orc.ptr(s), pb, orc.ptr(want[3][i]))
+    return _limbs(ra), _limbs(rb), want
+
+
+def fq29_run(apg, ctx_handle, op, a, b):
+    out = np.zeros_like(a)
+    rc = apg.lib().apg_debug_fq29_batch(ctx_handle, op, a.ctypes.data, b.ctypes.data, a.shape[0], out.ctypes.data)
+    assert rc == 0, apg.lib().apg_last_error()
+    return out
+
+
+def test_fq29_lazy_arithmetic_matches_oracle(apg, orc):
+    """The MSM accumulation's Fq in nine 29-bit words with lazy reduction
+    (results kept in [0, 2p)), compiled for the host, against the oracle's
+    u64 Montgomery Fq: multiply, add, subtract and a chain through the whole
+    lazy range, with non-canonical inputs and the edge grid."""
+    a, b, want = fq29_cases(orc)
+    for op in range(4):
+        assert np.array_equal(fq29_run(apg, None, op, a, b), want[op]), op
+    assert apg.lib().apg_debug_fq29_batch(None, 4, a.ctypes.data, b.ctypes.data, 1, a.ctypes.data) != 0
+
+
+def accumulate29_cases(orc):
+    """Chains of signed affine points and the oracle's sums (Jacobian chain of
+    affine additions, crates/curve/src/g1.rs:203-243). Points are small
+    multiples of G with random signs, so the accumulator hits +-P often
+    (doubling and identity), plus random points with infinities mixed in."""
+    L = orc.lib()
+    rng = np.random.default_rng(29)
+    small = np.zeros((4, 4), dtype=np.uint64)
+    for i in range(4):
+        L.orc_from_u64(0, i + 1, orc.ptr(small[i]))
+    big = np.zeros((64, 4), dtype=np.uint64)
+    for i in range(64):
+        L.orc_from_u64(0, int(rng.integers(1, 2**63)), orc.ptr(big[i]))
+    pts_small = np.zeros((4, 8), dtype=np.uint64)
+    pts_big = np.zeros((64, 8), dtype=np.uint64)
+    L.orc_g1_gen_mul_batch(orc.ptr(small), 4, 1, orc.ptr(pts_small))
+    L.orc_g1_gen_mul_batch(orc.ptr(big), 64, 1, orc.ptr(pts_big))
+    inf = np.zeros(8, dtype=np.uint64)
+
+    def oracle_sum(pts, neg):
+        acc = np.zeros(12, dtype=np.uint64)
+        L.orc_g1_from_affine(orc.ptr(inf), orc.ptr(acc))
+        for p, s in zip(pts, neg):
+            q = p.copy()
+            if s and q.any():
+                L.orc_neg(1, orc.ptr(p[4:]), orc.ptr(q[4:]))
+            nxt = np.zeros(12, dtype=np.uint64)
+            L.orc_g1_add_affine(orc.ptr(acc), orc.ptr(q), orc.ptr(nxt))
+            acc = nxt
+        out = np.zeros(8, dtype=np.uint64)
+        L.orc_g1_to_affine(orc.ptr(acc), orc.ptr(out))
+        return out
+
+    P, Q = pts_small[0], pts_small[1]
+    cases = [
+        ([], []),
+        ([P], [0]),
+        ([P, P], [0, 0]),            # doubling
+        ([P, P], [1, 1]),            # doubling of -P
+        ([P, P], [0, 1]),            # identity
+        ([P, P, Q], [0, 1, 0]),      # identity, then restart
+        ([P, P, P, P], [0, 1, 0, 0]),
+        ([P, inf, Q, inf], [0, 0, 1, 1]),
+        ([inf, inf], [0, 1]),
+    ]
+    for _ in range(40):
+        m = int(rng.integers(1, 40))
+        idx = rng.integers(0, 4, size=m)
+        cases.append(([pts_small[i] for i in idx], list(rng.integers(0, 2, size=m))))
+    for _ in range(8):
+        m = int(rng.integers(1, 64))
+        idx = rng.integers(0, 64, size=m)
+        pts = [pts_big[i] for i in idx]
+        for j in rng.integers(0, m, size=m // 8):
+            pts[j] = inf
+        cases.append((pts, list(rng.integers(0, 2, size=m))))
+    flat = [p for pts, _ in cases for p in pts]
+    points = np.ascontiguousarray(np.array(flat, dtype=np.uint64).reshape(-1, 8))
+    neg = np.ascontiguousarray(np.array([s for _, ng in cases for s in ng], dtype=np.uint8))
+    offsets = np.zeros(len(cases) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum([len(pts) for pts, _ in cases])
+    want = np.array([oracle_sum(pts, ng) for pts, ng in cases], dtype=np.uint64).reshape(-1, 8)
+    return points, neg, offsets, want
+
+
+def accumulate29_run(apg, ctx_handle, form, points, neg, offsets):
+    n = offsets.shape[0] - 1
+    out = np.zeros((n, 8), dtype=np.uint64)
+    rc = apg.lib().apg_debug_accumulate29(ctx_handle, form, points.ctypes.data, neg.ctypes.data,
+                                          offsets.ctypes.data, n, out.ctypes.data)
+    assert rc == 0, apg.lib().apg_last_error()
+    return out
+
+
+def test_accumulate29_matches_oracle_chain(apg, orc):
+    """The MSM bucket loop in the lazy 29-bit arithmetic (mixed XYZZ adds,
+    affine doubling when the point equals the accumulator, identity when it is
+    its negation, infinity skipped), compiled for the host, against the
+    oracle's chains; the empty chain is the identity (all-zero)."""
+    points, neg, offsets, want = accumulate29_cases(orc)
+    got = accumulate29_run(apg, None, 0, points, neg, offsets)
+    for c in range(want.shape[0]):
+        assert np.array_equal(got[c], want[c]), c
+    assert not got[0].any()
+
 @pytest.mark.parametrize("kind", ["uniform", "witness"])
 def test_synthetic_inputs_match_oracle(apg, orc, kind):
     """The benchmark's counter-mode ChaCha20 draws (host path of the library)
