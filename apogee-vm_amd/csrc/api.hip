@@ -828,26 +828,43 @@ __global__ void k_debug_fq29(int op, const Fe *a, const Fe *b, size_t n, Fe *out
     if (i < n) out[i] = f29_test_chain<Fq>(a[i], b[i], op);
 }
 
-// one chain per thread: points [off[c], off[c + 1]) summed in the lazy 29-bit
-// form (form 0) or the 32-bit form (form 1)
-__global__ void k_debug_accumulate(int form, const Affine *pts, const uint8_t *neg, const uint64_t *off,
-                                   size_t ncases, Xyzz *out) {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncases) return;
+// One chain: points [lo, hi) summed in the lazy 29-bit form with mixed adds
+// (form 0), in the 32-bit form with mixed adds (form 1), or in the lazy form
+// with general adds of the points scaled to Z = index + 2 (form 2).
+APG_HD Xyzz debug_chain(int form, const Affine *pts, const uint8_t *neg, uint64_t lo, uint64_t hi) {
     if (form == 0) {
         Xyzz29 acc{};
         bool id = true;
-        for (uint64_t i = off[c]; i < off[c + 1]; i++) xyzz29_accumulate(acc, id, pts[i], neg[i] != 0);
-        out[c] = xyzz29_to32(acc, id);
-    } else {
+        for (uint64_t i = lo; i < hi; i++) xyzz29_accumulate(acc, id, pts[i], neg[i] != 0);
+        return xyzz29_to32(acc, id);
+    }
+    if (form == 1) {
         Xyzz acc = xyzz_identity();
-        for (uint64_t i = off[c]; i < off[c + 1]; i++) {
+        for (uint64_t i = lo; i < hi; i++) {
             Affine p = pts[i];
             if (neg[i]) p.y = f_neg<Fq>(p.y);
             acc = xyzz_add_affine(acc, p);
         }
-        out[c] = acc;
+        return acc;
     }
+    Xyzz29 acc = xyzz29_identity();
+    for (uint64_t i = lo; i < hi; i++) {
+        if (aff_is_inf(pts[i])) continue;
+        const Fe z = f_from_u64<Fq>(i + 2), z2 = f_mul<Fq>(z, z), z3 = f_mul<Fq>(z2, z);
+        Xyzz q;
+        q.x = f_mul<Fq>(pts[i].x, z2);
+        q.y = f_mul<Fq>(neg[i] ? f_neg<Fq>(pts[i].y) : pts[i].y, z3);
+        q.zz = z2;
+        q.zzz = z3;
+        acc = xyzz29_add(acc, xyzz29_from32(q));
+    }
+    return xyzz29_store(acc);
+}
+
+__global__ void k_debug_accumulate(int form, const Affine *pts, const uint8_t *neg, const uint64_t *off,
+                                   size_t ncases, Xyzz *out) {
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncases) out[c] = debug_chain(form, pts, neg, off[c], off[c + 1]);
 }
 
 // Device scratch for the debug entries: copies in, one launch, copies out.
@@ -889,23 +906,15 @@ int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a, const uint8_t *
 
 int apg_debug_accumulate29(apg_ctx *ctx, int form, const uint8_t *points, const uint8_t *neg,
                            const uint64_t *offsets, size_t ncases, uint8_t *out_affine) {
-    if (form != 0 && form != 1) return APG_ERR_ARG;
+    if (form < 0 || form > 2) return APG_ERR_ARG;
     for (size_t c = 0; c < ncases; c++)
         if (offsets[c + 1] < offsets[c]) return APG_ERR_ARG;
     const size_t npts = ncases ? (size_t)offsets[ncases] : 0;
     std::vector<Xyzz> sums(ncases);
     if (!ctx) {
-        if (form != 0) return APG_ERR_ARG;
-        for (size_t c = 0; c < ncases; c++) {
-            Xyzz29 acc{};
-            bool id = true;
-            for (uint64_t i = offsets[c]; i < offsets[c + 1]; i++) {
-                Affine a;
-                std::memcpy(&a, points + 64 * i, 64);
-                xyzz29_accumulate(acc, id, a, neg[i] != 0);
-            }
-            sums[c] = xyzz29_to32(acc, id);
-        }
+        std::vector<Affine> pts(npts);
+        if (npts) std::memcpy(pts.data(), points, 64 * npts);
+        for (size_t c = 0; c < ncases; c++) sums[c] = debug_chain(form, pts.data(), neg, offsets[c], offsets[c + 1]);
     } else {
         const int rc = guarded([&] {
             std::lock_guard<std::mutex> lk(ctx->c.mu);

@@ -126,10 +126,6 @@ APG_HD Xyzz xyzz_add(const Xyzz &p, const Xyzz &q) {
     return o;
 }
 
-// Out-of-line addition for the LDS-tree kernels, which add at two call sites:
-// one copy of the formula keeps their register budget at one addition's.
-__device__ __noinline__ Xyzz xyzz_add_ool(const Xyzz &p, const Xyzz &q) { return xyzz_add(p, q); }
-
 APG_HD Affine aff_neg(const Affine &a) {
     Affine r = a;
     r.y = f_neg<Fq>(a.y);  // zero stays zero, so infinity stays infinity

@@ -95,6 +95,77 @@ APG_HD void xyzz29_accumulate(Xyzz29 &acc, bool &id, const Affine &a, bool neg) 
     xyzz29_add_affine(acc, id, x, y);
 }
 
+// General XYZZ points in the lazy form for the bucket reduction, where the
+// identity is ZZ = 0 mod p as in the 32-bit form.
+APG_HD F29 f29_zero() {
+    F29 r;
+#pragma unroll
+    for (int j = 0; j < 9; j++) r.v[j] = 0;
+    return r;
+}
+
+APG_HD Xyzz29 xyzz29_identity() {
+    Xyzz29 r;
+    r.x = f29_one_fq();
+    r.y = r.x;
+    r.zz = f29_zero();
+    r.zzz = r.zz;
+    return r;
+}
+
+APG_HD bool xyzz29_is_id(const Xyzz29 &p) { return f29_is_zero<Fq>(p.zz); }
+
+APG_HD Xyzz29 xyzz29_from32(const Xyzz &p) {
+    Xyzz29 r;
+    r.x = f29_from32(p.x);
+    r.y = f29_from32(p.y);
+    r.zz = f29_from32(p.zz);
+    r.zzz = f29_from32(p.zzz);
+    return r;
+}
+
+// dbl-2008-s-1 (a = 0), as curve.cuh's xyzz_dbl
+APG_HD Xyzz29 xyzz29_dbl(const Xyzz29 &p) {
+    if (xyzz29_is_id(p)) return p;
+    const F29 u = f29_add<Fq>(p.y, p.y);
+    const F29 v = f29_mul<Fq>(u, u);
+    const F29 w = f29_mul<Fq>(u, v);
+    const F29 s = f29_mul<Fq>(p.x, v);
+    const F29 x2 = f29_mul<Fq>(p.x, p.x);
+    const F29 m = f29_add<Fq>(f29_add<Fq>(x2, x2), x2);
+    Xyzz29 r;
+    r.x = f29_sub<Fq>(f29_sub<Fq>(f29_mul<Fq>(m, m), s), s);
+    r.y = f29_sub<Fq>(f29_mul<Fq>(m, f29_sub<Fq>(s, r.x)), f29_mul<Fq>(w, p.y));
+    r.zz = f29_mul<Fq>(v, p.zz);
+    r.zzz = f29_mul<Fq>(w, p.zzz);
+    return r;
+}
+
+// p + q (add-2008-s), as curve.cuh's xyzz_add
+APG_HD Xyzz29 xyzz29_add(const Xyzz29 &p, const Xyzz29 &q) {
+    if (xyzz29_is_id(p)) return q;
+    if (xyzz29_is_id(q)) return p;
+    const F29 u1 = f29_mul<Fq>(p.x, q.zz);
+    const F29 u2 = f29_mul<Fq>(q.x, p.zz);
+    const F29 s1 = f29_mul<Fq>(p.y, q.zzz);
+    const F29 s2 = f29_mul<Fq>(q.y, p.zzz);
+    const F29 pd = f29_sub<Fq>(u2, u1);
+    const F29 r = f29_sub<Fq>(s2, s1);
+    if (f29_is_zero<Fq>(pd)) {
+        if (f29_is_zero<Fq>(r)) return xyzz29_dbl(p);
+        return xyzz29_identity();
+    }
+    const F29 pp = f29_mul<Fq>(pd, pd);
+    const F29 ppp = f29_mul<Fq>(pd, pp);
+    const F29 qq = f29_mul<Fq>(u1, pp);
+    Xyzz29 o;
+    o.x = f29_sub<Fq>(f29_sub<Fq>(f29_sub<Fq>(f29_mul<Fq>(r, r), ppp), qq), qq);
+    o.y = f29_sub<Fq>(f29_mul<Fq>(r, f29_sub<Fq>(qq, o.x)), f29_mul<Fq>(s1, ppp));
+    o.zz = f29_mul<Fq>(f29_mul<Fq>(p.zz, q.zz), pp);
+    o.zzz = f29_mul<Fq>(f29_mul<Fq>(p.zzz, q.zzz), ppp);
+    return o;
+}
+
 APG_HD Xyzz xyzz29_to32(const Xyzz29 &a, bool id) {
     if (id) return xyzz_identity();
     Xyzz r;
@@ -104,5 +175,8 @@ APG_HD Xyzz xyzz29_to32(const Xyzz29 &a, bool id) {
     r.zzz = f29_to32<Fq>(a.zzz);
     return r;
 }
+
+// fully reduced 32-bit form; the identity is written as xyzz_identity()
+APG_HD Xyzz xyzz29_store(const Xyzz29 &a) { return xyzz29_to32(a, xyzz29_is_id(a)); }
 
 }  // namespace apg

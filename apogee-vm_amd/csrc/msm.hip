@@ -226,20 +226,20 @@ __global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restr
     if (k >= nkeys) return;
     const uint32_t a = qoff[k], q = qoff[k + 1] - a;
     if (q > LIGHT_CHUNKS) return;
-    Xyzz acc = xyzz_identity();
-    for (uint32_t s = 0; s < q; s++) acc = xyzz_add(acc, chunk_sums[a + s]);
-    buckets[k] = acc;
+    Xyzz29 acc = xyzz29_identity();
+    for (uint32_t s = 0; s < q; s++) acc = xyzz29_add(acc, xyzz29_from32(chunk_sums[a + s]));
+    buckets[k] = xyzz29_store(acc);
 }
 
-// Xyzz exchange between lanes of one wave.
-__device__ __forceinline__ Xyzz shfl_xor_xyzz(const Xyzz &p, int mask) {
-    Xyzz r;
+// Point exchange between lanes of one wave.
+__device__ __forceinline__ Xyzz29 shfl_xor_xyzz(const Xyzz29 &p, int mask) {
+    Xyzz29 r;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        r.x.l[q] = __shfl_xor(p.x.l[q], mask, 64);
-        r.y.l[q] = __shfl_xor(p.y.l[q], mask, 64);
-        r.zz.l[q] = __shfl_xor(p.zz.l[q], mask, 64);
-        r.zzz.l[q] = __shfl_xor(p.zzz.l[q], mask, 64);
+    for (int q = 0; q < 9; q++) {
+        r.x.v[q] = __shfl_xor(p.x.v[q], mask, 64);
+        r.y.v[q] = __shfl_xor(p.y.v[q], mask, 64);
+        r.zz.v[q] = __shfl_xor(p.zz.v[q], mask, 64);
+        r.zzz.v[q] = __shfl_xor(p.zzz.v[q], mask, 64);
     }
     return r;
 }
@@ -271,14 +271,14 @@ __global__ void k_classify(const uint32_t *__restrict__ qoff, uint32_t nkeys, He
 }
 
 // Sum of the 256 lanes' points of a block (all lanes call it).
-__device__ __forceinline__ Xyzz block_sum256(Xyzz acc, Xyzz *lds) {
+__device__ __forceinline__ Xyzz29 block_sum256(Xyzz29 acc, Xyzz29 *lds) {
 #pragma unroll
-    for (int m = 1; m < 64; m <<= 1) acc = xyzz_add(acc, shfl_xor_xyzz(acc, m));
+    for (int m = 1; m < 64; m <<= 1) acc = xyzz29_add(acc, shfl_xor_xyzz(acc, m));
     const uint32_t tid = threadIdx.x;
     if ((tid & 63) == 0) lds[tid >> 6] = acc;
     __syncthreads();
-    Xyzz r = lds[0];
-    for (int w = 1; w < TPB / 64; w++) r = xyzz_add(r, lds[w]);
+    Xyzz29 r = lds[0];
+    for (int w = 1; w < TPB / 64; w++) r = xyzz29_add(r, lds[w]);
     __syncthreads();
     return r;
 }
@@ -289,17 +289,17 @@ __global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restr
                                                        const uint4 *__restrict__ heavy,
                                                        const uint2 *__restrict__ piece_of,
                                                        Xyzz *__restrict__ piece_sums) {
-    __shared__ Xyzz lds[TPB / 64];
+    __shared__ Xyzz29 lds[TPB / 64];
     const uint32_t np = plan->npieces, tid = threadIdx.x;
     for (uint32_t pc = blockIdx.x; pc < np; pc += gridDim.x) {
         const uint2 po = piece_of[pc];
         const uint32_t k = heavy[po.x].x;
         const uint32_t a = qoff[k] + po.y * PIECE;
         const uint32_t end = min(qoff[k + 1], a + PIECE);
-        Xyzz acc = xyzz_identity();
-        for (uint32_t c = a + tid; c < end; c += TPB) acc = xyzz_add(acc, chunk_sums[c]);
-        const Xyzz r = block_sum256(acc, lds);
-        if (tid == 0) piece_sums[pc] = r;
+        Xyzz29 acc = xyzz29_identity();
+        for (uint32_t c = a + tid; c < end; c += TPB) acc = xyzz29_add(acc, xyzz29_from32(chunk_sums[c]));
+        const Xyzz29 r = block_sum256(acc, lds);
+        if (tid == 0) piece_sums[pc] = xyzz29_store(r);
     }
 }
 
@@ -310,9 +310,9 @@ __global__ void __launch_bounds__(TPB, 2) k_heavy_final(const HeavyPlan *plan,
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= plan->nheavy) return;
     const uint4 e = heavy[h];
-    Xyzz acc = xyzz_identity();
-    for (uint32_t j = 0; j < e.z; j++) acc = xyzz_add(acc, piece_sums[e.y + j]);
-    buckets[e.x] = acc;
+    Xyzz29 acc = xyzz29_identity();
+    for (uint32_t j = 0; j < e.z; j++) acc = xyzz29_add(acc, xyzz29_from32(piece_sums[e.y + j]));
+    buckets[e.x] = xyzz29_store(acc);
 }
 
 constexpr int RGS = 32;  // threads per row / column sum
@@ -331,20 +331,20 @@ __global__ void __launch_bounds__(TPB, 2) k_reduce_rows(const Xyzz *__restrict__
     if (o >= (size_t)nprob * per) return;  // whole groups only: RGS divides TPB
     const uint32_t p = (uint32_t)(o / per), w = (uint32_t)(o % per);
     const Xyzz *base = S + (size_t)p * R * 256;
-    Xyzz acc = xyzz_identity();
+    Xyzz29 acc = xyzz29_identity();
     if (w < R) {
-        for (uint32_t t = j; t < 256; t += RGS) acc = xyzz_add(acc, base[(size_t)w * 256 + t]);
+        for (uint32_t t = j; t < 256; t += RGS) acc = xyzz29_add(acc, xyzz29_from32(base[(size_t)w * 256 + t]));
     } else {
         const uint32_t t = w - R;
-        for (uint32_t r = j; r < R; r += RGS) acc = xyzz_add(acc, base[(size_t)r * 256 + t]);
+        for (uint32_t r = j; r < R; r += RGS) acc = xyzz29_add(acc, xyzz29_from32(base[(size_t)r * 256 + t]));
     }
 #pragma unroll
-    for (int m = 1; m < RGS; m <<= 1) acc = xyzz_add(acc, shfl_xor_xyzz(acc, m));
+    for (int m = 1; m < RGS; m <<= 1) acc = xyzz29_add(acc, shfl_xor_xyzz(acc, m));
     if (j == 0) {
         if (w < R)
-            Z[(size_t)p * R + w] = acc;
+            Z[(size_t)p * R + w] = xyzz29_store(acc);
         else
-            U[(size_t)p * 256 + (w - R)] = acc;
+            U[(size_t)p * 256 + (w - R)] = xyzz29_store(acc);
     }
 }
 
@@ -355,33 +355,33 @@ __global__ void __launch_bounds__(TPB, 2) k_reduce_rows(const Xyzz *__restrict__
 __global__ void __launch_bounds__(TPB) k_weighted(const Xyzz *__restrict__ a_src, uint32_t ma,
                                                 uint32_t na, const Xyzz *__restrict__ b_src,
                                                 uint32_t mb, Xyzz *__restrict__ out) {
-    __shared__ Xyzz lds[TPB];
+    __shared__ Xyzz29 lds[TPB];
     const uint32_t tid = threadIdx.x;
     const bool first = blockIdx.x < na;
     const uint32_t m = first ? ma : mb;
     const Xyzz *src = first ? a_src : b_src;
     const size_t base = (size_t)(first ? blockIdx.x : blockIdx.x - na) * m;
-    lds[tid] = tid < m ? src[base + tid] : xyzz_identity();
+    lds[tid] = tid < m ? xyzz29_from32(src[base + tid]) : xyzz29_identity();
     __syncthreads();
     for (uint32_t s = 1; s < m; s <<= 1) {
-        Xyzz v = lds[tid];
+        Xyzz29 v = lds[tid];
         const bool take = tid + s < m;
-        Xyzz o = take ? lds[tid + s] : v;
+        Xyzz29 o = take ? lds[tid + s] : v;
         __syncthreads();
-        if (take) lds[tid] = xyzz_add_ool(v, o);
+        if (take) lds[tid] = xyzz29_add(v, o);
         __syncthreads();
     }
-    const Xyzz tot = lds[0];
+    const Xyzz29 tot = lds[0];
     __syncthreads();
-    if (tid == 0) lds[0] = xyzz_identity();
+    if (tid == 0) lds[0] = xyzz29_identity();
     __syncthreads();
     for (uint32_t s = TPB / 2; s > 0; s >>= 1) {
-        if (tid < s && tid + s < m) lds[tid] = xyzz_add_ool(lds[tid], lds[tid + s]);
+        if (tid < s && tid + s < m) lds[tid] = xyzz29_add(lds[tid], lds[tid + s]);
         __syncthreads();
     }
     if (tid == 0) {
-        out[2 * blockIdx.x] = lds[0];
-        out[2 * blockIdx.x + 1] = tot;
+        out[2 * blockIdx.x] = xyzz29_store(lds[0]);
+        out[2 * blockIdx.x + 1] = xyzz29_store(tot);
     }
 }
 

@@ -46,9 +46,12 @@ sys.path.insert(0, ROOT)
 os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "shards/s (commit MSM+sumcheck) on mixed 2^8–2^22 batch; G1 MSM pts/s; p99 lat"
-# v_mad_u64_u32 issues at half rate on gfx950: 256 CUs x 4 SIMDs x 16 lanes / 2
-# per clock at 2.4 GHz = 1.966e13 32x32->64 multiply-adds per second.
-PEAK_IMUL = 256 * 4 * 16 / 2 * 2.4e9
+# v_mad_u64_u32 (32x32->64 multiply-add) issues at the FP64-FMA rate on gfx950:
+# 64 lanes per CU per clock, half the rate of 32-bit adds and logic. At
+# 2.4 GHz that is 256 x 64 x 2.4e9 = 3.93e13/s, the published FP64 vector
+# 78.6 TFLOP/s counted as FMAs; 32 independent chains per lane sustain
+# 3.3e13/s under load.
+PEAK_IMUL = 256 * 64 * 2.4e9
 PEAK_HBM = 8.0e12
 IMUL_PER_MADD = 1496  # madd-2007-bl: 7M + 4S Fq multiplies x 136
 

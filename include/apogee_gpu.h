@@ -87,8 +87,9 @@ int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a_mont, const uint
 /* The MSM bucket loop's chains: for each case c, the sum over points
  * i in [offsets[c], offsets[c+1]) of (neg[i] ? -P_i : P_i), P_i affine (64 B
  * Montgomery x || y, all-zero = infinity), as an affine point (all-zero for
- * the identity). form 0 = lazy 29-bit arithmetic, 1 = 32-bit (device only);
- * ctx NULL runs on the host. */
+ * the identity). form 0 = lazy 29-bit arithmetic with mixed adds, 1 = 32-bit
+ * with mixed adds, 2 = lazy 29-bit with general XYZZ adds of the points
+ * scaled to Z = i + 2; ctx NULL runs on the host. */
 int apg_debug_accumulate29(apg_ctx *ctx, int form, const uint8_t *points, const uint8_t *neg,
                            const uint64_t *offsets, size_t ncases, uint8_t *out_affine);
 

@@ -31,9 +31,10 @@ def test_fq29_on_device_matches_oracle(apg, orc, ctx):
         assert bad.size == 0, (op, bad[:8].tolist())
 
 
-@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("form", [0, 1, 2])
 def test_accumulate_chains_on_device_match_oracle(apg, orc, ctx, form):
-    """form 0 is the lazy 29-bit loop the MSM runs, form 1 the 32-bit one."""
+    """form 0 is the lazy 29-bit bucket loop the MSM runs, form 1 the 32-bit
+    one, form 2 the lazy general XYZZ add of the bucket reduction."""
     points, neg, offsets, want = accumulate29_cases(orc)
     got = accumulate29_run(apg, ctx.h, form, points, neg, offsets)
     bad = [c for c in range(want.shape[0]) if not np.array_equal(got[c], want[c])]

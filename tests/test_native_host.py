@@ -198,6 +198,9 @@ def test_fq29_lazy_arithmetic_matches_oracle(apg, orc):
     for op in range(4):
         assert np.array_equal(fq29_run(apg, None, op, a, b), want[op]), op
     assert apg.lib().apg_debug_fq29_batch(None, 4, a.ctypes.data, b.ctypes.data, 1, a.ctypes.data) != 0
+    off = np.zeros(2, dtype=np.uint64)
+    assert apg.lib().apg_debug_accumulate29(None, 3, a.ctypes.data, a.ctypes.data, off.ctypes.data, 1,
+                                            a.ctypes.data) != 0
 
 
 def accumulate29_cases(orc):
@@ -274,13 +277,17 @@ def accumulate29_run(apg, ctx_handle, form, points, neg, offsets):
     return out
 
 
-def test_accumulate29_matches_oracle_chain(apg, orc):
-    """The MSM bucket loop in the lazy 29-bit arithmetic (mixed XYZZ adds,
-    affine doubling when the point equals the accumulator, identity when it is
-    its negation, infinity skipped), compiled for the host, against the
-    oracle's chains; the empty chain is the identity (all-zero)."""
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_accumulate29_matches_oracle_chain(apg, orc, form):
+    """The MSM's point sums, compiled for the host, against the oracle's
+    chains: form 0 is the bucket loop in the lazy 29-bit arithmetic (mixed
+    XYZZ adds, affine doubling when the point equals the accumulator, identity
+    when it is its negation, infinity skipped), form 1 the same in the 32-bit
+    form, form 2 the bucket reduction's general XYZZ add in the lazy form, on
+    the points rescaled to Z != 1 (so equal points with different Z reach its
+    doubling branch). The empty chain is the identity (all-zero)."""
     points, neg, offsets, want = accumulate29_cases(orc)
-    got = accumulate29_run(apg, None, 0, points, neg, offsets)
+    got = accumulate29_run(apg, None, form, points, neg, offsets)
     for c in range(want.shape[0]):
         assert np.array_equal(got[c], want[c]), c
     assert not got[0].any()
