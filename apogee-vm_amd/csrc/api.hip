@@ -203,7 +203,7 @@ size_t shard_hbm_estimate(size_t num_vars, size_t k, int degree) {
     const int c = msm_pick_c(n, k);
     const size_t windows = (size_t)msm_window_count(254, c);
     const size_t slots = std::min(n * k * windows, size_t(1) << 28);
-    const size_t msm = slots * 16 + slots / 4 * 128 + k * (size_t(1) << (c - 1)) * 128;
+    const size_t msm = slots * 16 + slots / 4 * 128 + k * (size_t(1) << (c - 1)) * 144;
     const size_t f = (size_t)degree - 1;
     const size_t sc = f * n * 32 + (f + 1) * n * 32 * 2;
     return msm + sc + (size_t(64) << 20);

@@ -179,4 +179,54 @@ APG_HD Xyzz xyzz29_to32(const Xyzz29 &a, bool id) {
 // fully reduced 32-bit form; the identity is written as xyzz_identity()
 APG_HD Xyzz xyzz29_store(const Xyzz29 &a) { return xyzz29_to32(a, xyzz29_is_id(a)); }
 
+// Memory image of an Xyzz29 (36 words, 144 B): the bucket phase keeps points
+// in the lazy form between kernels, so a write is stores only. Arrays of it
+// are 16-byte aligned and move as nine 16-byte vectors.
+__device__ __forceinline__ Xyzz29 ld29(const Xyzz29 *p) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    uint32_t w[36];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const uint4 v = q[i];
+        w[4 * i] = v.x;
+        w[4 * i + 1] = v.y;
+        w[4 * i + 2] = v.z;
+        w[4 * i + 3] = v.w;
+    }
+    Xyzz29 r;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        r.x.v[j] = w[j];
+        r.y.v[j] = w[9 + j];
+        r.zz.v[j] = w[18 + j];
+        r.zzz.v[j] = w[27 + j];
+    }
+    return r;
+}
+
+__device__ __forceinline__ void st29(Xyzz29 *p, const Xyzz29 &a) {
+    uint32_t w[36];
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        w[j] = a.x.v[j];
+        w[9 + j] = a.y.v[j];
+        w[18 + j] = a.zz.v[j];
+        w[27 + j] = a.zzz.v[j];
+    }
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+#pragma unroll
+    for (int i = 0; i < 9; i++) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
+// the accumulator of a bucket loop as stored: the identity when id is set
+__device__ __forceinline__ void st29_acc(Xyzz29 *p, const Xyzz29 &a, bool id) {
+    Xyzz29 v = a;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        v.zz.v[j] = id ? 0u : a.zz.v[j];
+        v.zzz.v[j] = id ? 0u : a.zzz.v[j];
+    }
+    st29(p, v);
+}
+
 }  // namespace apg

@@ -10,22 +10,24 @@
 //               pair per nonzero digit, packed densely per block.
 //  2. sort    : an LSD radix sort of the pairs by bucket key (hipCUB), then a
 //               binary search per key for the bucket boundaries.
-//  3. chunks  : every bucket is cut into near-equal chunks of at most K1
-//               entries, so the work per thread is bounded whatever the digit
-//               skew (the witness-like mix piles 10% of all points into one
-//               bucket) and the lanes of a wave finish together.
-//  4. accumulate: one thread per chunk sums its points with mixed XYZZ adds;
-//               the gather of 64-byte affine bases is the only HBM traffic.
-//  5. buckets : chunk sums are combined per bucket: a thread per light bucket;
-//               heavy buckets cut into pieces over the whole grid (block
-//               slices + LDS tree, then the few piece sums), on a forked stream.
-//  6. reduce  : sum_b (b+1) S_b per group by a two-level split
+//  3. accumulate: the sorted entries are cut into equal slices, one thread
+//               per slice summing its points with mixed XYZZ adds, so the
+//               lanes of a wave finish together whatever the digit skew (the
+//               witness-like mix piles 10% of all points into one bucket). A
+//               bucket inside one slice is written directly; one that crosses
+//               slices leaves a head and a tail partial sum per slice. The
+//               gather of 64-byte affine bases is the only large HBM traffic.
+//  4. buckets : crossing buckets combine their partials: a thread per light
+//               bucket; heavy buckets cut into pieces over the whole grid
+//               (block slices + LDS tree, then the few piece sums), on a
+//               forked stream.
+//  5. reduce  : sum_b (b+1) S_b per group by a two-level split
 //               (b = 256 r + t: 256 * sum_r r Z_r + sum_t t U_t + sum_r Z_r):
 //               row and column sums by lane groups (serial slices, then wave
 //               shuffles), then an LDS suffix scan per 256-item problem:
 //               ~30 dependent adds instead of the 2 * 2^(c-1) serial running
 //               sum of msm.rs:304-310.
-//  7. host    : groups combine with 2^(c P) doublings (none when the table holds
+//  6. host    : groups combine with 2^(c P) doublings (none when the table holds
 //               every window shifted), one batched inversion gives affine.
 #include <hipcub/hipcub.hpp>
 
@@ -42,10 +44,6 @@ namespace {
 
 constexpr uint32_t SIGN = 0x80000000u;
 constexpr int TPB = 256;
-
-struct Chunk {
-    uint32_t key, start, len;
-};
 
 __device__ __forceinline__ uint32_t window_of32(const uint32_t l[8], int off, int c) {
     const int limb = off >> 5, sh = off & 31;
@@ -96,14 +94,20 @@ __device__ __forceinline__ void recode_digits(const uint32_t l[8], int c, int wi
 // Emits one (bucket key, table index | sign) pair per nonzero digit. Each
 // block packs its pairs contiguously at an offset claimed with one atomic, so
 // the output is dense (zero digits, 20-70% of all digits for witness-like
-// scalars, never reach the sort).
+// scalars, never reach the sort). With stage != 0 (windows <= STAGE_WINDOWS)
+// the block first gathers its pairs in LDS, keys then values, and writes them
+// out with consecutive lanes on consecutive words; otherwise each thread
+// writes its own run.
+constexpr int STAGE_WINDOWS = 31;  // 256 x 31 x 8 B + the scan: within 64 KiB of LDS
+
 __global__ void __launch_bounds__(TPB) k_emit(const Fe *__restrict__ fr,
                                             const uint32_t *__restrict__ u32, uint32_t n,
                                             uint32_t K, int c, int windows, int P, int G,
-                                            uint32_t B, uint32_t tab_stride,
+                                            uint32_t B, uint32_t tab_stride, int stage,
                                             uint32_t *__restrict__ counter,
                                             uint32_t *__restrict__ keys,
                                             uint32_t *__restrict__ vals) {
+    extern __shared__ uint32_t staged[];  // [TPB * windows] keys, then as many values
     __shared__ uint32_t scan[TPB];
     __shared__ uint32_t base;
     const uint32_t tid = threadIdx.x;
@@ -122,12 +126,30 @@ __global__ void __launch_bounds__(TPB) k_emit(const Fe *__restrict__ fr,
         scan[tid] += v;
         __syncthreads();
     }
-    if (tid == TPB - 1) base = atomicAdd(counter, scan[TPB - 1]);
+    const uint32_t btotal = scan[TPB - 1];
+    if (tid == TPB - 1) base = atomicAdd(counter, btotal);
+    const uint32_t col = (uint32_t)(idx / n);
+    const uint32_t i = (uint32_t)(idx - (size_t)col * n);
+    if (stage) {
+        const uint32_t cap = TPB * (uint32_t)windows;
+        uint32_t pos = scan[tid] - cnt;
+        if (live && cnt)
+            recode_digits(l, c, windows, [&](int j, uint32_t mag, bool neg) {
+                staged[pos] = ((col * (uint32_t)G) + (uint32_t)(j / P)) * B + (mag - 1);
+                staged[cap + pos] = ((uint32_t)(j % P) * tab_stride + i) | (neg ? SIGN : 0u);
+                pos++;
+            });
+        __syncthreads();
+        const uint32_t b0 = base;
+        for (uint32_t e = tid; e < btotal; e += TPB) {
+            keys[b0 + e] = staged[e];
+            vals[b0 + e] = staged[cap + e];
+        }
+        return;
+    }
     __syncthreads();
     if (!live || !cnt) return;
     uint32_t pos = base + scan[tid] - cnt;
-    const uint32_t col = (uint32_t)(idx / n);
-    const uint32_t i = (uint32_t)(idx - (size_t)col * n);
     recode_digits(l, c, windows, [&](int j, uint32_t mag, bool neg) {
         keys[pos] = ((col * (uint32_t)G) + (uint32_t)(j / P)) * B + (mag - 1);
         vals[pos] = ((uint32_t)(j % P) * tab_stride + i) | (neg ? SIGN : 0u);
@@ -151,44 +173,6 @@ __global__ void k_bounds(const uint32_t *__restrict__ skeys, uint32_t total, uin
     off[k] = lo;
 }
 
-__global__ void k_chunk_count(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t K1,
-                              uint32_t *__restrict__ q) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > nkeys) return;
-    if (k == nkeys) {
-        q[k] = 0;
-        return;
-    }
-    const uint32_t m = off[k + 1] - off[k];
-    q[k] = (m + K1 - 1) / K1;
-}
-
-// One thread per chunk: its bucket is the last key whose first chunk is at or
-// before it (binary search over the chunk offsets).
-__global__ void k_chunk_fill(const uint32_t *__restrict__ off, const uint32_t *__restrict__ qoff,
-                             uint32_t nkeys, uint32_t nchunks, Chunk *__restrict__ desc) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nchunks) return;
-    uint32_t lo = 0, hi = nkeys;  // largest k < nkeys with qoff[k] <= t
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (qoff[mid] <= t)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    const uint32_t k = lo, s = t - qoff[k];
-    // the bucket's len entries in q near-equal chunks: the first len % q get
-    // one entry more, so the lanes of a wave finish together
-    const uint32_t len = off[k + 1] - off[k], q = qoff[k + 1] - qoff[k];
-    const uint32_t base = len / q, rem = len % q;
-    Chunk ch;
-    ch.key = k;
-    ch.start = off[k] + s * base + min(s, rem);
-    ch.len = base + (s < rem ? 1u : 0u);
-    desc[t] = ch;
-}
-
 __device__ __forceinline__ Affine load_affine(const Affine *__restrict__ p, uint32_t idx) {
     const uint4 *q = reinterpret_cast<const uint4 *>(p + idx);
     uint4 a = q[0], b = q[1], c = q[2], d = q[3];
@@ -200,35 +184,94 @@ __device__ __forceinline__ Affine load_affine(const Affine *__restrict__ p, uint
     return r;
 }
 
-__global__ void __launch_bounds__(TPB) k_accumulate(const Chunk *__restrict__ desc, uint32_t nchunks,
-                                                  const uint32_t *__restrict__ entries,
-                                                  const Affine *__restrict__ bases,
-                                                  Xyzz *__restrict__ chunk_sums) {
+// Accumulation over equal slices: thread t sums the sorted entries
+// [t L, min((t + 1) L, total)), so every lane of a wave runs the same number
+// of additions. A bucket that lies inside one slice is written to buckets[]
+// directly. A bucket that crosses slice boundaries leaves partial sums:
+// part[2t] (head) is the slice's first run when that bucket started in an
+// earlier slice (the whole slice when it also continues past it), and
+// part[2t + 1] (tail) its last run when that bucket continues into the next
+// slice; unused slots hold the identity. Bucket k's partials are then the
+// contiguous part[2 t0 + 1 .. 2 t1], t0 and t1 the slices of its first and last
+// entries.
+__global__ void __launch_bounds__(TPB) k_accumulate(const uint32_t *__restrict__ skeys,
+                                                  const uint32_t *__restrict__ entries, uint32_t total,
+                                                  uint32_t L, const Affine *__restrict__ bases,
+                                                  Xyzz29 *__restrict__ buckets, Xyzz29 *__restrict__ part) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nchunks) return;
-    const Chunk ch = desc[t];
-    // lazy 29-bit Fq (curve29.cuh): no carry chains in the multiplies, one
-    // full reduction at the write-back
+    if ((size_t)t * L >= total) return;
+    const uint32_t lo = t * L, hi = lo + min(total - lo, L);
+    const bool head_open = lo > 0 && skeys[lo - 1] == skeys[lo];
+    const bool tail_open = hi < total && skeys[hi] == skeys[hi - 1];
+    uint32_t cur = skeys[lo];
+    bool in_head = head_open, head_done = false;
+    // lazy 29-bit Fq (curve29.cuh): no carry chains in the multiplies; sums
+    // are stored in that form
     Xyzz29 acc{};
     bool id = true;
-    for (uint32_t e = 0; e < ch.len; e++) {
-        const uint32_t ent = entries[ch.start + e];
+    for (uint32_t e = lo; e < hi; e++) {
+        const uint32_t key = skeys[e];
+        if (key != cur) {
+            if (in_head) {
+                st29_acc(part + 2 * (size_t)t, acc, id);
+                head_done = true;
+                in_head = false;
+            } else {
+                st29_acc(buckets + cur, acc, id);
+            }
+            cur = key;
+            id = true;
+        }
+        const uint32_t ent = entries[e];
         xyzz29_accumulate(acc, id, load_affine(bases, ent & ~SIGN), (ent & SIGN) != 0);
     }
-    chunk_sums[t] = xyzz29_to32(acc, id);
+    bool tail_done = false;
+    if (in_head) {
+        st29_acc(part + 2 * (size_t)t, acc, id);  // the whole slice belongs to one crossing bucket
+        head_done = true;
+    } else if (tail_open) {
+        st29_acc(part + 2 * (size_t)t + 1, acc, id);
+        tail_done = true;
+    } else {
+        st29_acc(buckets + cur, acc, id);
+    }
+    if (!head_done) st29(part + 2 * (size_t)t, xyzz29_identity());
+    if (!tail_done) st29(part + 2 * (size_t)t + 1, xyzz29_identity());
 }
 
-constexpr uint32_t LIGHT_CHUNKS = 8;  // more chunks than this: a heavy bucket
+// Partial sums of bucket k in part[]: count 0 when it is empty or lies in
+// one slice (then k_accumulate wrote it, or it stays empty).
+struct Span {
+    uint32_t first, count;
+};
 
-__global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restrict__ qoff, uint32_t nkeys,
-                               const Xyzz *__restrict__ chunk_sums, Xyzz *__restrict__ buckets) {
+__device__ __forceinline__ Span merge_span(const uint32_t *__restrict__ off, uint32_t k, uint32_t L) {
+    const uint32_t a = off[k], b = off[k + 1];
+    if (a == b) return {0, 0};
+    const uint32_t t0 = a / L, t1 = (b - 1) / L;
+    if (t0 == t1) return {0, 0};
+    return {2 * t0 + 1, 2 * (t1 - t0)};
+}
+
+constexpr uint32_t LIGHT_ITEMS = 16;  // more partials than this: a heavy bucket
+constexpr size_t ACC_ROUNDS = 4, MIN_SLICE = 16, MAX_SLICE = 4096;
+
+// One thread per bucket: empty buckets become the identity, buckets with at
+// most LIGHT_ITEMS partials are summed here, heavier ones by the pieces below.
+__global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restrict__ off, uint32_t nkeys,
+                                                       uint32_t L, const Xyzz29 *__restrict__ part,
+                                                       Xyzz29 *__restrict__ buckets) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
-    const uint32_t a = qoff[k], q = qoff[k + 1] - a;
-    if (q > LIGHT_CHUNKS) return;
+    if (off[k] == off[k + 1]) {
+        st29(buckets + k, xyzz29_identity());
+        return;
+    }
+    const Span sp = merge_span(off, k, L);
+    if (sp.count == 0 || sp.count > LIGHT_ITEMS) return;
     Xyzz29 acc = xyzz29_identity();
-    for (uint32_t s = 0; s < q; s++) acc = xyzz29_add(acc, xyzz29_from32(chunk_sums[a + s]));
-    buckets[k] = xyzz29_store(acc);
+    for (uint32_t s = 0; s < sp.count; s++) acc = xyzz29_add(acc, ld29(part + sp.first + s));
+    st29(buckets + k, acc);
 }
 
 // Point exchange between lanes of one wave.
@@ -244,11 +287,12 @@ __device__ __forceinline__ Xyzz29 shfl_xor_xyzz(const Xyzz29 &p, int mask) {
     return r;
 }
 
-// Heavy buckets (more than LIGHT_CHUNKS chunks; the witness-like mix puts
-// every scalar equal to 1 into one bucket per column) are cut into pieces of
-// PIECE chunks spread over the whole grid: each piece is one block's serial
-// slices plus a shuffle/LDS tree, and a last pass adds each bucket's few
-// piece sums. k_classify allocates the pieces as it finds the buckets.
+// Heavy buckets (more than LIGHT_ITEMS partials; the witness-like mix puts
+// every scalar equal to 1 into one bucket per column, spanning many slices)
+// are cut into pieces of PIECE partials spread over the whole grid: each piece
+// is one block's serial slices plus a shuffle/LDS tree, and a last pass adds
+// each bucket's few piece sums. k_classify allocates the pieces as it finds
+// the buckets.
 constexpr uint32_t PIECE = 4 * TPB;
 
 struct HeavyPlan {
@@ -257,12 +301,12 @@ struct HeavyPlan {
 
 // heavy[h] = (bucket, first piece, pieces) of heavy bucket h; piece_of[p] =
 // (heavy index, piece within the bucket).
-__global__ void k_classify(const uint32_t *__restrict__ qoff, uint32_t nkeys, HeavyPlan *plan,
+__global__ void k_classify(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t L, HeavyPlan *plan,
                            uint4 *__restrict__ heavy, uint2 *__restrict__ piece_of) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
-    const uint32_t q = qoff[k + 1] - qoff[k];
-    if (q <= LIGHT_CHUNKS) return;
+    const uint32_t q = merge_span(off, k, L).count;
+    if (q <= LIGHT_ITEMS) return;
     const uint32_t np = (q + PIECE - 1) / PIECE;
     const uint32_t h = atomicAdd(&plan->nheavy, 1u);
     const uint32_t p0 = atomicAdd(&plan->npieces, np);
@@ -283,36 +327,36 @@ __device__ __forceinline__ Xyzz29 block_sum256(Xyzz29 acc, Xyzz29 *lds) {
     return r;
 }
 
-__global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restrict__ qoff,
-                                                       const Xyzz *__restrict__ chunk_sums,
+__global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restrict__ off, uint32_t L,
+                                                       const Xyzz29 *__restrict__ part,
                                                        const HeavyPlan *plan,
                                                        const uint4 *__restrict__ heavy,
                                                        const uint2 *__restrict__ piece_of,
-                                                       Xyzz *__restrict__ piece_sums) {
+                                                       Xyzz29 *__restrict__ piece_sums) {
     __shared__ Xyzz29 lds[TPB / 64];
     const uint32_t np = plan->npieces, tid = threadIdx.x;
     for (uint32_t pc = blockIdx.x; pc < np; pc += gridDim.x) {
         const uint2 po = piece_of[pc];
-        const uint32_t k = heavy[po.x].x;
-        const uint32_t a = qoff[k] + po.y * PIECE;
-        const uint32_t end = min(qoff[k + 1], a + PIECE);
+        const Span sp = merge_span(off, heavy[po.x].x, L);
+        const uint32_t a = sp.first + po.y * PIECE;
+        const uint32_t end = min(sp.first + sp.count, a + PIECE);
         Xyzz29 acc = xyzz29_identity();
-        for (uint32_t c = a + tid; c < end; c += TPB) acc = xyzz29_add(acc, xyzz29_from32(chunk_sums[c]));
+        for (uint32_t c = a + tid; c < end; c += TPB) acc = xyzz29_add(acc, ld29(part + c));
         const Xyzz29 r = block_sum256(acc, lds);
-        if (tid == 0) piece_sums[pc] = xyzz29_store(r);
+        if (tid == 0) st29(piece_sums + pc, r);
     }
 }
 
 __global__ void __launch_bounds__(TPB, 2) k_heavy_final(const HeavyPlan *plan,
                                                       const uint4 *__restrict__ heavy,
-                                                      const Xyzz *__restrict__ piece_sums,
-                                                      Xyzz *__restrict__ buckets) {
+                                                      const Xyzz29 *__restrict__ piece_sums,
+                                                      Xyzz29 *__restrict__ buckets) {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= plan->nheavy) return;
     const uint4 e = heavy[h];
     Xyzz29 acc = xyzz29_identity();
-    for (uint32_t j = 0; j < e.z; j++) acc = xyzz29_add(acc, xyzz29_from32(piece_sums[e.y + j]));
-    buckets[e.x] = xyzz29_store(acc);
+    for (uint32_t j = 0; j < e.z; j++) acc = xyzz29_add(acc, ld29(piece_sums + e.y + j));
+    st29(buckets + e.x, acc);
 }
 
 constexpr int RGS = 32;  // threads per row / column sum
@@ -321,30 +365,30 @@ constexpr int RGS = 32;  // threads per row / column sum
 // an R x 256 matrix S[r][t] = S[256 r + t]: row sums Z_r and column sums U_t.
 // RGS consecutive lanes share one output: each sums a strided slice of it
 // serially (full waves), then a shuffle tree across the RGS lanes.
-__global__ void __launch_bounds__(TPB, 2) k_reduce_rows(const Xyzz *__restrict__ S, uint32_t R,
-                                                      uint32_t nprob, Xyzz *__restrict__ Z,
-                                                      Xyzz *__restrict__ U) {
+__global__ void __launch_bounds__(TPB, 2) k_reduce_rows(const Xyzz29 *__restrict__ S, uint32_t R,
+                                                      uint32_t nprob, Xyzz29 *__restrict__ Z,
+                                                      Xyzz29 *__restrict__ U) {
     const size_t gt = (size_t)blockIdx.x * TPB + threadIdx.x;
     const uint32_t per = R + 256;
     const size_t o = gt / RGS;
     const uint32_t j = (uint32_t)(gt % RGS);
     if (o >= (size_t)nprob * per) return;  // whole groups only: RGS divides TPB
     const uint32_t p = (uint32_t)(o / per), w = (uint32_t)(o % per);
-    const Xyzz *base = S + (size_t)p * R * 256;
+    const Xyzz29 *base = S + (size_t)p * R * 256;
     Xyzz29 acc = xyzz29_identity();
     if (w < R) {
-        for (uint32_t t = j; t < 256; t += RGS) acc = xyzz29_add(acc, xyzz29_from32(base[(size_t)w * 256 + t]));
+        for (uint32_t t = j; t < 256; t += RGS) acc = xyzz29_add(acc, ld29(base + (size_t)w * 256 + t));
     } else {
         const uint32_t t = w - R;
-        for (uint32_t r = j; r < R; r += RGS) acc = xyzz29_add(acc, xyzz29_from32(base[(size_t)r * 256 + t]));
+        for (uint32_t r = j; r < R; r += RGS) acc = xyzz29_add(acc, ld29(base + (size_t)r * 256 + t));
     }
 #pragma unroll
     for (int m = 1; m < RGS; m <<= 1) acc = xyzz29_add(acc, shfl_xor_xyzz(acc, m));
     if (j == 0) {
         if (w < R)
-            Z[(size_t)p * R + w] = xyzz29_store(acc);
+            st29(Z + (size_t)p * R + w, acc);
         else
-            U[(size_t)p * 256 + (w - R)] = xyzz29_store(acc);
+            st29(U + (size_t)p * 256 + (w - R), acc);
     }
 }
 
@@ -352,16 +396,16 @@ __global__ void __launch_bounds__(TPB, 2) k_reduce_rows(const Xyzz *__restrict__
 // LDS suffix scan (Suf_i = sum_{j >= i} X_j; WS = sum_{i >= 1} Suf_i).
 // Block b < na reads problem b of a (ma items each), block na + b problem b of
 // b_src (mb items each); block b writes out[2b] = WS, out[2b + 1] = TOT.
-__global__ void __launch_bounds__(TPB) k_weighted(const Xyzz *__restrict__ a_src, uint32_t ma,
-                                                uint32_t na, const Xyzz *__restrict__ b_src,
+__global__ void __launch_bounds__(TPB) k_weighted(const Xyzz29 *__restrict__ a_src, uint32_t ma,
+                                                uint32_t na, const Xyzz29 *__restrict__ b_src,
                                                 uint32_t mb, Xyzz *__restrict__ out) {
     __shared__ Xyzz29 lds[TPB];
     const uint32_t tid = threadIdx.x;
     const bool first = blockIdx.x < na;
     const uint32_t m = first ? ma : mb;
-    const Xyzz *src = first ? a_src : b_src;
+    const Xyzz29 *src = first ? a_src : b_src;
     const size_t base = (size_t)(first ? blockIdx.x : blockIdx.x - na) * m;
-    lds[tid] = tid < m ? xyzz29_from32(src[base + tid]) : xyzz29_identity();
+    lds[tid] = tid < m ? ld29(src + base + tid) : xyzz29_identity();
     __syncthreads();
     for (uint32_t s = 1; s < m; s <<= 1) {
         Xyzz29 v = lds[tid];
@@ -524,9 +568,11 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     uint32_t *keys_in = ctx.d<uint32_t>("msm.keys_in", max_pairs);
     uint32_t *vals_in = ctx.d<uint32_t>("msm.vals_in", max_pairs);
     HIP_CHECK(hipMemsetAsync(counter, 0, sizeof(uint32_t), st));
-    hipLaunchKernelGGL(k_emit, dim3(grid_for(nthreads, TPB)), dim3(TPB), 0, st, fr_dev, u32_dev,
+    const int stage = sh.windows <= STAGE_WINDOWS ? 1 : 0;
+    const size_t stage_lds = stage ? (size_t)TPB * sh.windows * 2 * sizeof(uint32_t) : 0;
+    hipLaunchKernelGGL(k_emit, dim3(grid_for(nthreads, TPB)), dim3(TPB), stage_lds, st, fr_dev, u32_dev,
                        (uint32_t)n, (uint32_t)K, sh.c, sh.windows, sh.P, sh.G, sh.B,
-                       (uint32_t)tab.stride, counter, keys_in, vals_in);
+                       (uint32_t)tab.stride, stage, counter, keys_in, vals_in);
     HIP_CHECK(hipGetLastError());
     uint32_t *h_total = ctx.h<uint32_t>("msm.total", 1);
     HIP_CHECK(hipMemcpyAsync(h_total, counter, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -543,11 +589,7 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     size_t tmp_bytes = 0;
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_in, skeys, vals_in, entries,
                                                  (int)total, 0, key_bits, st));
-    size_t tmp2 = 0;
-    uint32_t *q = ctx.d<uint32_t>("msm.q", nkeys + 1);
-    uint32_t *qoff = ctx.d<uint32_t>("msm.qoff", nkeys + 1);
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, q, qoff, nkeys + 1, st));
-    void *tmp = ctx.d<uint8_t>("msm.cubtmp", std::max(tmp_bytes, tmp2));
+    void *tmp = ctx.d<uint8_t>("msm.cubtmp", std::max<size_t>(tmp_bytes, 1));
     if (total)
         HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, skeys, vals_in, entries,
                                                      (int)total, 0, key_bits, st));
@@ -556,71 +598,64 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
                        nkeys, off);
     HIP_CHECK(hipGetLastError());
 
-    // 3. chunks
-    const uint32_t avg = total / std::max<uint32_t>(nkeys, 1);
-    uint32_t K1 = std::max<uint32_t>(16, avg / 4);
-    if (K1 > 256) K1 = 256;
-    hipLaunchKernelGGL(k_chunk_count, dim3(grid_for(nkeys + 1, TPB)), dim3(TPB), 0, st, off,
-                       nkeys, K1, q);
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmp2, q, qoff, nkeys + 1, st));
-    HIP_CHECK(hipMemcpyAsync(h_total, qoff + nkeys, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    ctx.sync();
-    const uint32_t nchunks = *h_total;
-    Chunk *desc = ctx.d<Chunk>("msm.desc", std::max<uint32_t>(nchunks, 1));
-    if (nchunks)
-        hipLaunchKernelGGL(k_chunk_fill, dim3(grid_for(nchunks, TPB)), dim3(TPB), 0, st, off, qoff,
-                           nkeys, nchunks, desc);
-
-    // 4. accumulate
-    Xyzz *csum = ctx.d<Xyzz>("msm.csum", std::max<uint32_t>(nchunks, 1));
+    // 3. accumulate over equal slices of the sorted entries: about ACC_ROUNDS
+    // slices per resident thread (4 waves per SIMD), at least MIN_SLICE entries
+    // and half the mean bucket, so a typical bucket crosses at most two slice
+    // boundaries and stays light
+    const size_t resident = (size_t)ctx.cu_count * 4 * 4 * 64;
+    const size_t by_grid = (total + ACC_ROUNDS * resident - 1) / (ACC_ROUNDS * resident);
+    const size_t by_bucket = total / (2 * (size_t)std::max<uint32_t>(nkeys, 1));
+    const uint32_t L = (uint32_t)std::min<size_t>(MAX_SLICE, std::max({MIN_SLICE, by_grid, by_bucket}));
+    const uint32_t nslices = (total + L - 1) / L;
+    Xyzz29 *buckets = ctx.d<Xyzz29>("msm.buckets", nkeys);
+    Xyzz29 *part = ctx.d<Xyzz29>("msm.part", 2 * (size_t)std::max<uint32_t>(nslices, 1));
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (ctx.profiling) {
         HIP_CHECK(hipEventCreate(&ev0));
         HIP_CHECK(hipEventCreate(&ev1));
         HIP_CHECK(hipEventRecord(ev0, st));
     }
-    if (nchunks)
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(nchunks, TPB)), dim3(TPB), 0, st, desc,
-                           nchunks, entries, tab.points, csum);
+    if (nslices)
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(nslices, TPB)), dim3(TPB), 0, st, skeys, entries,
+                           total, L, tab.points, buckets, part);
     HIP_CHECK(hipGetLastError());
     if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
 
-    // 5. buckets
-    Xyzz *buckets = ctx.d<Xyzz>("msm.buckets", nkeys);
-    // heavy buckets have > LIGHT_CHUNKS chunks each, so at most max_heavy of
-    // them, cut into at most max_pieces pieces
-    const uint32_t max_heavy = std::min<uint32_t>(nkeys, nchunks / (LIGHT_CHUNKS + 1) + 1);
-    const uint32_t max_pieces = nchunks / PIECE + max_heavy + 1;
+    // 4. buckets: empty ones set, crossing ones merged from their partials. A
+    // heavy bucket spans more than LIGHT_ITEMS / 2 slices, each slice belongs
+    // to at most one heavy span, and spans hold 2 partials per slice.
+    const uint32_t max_heavy = std::min<uint32_t>(nkeys, nslices / (LIGHT_ITEMS / 2) + 1);
+    const uint32_t max_pieces = 2 * nslices / PIECE + max_heavy + 1;
     HeavyPlan *plan = ctx.d<HeavyPlan>("msm.plan", 1);
     uint4 *heavy = ctx.d<uint4>("msm.heavy", max_heavy);
     uint2 *piece_of = ctx.d<uint2>("msm.piece_of", max_pieces);
-    Xyzz *piece_sums = ctx.d<Xyzz>("msm.piece_sums", max_pieces);
+    Xyzz29 *piece_sums = ctx.d<Xyzz29>("msm.piece_sums", max_pieces);
     HIP_CHECK(hipMemsetAsync(plan, 0, sizeof(HeavyPlan), st));
-    hipLaunchKernelGGL(k_classify, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys, plan, heavy,
+    hipLaunchKernelGGL(k_classify, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, off, nkeys, L, plan, heavy,
                        piece_of);
     // heavy pieces (a few latency-bound blocks) on the side stream, queued
     // first, beside the light buckets (throughput-bound) here
     HIP_CHECK(hipEventRecord(ctx.ev_fork, st));
     HIP_CHECK(hipStreamWaitEvent(ctx.side, ctx.ev_fork, 0));
     hipLaunchKernelGGL(k_heavy_pieces, dim3(std::min<uint32_t>(max_pieces, 4u * ctx.cu_count)), dim3(TPB), 0,
-                       ctx.side, qoff, csum, plan, heavy, piece_of, piece_sums);
+                       ctx.side, off, L, part, plan, heavy, piece_of, piece_sums);
     hipLaunchKernelGGL(k_heavy_final, dim3(grid_for(max_heavy, TPB)), dim3(TPB), 0, ctx.side, plan, heavy,
                        piece_sums, buckets);
     HIP_CHECK(hipEventRecord(ctx.ev_join, ctx.side));
-    hipLaunchKernelGGL(k_bucket_light, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, qoff, nkeys,
-                       csum, buckets);
+    hipLaunchKernelGGL(k_bucket_light, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, off, nkeys, L, part,
+                       buckets);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamWaitEvent(st, ctx.ev_join, 0));
 
-    // 6. reduce: per problem (column, group): W = sum_b (b + 1) S_b
+    // 5. reduce: per problem (column, group): W = sum_b (b + 1) S_b
     const uint32_t nprob = (uint32_t)(K * sh.G);
     const uint32_t B = sh.B;
     Xyzz *wout = ctx.d<Xyzz>("msm.wout", (size_t)nprob * 4);
     Xyzz *h_w = ctx.h<Xyzz>("msm.hw", (size_t)nprob * 4);
     if (B > (uint32_t)TPB) {
         const uint32_t R = B / TPB;
-        Xyzz *Z = ctx.d<Xyzz>("msm.Z", (size_t)nprob * R);
-        Xyzz *U = ctx.d<Xyzz>("msm.U", (size_t)nprob * TPB);
+        Xyzz29 *Z = ctx.d<Xyzz29>("msm.Z", (size_t)nprob * R);
+        Xyzz29 *U = ctx.d<Xyzz29>("msm.U", (size_t)nprob * TPB);
         const size_t threads = (size_t)nprob * (R + TPB) * RGS;
         hipLaunchKernelGGL(k_reduce_rows, dim3(grid_for(threads, TPB)), dim3(TPB), 0, st, buckets, R,
                            nprob, Z, U);
@@ -635,7 +670,7 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     HIP_CHECK(hipMemcpyAsync(h_w, wout, (size_t)nprob * 4 * sizeof(Xyzz), hipMemcpyDeviceToHost, st));
     ctx.sync();
 
-    // 7. host tail
+    // 6. host tail
     std::vector<Xyzz> res(K);
     for (size_t col = 0; col < K; col++) {
         Xyzz acc = xyzz_identity();
@@ -659,7 +694,7 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
         ctx.stats.accumulate_ms += ms;
-        ctx.stats.accumulate_launches += nchunks ? 1 : 0;
+        ctx.stats.accumulate_launches += nslices ? 1 : 0;
         ctx.stats.accumulate_entries += total;
         (void)hipEventDestroy(ev0);
         (void)hipEventDestroy(ev1);
