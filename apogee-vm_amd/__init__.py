@@ -158,6 +158,8 @@ def _declare(L):
     d("apg_ctx_reset_stats", [VP])
     d("apg_synthetic_fill", [U64, I32, SZ, VP])
     d("apg_debug_mul32_batch", [I32, VP, VP, SZ, VP])
+    d("apg_pcs_open", [VP, VP, VP, SZ, VP, VP, VP, VP, VP])
+    d("apg_pcs_open_dev", [VP, VP, VP, SZ, VP, VP, VP, VP, VP])
     d("apg_debug_fq29_batch", [VP, I32, VP, VP, SZ, VP])
     d("apg_debug_accumulate29", [VP, I32, VP, VP, VP, SZ, VP])
     d("apg_synthetic_fill_dev", [VP, U64, I32, SZ, VP])
@@ -546,6 +548,61 @@ def commit_device(srs: Srs, columns: DeviceBuffer, n: int, k: int, ctx: Optional
 
 
 # ------------------------------------------------------------------ transcript
+class DegenerateChallenge(PcsError):
+    """PcsError::DegenerateChallenge (crates/pcs-verify/src/lib.rs:40-87)."""
+
+
+class PointLengthMismatch(PcsError):
+    def __init__(self, point: int, num_vars: int):
+        super().__init__(f"PointLengthMismatch {{ point: {point}, num_vars: {num_vars} }}")
+        self.point, self.num_vars = point, num_vars
+
+
+MERCURY_PROOF_BYTES = 704
+APG_ERR_DEGENERATE = -7
+
+
+def open(srs: "Srs", f, cm: bytes, u, transcript: "Transcript", ctx: Optional["Context"] = None,
+         num_vars: Optional[int] = None):
+    """pcs::open (crates/pcs/src/lib.rs:313-492): the Mercury opening of the
+    commitment ``cm`` (64-byte wire form, absorbed as passed) at the point
+    ``u`` for the multilinear ``f``, advancing ``transcript``. ``f`` is the
+    evaluation table (Montgomery Fr, numpy ``(2^num_vars, 4)``) or a
+    ``DeviceBuffer`` holding it (then give ``num_vars``). Returns ``(v,
+    proof)``: v = fhat(u) as Montgomery limbs and the 704 bytes of
+    ``MercuryProof::to_bytes``. Raises the reference's errors."""
+    ctx = ctx or srs.ctx
+    u = _fe_array(u) if len(u) else np.zeros((0, 4), dtype=np.uint64)
+    if isinstance(f, DeviceBuffer):
+        if num_vars is None:
+            raise ValueError("open: give num_vars with a device table")
+        nv = num_vars
+    else:
+        f = _fe_array(f)
+        nv = f.shape[0].bit_length() - 1
+        if f.shape[0] != 1 << nv:
+            raise ValueError("a multilinear table holds a power of two entries")
+    n = check_num_vars(nv)
+    if u.shape[0] != nv:
+        raise PointLengthMismatch(u.shape[0], nv)
+    if len(srs) < n:
+        raise SrsTooSmall(n, len(srs))
+    cmb = np.frombuffer(bytes(cm), dtype=np.uint8).copy()
+    v = np.zeros(4, dtype=np.uint64)
+    proof = np.zeros(MERCURY_PROOF_BYTES, dtype=np.uint8)
+    if isinstance(f, DeviceBuffer):
+        rc = lib().apg_pcs_open_dev(ctx.h, srs.h, f.ptr, nv, _p(cmb), _p(u), transcript.h, _p(v), _p(proof))
+    else:
+        rc = lib().apg_pcs_open(ctx.h, srs.h, _p(f), nv, _p(cmb), _p(u), transcript.h, _p(v), _p(proof))
+    if rc == APG_ERR_DEGENERATE:
+        raise DegenerateChallenge("DegenerateChallenge")
+    _check(rc)
+    return v, proof.tobytes()
+
+
+pcs_open = open  # the module-level name without shadowing concerns for importers
+
+
 class Transcript:
     """The reference's Poseidon2 duplex transcript, native."""
 

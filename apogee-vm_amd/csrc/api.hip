@@ -11,6 +11,7 @@
 
 #include "../../include/apogee_gpu.h"
 #include "engine.h"
+#include "mercury.h"
 #include "curve29.cuh"
 #include "msm.h"
 #include "sumcheck.h"
@@ -476,6 +477,54 @@ int apg_msm_g1_fr(apg_ctx *ctx, const uint8_t *bases_mont, const uint8_t *scalar
         Affine res;
         msm_run(c, tab, n, 1, sc, nullptr, &res);
         std::memcpy(out_mont, &res, 64);
+    });
+}
+
+static void pcs_open_common(Context &c, apg_srs *srs, const Fe *f_dev, size_t num_vars, const uint8_t *cm_wire,
+                            const uint8_t *u_mont, apg_transcript *tr, uint8_t *v_mont, uint8_t *proof) {
+    if (num_vars < 2 || num_vars > 30 || (num_vars & 1))
+        throw ApgError(APG_ERR_ARG, "pcs open: num_vars must be even and in [2, 30]");
+    if (srs->n < (size_t(1) << num_vars)) throw ApgError(APG_ERR_ARG, "pcs open: SRS too small");
+    std::vector<Fe> u(num_vars);
+    for (size_t i = 0; i < num_vars; i++) u[i] = load_fe(u_mont + 32 * i);
+    CommitFn commit = [&](const Fe *dev, size_t len, uint8_t wire[64]) {
+        uint8_t m[64];
+        commit_device(c, srs, dev, nullptr, len, 1, m);
+        if (apg_g1_to_wire(m, wire) != APG_OK) throw ApgError(APG_ERR_INTERNAL, "pcs open: bad point");
+    };
+    Fe v;
+    mercury_open(c, commit, f_dev, (int)num_vars, cm_wire, u.data(), tr->t, v, proof);
+    store_fe(v_mont, v);
+}
+
+int apg_pcs_open(apg_ctx *ctx, apg_srs *srs, const uint8_t *table_mont, size_t num_vars, const uint8_t cm_wire[64],
+                 const uint8_t *u_mont, apg_transcript *tr, uint8_t v_mont[32], uint8_t proof[704]) {
+    return guarded([&] {
+        need(ctx);
+        need(srs);
+        need(tr);
+        Context &c = ctx->c;
+        std::lock_guard<std::mutex> lk(c.mu);
+        HIP_CHECK(hipSetDevice(c.device));
+        if (num_vars > 30) throw ApgError(APG_ERR_ARG, "pcs open: num_vars out of range");
+        const size_t n = size_t(1) << num_vars;
+        Fe *f = c.d<Fe>("pcs.table", n);
+        HIP_CHECK(hipMemcpyAsync(f, table_mont, n * 32, hipMemcpyHostToDevice, c.stream));
+        pcs_open_common(c, srs, f, num_vars, cm_wire, u_mont, tr, v_mont, proof);
+    });
+}
+
+int apg_pcs_open_dev(apg_ctx *ctx, apg_srs *srs, const void *table_dev, size_t num_vars, const uint8_t cm_wire[64],
+                     const uint8_t *u_mont, apg_transcript *tr, uint8_t v_mont[32], uint8_t proof[704]) {
+    return guarded([&] {
+        need(ctx);
+        need(srs);
+        need(tr);
+        need(table_dev);
+        Context &c = ctx->c;
+        std::lock_guard<std::mutex> lk(c.mu);
+        HIP_CHECK(hipSetDevice(c.device));
+        pcs_open_common(c, srs, static_cast<const Fe *>(table_dev), num_vars, cm_wire, u_mont, tr, v_mont, proof);
     });
 }
 

@@ -30,6 +30,7 @@ extern "C" {
 #define APG_ERR_OOM (-4)
 #define APG_ERR_INTERNAL (-5)
 #define APG_ERR_CALLBACK (-6)
+#define APG_ERR_DEGENERATE (-7) /* pcs::open: PcsError::DegenerateChallenge */
 
 typedef struct apg_ctx apg_ctx;
 typedef struct apg_srs apg_srs;
@@ -78,6 +79,24 @@ int apg_synthetic_fill_dev(apg_ctx *ctx, uint64_t seed, int kind, size_t n, void
  * Fq. No counterpart in the reference. */
 int apg_debug_mul32_batch(int which, const uint8_t *a_mont, const uint8_t *b_mont, size_t n,
                           uint8_t *out_mont);
+/* pcs::open (crates/pcs/src/lib.rs:313-492): the Mercury opening of the
+ * commitment cm_wire (64-byte wire form, absorbed as passed) at u (num_vars
+ * Montgomery Fr) for the 2^num_vars evaluation table, read as coefficients.
+ * The transcript advances exactly as the reference's, including the final
+ * PAIRING_MERGE squeeze. Out: v = fhat(u) (Montgomery Fr) and the 704-byte
+ * MercuryProof::to_bytes (8 wire points h, q, g, s, d, pi_z, w, w', then the
+ * canonical g(z), g(1/z), h(z), h(1/z), S(z), S(1/z)). num_vars must be even
+ * and in [2, 30] (check_num_vars, crates/pcs-verify/src/lib.rs:97) and the SRS
+ * hold 2^num_vars powers (PcsError::SrsTooSmall): APG_ERR_ARG otherwise;
+ * APG_ERR_DEGENERATE for the reference's DegenerateChallenge.
+ * _dev: the table is device memory (2^num_vars x 32 B). */
+int apg_pcs_open(apg_ctx *ctx, apg_srs *srs, const uint8_t *table_mont, size_t num_vars,
+                 const uint8_t cm_wire[64], const uint8_t *u_mont, apg_transcript *tr, uint8_t v_mont[32],
+                 uint8_t proof[704]);
+int apg_pcs_open_dev(apg_ctx *ctx, apg_srs *srs, const void *table_dev, size_t num_vars,
+                     const uint8_t cm_wire[64], const uint8_t *u_mont, apg_transcript *tr, uint8_t v_mont[32],
+                     uint8_t proof[704]);
+
 /* The MSM accumulation's Fq arithmetic in nine 29-bit words with lazy
  * reduction (field29.cuh), on the host (ctx NULL) or in a kernel on ctx's
  * device (host buffers in and out): op 0 = a b, 1 = a + b, 2 = a - b,

@@ -97,6 +97,7 @@ def _declare(L):
     d("orc_tr_append_g1", [P, U64, P, SZ])
     d("orc_interpolate", [P, SZ, P])
     d("orc_prove_zerocheck", [P, SZ, P, SZ, SZ, P, P, P])
+    d("orc_mercury_open", [P, SZ, P, SZ, P, P, P, SZ, P, P], C.c_int)
     d("orc_prove_sumcheck", [P, SZ, P, SZ, P, SZ, P, SZ, P, P])
     d("orc_witness_digest", [P, SZ, SZ, P])
     d("orc_combine", [P, SZ, SZ, P, SZ, P])
@@ -366,6 +367,24 @@ class Transcript:
 
     def snapshot(self) -> bytes:
         return bytes(self.t)
+
+
+def mercury_open(srs: np.ndarray, table: np.ndarray, cm_wire: bytes, u: np.ndarray,
+                 transcript: "Transcript", workers: int = 8):
+    """pcs::open (crates/pcs/src/lib.rs:313-492): returns (v, proof bytes),
+    advancing ``transcript``. Raises ValueError with the oracle's code."""
+    srs = np.ascontiguousarray(srs, dtype=np.uint64)
+    table = np.ascontiguousarray(table, dtype=np.uint64)
+    u = np.ascontiguousarray(np.asarray(u, dtype=np.uint64).reshape(-1, 4))
+    nv = u.shape[0]
+    cm = np.frombuffer(bytes(cm_wire), dtype=np.uint8).copy()
+    v = fe()
+    proof = np.zeros(704, dtype=np.uint8)
+    rc = lib().orc_mercury_open(ptr(srs), srs.shape[0], ptr(table), nv, ptr(cm), ptr(u),
+                                transcript.ref(), workers, ptr(v), ptr(proof))
+    if rc != 0:
+        raise ValueError(f"orc_mercury_open: {rc}")
+    return v, proof.tobytes()
 
 
 def poseidon2_permute(state: np.ndarray) -> np.ndarray:

@@ -128,6 +128,17 @@ void orc_fill_witness(uint64_t seed, uint64_t *out, size_t n);
  * Fr, 1 witness mix */
 void orc_fill_ctr(uint64_t seed, int kind, uint64_t *out, size_t n);
 
+/* ---- Mercury opening (crates/pcs/src/lib.rs:313-492), oracle/mercury.c ----
+ * srs: srs_len affine powers; table: 2^num_vars Fr (the evaluation table read
+ * as coefficients); cm_wire: the commitment as passed (64 B wire form); u:
+ * num_vars Fr; tr: the transcript, advanced as the reference advances it.
+ * Out: v (Fr) and the 704-byte proof (8 wire points, then 6 canonical Fr).
+ * Returns 0, or -2 unsupported num_vars, -3 SRS too small, -4 degenerate
+ * challenge, -5..-8 a broken internal identity. */
+int orc_mercury_open(const uint64_t *srs_aff, size_t srs_len, const uint64_t *table, size_t num_vars,
+                     const uint8_t *cm_wire, const uint64_t *u, orc_transcript *tr, size_t workers,
+                     uint64_t *v_out, uint8_t *proof_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,3 +149,53 @@ def test_mercury_commitment_kat(orc):
     srs = orc.srs_powers(tau, 1 << nv)
     cm = orc.jac_to_wire(orc.msm(srs, f)).hex()
     assert cm == next(r for r in recs if r[0] == "commitment")[1]
+
+
+def _mercury_kat(orc, name="mercury_proof.txt"):
+    recs = list(golden_lines(name))
+    one = lambda key: next(r for r in recs if r[0] == key)[1]
+    nv = int(one("numvars"))
+    vals = {int(r[1]): r[2] for r in recs if r[0] == "value"}
+    pts = {int(r[1]): r[2] for r in recs if r[0] == "point"}
+    return dict(
+        tau=orc.from_hex_le(orc.FR, one("tau")),
+        nv=nv,
+        f=np.stack([orc.from_hex_le(orc.FR, vals[i]) for i in range(1 << nv)]),
+        u=np.stack([orc.from_hex_le(orc.FR, pts[i]) for i in range(nv)]),
+        claim=one("claim"),
+        cm=bytes.fromhex(one("commitment")),
+        proof=one("proof"),
+        probe=one("probe"),
+    )
+
+
+def test_mercury_open_kat(orc):
+    """pcs::open over the toy SRS replays the committed full proof at n = 2^4
+    byte for byte: the claimed value, the 704 proof bytes (eight points, six
+    values) and the terminal transcript sample, which moves with any tag,
+    absorb order or squeeze the schedule changes
+    (crates/pcs/tests/kats.rs:246-290)."""
+    k = _mercury_kat(orc)
+    srs = orc.srs_powers(k["tau"], 1 << k["nv"])
+    tr = orc.Transcript()
+    v, proof = orc.mercury_open(srs, k["f"], k["cm"], k["u"], tr)
+    assert orc.to_hex_le(orc.FR, v) == k["claim"]
+    assert proof.hex() == k["proof"]
+    assert orc.to_hex_le(orc.FR, tr.sample()) == k["probe"]
+
+
+def test_mercury_open_kat_detects_changes(orc):
+    """The replay can fail: one changed witness value or coordinate of u
+    moves every proof byte after h."""
+    k = _mercury_kat(orc)
+    srs = orc.srs_powers(k["tau"], 1 << k["nv"])
+    f = k["f"].copy()
+    f[3] = orc.from_u64(orc.FR, 12345)
+    _, proof = orc.mercury_open(srs, f, k["cm"], k["u"], orc.Transcript())
+    assert proof.hex() != k["proof"]
+    u = k["u"].copy()
+    u[0] = orc.from_u64(orc.FR, 7)
+    v, proof = orc.mercury_open(srs, k["f"], k["cm"], u, orc.Transcript())
+    assert proof.hex()[:128] != k["proof"][:128] or orc.to_hex_le(orc.FR, v) != k["claim"]
+    with pytest.raises(ValueError):
+        orc.mercury_open(srs, k["f"][:8], k["cm"], k["u"][:3], orc.Transcript())
