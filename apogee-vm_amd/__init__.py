@@ -160,6 +160,8 @@ def _declare(L):
     d("apg_debug_mul32_batch", [I32, VP, VP, SZ, VP])
     d("apg_pcs_open", [VP, VP, VP, SZ, VP, VP, VP, VP, VP])
     d("apg_pcs_open_dev", [VP, VP, VP, SZ, VP, VP, VP, VP, VP])
+    d("apg_pcs_batch_open_stacked", [VP, VP, VP, VP, SZ, SZ, VP, VP, VP, SZ, VP, VP, VP])
+    d("apg_pcs_batch_open_stacked_dev", [VP, VP, VP, VP, SZ, SZ, VP, VP, VP, SZ, VP, VP, VP])
     d("apg_debug_fq29_batch", [VP, I32, VP, VP, SZ, VP])
     d("apg_debug_accumulate29", [VP, I32, VP, VP, VP, SZ, VP])
     d("apg_synthetic_fill_dev", [VP, U64, I32, SZ, VP])
@@ -601,6 +603,72 @@ def open(srs: "Srs", f, cm: bytes, u, transcript: "Transcript", ctx: Optional["C
 
 
 pcs_open = open  # the module-level name without shadowing concerns for importers
+
+
+class EmptyBatch(PcsError):
+    """PcsError::EmptyBatch."""
+
+
+class BatchLengthMismatch(PcsError):
+    def __init__(self, commitments: int, paired: int):
+        super().__init__(f"BatchLengthMismatch {{ commitments: {commitments}, paired: {paired} }}")
+        self.commitments, self.paired = commitments, paired
+
+
+class MixedColumnSizes(PcsError):
+    def __init__(self, expected: int, found: int):
+        super().__init__(f"MixedColumnSizes {{ expected: {expected}, found: {found} }}")
+        self.expected, self.found = expected, found
+
+
+def batch_open_stacked(srs: "Srs", stacks, cms: Sequence[bytes], u, r, transcript: "Transcript",
+                       ctx: Optional["Context"] = None):
+    """pcs::batch_open_stacked (crates/pcs/src/lib.rs:701-789): stacks[i] is
+    stack i's columns in slot order (Montgomery Fr tables of one size), opened
+    as one Mercury instance at u || r. Returns (vs, proof): one value per stack
+    and the 704-byte proof of cm* = sum rho^i cm_i. The reference's refusals,
+    in its order."""
+    ctx = ctx or srs.ctx
+    if not stacks or any(len(s) == 0 for s in stacks):
+        raise EmptyBatch("EmptyBatch")
+    if len(stacks) != len(cms):
+        raise BatchLengthMismatch(len(cms), len(stacks))
+    cols = [_fe_array(c) for s in stacks for c in s]
+    width = cols[0].shape[0].bit_length() - 1
+    for c in cols:
+        if c.shape[0] != cols[0].shape[0]:
+            raise MixedColumnSizes(width, c.shape[0].bit_length() - 1)
+    if cols[0].shape[0] != 1 << width:
+        raise ValueError("a multilinear table holds a power of two entries")
+    n = check_num_vars(width + len(r))
+    u = _fe_array(u) if len(u) else np.zeros((0, 4), dtype=np.uint64)
+    if u.shape[0] != width:
+        raise PointLengthMismatch(u.shape[0], width)
+    slots = 1 << len(r)
+    for s in stacks:
+        if len(s) > slots:
+            raise BatchLengthMismatch(slots, len(s))
+    if len(srs) < n:
+        raise SrsTooSmall(n, len(srs))
+    flat = np.ascontiguousarray(np.concatenate(cols))
+    sizes = np.array([len(s) for s in stacks], dtype=np.uint32)
+    cmb = np.frombuffer(b"".join(bytes(c) for c in cms), dtype=np.uint8).copy()
+    ra = _fe_array(r) if len(r) else np.zeros((1, 4), dtype=np.uint64)
+    ua = u if u.shape[0] else np.zeros((1, 4), dtype=np.uint64)
+    vs = np.zeros((len(stacks), 4), dtype=np.uint64)
+    proof = np.zeros(MERCURY_PROOF_BYTES, dtype=np.uint8)
+    rc = lib().apg_pcs_batch_open_stacked(ctx.h, srs.h, _p(flat), _p(sizes), len(stacks), width, _p(cmb), _p(ua),
+                                          _p(ra), len(r), transcript.h, _p(vs), _p(proof))
+    if rc == APG_ERR_DEGENERATE:
+        raise DegenerateChallenge("DegenerateChallenge")
+    _check(rc)
+    return vs, proof.tobytes()
+
+
+def batch_open(srs: "Srs", cols, cms: Sequence[bytes], u, transcript: "Transcript", ctx: Optional["Context"] = None):
+    """pcs::batch_open (crates/pcs/src/lib.rs:689-698): k same-size columns at
+    one point, one proof of cm* = sum rho^i cm_i."""
+    return batch_open_stacked(srs, [[c] for c in cols], cms, u, [], transcript, ctx)
 
 
 class Transcript:

@@ -528,6 +528,67 @@ int apg_pcs_open_dev(apg_ctx *ctx, apg_srs *srs, const void *table_dev, size_t n
     });
 }
 
+static void batch_open_common(Context &c, apg_srs *srs, const Fe *cols_dev, const uint32_t *stack_sizes,
+                              size_t nstacks, size_t width, const uint8_t *cms_wire, const uint8_t *u_mont,
+                              const uint8_t *r_mont, size_t nr, apg_transcript *tr, uint8_t *vs_mont,
+                              uint8_t *proof) {
+    if (width > 30 || nr > 30 || width + nr > 30) throw ApgError(APG_ERR_ARG, "UnsupportedNumVars");
+    if (srs->n < (size_t(1) << (width + nr))) throw ApgError(APG_ERR_ARG, "SrsTooSmall");
+    std::vector<Fe> u(width), r(nr), vs(nstacks);
+    for (size_t i = 0; i < width; i++) u[i] = load_fe(u_mont + 32 * i);
+    for (size_t i = 0; i < nr; i++) r[i] = load_fe(r_mont + 32 * i);
+    CommitFn commit = [&](const Fe *dev, size_t len, uint8_t wire[64]) {
+        uint8_t m[64];
+        commit_device(c, srs, dev, nullptr, len, 1, m);
+        if (apg_g1_to_wire(m, wire) != APG_OK) throw ApgError(APG_ERR_INTERNAL, "pcs open: bad point");
+    };
+    mercury_batch_open_stacked(c, commit, cols_dev, stack_sizes, nstacks, (int)width, cms_wire, u.data(), r.data(),
+                               (int)nr, tr->t, vs.data(), proof);
+    for (size_t s = 0; s < nstacks; s++) store_fe(vs_mont + 32 * s, vs[s]);
+}
+
+int apg_pcs_batch_open_stacked(apg_ctx *ctx, apg_srs *srs, const uint8_t *cols_mont, const uint32_t *stack_sizes,
+                               size_t nstacks, size_t width, const uint8_t *cms_wire, const uint8_t *u_mont,
+                               const uint8_t *r_mont, size_t nr, apg_transcript *tr, uint8_t *vs_mont,
+                               uint8_t proof[704]) {
+    return guarded([&] {
+        need(ctx);
+        need(srs);
+        need(tr);
+        if (nstacks == 0) throw ApgError(APG_ERR_ARG, "EmptyBatch");
+        need(stack_sizes);
+        if (width > 30) throw ApgError(APG_ERR_ARG, "UnsupportedNumVars");
+        size_t ncols = 0;
+        for (size_t s = 0; s < nstacks; s++) ncols += stack_sizes[s];
+        Context &c = ctx->c;
+        std::lock_guard<std::mutex> lk(c.mu);
+        HIP_CHECK(hipSetDevice(c.device));
+        const size_t bytes = ncols * (size_t(32) << width);
+        Fe *cols = c.d<Fe>("pcs.cols", std::max<size_t>(ncols << width, 1));
+        if (bytes) HIP_CHECK(hipMemcpyAsync(cols, cols_mont, bytes, hipMemcpyHostToDevice, c.stream));
+        batch_open_common(c, srs, cols, stack_sizes, nstacks, width, cms_wire, u_mont, r_mont, nr, tr, vs_mont, proof);
+    });
+}
+
+int apg_pcs_batch_open_stacked_dev(apg_ctx *ctx, apg_srs *srs, const void *cols_dev, const uint32_t *stack_sizes,
+                                   size_t nstacks, size_t width, const uint8_t *cms_wire, const uint8_t *u_mont,
+                                   const uint8_t *r_mont, size_t nr, apg_transcript *tr, uint8_t *vs_mont,
+                                   uint8_t proof[704]) {
+    return guarded([&] {
+        need(ctx);
+        need(srs);
+        need(tr);
+        if (nstacks == 0) throw ApgError(APG_ERR_ARG, "EmptyBatch");
+        need(stack_sizes);
+        need(cols_dev);
+        Context &c = ctx->c;
+        std::lock_guard<std::mutex> lk(c.mu);
+        HIP_CHECK(hipSetDevice(c.device));
+        batch_open_common(c, srs, static_cast<const Fe *>(cols_dev), stack_sizes, nstacks, width, cms_wire, u_mont,
+                          r_mont, nr, tr, vs_mont, proof);
+    });
+}
+
 int apg_msm_g1_u32(apg_ctx *ctx, const uint8_t *bases_mont, const uint32_t *scalars, size_t n,
                    uint8_t out_mont[64]) {
     return guarded([&] {

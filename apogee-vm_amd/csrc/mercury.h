@@ -34,6 +34,17 @@ void mercury_open(Context &ctx, const CommitFn &commit, const Fe *f_dev, int num
                   const uint8_t cm_wire[64], const Fe *u, Transcript &tr, Fe &v,
                   uint8_t proof[MERCURY_PROOF_BYTES]);
 
+// batch_open_stacked (crates/pcs/src/lib.rs:701-789): cols_dev holds the
+// stacks' columns back to back in slot order, 2^width each; stack s has
+// stack_sizes[s] columns (at most 2^nr); one commitment per stack. Writes each
+// stack's value at u || r and the proof of cm* = sum rho^i cm_i. Throws
+// ApgError(APG_ERR_ARG) for EmptyBatch, BatchLengthMismatch and
+// UnsupportedNumVars, as the reference refuses them.
+void mercury_batch_open_stacked(Context &ctx, const CommitFn &commit, const Fe *cols_dev,
+                                const uint32_t *stack_sizes, size_t nstacks, int width, const uint8_t *cms_wire,
+                                const Fe *u, const Fe *r, int nr, Transcript &tr, Fe *vs_out,
+                                uint8_t proof[MERCURY_PROOF_BYTES]);
+
 // In place over ncols independent sequences, element j of column i at
 // v[j * stride + i]: A_j = v_j + x A_{j+1} from j = count - 1 down to 0
 // (A_count = 0). Exposed for tests of the scan.

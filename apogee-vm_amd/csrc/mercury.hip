@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/apogee_gpu.h"
+#include "curve.cuh"
 #include "mercury.h"
 
 namespace apg {
@@ -120,6 +121,42 @@ __global__ void k_numerator(const Fe *__restrict__ f, const Fe *__restrict__ q, 
     if (i < nq) val = f_sub<Fr>(val, f_mul<Fr>(zb_a, q[i]));
     if (i == 0) val = f_sub<Fr>(val, g_z);
     num[i] = val;
+}
+
+// eq(u, i) = prod_j (bit j of i ? u_j : 1 - u_j), the entries of eq_table(u)
+// (crates/poly/src/lib.rs:233-248) in closed form
+__global__ void k_eq_point(const Fe *__restrict__ u, int width, size_t len, Fe *__restrict__ eq) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    const Fe one = f_one<Fr>();
+    Fe acc = one;
+    for (int j = 0; j < width; j++) acc = f_mul<Fr>(acc, ((i >> j) & 1) ? u[j] : f_sub<Fr>(one, u[j]));
+    eq[i] = acc;
+}
+
+// partial sums of x . y, one per block (grid-stride)
+__global__ void __launch_bounds__(MTPB) k_dot_partial(const Fe *__restrict__ x, const Fe *__restrict__ y, size_t len,
+                                                    Fe *__restrict__ partials) {
+    __shared__ Fe lds[MTPB / 64];
+    Fe acc = f_zero<Fr>();
+    for (size_t i = (size_t)blockIdx.x * MTPB + threadIdx.x; i < len; i += (size_t)gridDim.x * MTPB)
+        acc = f_add<Fr>(acc, f_mul<Fr>(x[i], y[i]));
+    const Fe s = block_sum_fr(acc, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// f*(i) = sum_s w_s stack_s[slot](row), slot = i / column, row = i % column
+// (crates/pcs/src/lib.rs:769-780); stack s's columns start at column firsts[s]
+__global__ void k_stack_combine(const Fe *__restrict__ cols, size_t column, size_t n,
+                                const uint32_t *__restrict__ firsts, const uint32_t *__restrict__ sizes,
+                                const Fe *__restrict__ w, uint32_t nstacks, Fe *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t slot = i / column, row = i - slot * column;
+    Fe acc = f_zero<Fr>();
+    for (uint32_t s = 0; s < nstacks; s++)
+        if (slot < sizes[s]) acc = f_add<Fr>(acc, f_mul<Fr>(w[s], cols[((size_t)firsts[s] + slot) * column + row]));
+    out[i] = acc;
 }
 
 // ---- host algebra over Fr (Montgomery Fe, the 32-bit-word host build) ----
@@ -433,6 +470,105 @@ void mercury_open(Context &ctx, const CommitFn &commit, const Fe *f_dev, int num
     // MercuryProof::to_bytes (crates/pcs/src/lib.rs:152-161)
     std::memcpy(proof, pts, sizeof(pts));
     for (int i = 0; i < 6; i++) fe_to_bytes_fr(ev[i], proof + 8 * 64 + 32 * i);
+}
+
+void mercury_batch_open_stacked(Context &ctx, const CommitFn &commit, const Fe *cols_dev,
+                                const uint32_t *stack_sizes, size_t nstacks, int width, const uint8_t *cms_wire,
+                                const Fe *u, const Fe *r, int nr, Transcript &tr, Fe *vs_out,
+                                uint8_t proof[MERCURY_PROOF_BYTES]) {
+    if (nstacks == 0) throw ApgError(APG_ERR_ARG, "EmptyBatch");
+    for (size_t s = 0; s < nstacks; s++)
+        if (stack_sizes[s] == 0) throw ApgError(APG_ERR_ARG, "EmptyBatch");
+    const int nv = width + nr;
+    if (width < 0 || nr < 0 || nv < 2 || nv > 30 || (nv & 1))
+        throw ApgError(APG_ERR_ARG, "UnsupportedNumVars");
+    const size_t slots = (size_t)1 << nr, column = (size_t)1 << width, n = (size_t)1 << nv;
+    std::vector<uint32_t> firsts(nstacks);
+    uint32_t ncols = 0;
+    for (size_t s = 0; s < nstacks; s++) {
+        if (stack_sizes[s] > slots) throw ApgError(APG_ERR_ARG, "BatchLengthMismatch");
+        firsts[s] = ncols;
+        ncols += stack_sizes[s];
+    }
+    std::vector<Fe> point(u, u + width);
+    point.insert(point.end(), r, r + nr);
+
+    // every column's value at u: dot with eq(u, .) (MultilinearPoly::evaluate)
+    Fe *u_dev = ctx.d<Fe>("mercury.bu", std::max(width, 1));
+    Fe *eq_u = ctx.d<Fe>("mercury.beq", column);
+    const unsigned G = (unsigned)std::min<size_t>(1024, (column + MTPB - 1) / MTPB);
+    Fe *partials = ctx.d<Fe>("mercury.bpart", (size_t)G * ncols);
+    if (width) HIP_CHECK(hipMemcpyAsync(u_dev, u, width * sizeof(Fe), hipMemcpyHostToDevice, ctx.stream));
+    hipLaunchKernelGGL(k_eq_point, dim3(grid(column)), dim3(MTPB), 0, ctx.stream, u_dev, width, column, eq_u);
+    for (uint32_t c = 0; c < ncols; c++)
+        hipLaunchKernelGGL(k_dot_partial, dim3(G), dim3(MTPB), 0, ctx.stream, cols_dev + (size_t)c * column, eq_u,
+                           column, partials + (size_t)c * G);
+    HIP_CHECK(hipGetLastError());
+    Poly part;
+    download(ctx, part, partials, (size_t)G * ncols);
+    std::vector<Fe> colval(ncols, zero());
+    for (uint32_t c = 0; c < ncols; c++)
+        for (unsigned q = 0; q < G; q++) colval[c] = add(colval[c], part[(size_t)c * G + q]);
+    const Poly eq_r = eq_table(r, (size_t)nr);
+    std::vector<Fe> vs(nstacks, zero());
+    for (size_t s = 0; s < nstacks; s++)
+        for (uint32_t j = 0; j < stack_sizes[s]; j++) vs[s] = add(vs[s], mul(eq_r[j], colval[firsts[s] + j]));
+
+    // the preamble (crates/pcs-verify/src/lib.rs:333-345), then cm* = sum rho^i cm_i
+    tr.append_g1_wire(TAG_COMMITMENT, cms_wire, nstacks);
+    std::vector<Fe> claim = point;
+    claim.insert(claim.end(), vs.begin(), vs.end());
+    tr.append_scalars(TAG_EVALUATION_CLAIM, claim.data(), claim.size());
+    const Fe rho = tr.challenge(TAG_MERCURY_BATCH);
+    std::vector<Fe> w(nstacks);
+    Fe v_star = zero(), pw = one();
+    for (size_t s = 0; s < nstacks; s++) {
+        w[s] = pw;
+        v_star = add(v_star, mul(pw, vs[s]));
+        pw = mul(pw, rho);
+    }
+    Xyzz cm = xyzz_identity();
+    for (size_t s = 0; s < nstacks; s++) {
+        Affine a;
+        const uint8_t *wire = cms_wire + 64 * s;
+        bool inf = true;
+        for (int q = 0; q < 64; q++) inf = inf && wire[q] == 0;
+        if (inf) continue;
+        if (!fe_from_bytes_fq(wire, a.x) || !fe_from_bytes_fq(wire + 32, a.y))
+            throw ApgError(APG_ERR_ARG, "batch open: a commitment is not a canonical point");
+        const Fe y2 = f_sqr<Fq>(a.y), x3 = f_add<Fq>(f_mul<Fq>(f_sqr<Fq>(a.x), a.x), f_from_u64<Fq>(3));
+        if (!f_eq(y2, x3)) throw ApgError(APG_ERR_ARG, "batch open: a commitment is off the curve");
+        const Fe k = f_from_mont<Fr>(w[s]);
+        Xyzz acc = xyzz_identity();
+        for (int bit = 255; bit >= 0; bit--) {
+            acc = xyzz_dbl(acc);
+            if ((k.l[bit >> 5] >> (bit & 31)) & 1) acc = xyzz_add_affine(acc, a);
+        }
+        cm = xyzz_add(cm, acc);
+    }
+    uint8_t cm_star[64];
+    const Affine cs = xyzz_to_affine(cm);
+    if (aff_is_inf(cs)) {
+        std::memset(cm_star, 0, 64);
+    } else {
+        fe_to_bytes_fq(cs.x, cm_star);
+        fe_to_bytes_fq(cs.y, cm_star + 32);
+    }
+
+    // the combined column, then one opening at u || r
+    uint32_t *meta = ctx.d<uint32_t>("mercury.bmeta", 2 * nstacks);
+    Fe *w_dev = ctx.d<Fe>("mercury.bw", nstacks);
+    HIP_CHECK(hipMemcpyAsync(meta, firsts.data(), nstacks * 4, hipMemcpyHostToDevice, ctx.stream));
+    HIP_CHECK(hipMemcpyAsync(meta + nstacks, stack_sizes, nstacks * 4, hipMemcpyHostToDevice, ctx.stream));
+    HIP_CHECK(hipMemcpyAsync(w_dev, w.data(), nstacks * sizeof(Fe), hipMemcpyHostToDevice, ctx.stream));
+    Fe *comb = ctx.d<Fe>("mercury.bcomb", n);
+    hipLaunchKernelGGL(k_stack_combine, dim3(grid(n)), dim3(MTPB), 0, ctx.stream, cols_dev, column, n, meta,
+                       meta + nstacks, w_dev, (uint32_t)nstacks, comb);
+    HIP_CHECK(hipGetLastError());
+    Fe v;
+    mercury_open(ctx, commit, comb, nv, cm_star, point.data(), tr, v, proof);
+    if (!f_eq(v, v_star)) throw ApgError(APG_ERR_INTERNAL, "batch open: the combined value is off");
+    for (size_t s = 0; s < nstacks; s++) vs_out[s] = vs[s];
 }
 
 }  // namespace apg

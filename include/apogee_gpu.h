@@ -96,6 +96,23 @@ int apg_pcs_open(apg_ctx *ctx, apg_srs *srs, const uint8_t *table_mont, size_t n
 int apg_pcs_open_dev(apg_ctx *ctx, apg_srs *srs, const void *table_dev, size_t num_vars,
                      const uint8_t cm_wire[64], const uint8_t *u_mont, apg_transcript *tr, uint8_t v_mont[32],
                      uint8_t proof[704]);
+/* batch_open_stacked (crates/pcs/src/lib.rs:701-789; batch_open, :689, is
+ * nr = 0 with one column per stack): cols holds the stacks' columns back to
+ * back in slot order, 2^width Montgomery Fr each; stack s has stack_sizes[s]
+ * columns (1 .. 2^nr); cms_wire one 64-byte commitment per stack, absorbed as
+ * passed. Opens cm* = sum rho^i cm_i at u (width) || r (nr). Out: vs_mont, one
+ * value per stack (sum_j eq(r, j) col_j(u)), and the 704-byte proof.
+ * APG_ERR_ARG for the reference's EmptyBatch, BatchLengthMismatch,
+ * UnsupportedNumVars (width + nr) and SrsTooSmall; APG_ERR_DEGENERATE as open.
+ * _dev: cols is device memory. */
+int apg_pcs_batch_open_stacked(apg_ctx *ctx, apg_srs *srs, const uint8_t *cols_mont, const uint32_t *stack_sizes,
+                               size_t nstacks, size_t width, const uint8_t *cms_wire, const uint8_t *u_mont,
+                               const uint8_t *r_mont, size_t nr, apg_transcript *tr, uint8_t *vs_mont,
+                               uint8_t proof[704]);
+int apg_pcs_batch_open_stacked_dev(apg_ctx *ctx, apg_srs *srs, const void *cols_dev, const uint32_t *stack_sizes,
+                                   size_t nstacks, size_t width, const uint8_t *cms_wire, const uint8_t *u_mont,
+                                   const uint8_t *r_mont, size_t nr, apg_transcript *tr, uint8_t *vs_mont,
+                                   uint8_t proof[704]);
 
 /* The MSM accumulation's Fq arithmetic in nine 29-bit words with lazy
  * reduction (field29.cuh), on the host (ctx NULL) or in a kernel on ctx's

@@ -98,6 +98,7 @@ def _declare(L):
     d("orc_interpolate", [P, SZ, P])
     d("orc_prove_zerocheck", [P, SZ, P, SZ, SZ, P, P, P])
     d("orc_mercury_open", [P, SZ, P, SZ, P, P, P, SZ, P, P], C.c_int)
+    d("orc_mercury_batch_open_stacked", [P, SZ, P, P, SZ, SZ, P, P, P, SZ, P, SZ, P, P], C.c_int)
     d("orc_prove_sumcheck", [P, SZ, P, SZ, P, SZ, P, SZ, P, P])
     d("orc_witness_digest", [P, SZ, SZ, P])
     d("orc_combine", [P, SZ, SZ, P, SZ, P])
@@ -385,6 +386,28 @@ def mercury_open(srs: np.ndarray, table: np.ndarray, cm_wire: bytes, u: np.ndarr
     if rc != 0:
         raise ValueError(f"orc_mercury_open: {rc}")
     return v, proof.tobytes()
+
+
+def mercury_batch_open_stacked(srs: np.ndarray, stacks, cms_wire, u: np.ndarray, r, transcript: "Transcript",
+                               workers: int = 8):
+    """batch_open_stacked (crates/pcs/src/lib.rs:701-789); stacks is a list of
+    lists of columns ((2^width, 4) arrays). Returns (vs, proof bytes)."""
+    srs = np.ascontiguousarray(srs, dtype=np.uint64)
+    sizes = np.array([len(s) for s in stacks], dtype=np.uint32)
+    flat = [c for s in stacks for c in s]
+    cols = np.ascontiguousarray(np.concatenate(flat) if flat else np.zeros((1, 4), dtype=np.uint64), dtype=np.uint64)
+    u = np.ascontiguousarray(np.asarray(u, dtype=np.uint64).reshape(-1, 4))
+    nr = len(r)
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1, 4)) if nr else np.zeros((1, 4), np.uint64)
+    cms = np.frombuffer(b"".join(bytes(c) for c in cms_wire) + b"\0", dtype=np.uint8).copy()
+    vs = fe(max(len(stacks), 1))
+    proof = np.zeros(704, dtype=np.uint8)
+    rc = lib().orc_mercury_batch_open_stacked(ptr(srs), srs.shape[0], ptr(cols), ptr(sizes) if len(sizes) else None,
+                                              len(stacks), u.shape[0], ptr(cms), ptr(u), ptr(r), nr,
+                                              transcript.ref(), workers, ptr(vs), ptr(proof))
+    if rc != 0:
+        raise ValueError(f"orc_mercury_batch_open_stacked: {rc}")
+    return vs[:len(stacks)], proof.tobytes()
 
 
 def poseidon2_permute(state: np.ndarray) -> np.ndarray:

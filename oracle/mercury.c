@@ -37,6 +37,7 @@ enum {
     TAG_BDFG_BATCH = 14,
     TAG_BDFG_POINT = 15,
     TAG_PAIRING_MERGE = 16,
+    TAG_MERCURY_BATCH = 17,
 };
 
 static fe F0(void) {
@@ -369,7 +370,7 @@ int orc_mercury_open(const uint64_t *srs_aff, size_t srs_len, const uint64_t *ta
     /* step 2: the fold, b Horner divisions by X - alpha interleaved over rows */
     fe *acc = (fe *)malloc(b * sizeof(fe));
     memcpy(acc, coeffs + (b - 1) * b, b * sizeof(fe));
-    fe *q = (fe *)calloc((b - 1) * b ? (b - 1) * b : 1, sizeof(fe));
+    fe *q = (fe *)calloc(b > 1 ? (b - 1) * b : 1, sizeof(fe));
     for (size_t j = b - 1; j-- > 0;) {
         memcpy(q + j * b, acc, b * sizeof(fe));
         for (size_t i = 0; i < b; i++) acc[i] = fadd(coeffs[j * b + i], fmul(acc[i], alpha));
@@ -484,5 +485,105 @@ out:
     free(h);
     free(eq1);
     free(eq2);
+    return rc;
+}
+
+/* pcs/src/lib.rs:701-789 batch_open_stacked (batch_open is r = [], one column
+ * per stack, :689-698): each column's value at u, the stacks' values
+ * sum_j eq(r, j) col_j(u), the batch preamble (pcs-verify/src/lib.rs:333-345:
+ * the commitments as one message, u || r then the values, rho), cm* = sum
+ * rho^i cm_i by an MSM, the combined column, then one opening at u || r.
+ * cols: the stacks' columns back to back in slot order, 2^width each.
+ * Returns 0, -9 EmptyBatch, -10 BatchLengthMismatch, -11 bad commitment,
+ * -2 / -3 as open, or open's own errors. */
+int orc_mercury_batch_open_stacked(const uint64_t *srs_aff, size_t srs_len, const uint64_t *cols,
+                                   const uint32_t *stack_sizes, size_t nstacks, size_t width,
+                                   const uint8_t *cms_wire, const uint64_t *u_in, const uint64_t *r_in,
+                                   size_t nr, orc_transcript *tr, size_t workers, uint64_t *vs_out,
+                                   uint8_t *proof_out) {
+    if (nstacks == 0) return -9;
+    for (size_t s = 0; s < nstacks; s++)
+        if (stack_sizes[s] == 0) return -9;
+    const size_t nv = width + nr;
+    if (nv < 2 || nv > 54 || (nv & 1)) return -2;
+    const size_t slots = (size_t)1 << nr;
+    for (size_t s = 0; s < nstacks; s++)
+        if (stack_sizes[s] > slots) return -10;
+    const size_t n = (size_t)1 << nv, column = (size_t)1 << width;
+    if (srs_len < n) return -3;
+    const fe *c = (const fe *)cols;
+
+    fe *point = (fe *)malloc((nv ? nv : 1) * sizeof(fe));
+    memcpy(point, u_in, width * sizeof(fe));
+    memcpy(point + width, r_in, nr * sizeof(fe));
+    fe *eq_r = (fe *)malloc(slots * sizeof(fe));
+    orc_eq_table(r_in, nr, (u64 *)eq_r);
+    fe *vs = (fe *)malloc(nstacks * sizeof(fe));
+    size_t first = 0;
+    for (size_t s = 0; s < nstacks; s++) {
+        fe acc = F0();
+        for (size_t j = 0; j < stack_sizes[s]; j++) {
+            fe val;
+            orc_evaluate((const u64 *)(c + (first + j) * column), width, u_in, val.v);
+            acc = fadd(acc, fmul(eq_r[j], val));
+        }
+        vs[s] = acc;
+        first += stack_sizes[s];
+    }
+
+    /* the preamble */
+    absorb_points(tr, TAG_COMMITMENT, cms_wire, nstacks);
+    fe *claim = (fe *)malloc((nv + nstacks) * sizeof(fe));
+    memcpy(claim, point, nv * sizeof(fe));
+    memcpy(claim + nv, vs, nstacks * sizeof(fe));
+    orc_tr_append_scalars(tr, TAG_EVALUATION_CLAIM, (const u64 *)claim, nv + nstacks);
+    free(claim);
+    fe rho;
+    orc_tr_challenge(tr, TAG_MERCURY_BATCH, rho.v);
+    fe *w = (fe *)malloc(nstacks * sizeof(fe));
+    fe v_star = F0(), pw = Fu(1);
+    for (size_t s = 0; s < nstacks; s++) {
+        w[s] = pw;
+        v_star = fadd(v_star, fmul(pw, vs[s]));
+        pw = fmul(pw, rho);
+    }
+    int rc = 0;
+    uint8_t cm_star[64];
+    {
+        u64 *bases = (u64 *)malloc(nstacks * 64);
+        for (size_t s = 0; s < nstacks && !rc; s++)
+            if (!orc_g1_affine_from_wire(cms_wire + 64 * s, bases + 8 * s)) rc = -11;
+        if (!rc) {
+            u64 jac[12], aff[8];
+            orc_msm(bases, (const u64 *)w, nstacks, workers, workers, jac);
+            orc_g1_to_affine(jac, aff);
+            orc_g1_affine_to_wire(aff, cm_star);
+        }
+        free(bases);
+    }
+    fe *comb = NULL;
+    if (!rc) {
+        comb = (fe *)calloc(n, sizeof(fe));
+        for (size_t i = 0; i < n; i++) {
+            const size_t slot = i / column, row = i % column;
+            fe acc = F0();
+            size_t f0 = 0;
+            for (size_t s = 0; s < nstacks; s++) {
+                if (slot < stack_sizes[s]) acc = fadd(acc, fmul(w[s], c[(f0 + slot) * column + row]));
+                f0 += stack_sizes[s];
+            }
+            comb[i] = acc;
+        }
+        fe v;
+        rc = orc_mercury_open(srs_aff, srs_len, (const u64 *)comb, nv, cm_star, (const u64 *)point, tr, workers,
+                              v.v, proof_out);
+        if (!rc && !feq(v, v_star)) rc = -12;
+        if (!rc) memcpy(vs_out, vs, nstacks * sizeof(fe));
+    }
+    free(comb);
+    free(w);
+    free(vs);
+    free(eq_r);
+    free(point);
     return rc;
 }

@@ -139,6 +139,17 @@ int orc_mercury_open(const uint64_t *srs_aff, size_t srs_len, const uint64_t *ta
                      const uint8_t *cm_wire, const uint64_t *u, orc_transcript *tr, size_t workers,
                      uint64_t *v_out, uint8_t *proof_out);
 
+/* batch_open_stacked (crates/pcs/src/lib.rs:701-789): cols holds the stacks'
+ * columns back to back in slot order, 2^width Fr each; stack_sizes[s] columns
+ * in stack s (at most 2^nr); cms_wire: one 64-byte commitment per stack; the
+ * opening point is u (width) || r (nr). Out: vs (one Fr per stack) and the
+ * 704-byte proof. Returns 0, -9 EmptyBatch, -10 BatchLengthMismatch, -11 an
+ * undecodable commitment, or open's codes. */
+int orc_mercury_batch_open_stacked(const uint64_t *srs_aff, size_t srs_len, const uint64_t *cols,
+                                   const uint32_t *stack_sizes, size_t nstacks, size_t width,
+                                   const uint8_t *cms_wire, const uint64_t *u, const uint64_t *r, size_t nr,
+                                   orc_transcript *tr, size_t workers, uint64_t *vs_out, uint8_t *proof_out);
+
 #ifdef __cplusplus
 }
 #endif

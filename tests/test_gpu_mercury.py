@@ -104,3 +104,60 @@ def test_open_errors(apg, orc, srs16):
     small = apg.Srs.generate(rng.next_fr(), 8)
     with pytest.raises(apg.SrsTooSmall):
         apg.open(small, f4, bytes(64), np.stack([rng.next_fr() for _ in range(4)]), apg.Transcript())
+
+
+def test_batch_open_replays_committed_proof(apg, orc):
+    """batch_open of one column replays mercury_batch.txt: the preamble before
+    the opening moves every challenge (crates/pcs/tests/vectors/mercury_batch.txt)."""
+    recs = list(golden_lines("mercury_batch.txt"))
+    one = lambda key: next(r for r in recs if r[0] == key)[1]
+    nv = int(one("numvars"))
+    vals = {int(r[1]): r[2] for r in recs if r[0] == "value"}
+    pts = {int(r[1]): r[2] for r in recs if r[0] == "point"}
+    f = np.stack([orc.from_hex_le(orc.FR, vals[i]) for i in range(1 << nv)])
+    u = np.stack([orc.from_hex_le(orc.FR, pts[i]) for i in range(nv)])
+    srs = apg.Srs.generate(orc.from_hex_le(orc.FR, one("tau")), 1 << nv)
+    cm = bytes.fromhex(one("commitment"))
+    tr = apg.Transcript()
+    vs, proof = apg.batch_open(srs, [f], [cm], u, tr)
+    assert orc.to_hex_le(orc.FR, vs[0]) == one("claim")
+    assert proof.hex() == one("proof")
+    assert orc.to_hex_le(orc.FR, tr.sample()) == one("probe")
+
+
+@pytest.mark.parametrize("width,nr,shape", [(6, 0, [1, 1, 1]), (5, 1, [2, 1]), (4, 2, [4, 3, 1]),
+                                             (11, 1, [2, 2, 1, 2])])
+def test_batch_open_stacked_matches_oracle(apg, orc, srs16, width, nr, shape):
+    _, srs, powers = srs16
+    rng = orc.SplitMix(100 * width + nr)
+    stacks = [[np.stack([rng.next_fr() for _ in range(1 << width)]) for _ in range(k)] for k in shape]
+    cms = []
+    for s in stacks:
+        table = np.concatenate(s + [np.zeros((1 << width, 4), dtype=np.uint64)] * ((1 << nr) - len(s)))
+        cms.append(apg.g1_to_wire(apg.commit(srs, table)))
+    u = np.stack([rng.next_fr() for _ in range(width)])
+    r = [rng.next_fr() for _ in range(nr)]
+    tr_g, tr_o = apg.Transcript(), orc.Transcript()
+    vs, proof = apg.batch_open_stacked(srs, stacks, cms, u, r, tr_g)
+    vs_o, proof_o = orc.mercury_batch_open_stacked(powers, stacks, cms, u, r, tr_o, workers=16)
+    assert np.array_equal(vs, vs_o)
+    assert proof == proof_o
+    assert np.array_equal(tr_g.sample(), tr_o.sample())
+
+
+def test_batch_open_errors(apg, orc, srs16):
+    _, srs, _ = srs16
+    rng = orc.SplitMix(3)
+    c = np.stack([rng.next_fr() for _ in range(16)])
+    u = np.stack([rng.next_fr() for _ in range(4)])
+    with pytest.raises(apg.EmptyBatch):
+        apg.batch_open_stacked(srs, [], [], u, [], apg.Transcript())
+    with pytest.raises(apg.BatchLengthMismatch):
+        apg.batch_open(srs, [c, c], [bytes(64)], u, apg.Transcript())
+    with pytest.raises(apg.MixedColumnSizes):
+        apg.batch_open(srs, [c, c[:8]], [bytes(64)] * 2, u, apg.Transcript())
+    c8 = c[:8]
+    with pytest.raises(apg.BatchLengthMismatch):
+        apg.batch_open_stacked(srs, [[c8, c8, c8]], [bytes(64)], u[:3], [rng.next_fr()], apg.Transcript())
+    with pytest.raises(apg.UnsupportedNumVars):
+        apg.batch_open_stacked(srs, [[c, c]], [bytes(64)], u, [rng.next_fr()], apg.Transcript())

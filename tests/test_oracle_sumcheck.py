@@ -199,3 +199,44 @@ def test_mercury_open_kat_detects_changes(orc):
     assert proof.hex()[:128] != k["proof"][:128] or orc.to_hex_le(orc.FR, v) != k["claim"]
     with pytest.raises(ValueError):
         orc.mercury_open(srs, k["f"][:8], k["cm"], k["u"][:3], orc.Transcript())
+
+
+def test_mercury_batch_open_kat(orc):
+    """batch_open of one column (crates/pcs/src/lib.rs:689-789) replays the
+    committed batched proof at n = 2^4: the preamble (the commitment list, u
+    then the values, rho) runs before the opening and moves every challenge,
+    so this pins its order, tags and encodings (mercury_batch.txt)."""
+    k = _mercury_kat(orc, "mercury_batch.txt")
+    srs = orc.srs_powers(k["tau"], 1 << k["nv"])
+    tr = orc.Transcript()
+    vs, proof = orc.mercury_batch_open_stacked(srs, [[k["f"]]], [k["cm"]], k["u"], [], tr)
+    assert orc.to_hex_le(orc.FR, vs[0]) == k["claim"]
+    assert proof.hex() == k["proof"]
+    assert orc.to_hex_le(orc.FR, tr.sample()) == k["probe"]
+
+
+def test_mercury_batch_open_stacked_properties(orc):
+    """Stacks at u || r: each stack's value is sum_j eq(r, j) col_j(u), which
+    is the stack's multilinear extension evaluated at u || r (its columns laid
+    out slot-major); with one stack of one column and r = [] the batch is a
+    bare batch_open. Refusals: an empty batch, a stack with more columns than
+    slots."""
+    rng = orc.SplitMix(77)
+    width, nr = 3, 1
+    cols = [np.stack([rng.next_fr() for _ in range(1 << width)]) for _ in range(3)]
+    stacks = [[cols[0], cols[1]], [cols[2]]]
+    srs = orc.srs_powers(rng.next_fr(), 1 << (width + nr))
+    cms = [orc.jac_to_wire(orc.msm(srs[: 1 << (width + nr)], np.concatenate(
+        [np.concatenate(s), np.zeros(((2 - len(s)) << width, 4), dtype=np.uint64)]))) for s in stacks]
+    u = np.stack([rng.next_fr() for _ in range(width)])
+    r = np.stack([rng.next_fr() for _ in range(nr)])
+    vs, proof = orc.mercury_batch_open_stacked(srs, stacks, cms, u, r, orc.Transcript())
+    for s, v in zip(stacks, vs):
+        table = np.concatenate([np.concatenate(s), np.zeros(((2 - len(s)) << width, 4), dtype=np.uint64)])
+        want = orc.evaluate(table, np.concatenate([u, r]))
+        assert np.array_equal(v, want)
+    assert len(proof) == 704
+    with pytest.raises(ValueError):
+        orc.mercury_batch_open_stacked(srs, [], [], u, r, orc.Transcript())
+    with pytest.raises(ValueError):
+        orc.mercury_batch_open_stacked(srs, [[cols[0], cols[1], cols[2]]], cms[:1], u, r, orc.Transcript())
