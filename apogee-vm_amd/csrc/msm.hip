@@ -253,7 +253,7 @@ __device__ __forceinline__ Span merge_span(const uint32_t *__restrict__ off, uin
     return {2 * t0 + 1, 2 * (t1 - t0)};
 }
 
-constexpr uint32_t LIGHT_ITEMS = 16;  // more partials than this: a heavy bucket
+constexpr uint32_t LIGHT_ITEMS = 32;  // more partials than this: a heavy bucket
 constexpr size_t ACC_ROUNDS = 4, MIN_SLICE = 16, MAX_SLICE = 4096;
 
 // One thread per bucket: empty buckets become the identity, buckets with at
@@ -600,11 +600,11 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
 
     // 3. accumulate over equal slices of the sorted entries: about ACC_ROUNDS
     // slices per resident thread (4 waves per SIMD), at least MIN_SLICE entries
-    // and half the mean bucket, so a typical bucket crosses at most two slice
-    // boundaries and stays light
+    // and an eighth of the mean bucket, so a typical bucket crosses about eight
+    // slice boundaries and stays light (16 partials of at most LIGHT_ITEMS)
     const size_t resident = (size_t)ctx.cu_count * 4 * 4 * 64;
     const size_t by_grid = (total + ACC_ROUNDS * resident - 1) / (ACC_ROUNDS * resident);
-    const size_t by_bucket = total / (2 * (size_t)std::max<uint32_t>(nkeys, 1));
+    const size_t by_bucket = total / (8 * (size_t)std::max<uint32_t>(nkeys, 1));
     const uint32_t L = (uint32_t)std::min<size_t>(MAX_SLICE, std::max({MIN_SLICE, by_grid, by_bucket}));
     const uint32_t nslices = (total + L - 1) / L;
     Xyzz29 *buckets = ctx.d<Xyzz29>("msm.buckets", nkeys);

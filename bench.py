@@ -16,6 +16,9 @@ Workloads (--workload):
          latency (and latency over the shard's isolated time), HBM occupancy.
   copy   1 GiB device copies by the library's k_copy16 kernel: the
          known-size reference for rocprofv3 FETCH_SIZE / WRITE_SIZE passes.
+  open   pcs::open (the Mercury opening) of one witness-mix column of
+         2^--open-vars rows resident in HBM, a step per opening; the oracle
+         opens a 2^16 column on the host beside it (parity checked).
 Inputs are generated into HBM before any timed region. One JSON line on
 stdout; progress on stderr. The mixed run reports the bucket-accumulation
 kernel's roofline over the engine's timed passes (HIP events on each launch's
@@ -24,7 +27,7 @@ single 2^20 x 8 shard), the single large shard's numbers, and a CPU baseline:
 the oracle's restatement of the reference algorithms timed on a sample of the
 batch's shards on the host cores, whose outputs are checked against the GPU's.
 
-    python bench.py [--workload mixed|shard|serve|copy] [--steps K] [--warmup W]
+    python bench.py [--workload mixed|shard|serve|copy|open] [--steps K] [--warmup W]
                     [--workers N] [--policy fifo|shortest] [--reserve R] [--lane-cus C]
 """
 from __future__ import annotations
@@ -255,13 +258,66 @@ def copy_calibration(ctx, args, nbytes=1 << 30):
             "ms_per_step": round(1e3 * dt, 4), "frac_of_hbm_peak": round(2 * nbytes / dt / PEAK_HBM, 4)}
 
 
+def run_open(ctx, args):
+    """pcs::open of one witness-mix column of 2^open_vars rows resident in
+    HBM: warmup, then `steps` timed openings (a fresh transcript each), every
+    proof compared with the first. The CPU leg is the oracle's opening of a
+    2^16 column (the same inputs opened on the GPU too, parity checked),
+    scaled to the timed size by rows."""
+    nv = args.open_vars
+    n = 1 << nv
+    tau = apg.synthetic(args.seed, 1, "uniform")[0]
+    srs = apg.Srs.generate(tau, n, ctx)
+    f = apg.synthetic_device(ctx, args.seed + 1, n, "witness")
+    cm = apg.g1_to_wire(apg.commit_device(srs, f, n, 1, ctx)[0])
+    u = apg.synthetic(args.seed + 2, nv, "uniform")
+    log(f"open: 2^{nv} rows, warmup {args.warmup}")
+    first = None
+    for _ in range(max(args.warmup, 1)):
+        _, first = apg.open(srs, f, cm, u, apg.Transcript(), ctx, num_vars=nv)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    same_all = True
+    for _ in range(args.steps):
+        _, proof = apg.open(srs, f, cm, u, apg.Transcript(), ctx, num_vars=nv)
+        same_all = same_all and proof == first
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f"open: {dt * 1e3:.2f} ms per opening")
+    cpu = None
+    if args.cpu_budget > 0:
+        import oracle
+        snv = 16
+        sf = apg.synthetic(args.seed + 3, 1 << snv, "witness")
+        su = apg.synthetic(args.seed + 4, snv, "uniform")
+        ssrs = apg.Srs.generate(tau, 1 << snv, ctx)
+        powers = ssrs.g1()
+        scm = apg.g1_to_wire(apg.commit(ssrs, sf))
+        _, gpu_proof = apg.open(ssrs, sf, scm, su, apg.Transcript(), ctx)
+        workers = min(16, os.cpu_count() or 1)
+        t1 = time.perf_counter()
+        _, cpu_proof = oracle.mercury_open(powers, sf, scm, su, oracle.Transcript(), workers=workers)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": round((1 << snv) / n / cpu_s, 5), "unit": "openings/s", "cores": workers, "kind": "port",
+               "sample": f"one 2^{snv}-row opening in {cpu_s:.2f} s, scaled to 2^{nv} rows by row count",
+               "parity": gpu_proof == cpu_proof}
+    return {"metric": f"Mercury openings/s (pcs::open of one 2^{nv}-row column resident in HBM)",
+            "value": round(1.0 / dt, 3), "unit": "openings/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u32x8 (BN254 Fr/Fq Montgomery, exact)",
+            "data": "synthetic (counter-mode ChaCha20 witness mix in HBM)",
+            "config": {"workload": f"pcs::open, one column of 2^{nv} rows", "parallelism": "1 GPU"},
+            "deterministic": same_all, "cpu_baseline": cpu}
+
+
 def main():
     global apg, wl
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve", "copy"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve", "copy", "open"])
+    ap.add_argument("--open-vars", type=int, default=20, help="log2 rows of the opened table (--workload open)")
     # engine defaults: throughput (mixed) 8 workers in arrival order; serving
     # 4 workers (within the hardware queues the device services at once), one
     # of them a fast lane on 32 CUs, shortest job first
@@ -289,6 +345,10 @@ def main():
 
     if args.workload == "copy":
         print(json.dumps(copy_calibration(ctx, args)), flush=True)
+        return
+
+    if args.workload == "open":
+        print(json.dumps(run_open(ctx, args)), flush=True)
         return
 
     if args.workload == "shard":
