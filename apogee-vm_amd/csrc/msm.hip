@@ -209,8 +209,18 @@ __global__ void __launch_bounds__(TPB) k_accumulate(const uint32_t *__restrict__
     // are stored in that form
     Xyzz29 acc{};
     bool id = true;
+    // keys and entries arrive four at a time as 16-byte vectors (L is a
+    // multiple of 4, so lo is 16-byte aligned; the arrays carry 4 words of
+    // slack past total): one line fetch serves four iterations
+    uint4 kv = make_uint4(0, 0, 0, 0), ev = kv;
     for (uint32_t e = lo; e < hi; e++) {
-        const uint32_t key = skeys[e];
+        const uint32_t q = (e - lo) & 3;
+        if (q == 0) {
+            kv = *reinterpret_cast<const uint4 *>(skeys + e);
+            ev = *reinterpret_cast<const uint4 *>(entries + e);
+        }
+        const uint32_t key = q == 0 ? kv.x : q == 1 ? kv.y : q == 2 ? kv.z : kv.w;
+        const uint32_t ent = q == 0 ? ev.x : q == 1 ? ev.y : q == 2 ? ev.z : ev.w;
         if (key != cur) {
             if (in_head) {
                 st29_acc(part + 2 * (size_t)t, acc, id);
@@ -222,7 +232,6 @@ __global__ void __launch_bounds__(TPB) k_accumulate(const uint32_t *__restrict__
             cur = key;
             id = true;
         }
-        const uint32_t ent = entries[e];
         xyzz29_accumulate(acc, id, load_affine(bases, ent & ~SIGN), (ent & SIGN) != 0);
     }
     bool tail_done = false;
@@ -584,8 +593,9 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     // saved pass), then bucket boundaries
     int key_bits = 1;
     while ((1ull << key_bits) < (uint64_t)nkeys) key_bits++;
-    uint32_t *skeys = ctx.d<uint32_t>("msm.skeys", std::max<uint32_t>(total, 1));
-    uint32_t *entries = ctx.d<uint32_t>("msm.entries", std::max<uint32_t>(total, 1));
+    // 4 words of slack: the accumulation reads keys and entries as 16-byte vectors
+    uint32_t *skeys = ctx.d<uint32_t>("msm.skeys", (size_t)total + 4);
+    uint32_t *entries = ctx.d<uint32_t>("msm.entries", (size_t)total + 4);
     size_t tmp_bytes = 0;
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_in, skeys, vals_in, entries,
                                                  (int)total, 0, key_bits, st));
@@ -605,7 +615,7 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     const size_t resident = (size_t)ctx.cu_count * 4 * 4 * 64;
     const size_t by_grid = (total + ACC_ROUNDS * resident - 1) / (ACC_ROUNDS * resident);
     const size_t by_bucket = total / (8 * (size_t)std::max<uint32_t>(nkeys, 1));
-    const uint32_t L = (uint32_t)std::min<size_t>(MAX_SLICE, std::max({MIN_SLICE, by_grid, by_bucket}));
+    const uint32_t L = (uint32_t)(std::min<size_t>(MAX_SLICE, std::max({MIN_SLICE, by_grid, by_bucket})) + 3) & ~3u;
     const uint32_t nslices = (total + L - 1) / L;
     Xyzz29 *buckets = ctx.d<Xyzz29>("msm.buckets", nkeys);
     Xyzz29 *part = ctx.d<Xyzz29>("msm.part", 2 * (size_t)std::max<uint32_t>(nslices, 1));
