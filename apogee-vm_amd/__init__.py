@@ -146,6 +146,7 @@ def _declare(L):
     d("apg_transcript_append_g1", [VP, U64, VP, SZ], None)
     d("apg_transcript_round", [VP, VP, SZ, VP])
     d("apg_transcript_state", [VP, VP])
+    d("apg_transcript_set_state", [VP, VP])
     d("apg_fr_to_bytes", [VP, VP])
     d("apg_fr_from_bytes", [VP, VP])
     d("apg_fq_to_bytes", [VP, VP])
@@ -724,6 +725,13 @@ class Transcript:
         out = np.zeros(232, dtype=np.uint8)
         _check(lib().apg_transcript_state(self.h, _p(out)))
         return out.tobytes()
+
+    def set_state(self, state: bytes):
+        """The inverse of state(): the 232-byte sponge state, buffers and fill counts."""
+        buf = np.frombuffer(bytes(state), dtype=np.uint8).copy()
+        if buf.size != 232:
+            raise ValueError("a transcript state is 232 bytes")
+        _check(lib().apg_transcript_set_state(self.h, _p(buf)))
 
 
 # ------------------------------------------------------------------ sumcheck

@@ -905,6 +905,32 @@ int apg_transcript_state(const apg_transcript *t, uint8_t out[232]) {
     return APG_OK;
 }
 
+int apg_transcript_set_state(apg_transcript *t, const uint8_t in[232]) {
+    if (!t || !in) return APG_ERR_ARG;
+    Fe st[3], ip[2], op[2];
+    for (int i = 0; i < 3; i++) st[i] = load_fe(in + 32 * i);
+    for (int i = 0; i < 2; i++) ip[i] = load_fe(in + 96 + 32 * i);
+    for (int i = 0; i < 2; i++) op[i] = load_fe(in + 160 + 32 * i);
+    uint32_t il, ol;
+    std::memcpy(&il, in + 224, 4);
+    std::memcpy(&ol, in + 228, 4);
+    if (il > 2 || ol > 2) return APG_ERR_ARG;
+    for (const Fe &x : st)
+        if (!f_is_canonical<Fr>(x)) return APG_ERR_ARG;
+    for (const Fe &x : ip)
+        if (!f_is_canonical<Fr>(x)) return APG_ERR_ARG;
+    for (const Fe &x : op)
+        if (!f_is_canonical<Fr>(x)) return APG_ERR_ARG;
+    for (int i = 0; i < 3; i++) t->t.state[i] = st[i];
+    for (int i = 0; i < 2; i++) {
+        t->t.input[i] = ip[i];
+        t->t.output[i] = op[i];
+    }
+    t->t.input_len = il;
+    t->t.output_len = ol;
+    return APG_OK;
+}
+
 int apg_fr_to_bytes(const uint8_t mont[32], uint8_t canonical[32]) {
     Fe a = load_fe(mont);
     if (!f_is_canonical<Fr>(a)) return APG_ERR_ARG;

@@ -318,6 +318,12 @@ int apg_transcript_round(void *transcript, const uint8_t *coeffs_mont, size_t n_
 /* sponge state: state[3], input[2], output[2] (Montgomery, 32 B each), then
  * input_len and output_len as u32: 232 bytes */
 int apg_transcript_state(const apg_transcript *t, uint8_t out[232]);
+/* The inverse of apg_transcript_state: sets the sponge state, both buffers
+ * and their fill counts from the same 232-byte layout, so a caller can hand
+ * its own transcript's state across the boundary and take it back
+ * (crates/transcript/src/lib.rs:197-346 keeps these fields). APG_ERR_ARG for
+ * a non-canonical element or a fill count above 2. */
+int apg_transcript_set_state(apg_transcript *t, const uint8_t in[232]);
 
 /* ---- encodings ---- */
 int apg_fr_to_bytes(const uint8_t mont[32], uint8_t canonical[32]);

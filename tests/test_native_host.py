@@ -324,3 +324,34 @@ def test_witness_digest_matches_oracle(apg, orc, k, nv):
 def test_witness_digest_rejects_no_columns(apg):
     with pytest.raises(ValueError):
         apg.witness_digest([])
+
+
+def test_transcript_state_round_trip(apg, orc):
+    """apg_transcript_set_state inverts apg_transcript_state: a transcript
+    moved across mid-message (one scalar in the rate buffer, a squeeze
+    pending) continues exactly as the original and as the oracle; malformed
+    states are refused."""
+    rng = orc.SplitMix(31)
+    a, o = apg.Transcript(), orc.Transcript()
+    for step in range(5):
+        xs = [rng.next_fr() for _ in range(step + 1)]
+        a.append_scalars(6, xs)
+        o.append_scalars(6, xs)
+        a.observe(xs[0])
+        o.observe(xs[0])
+        b = apg.Transcript()
+        b.set_state(a.state())
+        assert b.state() == a.state()
+        for t in (a, b):
+            t.append_scalars(4, xs[:1])
+        o.append_scalars(4, xs[:1])
+        ca, cb, co = a.challenge_scalar(5), b.challenge_scalar(5), o.challenge(5)
+        assert np.array_equal(ca, cb) and np.array_equal(ca, co)
+    bad = bytearray(a.state())
+    bad[224] = 3
+    with pytest.raises(apg.ApgError):
+        apg.Transcript().set_state(bytes(bad))
+    bad = bytearray(a.state())
+    bad[0:32] = b"\xff" * 32
+    with pytest.raises(apg.ApgError):
+        apg.Transcript().set_state(bytes(bad))
