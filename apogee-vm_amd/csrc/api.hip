@@ -473,6 +473,7 @@ int apg_msm_g1_fr(apg_ctx *ctx, const uint8_t *bases_mont, const uint8_t *scalar
         tab.copies = 1;
         Fe *sc = c.d<Fe>("api.scalars", n);
         HIP_CHECK(hipMemcpyAsync(tab.points, bases_mont, n * 64, hipMemcpyHostToDevice, c.stream));
+        msm_table_convert(c, tab.points, n);
         HIP_CHECK(hipMemcpyAsync(sc, scalars_mont, n * 32, hipMemcpyHostToDevice, c.stream));
         Affine res;
         msm_run(c, tab, n, 1, sc, nullptr, &res);
@@ -607,6 +608,7 @@ int apg_msm_g1_u32(apg_ctx *ctx, const uint8_t *bases_mont, const uint32_t *scal
         tab.copies = 1;
         uint32_t *sc = c.d<uint32_t>("api.u32", n);
         HIP_CHECK(hipMemcpyAsync(tab.points, bases_mont, n * 64, hipMemcpyHostToDevice, c.stream));
+        msm_table_convert(c, tab.points, n);
         HIP_CHECK(hipMemcpyAsync(sc, scalars, n * 4, hipMemcpyHostToDevice, c.stream));
         Affine res;
         msm_run(c, tab, n, 1, nullptr, sc, &res);
@@ -971,7 +973,7 @@ APG_HD Xyzz debug_chain(int form, const Affine *pts, const uint8_t *neg, uint64_
     if (form == 0) {
         Xyzz29 acc{};
         bool id = true;
-        for (uint64_t i = lo; i < hi; i++) xyzz29_accumulate(acc, id, pts[i], neg[i] != 0);
+        for (uint64_t i = lo; i < hi; i++) xyzz29_accumulate(acc, id, affine_to_table(pts[i]), neg[i] != 0);
         return xyzz29_to32(acc, id);
     }
     if (form == 1) {

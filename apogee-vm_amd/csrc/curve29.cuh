@@ -1,8 +1,15 @@
-// The MSM's bucket accumulation in XYZZ coordinates over the lazy 29-bit Fq
-// (field29.cuh): the same formulas as curve.cuh's mixed addition and affine
-// doubling, with every intermediate kept in [0, 2p) and one full reduction
-// when the accumulator is written back in the 32-bit form. The group element,
-// and so its fully reduced coordinates, are the same as the 32-bit path's.
+// The MSM's bucket phase in XYZZ coordinates over the lazy 29-bit Fq of
+// field29.cuh (Montgomery domain R' = 2^261): the same formulas as
+// curve.cuh's mixed addition, addition and doublings. The group element, and
+// so its fully reduced coordinates after leaving the domain, are the same as
+// the 32-bit path's.
+//
+// Values are kept below fixed multiples of p instead of below p. With table
+// coordinates below p, every accumulator or bucket point of the phase has
+// X < 7.2p, Y < 3.3p, ZZ < 1.1p and ZZZ < 1.1p (the bounds each formula
+// below restates); a product of two values below 12p is below 1.1p, so the
+// subtractions add 2p, 4p or 8p above the subtrahend's bound and nothing is
+// reduced until the phase's output. The identity is ZZ = ZZZ = 0.
 #pragma once
 #include "curve.cuh"
 #include "field29.cuh"
@@ -13,75 +20,78 @@ struct Xyzz29 {
     F29 x, y, zz, zzz;
 };
 
-// p - a for a <= p (the negation of a fully reduced coordinate)
-template <class P>
-APG_HD F29 f29_neg_reduced(const F29 &a) {
-    uint32_t pw[9], d[9];
-#pragma unroll
-    for (int j = 0; j < 9; j++) pw[j] = mod29<P>(j);
-    (void)sub29_words(pw, a.v, d);
+APG_HD F29 f29_zero() {
     F29 r;
 #pragma unroll
-    for (int j = 0; j < 9; j++) r.v[j] = d[j];
+    for (int j = 0; j < 9; j++) r.v[j] = 0;
     return r;
-}
-
-APG_HD F29 f29_one_fq() {
-    const Fe one = f_one<Fq>();
-    return f29_from32(one);
 }
 
 APG_HD Xyzz29 xyzz29_from_affine(const F29 &x, const F29 &y) {
     Xyzz29 r;
     r.x = x;
     r.y = y;
-    r.zz = f29_one_fq();
+    r.zz = f29_const_one<Fq>();
     r.zzz = r.zz;
     return r;
 }
 
-// dbl-2008-s-1 with ZZ = ZZZ = 1: U = 2y, V = U^2, W = U V, S = x V,
-// M = 3 x^2, X3 = M^2 - 2S, Y3 = M (S - X3) - W y, ZZ3 = V, ZZZ3 = W.
-APG_HD Xyzz29 xyzz29_mdbl(const F29 &x, const F29 &y) {
-    const F29 u = f29_add<Fq>(y, y);
-    const F29 v = f29_mul<Fq>(u, u);
-    const F29 w = f29_mul<Fq>(u, v);
-    const F29 s = f29_mul<Fq>(x, v);
-    const F29 x2 = f29_mul<Fq>(x, x);
-    const F29 m = f29_add<Fq>(f29_add<Fq>(x2, x2), x2);
+APG_HD Xyzz29 xyzz29_identity() {
     Xyzz29 r;
-    r.x = f29_sub<Fq>(f29_sub<Fq>(f29_mul<Fq>(m, m), s), s);
-    r.y = f29_sub<Fq>(f29_mul<Fq>(m, f29_sub<Fq>(s, r.x)), f29_mul<Fq>(w, y));
+    r.x = f29_const_one<Fq>();
+    r.y = r.x;
+    r.zz = f29_zero();
+    r.zzz = r.zz;
+    return r;
+}
+
+// ZZ = 0 mod p; ZZ < 1.1p, so ZZ is 0 or p
+APG_HD bool xyzz29_is_id(const Xyzz29 &p) { return f29_is_zero<Fq, 2>(p.zz); }
+
+// dbl-2008-s-1 with ZZ = ZZZ = 1 for x, y < p (bounds in the comments):
+// U = 2y, V = U^2, W = U V, S = x V, M = 3 x^2, X3 = M^2 - 2S,
+// Y3 = M (S - X3) - W y, ZZ3 = V, ZZZ3 = W.
+APG_HD Xyzz29 xyzz29_mdbl(const F29 &x, const F29 &y) {
+    const F29 u = f29_add(y, y);                 // < 2p
+    const F29 v = f29_mul<Fq>(u, u);             // < 1.03p
+    const F29 w = f29_mul<Fq>(u, v);             // < 1.02p
+    const F29 s = f29_mul<Fq>(x, v);             // < 1.01p
+    const F29 x2 = f29_mul<Fq>(x, x);            // < 1.01p
+    const F29 m = f29_add(f29_add(x2, x2), x2);  // < 3.03p
+    Xyzz29 r;
+    r.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(m, m), s), s);                     // < 5.06p
+    r.y = f29_sub<Fq, 2>(f29_mul<Fq>(m, f29_sub<Fq, 8>(s, r.x)), f29_mul<Fq>(w, y));  // < 3.17p
     r.zz = v;
     r.zzz = w;
     return r;
 }
 
-// acc + (x, y) for a non-identity accumulator (add-2008-s with Z2 = 1);
-// P = -Q leaves the identity, signalled through id.
+// acc + (x, y) for a non-identity accumulator (add-2008-s with Z2 = 1), the
+// point below p; P = -Q leaves the identity, signalled through id.
 APG_HD void xyzz29_add_affine(Xyzz29 &acc, bool &id, const F29 &x, const F29 &y) {
-    const F29 u2 = f29_mul<Fq>(x, acc.zz);
-    const F29 s2 = f29_mul<Fq>(y, acc.zzz);
-    const F29 pd = f29_sub<Fq>(u2, acc.x);
-    const F29 r = f29_sub<Fq>(s2, acc.y);
-    if (f29_is_zero<Fq>(pd)) {
-        if (f29_is_zero<Fq>(r))
+    const F29 u2 = f29_mul<Fq>(x, acc.zz);     // < 1.01p
+    const F29 s2 = f29_mul<Fq>(y, acc.zzz);    // < 1.01p
+    const F29 pd = f29_sub<Fq, 8>(u2, acc.x);  // < 9.01p (acc.x < 7.2p)
+    const F29 r = f29_sub<Fq, 4>(s2, acc.y);   // < 5.01p (acc.y < 3.3p)
+    if (f29_is_zero<Fq, 9>(pd)) {
+        if (f29_is_zero<Fq, 5>(r))
             acc = xyzz29_mdbl(x, y);
         else
             id = true;
         return;
     }
-    const F29 pp = f29_mul<Fq>(pd, pd);
-    const F29 ppp = f29_mul<Fq>(pd, pp);
-    const F29 q = f29_mul<Fq>(acc.x, pp);
-    const F29 x3 = f29_sub<Fq>(f29_sub<Fq>(f29_sub<Fq>(f29_mul<Fq>(r, r), ppp), q), q);
-    acc.y = f29_sub<Fq>(f29_mul<Fq>(r, f29_sub<Fq>(q, x3)), f29_mul<Fq>(acc.y, ppp));
+    const F29 pp = f29_mul<Fq>(pd, pd);      // < 1.49p
+    const F29 ppp = f29_mul<Fq>(pd, pp);     // < 1.08p
+    const F29 q = f29_mul<Fq>(acc.x, pp);    // < 1.07p
+    const F29 x3 = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(r, r), ppp), q), q);  // < 7.15p
+    acc.y = f29_sub<Fq, 2>(f29_mul<Fq>(r, f29_sub<Fq, 8>(q, x3)), f29_mul<Fq>(acc.y, ppp));      // < 3.27p
     acc.x = x3;
-    acc.zz = f29_mul<Fq>(acc.zz, pp);
-    acc.zzz = f29_mul<Fq>(acc.zzz, ppp);
+    acc.zz = f29_mul<Fq>(acc.zz, pp);        // < 1.01p
+    acc.zzz = f29_mul<Fq>(acc.zzz, ppp);     // < 1.01p
 }
 
-// Accumulates one affine point (sign-flipped if neg) into (acc, id).
+// Accumulates one base-table point (coordinates in the domain, below p;
+// all-zero = infinity), sign-flipped if neg, into (acc, id).
 APG_HD void xyzz29_accumulate(Xyzz29 &acc, bool &id, const Affine &a, bool neg) {
     if (aff_is_inf(a)) return;
     const F29 x = f29_from32(a.x);
@@ -95,77 +105,59 @@ APG_HD void xyzz29_accumulate(Xyzz29 &acc, bool &id, const Affine &a, bool neg) 
     xyzz29_add_affine(acc, id, x, y);
 }
 
-// General XYZZ points in the lazy form for the bucket reduction, where the
-// identity is ZZ = 0 mod p as in the 32-bit form.
-APG_HD F29 f29_zero() {
-    F29 r;
-#pragma unroll
-    for (int j = 0; j < 9; j++) r.v[j] = 0;
-    return r;
-}
-
-APG_HD Xyzz29 xyzz29_identity() {
-    Xyzz29 r;
-    r.x = f29_one_fq();
-    r.y = r.x;
-    r.zz = f29_zero();
-    r.zzz = r.zz;
-    return r;
-}
-
-APG_HD bool xyzz29_is_id(const Xyzz29 &p) { return f29_is_zero<Fq>(p.zz); }
-
-APG_HD Xyzz29 xyzz29_from32(const Xyzz &p) {
-    Xyzz29 r;
-    r.x = f29_from32(p.x);
-    r.y = f29_from32(p.y);
-    r.zz = f29_from32(p.zz);
-    r.zzz = f29_from32(p.zzz);
-    return r;
-}
-
-// dbl-2008-s-1 (a = 0), as curve.cuh's xyzz_dbl
+// dbl-2008-s-1 (a = 0), as curve.cuh's xyzz_dbl, on a point within the bounds
 APG_HD Xyzz29 xyzz29_dbl(const Xyzz29 &p) {
     if (xyzz29_is_id(p)) return p;
-    const F29 u = f29_add<Fq>(p.y, p.y);
-    const F29 v = f29_mul<Fq>(u, u);
-    const F29 w = f29_mul<Fq>(u, v);
-    const F29 s = f29_mul<Fq>(p.x, v);
-    const F29 x2 = f29_mul<Fq>(p.x, p.x);
-    const F29 m = f29_add<Fq>(f29_add<Fq>(x2, x2), x2);
+    const F29 u = f29_add(p.y, p.y);             // < 6.6p
+    const F29 v = f29_mul<Fq>(u, u);             // < 1.26p
+    const F29 w = f29_mul<Fq>(u, v);             // < 1.05p
+    const F29 s = f29_mul<Fq>(p.x, v);           // < 1.06p
+    const F29 x2 = f29_mul<Fq>(p.x, p.x);        // < 1.31p
+    const F29 m = f29_add(f29_add(x2, x2), x2);  // < 3.93p
     Xyzz29 r;
-    r.x = f29_sub<Fq>(f29_sub<Fq>(f29_mul<Fq>(m, m), s), s);
-    r.y = f29_sub<Fq>(f29_mul<Fq>(m, f29_sub<Fq>(s, r.x)), f29_mul<Fq>(w, p.y));
-    r.zz = f29_mul<Fq>(v, p.zz);
-    r.zzz = f29_mul<Fq>(w, p.zzz);
+    r.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(m, m), s), s);                       // < 5.1p
+    r.y = f29_sub<Fq, 2>(f29_mul<Fq>(m, f29_sub<Fq, 8>(s, r.x)), f29_mul<Fq>(w, p.y));  // < 3.22p
+    r.zz = f29_mul<Fq>(v, p.zz);                 // < 1.01p
+    r.zzz = f29_mul<Fq>(w, p.zzz);               // < 1.01p
     return r;
 }
 
-// p + q (add-2008-s), as curve.cuh's xyzz_add
+// p + q (add-2008-s), as curve.cuh's xyzz_add, on points within the bounds
 APG_HD Xyzz29 xyzz29_add(const Xyzz29 &p, const Xyzz29 &q) {
     if (xyzz29_is_id(p)) return q;
     if (xyzz29_is_id(q)) return p;
-    const F29 u1 = f29_mul<Fq>(p.x, q.zz);
+    const F29 u1 = f29_mul<Fq>(p.x, q.zz);   // < 1.05p
     const F29 u2 = f29_mul<Fq>(q.x, p.zz);
-    const F29 s1 = f29_mul<Fq>(p.y, q.zzz);
+    const F29 s1 = f29_mul<Fq>(p.y, q.zzz);  // < 1.03p
     const F29 s2 = f29_mul<Fq>(q.y, p.zzz);
-    const F29 pd = f29_sub<Fq>(u2, u1);
-    const F29 r = f29_sub<Fq>(s2, s1);
-    if (f29_is_zero<Fq>(pd)) {
-        if (f29_is_zero<Fq>(r)) return xyzz29_dbl(p);
+    const F29 pd = f29_sub<Fq, 2>(u2, u1);   // < 3.05p
+    const F29 r = f29_sub<Fq, 2>(s2, s1);    // < 3.03p
+    if (f29_is_zero<Fq, 3>(pd)) {
+        if (f29_is_zero<Fq, 3>(r)) return xyzz29_dbl(p);
         return xyzz29_identity();
     }
-    const F29 pp = f29_mul<Fq>(pd, pd);
-    const F29 ppp = f29_mul<Fq>(pd, pp);
-    const F29 qq = f29_mul<Fq>(u1, pp);
+    const F29 pp = f29_mul<Fq>(pd, pd);      // < 1.06p
+    const F29 ppp = f29_mul<Fq>(pd, pp);     // < 1.02p
+    const F29 qq = f29_mul<Fq>(u1, pp);      // < 1.01p
     Xyzz29 o;
-    o.x = f29_sub<Fq>(f29_sub<Fq>(f29_sub<Fq>(f29_mul<Fq>(r, r), ppp), qq), qq);
-    o.y = f29_sub<Fq>(f29_mul<Fq>(r, f29_sub<Fq>(qq, o.x)), f29_mul<Fq>(s1, ppp));
-    o.zz = f29_mul<Fq>(f29_mul<Fq>(p.zz, q.zz), pp);
-    o.zzz = f29_mul<Fq>(f29_mul<Fq>(p.zzz, q.zzz), ppp);
+    o.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(r, r), ppp), qq), qq);  // < 7.06p
+    o.y = f29_sub<Fq, 2>(f29_mul<Fq>(r, f29_sub<Fq, 8>(qq, o.x)), f29_mul<Fq>(s1, ppp));  // < 3.17p
+    o.zz = f29_mul<Fq>(f29_mul<Fq>(p.zz, q.zz), pp);      // < 1.01p
+    o.zzz = f29_mul<Fq>(f29_mul<Fq>(p.zzz, q.zzz), ppp);  // < 1.01p
     return o;
 }
 
+// a 32-bit Montgomery point (R = 2^256, coordinates below 2p) in the domain
+APG_HD Xyzz29 xyzz29_from32(const Xyzz &p) {
+    Xyzz29 r;
+    r.x = f29_from_mont32<Fq>(p.x);
+    r.y = f29_from_mont32<Fq>(p.y);
+    r.zz = f29_from_mont32<Fq>(p.zz);
+    r.zzz = f29_from_mont32<Fq>(p.zzz);
+    return r;
+}
+
+// leaves the domain: fully reduced 32-bit Montgomery coordinates
 APG_HD Xyzz xyzz29_to32(const Xyzz29 &a, bool id) {
     if (id) return xyzz_identity();
     Xyzz r;
@@ -176,8 +168,17 @@ APG_HD Xyzz xyzz29_to32(const Xyzz29 &a, bool id) {
     return r;
 }
 
-// fully reduced 32-bit form; the identity is written as xyzz_identity()
 APG_HD Xyzz xyzz29_store(const Xyzz29 &a) { return xyzz29_to32(a, xyzz29_is_id(a)); }
+
+// An affine point (32-bit Montgomery, R = 2^256) as the base tables hold it:
+// coordinates times 2^5 mod p, fully reduced; infinity stays all-zero.
+APG_HD Affine affine_to_table(const Affine &a) {
+    if (aff_is_inf(a)) return a;
+    Affine r;
+    r.x = f_to_r261<Fq>(a.x);
+    r.y = f_to_r261<Fq>(a.y);
+    return r;
+}
 
 // Memory image of an Xyzz29 (36 words, 144 B): the bucket phase keeps points
 // in the lazy form between kernels, so a write is stores only. Arrays of it

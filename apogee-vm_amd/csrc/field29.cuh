@@ -1,19 +1,23 @@
 // Fq in nine 29-bit words with lazy reduction: the form the MSM's bucket
-// accumulation computes in.
+// phase computes in.
 //
-// A value is nine words v[j] < 2^29 (bits [29j, 29j + 29)) of an integer
-// x < 2p. It stands for x mod p, Montgomery with the same R = 2^256 as the
-// 32-bit form (field.cuh), so converting is re-slicing bits, not a multiply.
+// A value is nine words v[j] < 2^29 (bits [29j, 29j + 29)) of an integer x
+// that stands for x mod p in the Montgomery domain of R' = 2^261 (x = a R'
+// mod p, up to a multiple of p): nine reduction rounds of one 29-bit word
+// each. The multiply accumulates in 64-bit words with no carry handling in
+// its rounds: a column takes at most 9 products a_j b_i and 9 products
+// m_i p_j, each below 2^58, so it stays below 2^63. Its result is
+// (a b + M p) / 2^261 < a b / R' + p, and R' / p is about 169, so inputs up to
+// ~12p give a result below 1.1 p: subtraction is a - b + K p with K p above
+// b's bound and no conditional correction, addition is a plain add, and only
+// the final conversion reduces fully. Each formula in curve29.cuh states the
+// bounds of its values. On gfx950 a multiply is ~230 VALU instructions against
+// ~430 for the 32-bit CIOS of field.cuh, whose add-with-carry chains cost a
+// carry select and a hazard wait per word.
 //
-// The multiply accumulates in 64-bit words with no carry handling in the
-// inner loops: a column takes at most 9 products a_j b_i and 9 products
-// m_i p_j, each below 2^58, so it stays below 2^63. Each of the nine rounds
-// reduces by one word (2^29), the last by 2^24 (8 * 29 + 24 = 256). For inputs
-// below 2p the result (a b + M p) / 2^256 is below 2p again because 4p <
-// 2^256, so no conditional subtraction is needed between multiplies. On gfx950
-// this is ~250 VALU instructions per multiply against ~430 for the 32-bit
-// CIOS, whose add-with-carry chains cost a carry select and a hazard wait per
-// word.
+// Entering the domain multiplies a 32-bit Montgomery value (R = 2^256) by 2^5
+// (five modular doublings, done where the MSM's base tables are built);
+// leaving it multiplies by 2^256 mod p in this domain, which divides by 2^5.
 #pragma once
 #include "field.cuh"
 
@@ -34,30 +38,78 @@ APG_HD uint32_t slice29(const uint32_t l[8]) {
     return (uint32_t)x & W29;
 }
 
-template <class P>
-APG_HD uint32_t mod29(int j) {
-    uint32_t l[8];
-    for (int i = 0; i < 8; i++) l[i] = P::mod(i);
+APG_HD uint32_t slice29_rt(const uint32_t l[8], int j) {
     const int bit = 29 * j, w = bit >> 5, s = bit & 31;
     uint64_t x = (uint64_t)l[w] >> s;
     if (w + 1 < 8) x |= (uint64_t)l[w + 1] << (32 - s);
     return (uint32_t)x & W29;
 }
 
-// words of 2p (for the lazy range [0, 2p))
-template <class P>
-APG_HD uint32_t mod2x29(int j) {
-    uint32_t l[8];
-    uint32_t c = 0;
+// words of K p (K p < 2^261)
+template <class P, uint32_t K = 1>
+APG_HD uint32_t modk29(int j) {
+    uint32_t l[9];
+    uint64_t c = 0;
     for (int i = 0; i < 8; i++) {
-        const uint64_t s = ((uint64_t)P::mod(i) << 1) | c;
+        const uint64_t s = (uint64_t)P::mod(i) * K + c;
         l[i] = (uint32_t)s;
-        c = P::mod(i) >> 31;
+        c = s >> 32;
     }
+    l[8] = (uint32_t)c;
     const int bit = 29 * j, w = bit >> 5, s = bit & 31;
     uint64_t x = (uint64_t)l[w] >> s;
-    if (w + 1 < 8) x |= (uint64_t)l[w + 1] << (32 - s);
+    if (w + 1 < 9) x |= (uint64_t)l[w + 1] << (32 - s);
     return (uint32_t)x & W29;
+}
+
+template <class P>
+APG_HD uint32_t mod29(int j) {
+    return modk29<P, 1>(j);
+}
+
+// 2^256 mod p (P::one) in words: the multiplier that leaves the domain
+template <class P>
+APG_HD uint32_t r256_29(int j) {
+    uint32_t l[8];
+    for (int i = 0; i < 8; i++) l[i] = P::one(i);
+    return slice29_rt(l, j);
+}
+
+// 2^261 mod p in words: one in the domain (32 x 2^256 mod p, five doublings)
+template <class P>
+APG_HD uint32_t one29w(int j) {
+    uint32_t l[8];
+    for (int i = 0; i < 8; i++) l[i] = P::one(i);
+    for (int d = 0; d < 5; d++) {
+        uint32_t t[8], c = 0;
+        for (int i = 0; i < 8; i++) {
+            t[i] = (l[i] << 1) | c;
+            c = l[i] >> 31;
+        }
+        // t >= p (t < 2p < 2^256): subtract
+        bool ge = true;
+        for (int i = 7; i >= 0; i--) {
+            if (t[i] != P::mod(i)) {
+                ge = t[i] > P::mod(i);
+                break;
+            }
+        }
+        uint32_t br = 0;
+        for (int i = 0; i < 8; i++) {
+            const uint64_t s = (uint64_t)t[i] - P::mod(i) - br;
+            l[i] = ge ? (uint32_t)s : t[i];
+            br = (uint32_t)(s >> 63);
+        }
+    }
+    return slice29_rt(l, j);
+}
+
+template <class P>
+APG_HD F29 f29_const_one() {
+    F29 r;
+#pragma unroll
+    for (int j = 0; j < 9; j++) r.v[j] = one29w<P>(j);
+    return r;
 }
 
 APG_HD F29 f29_from32(const Fe &a) {
@@ -74,6 +126,16 @@ APG_HD F29 f29_from32(const Fe &a) {
     return r;
 }
 
+// a 32-bit Montgomery value (R = 2^256, below 2p) times 2^5 mod p, still
+// in 32-bit words: the domain's representative, as the base tables hold it
+template <class P>
+APG_HD Fe f_to_r261(const Fe &a) {
+    Fe r = f_add<P>(a, f_zero<P>());  // a < 2p -> a mod p
+#pragma unroll
+    for (int d = 0; d < 5; d++) r = f_add<P>(r, r);
+    return r;
+}
+
 // d = a - b - borrow_in over the nine words; returns the borrow out.
 APG_HD uint32_t sub29_words(const uint32_t a[9], const uint32_t b[9], uint32_t d[9]) {
     uint32_t br = 0;
@@ -86,30 +148,6 @@ APG_HD uint32_t sub29_words(const uint32_t a[9], const uint32_t b[9], uint32_t d
     return br;
 }
 
-// x mod p as a fully reduced 32-bit-word value (x < 2p)
-template <class P>
-APG_HD Fe f29_to32(const F29 &a) {
-    uint32_t p[9], d[9];
-#pragma unroll
-    for (int j = 0; j < 9; j++) p[j] = mod29<P>(j);
-    const uint32_t br = sub29_words(a.v, p, d);
-    const uint32_t *s = br ? a.v : d;  // a < p: keep a
-    uint32_t w[9];
-#pragma unroll
-    for (int j = 0; j < 9; j++) w[j] = s[j];
-    Fe r;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        // 32-bit word i: bits [32i, 32i + 32)
-        const int bit = 32 * i, j = bit / 29, sh = bit % 29;
-        uint64_t x = (uint64_t)w[j] >> sh;
-        x |= (uint64_t)w[j + 1] << (29 - sh);
-        if (j + 2 < 9) x |= (uint64_t)w[j + 2] << (58 - sh);
-        r.l[i] = (uint32_t)x;
-    }
-    return r;
-}
-
 template <class P>
 APG_HD F29 f29_mul(const F29 &a, const F29 &b) {
     uint64_t t[10];
@@ -119,103 +157,133 @@ APG_HD F29 f29_mul(const F29 &a, const F29 &b) {
     for (int i = 0; i < 9; i++) {
 #pragma unroll
         for (int j = 0; j < 9; j++) t[j] += (uint64_t)a.v[j] * b.v[i];
-        const uint32_t mask = i < 8 ? W29 : (1u << 24) - 1;
-        uint32_t m = ((uint32_t)t[0] * P::inv) & mask;
-#if defined(__HIP_DEVICE_COMPILE__)
-        // Hide m's known width from the gfx950 backend. Told that the last
-        // round's m fits 24 bits, it selects 24-bit multiplies for the words
-        // of p below 2^24 (words 6 and 8 of Fq), drops the mask they make
-        // redundant, then fuses them into v_mad_u64_u32, which reads all 32
-        // bits of the unmasked value: wrong products. The empty asm keeps the
-        // masked value in a register of unknown width.
-        asm volatile("" : "+v"(m));
-#endif
+        const uint32_t m = ((uint32_t)t[0] * P::inv) & W29;
 #pragma unroll
         for (int j = 0; j < 9; j++) t[j] += (uint64_t)m * mod29<P>(j);
-        if (i < 8) {
-            const uint64_t c = t[0] >> 29;
+        const uint64_t c = t[0] >> 29;
 #pragma unroll
-            for (int j = 0; j < 9; j++) t[j] = t[j + 1];
-            t[0] += c;
-        }
+        for (int j = 0; j < 9; j++) t[j] = t[j + 1];
+        t[0] += c;
     }
-    // t[0] = 0 mod 2^24: normalise to 29-bit words, then shift right by 24
-    uint32_t n[10];
+    // normalise to 29-bit words (the value is below 2^255)
+    F29 r;
     uint64_t c = 0;
 #pragma unroll
     for (int j = 0; j < 9; j++) {
         const uint64_t s = t[j] + c;
-        n[j] = (uint32_t)s & W29;
+        r.v[j] = (uint32_t)s & W29;
         c = s >> 29;
     }
-    n[9] = (uint32_t)c;
-    F29 r;
-#pragma unroll
-    for (int j = 0; j < 9; j++) r.v[j] = ((n[j] >> 24) | (n[j + 1] << 5)) & W29;
     return r;
 }
 
-// (a + b) with a, b < 2p, reduced below 2p
-template <class P>
+// a + b, not reduced (the bound is the sum of the bounds)
 APG_HD F29 f29_add(const F29 &a, const F29 &b) {
-    uint32_t s[9], tp[9], d[9];
+    F29 r;
     uint32_t c = 0;
 #pragma unroll
     for (int j = 0; j < 9; j++) {
         const uint32_t x = a.v[j] + b.v[j] + c;
-        s[j] = x & W29;
+        r.v[j] = x & W29;
         c = x >> 29;
     }
-#pragma unroll
-    for (int j = 0; j < 9; j++) tp[j] = mod2x29<P>(j);
-    const uint32_t br = sub29_words(s, tp, d);  // s < 4p < 2^256: no tenth word
-    F29 r;
-#pragma unroll
-    for (int j = 0; j < 9; j++) r.v[j] = br ? s[j] : d[j];
     return r;
 }
 
-// (a - b) with a, b < 2p, reduced into [0, 2p)
-template <class P>
+// a - b + K p for b < K p (the bound is a's plus K p)
+template <class P, uint32_t K>
 APG_HD F29 f29_sub(const F29 &a, const F29 &b) {
-    uint32_t d[9], e[9];
-    const uint32_t br = sub29_words(a.v, b.v, d);
-    uint32_t c = 0;
+    F29 r;
+    int32_t c = 0;
 #pragma unroll
     for (int j = 0; j < 9; j++) {
-        const uint32_t x = d[j] + (br ? mod2x29<P>(j) : 0u) + c;
-        e[j] = x & W29;
-        c = x >> 29;
+        const int32_t x = (int32_t)(a.v[j] + modk29<P, K>(j)) - (int32_t)b.v[j] + c;
+        r.v[j] = (uint32_t)x & W29;
+        c = x >> 29;  // arithmetic: the borrow is -1
     }
-    F29 r;
-#pragma unroll
-    for (int j = 0; j < 9; j++) r.v[j] = e[j];
     return r;
 }
 
-// x = 0 mod p for x < 2p: x is 0 or p
+// p - a for a <= p (the negation of a fully reduced coordinate)
 template <class P>
-APG_HD bool f29_is_zero(const F29 &a) {
-    uint32_t z = 0, q = 0;
+APG_HD F29 f29_neg_reduced(const F29 &a) {
+    uint32_t pw[9], d[9];
 #pragma unroll
-    for (int j = 0; j < 9; j++) {
-        z |= a.v[j];
-        q |= a.v[j] ^ mod29<P>(j);
-    }
-    return z == 0 || q == 0;
+    for (int j = 0; j < 9; j++) pw[j] = mod29<P>(j);
+    (void)sub29_words(pw, a.v, d);
+    F29 r;
+#pragma unroll
+    for (int j = 0; j < 9; j++) r.v[j] = d[j];
+    return r;
 }
 
-// A test composition over the whole lazy range: ((a + b)(a - b) + a^2) - b,
-// every intermediate in [0, 2p); the host build checks it against the oracle.
+// x = k p for some 0 <= k <= MAXK, i.e. x = 0 mod p for x below (MAXK + 1) p.
+// The low words of k p are distinct (p is odd), so one word picks the only
+// candidate and the full comparison runs only when it matches.
+template <class P, int MAXK>
+APG_HD bool f29_is_zero(const F29 &x) {
+    const uint32_t p0 = mod29<P>(0);
+    int cand = -1;
+#pragma unroll
+    for (int k = 0; k <= MAXK; k++)
+        if (x.v[0] == (((uint32_t)k * p0) & W29)) cand = k;
+    if (cand < 0) return false;
+    uint64_t c = 0;
+    bool eq = true;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        const uint64_t s = (uint64_t)(uint32_t)cand * mod29<P>(j) + c;
+        eq = eq && x.v[j] == ((uint32_t)s & W29);
+        c = s >> 29;
+    }
+    return eq;
+}
+
+// leaves the domain: x / 2^5 as a fully reduced 32-bit Montgomery value
+// (R = 2^256); x may be any value below ~12p
+template <class P>
+APG_HD Fe f29_to32(const F29 &x) {
+    F29 c;
+#pragma unroll
+    for (int j = 0; j < 9; j++) c.v[j] = r256_29<P>(j);
+    const F29 m = f29_mul<P>(x, c);  // below 1.1 p
+    uint32_t p[9], d[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) p[j] = mod29<P>(j);
+    const uint32_t br = sub29_words(m.v, p, d);
+    uint32_t w[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[j] = br ? m.v[j] : d[j];
+    Fe r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int bit = 32 * i, j = bit / 29, sh = bit % 29;
+        uint64_t v = (uint64_t)w[j] >> sh;
+        v |= (uint64_t)w[j + 1] << (29 - sh);
+        if (j + 2 < 9) v |= (uint64_t)w[j + 2] << (58 - sh);
+        r.l[i] = (uint32_t)v;
+    }
+    return r;
+}
+
+// enters the domain from a fully reduced 32-bit Montgomery value
+template <class P>
+APG_HD F29 f29_from_mont32(const Fe &a) {
+    return f29_from32(f_to_r261<P>(a));
+}
+
+// A test composition over the lazy bounds: ((a + b)(a - b) + a^2) - b, in
+// and out as fully reduced 32-bit Montgomery values; the host build and a
+// kernel are checked against the oracle.
 template <class P>
 APG_HD Fe f29_test_chain(const Fe &a32, const Fe &b32, int op) {
-    const F29 a = f29_from32(a32), b = f29_from32(b32);
+    const F29 a = f29_from_mont32<P>(a32), b = f29_from_mont32<P>(b32);
     F29 r;
     switch (op) {
         case 0: r = f29_mul<P>(a, b); break;
-        case 1: r = f29_add<P>(a, b); break;
-        case 2: r = f29_sub<P>(a, b); break;
-        default: r = f29_sub<P>(f29_add<P>(f29_mul<P>(f29_add<P>(a, b), f29_sub<P>(a, b)), f29_mul<P>(a, a)), b);
+        case 1: r = f29_add(a, b); break;
+        case 2: r = f29_sub<P, 2>(a, b); break;
+        default: r = f29_sub<P, 2>(f29_add(f29_mul<P>(f29_add(a, b), f29_sub<P, 2>(a, b)), f29_mul<P>(a, a)), b);
     }
     return f29_to32<P>(r);
 }

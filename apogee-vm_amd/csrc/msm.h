@@ -10,7 +10,10 @@ namespace apg {
 // A table of affine bases held in HBM, optionally with window-shifted copies:
 // copy w holds 2^(c*w) * B_i at points + w * stride + i, w < copies. With every
 // window of a scalar precomputed (copies == windows) all windows share one
-// bucket set and the MSM has no doubling pass at all.
+// bucket set and the MSM has no doubling pass at all. Coordinates are held in
+// the bucket phase's domain: Montgomery 32-bit words times 2^5 mod p
+// (affine_to_table, curve29.cuh); msm_expand_bases and msm_table_convert
+// write them so.
 struct BaseTable {
     Affine *points = nullptr;  // device, [copies][stride]
     size_t stride = 0;         // points per copy (the prefix length covered)
@@ -45,6 +48,9 @@ void msm_set_window_bits(int log2n, int c);
 
 // Builds copies = windows(254, c) shifted copies of the first n bases (device
 // pointer `src`, affine Montgomery) into a new table.
+// msm_table_convert puts n affine Montgomery points (canonical coordinates) in
+// place into the table form, for a one-copy table over caller bases.
+void msm_table_convert(Context &ctx, Affine *points, size_t n);
 std::unique_ptr<BaseTable> msm_expand_bases(Context &ctx, const Affine *src, size_t n, int c,
                                             int copies);
 

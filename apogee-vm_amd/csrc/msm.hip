@@ -438,18 +438,24 @@ __global__ void __launch_bounds__(TPB) k_weighted(const Xyzz29 *__restrict__ a_s
     }
 }
 
-// Window-shifted copies: copy w of base i is 2^(c w) B_i, affine.
+// Window-shifted copies: copy w of base i is 2^(c w) B_i, affine, written in
+// the bucket phase's domain (coordinates times 2^5 mod p, affine_to_table).
 __global__ void k_expand(const Affine *__restrict__ src, size_t n, int c, int copies,
                          Affine *__restrict__ dst, size_t stride) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Affine a = src[i];
-    dst[i] = a;
+    dst[i] = affine_to_table(a);
     Xyzz p = xyzz_from_affine(a);
     for (int w = 1; w < copies; w++) {
         for (int d = 0; d < c; d++) p = xyzz_dbl(p);
-        dst[(size_t)w * stride + i] = xyzz_to_affine(p);
+        dst[(size_t)w * stride + i] = affine_to_table(xyzz_to_affine(p));
     }
+}
+
+__global__ void k_table_convert(Affine *__restrict__ pts, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pts[i] = affine_to_table(pts[i]);
 }
 
 }  // namespace
@@ -509,6 +515,12 @@ int msm_pick_c(size_t n, size_t columns) {
     if (c < 4) c = 4;
     if (c > 17) c = 17;
     return c;
+}
+
+void msm_table_convert(Context &ctx, Affine *points, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_table_convert, dim3(grid_for(n, TPB)), dim3(TPB), 0, ctx.stream, points, n);
+    HIP_CHECK(hipGetLastError());
 }
 
 std::unique_ptr<BaseTable> msm_expand_bases(Context &ctx, const Affine *src, size_t n, int c,
