@@ -35,7 +35,7 @@ struct MsmShape {
 };
 
 // Windows needed for a bits-bit scalar at width c (crates/curve/src/msm.rs:140-142).
-inline int msm_window_count(int bits, int c) { return (bits + 1 + c - 1) / c; }
+constexpr int msm_window_count(int bits, int c) { return (bits + 1 + c - 1) / c; }
 
 // Window width for n points under this library's size classes (by
 // floor(log2 n)); msm_set_window_bits overrides one class (c = 0 restores the
