@@ -7,9 +7,12 @@
 //  1. digits  : one thread per scalar converts Montgomery -> canonical, recodes
 //               into signed c-bit digits exactly as msm.rs:180-224 does (same
 //               carry rule) and emits one (bucket key, table index | sign)
-//               pair per nonzero digit, packed densely per block.
-//  2. sort    : an LSD radix sort of the pairs by bucket key (hipCUB), then a
-//               binary search per key for the bucket boundaries.
+//               pair per nonzero digit, packed densely in scalar order (a
+//               count pass, a scan of the blocks' counts, a write pass).
+//  2. sort    : an LSD radix sort of the pairs (hipCUB): by the bucket bits
+//               alone when every window has its table copy (stable, so the
+//               columns stay apart within a bucket), else by the whole key;
+//               then a binary search per key for its range.
 //  3. accumulate: the sorted entries are cut into equal slices, one thread
 //               per slice summing its points with mixed XYZZ adds, so the
 //               lanes of a wave finish together whatever the digit skew (the
@@ -206,20 +209,20 @@ __global__ void __launch_bounds__(TPB) k_emit(const Fe *__restrict__ fr,
     });
 }
 
-// off[k] = first position of key k in the sorted key array (lower bound).
-__global__ void k_bounds(const uint32_t *__restrict__ skeys, uint32_t total, uint32_t nkeys,
-                         uint32_t *__restrict__ off) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > nkeys) return;
-    uint32_t lo = 0, hi = total;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (skeys[mid] < k)
-            lo = mid + 1;
-        else
-            hi = mid;
+// off[2k], off[2k + 1] = the range of key k in the sorted key array, which
+// holds each key's entries contiguously (sorted fully, or by the bucket bits
+// alone with the columns in emission order within a bucket). One thread per
+// entry marks where the key changes; keys without entries keep the (0, 0)
+// the array is cleared to.
+__global__ void k_bounds(const uint32_t *__restrict__ skeys, uint32_t total, uint32_t *__restrict__ off) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t k = skeys[i];
+    if (i == 0 || skeys[i - 1] != k) {
+        off[2 * (size_t)k] = i;
+        if (i > 0) off[2 * (size_t)skeys[i - 1] + 1] = i;
     }
-    off[k] = lo;
+    if (i == total - 1) off[2 * (size_t)k + 1] = total;
 }
 
 __device__ __forceinline__ Affine load_affine(const Affine *__restrict__ p, uint32_t idx) {
@@ -304,7 +307,7 @@ struct Span {
 };
 
 __device__ __forceinline__ Span merge_span(const uint32_t *__restrict__ off, uint32_t k, uint32_t L) {
-    const uint32_t a = off[k], b = off[k + 1];
+    const uint32_t a = off[2 * (size_t)k], b = off[2 * (size_t)k + 1];
     if (a == b) return {0, 0};
     const uint32_t t0 = a / L, t1 = (b - 1) / L;
     if (t0 == t1) return {0, 0};
@@ -321,7 +324,7 @@ __global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restr
                                                        Xyzz29 *__restrict__ buckets) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
-    if (off[k] == off[k + 1]) {
+    if (off[2 * (size_t)k] == off[2 * (size_t)k + 1]) {
         st29(buckets + k, xyzz29_identity());
         return;
     }
@@ -686,19 +689,26 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     // saved pass), then bucket boundaries
     int key_bits = 1;
     while ((1ull << key_bits) < (uint64_t)nkeys) key_bits++;
+    // With one bucket group per column (every window has its table copy) the
+    // sort keys only the bucket's c - 1 bits: it is stable and the emission is
+    // column-major, so each (column, bucket) stays contiguous, in bucket-major
+    // order. 16 bits at c = 17 is two onesweep passes instead of three.
+    const int bbits = sh.c - 1;
+    const bool bucket_major = sh.G == 1 && K > 1 && bbits < key_bits;
+    const int sort_bits = bucket_major ? bbits : key_bits;
     // 4 words of slack: the accumulation reads keys and entries as 16-byte vectors
     uint32_t *skeys = ctx.d<uint32_t>("msm.skeys", (size_t)total + 4);
     uint32_t *entries = ctx.d<uint32_t>("msm.entries", (size_t)total + 4);
     size_t tmp_bytes = 0;
     HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_in, skeys, vals_in, entries,
-                                                 (int)total, 0, key_bits, st));
+                                                 (int)total, 0, sort_bits, st));
     void *tmp = ctx.d<uint8_t>("msm.cubtmp", std::max<size_t>(tmp_bytes, 1));
     if (total)
         HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, skeys, vals_in, entries,
-                                                     (int)total, 0, key_bits, st));
-    uint32_t *off = ctx.d<uint32_t>("msm.off", nkeys + 1);
-    hipLaunchKernelGGL(k_bounds, dim3(grid_for(nkeys + 1, TPB)), dim3(TPB), 0, st, skeys, total,
-                       nkeys, off);
+                                                     (int)total, 0, sort_bits, st));
+    uint32_t *off = ctx.d<uint32_t>("msm.off", 2 * (size_t)nkeys);
+    HIP_CHECK(hipMemsetAsync(off, 0, 2 * (size_t)nkeys * sizeof(uint32_t), st));
+    if (total) hipLaunchKernelGGL(k_bounds, dim3(grid_for(total, TPB)), dim3(TPB), 0, st, skeys, total, off);
     HIP_CHECK(hipGetLastError());
 
     // 3. accumulate over equal slices of the sorted entries: about ACC_ROUNDS
