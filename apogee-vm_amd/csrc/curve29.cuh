@@ -80,14 +80,17 @@ APG_HD void xyzz29_add_affine(Xyzz29 &acc, bool &id, const F29 &x, const F29 &y)
             id = true;
         return;
     }
+    // ordered so that each coordinate of acc is replaced as soon as its last
+    // use is done (fewer live values)
     const F29 pp = f29_mul<Fq>(pd, pd);      // < 1.49p
-    const F29 ppp = f29_mul<Fq>(pd, pp);     // < 1.08p
-    const F29 q = f29_mul<Fq>(acc.x, pp);    // < 1.07p
-    const F29 x3 = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(r, r), ppp), q), q);  // < 7.15p
-    acc.y = f29_sub<Fq, 2>(f29_mul<Fq>(r, f29_sub<Fq, 8>(q, x3)), f29_mul<Fq>(acc.y, ppp));      // < 3.27p
-    acc.x = x3;
     acc.zz = f29_mul<Fq>(acc.zz, pp);        // < 1.01p
+    const F29 ppp = f29_mul<Fq>(pd, pp);     // < 1.08p
     acc.zzz = f29_mul<Fq>(acc.zzz, ppp);     // < 1.01p
+    const F29 q = f29_mul<Fq>(acc.x, pp);    // < 1.07p
+    const F29 yp = f29_mul<Fq>(acc.y, ppp);  // < 1.02p
+    const F29 x3 = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(r, r), ppp), q), q);  // < 7.15p
+    acc.y = f29_sub<Fq, 2>(f29_mul<Fq>(r, f29_sub<Fq, 8>(q, x3)), yp);  // < 3.27p
+    acc.x = x3;
 }
 
 // Accumulates one base-table point (coordinates in the domain, below p;
