@@ -145,6 +145,10 @@ def _declare(L):
     d("apg_transcript_challenge", [VP, U64, VP], None)
     d("apg_transcript_append_g1", [VP, U64, VP, SZ], None)
     d("apg_transcript_round", [VP, VP, SZ, VP])
+    d("apg_gkr_forward_layer", [VP, VP, VP, SZ, SZ, VP])
+    d("apg_gkr_forward_layer_dev", [VP, VP, VP, SZ, SZ, VP])
+    d("apg_gkr_prove_sumcheck", [VP, VP, VP, VP, SZ, VP, VP, SZ, ROUND_FN, VP, VP, VP, VP])
+    d("apg_gkr_prove_sumcheck_dev", [VP, VP, VP, VP, SZ, VP, VP, SZ, ROUND_FN, VP, VP, VP, VP])
     d("apg_transcript_state", [VP, VP])
     d("apg_transcript_set_state", [VP, VP])
     d("apg_fr_to_bytes", [VP, VP])

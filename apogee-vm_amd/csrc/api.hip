@@ -11,6 +11,7 @@
 
 #include "../../include/apogee_gpu.h"
 #include "engine.h"
+#include "gkr.h"
 #include "mercury.h"
 #include "curve29.cuh"
 #include "msm.h"
@@ -682,6 +683,143 @@ int apg_sumcheck_prove_dev(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
         for (size_t t = 0; t < k; t++) tabs[t] = static_cast<Fe *>(tables_dev[t]);
         sumcheck_common(ctx->c, g, tabs.data(), k, num_vars, eq_point_mont, wrap_round(round_fn, user),
                         rounds_out, point_out, final_evals_out);
+    });
+}
+
+// The flat gate list of the C ABI as the device code holds it.
+static GkrList load_gkr_list(const apg_gkr_list *list) {
+    if (!list) throw ApgError(APG_ERR_ARG, "gkr: no gate list");
+    GkrList l;
+    const size_t all = (size_t)list->ncached + list->nproducing + list->nenforcing;
+    if (all && !list->gates) throw ApgError(APG_ERR_ARG, "gkr: no gates");
+    if ((list->nsources && !list->sources) || (list->ncoeffs && !list->coeffs_mont) ||
+        (list->nvirtual && !list->virtual_kinds))
+        throw ApgError(APG_ERR_ARG, "gkr: a null array");
+    l.ncached = list->ncached;
+    l.nproducing = list->nproducing;
+    l.nenforcing = list->nenforcing;
+    l.halving = list->halving != 0;
+    l.gates.resize(all);
+    for (size_t i = 0; i < all; i++) {
+        const apg_gkr_gate &g = list->gates[i];
+        l.gates[i] = GkrGate{g.shape, g.src_begin, g.src_end, g.coef_begin, g.split};
+    }
+    l.sources.assign(list->sources, list->sources + list->nsources);
+    l.virtual_kinds.assign(list->virtual_kinds, list->virtual_kinds + list->nvirtual);
+    l.coeffs.resize(list->ncoeffs);
+    for (uint32_t i = 0; i < list->ncoeffs; i++) l.coeffs[i] = load_fe(list->coeffs_mont + 32 * (size_t)i);
+    return l;
+}
+
+static void gkr_forward_common(Context &ctx, const apg_gkr_list *list, const Fe *const *cols, size_t ncols,
+                               size_t num_vars, Fe *const *out) {
+    if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+    const GkrList l = load_gkr_list(list);
+    gkr_forward_device(ctx, l, cols, ncols, (int)num_vars, out);
+}
+
+int apg_gkr_forward_layer_dev(apg_ctx *ctx, const apg_gkr_list *list, const void *const *cols_dev, size_t ncols,
+                              size_t num_vars, void *const *out_dev) {
+    return guarded([&] {
+        need(ctx);
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        std::vector<const Fe *> cols(ncols);
+        for (size_t c = 0; c < ncols; c++) cols[c] = static_cast<const Fe *>(cols_dev[c]);
+        const uint32_t np = list ? list->nproducing : 0;
+        std::vector<Fe *> out(np);
+        for (uint32_t g = 0; g < np; g++) out[g] = static_cast<Fe *>(out_dev[g]);
+        gkr_forward_common(ctx->c, list, cols.data(), ncols, num_vars, out.data());
+    });
+}
+
+int apg_gkr_forward_layer(apg_ctx *ctx, const apg_gkr_list *list, const uint8_t *const *cols_mont, size_t ncols,
+                          size_t num_vars, uint8_t *const *out_mont) {
+    return guarded([&] {
+        need(ctx);
+        if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+        if (list && list->halving && num_vars < 1) throw ApgError(APG_ERR_ARG, "gkr: a halving list needs num_vars >= 1");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        const size_t len = size_t(1) << num_vars;
+        const size_t rows = list && list->halving ? len / 2 : len;
+        const uint32_t np = list ? list->nproducing : 0;
+        std::vector<Fe *> cols = upload_tables(ctx->c, cols_mont, ncols, len);
+        Fe *pool = ctx->c.d<Fe>("gkr.fwd_out", std::max<size_t>((size_t)np * rows, 1));
+        std::vector<Fe *> out(np);
+        for (uint32_t g = 0; g < np; g++) out[g] = pool + (size_t)g * rows;
+        std::vector<const Fe *> ccols(cols.begin(), cols.end());
+        gkr_forward_common(ctx->c, list, ccols.data(), ncols, num_vars, out.data());
+        for (uint32_t g = 0; g < np; g++)
+            HIP_CHECK(hipMemcpyAsync(out_mont[g], out[g], rows * 32, hipMemcpyDeviceToHost, ctx->c.stream));
+        ctx->c.sync();
+    });
+}
+
+static void gkr_sumcheck_common(Context &ctx, const apg_gkr_list *list, const uint8_t *weights_mont,
+                                const uint8_t *eq_point_mont, size_t num_vars, Fe *const *lower,
+                                Fe *const *upper, size_t ncols, const RoundFn &fn, uint8_t *rounds_out,
+                                uint8_t *point_out, uint8_t *finals_out) {
+    if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+    const GkrList l = load_gkr_list(list);
+    const size_t nw = (size_t)l.nproducing + l.nenforcing;
+    if (nw && !weights_mont) throw ApgError(APG_ERR_ARG, "gkr: no weights");
+    std::vector<Fe> w(nw + 1), eqp(num_vars + 1), rounds(4 * num_vars + 1), point(num_vars + 1);
+    for (size_t i = 0; i < nw; i++) w[i] = load_fe(weights_mont + 32 * i);
+    for (size_t i = 0; i < num_vars; i++) eqp[i] = load_fe(eq_point_mont + 32 * i);
+    const size_t nt = ncols * (l.halving ? 2 : 1);
+    std::vector<Fe> finals(nt + 1);
+    const double t0 = now_ms();
+    gkr_prove_sumcheck_device(ctx, l, w.data(), lower, upper, ncols, (int)num_vars, eqp.data(), fn, rounds.data(),
+                              point.data(), finals.data());
+    if (ctx.profiling) ctx.stats.sumcheck_ms += now_ms() - t0;
+    for (size_t i = 0; i < 4 * num_vars; i++) store_fe(rounds_out + 32 * i, rounds[i]);
+    if (point_out)
+        for (size_t i = 0; i < num_vars; i++) store_fe(point_out + 32 * i, point[i]);
+    if (finals_out)
+        for (size_t i = 0; i < nt; i++) store_fe(finals_out + 32 * i, finals[i]);
+}
+
+int apg_gkr_prove_sumcheck_dev(apg_ctx *ctx, const apg_gkr_list *list, const uint8_t *weights_mont,
+                               const uint8_t *eq_point_mont, size_t num_vars, void *const *lower_dev,
+                               void *const *upper_dev, size_t ncols, apg_round_fn round_fn, void *user,
+                               uint8_t *rounds_out, uint8_t *point_out, uint8_t *finals_out) {
+    return guarded([&] {
+        need(ctx);
+        if (list && list->halving && ncols && !upper_dev) throw ApgError(APG_ERR_ARG, "gkr: a halving list needs child-1 tables");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        std::vector<Fe *> lo(ncols + 1), up(ncols + 1);
+        for (size_t c = 0; c < ncols; c++) {
+            lo[c] = static_cast<Fe *>(lower_dev[c]);
+            up[c] = list && list->halving ? static_cast<Fe *>(upper_dev[c]) : nullptr;
+        }
+        gkr_sumcheck_common(ctx->c, list, weights_mont, eq_point_mont, num_vars, lo.data(), up.data(), ncols,
+                            wrap_round(round_fn, user), rounds_out, point_out, finals_out);
+    });
+}
+
+int apg_gkr_prove_sumcheck(apg_ctx *ctx, const apg_gkr_list *list, const uint8_t *weights_mont,
+                           const uint8_t *eq_point_mont, size_t num_vars, const uint8_t *const *lower_mont,
+                           const uint8_t *const *upper_mont, size_t ncols, apg_round_fn round_fn, void *user,
+                           uint8_t *rounds_out, uint8_t *point_out, uint8_t *finals_out) {
+    return guarded([&] {
+        need(ctx);
+        if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+        const bool halving = list && list->halving;
+        if (halving && ncols && !upper_mont) throw ApgError(APG_ERR_ARG, "gkr: a halving list needs child-1 tables");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        const size_t len = size_t(1) << num_vars;
+        std::vector<const uint8_t *> all(lower_mont, lower_mont + ncols);
+        if (halving) all.insert(all.end(), upper_mont, upper_mont + ncols);
+        std::vector<Fe *> t = upload_tables(ctx->c, all.data(), all.size(), len);
+        std::vector<Fe *> lo(t.begin(), t.begin() + ncols), up(ncols + 1, nullptr);
+        if (halving)
+            for (size_t c = 0; c < ncols; c++) up[c] = t[ncols + c];
+        lo.push_back(nullptr);
+        gkr_sumcheck_common(ctx->c, list, weights_mont, eq_point_mont, num_vars, lo.data(), up.data(), ncols,
+                            wrap_round(round_fn, user), rounds_out, point_out, finals_out);
     });
 }
 

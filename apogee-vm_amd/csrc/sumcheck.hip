@@ -30,14 +30,11 @@
 #include <cstring>
 
 #include "sumcheck.h"
+#include "sumcheck_dev.cuh"
 
 namespace apg {
 
 namespace {
-
-constexpr int RTPB = 256;
-constexpr int RWAVES = RTPB / 64;
-constexpr int MAX_RBLOCKS = 2048;
 
 struct TablePtrs {
     const Fe *src[SC_MAX_TABLES];
@@ -48,34 +45,6 @@ struct GateArgs {
     int nterms;
     Term terms[SC_MAX_TERMS];
 };
-
-// E_s(i) for the pairs of one round: lo[i mod 2^lo_bits] * hi[i >> lo_bits],
-// or hi[i] when lo is null.
-struct EqArgs {
-    const Fe *lo;
-    const Fe *hi;
-    uint32_t lo_bits;
-};
-
-__device__ __forceinline__ Fe ldfe(const Fe *p, size_t i) {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p + i);
-    uint4 a = q[0], b = q[1];
-    Fe r;
-    r.l[0] = a.x; r.l[1] = a.y; r.l[2] = a.z; r.l[3] = a.w;
-    r.l[4] = b.x; r.l[5] = b.y; r.l[6] = b.z; r.l[7] = b.w;
-    return r;
-}
-
-__device__ __forceinline__ void stfe(Fe *p, size_t i, const Fe &v) {
-    uint4 *q = reinterpret_cast<uint4 *>(p + i);
-    q[0] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
-    q[1] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
-}
-
-__device__ __forceinline__ Fe eq_factor(const EqArgs &q, size_t i) {
-    if (!q.lo) return ldfe(q.hi, i);
-    return f_mul<Fr>(ldfe(q.lo, i & ((size_t(1) << q.lo_bits) - 1)), ldfe(q.hi, i >> q.lo_bits));
-}
 
 // lo/hi of pair i of table t, folding the previous round's table if asked.
 __device__ __forceinline__ void pair_of(const TablePtrs &tp, int t, size_t i, bool fold,
@@ -92,68 +61,6 @@ __device__ __forceinline__ void pair_of(const TablePtrs &tp, int t, size_t i, bo
         lo = ldfe(tp.src[t], 2 * i);
         hi = ldfe(tp.src[t], 2 * i + 1);
     }
-}
-
-__device__ __forceinline__ Fe wave_sum(Fe v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        Fe o;
-#pragma unroll
-        for (int q = 0; q < 8; q++) o.l[q] = __shfl_xor(v.l[q], off, 64);
-        v = f_add<Fr>(v, o);
-    }
-    return v;
-}
-
-// Sum of the first nn of NN per-thread accumulators over the grid into out[].
-template <int NN>
-__device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *counter, Fe *out) {
-    __shared__ Fe lds[RWAVES][NN];
-    __shared__ bool is_last;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-#pragma unroll
-    for (int x = 0; x < NN; x++) {
-        if (x < nn) {
-            const Fe v = wave_sum(acc[x]);
-            if (lane == 0) lds[w][x] = v;
-        }
-    }
-    __syncthreads();
-    const bool single = gridDim.x == 1;
-    if (tid < nn) {
-        Fe s = lds[0][tid];
-        for (int q = 1; q < RWAVES; q++) s = f_add<Fr>(s, lds[q][tid]);
-        if (single)
-            out[tid] = s;
-        else
-            partials[(size_t)blockIdx.x * NN + tid] = s;
-    }
-    if (single) return;
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    for (int x = 0; x < nn; x++) {
-        Fe a = f_zero<Fr>();
-        for (unsigned b = tid; b < gridDim.x; b += RTPB) {
-            // partials written by other blocks: read through the L2
-            const Fe *pp = partials + (size_t)b * NN + x;
-            Fe v;
-            for (int q = 0; q < 8; q++) v.l[q] = __atomic_load_n(&pp->l[q], __ATOMIC_RELAXED);
-            a = f_add<Fr>(a, v);
-        }
-        a = wave_sum(a);
-        if (lane == 0) lds[w][x] = a;
-    }
-    __syncthreads();
-    if (tid < nn) {
-        Fe s = lds[0][tid];
-        for (int q = 1; q < RWAVES; q++) s = f_add<Fr>(s, lds[q][tid]);
-        out[tid] = s;
-    }
-    if (tid == 0) *counter = 0;
 }
 
 // Product summand over KT tables (coef 1): the shards' eq * a * b (* c).
@@ -237,43 +144,6 @@ __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateAr
         }
     }
     reduce_nodes<NN>(acc, nn, partials, counter, out);
-}
-
-// The suffix-eq tables of one proof, as products over the variables they
-// cover (crates/poly/src/lib.rs:233-248: eq(r, x) = prod_j (r_j x_j +
-// (1 - r_j)(1 - x_j)), variable j = bit j):
-//   A_s (1 <= s < h0): 2^(h0-s) entries over variables s..h0-1, at A + (s-1) 2^(h0-1);
-//   B_s (h0 <= s <= n): 2^(n-s) entries over variables s..n-1, at B + (s-h0) 2^(n-h0).
-__global__ void k_eq_suffix(const Fe *__restrict__ r, int n, int h0, Fe *__restrict__ A,
-                            Fe *__restrict__ B) {
-    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t astride = size_t(1) << (h0 - 1), bstride = size_t(1) << (n - h0);
-    const size_t asz = (size_t)(h0 - 1) * astride, bsz = (size_t)(n - h0 + 1) * bstride;
-    int s, top;
-    size_t x;
-    Fe *dst;
-    if (tid < asz) {
-        s = 1 + (int)(tid / astride);
-        x = tid % astride;
-        top = h0;
-        dst = A + tid;
-    } else if (tid < asz + bsz) {
-        const size_t u = tid - asz;
-        s = h0 + (int)(u / bstride);
-        x = u % bstride;
-        top = n;
-        dst = B + u;
-    } else {
-        return;
-    }
-    if (x >= (size_t(1) << (top - s))) return;
-    const Fe one = f_one<Fr>();
-    Fe acc = one;
-    for (int l = s; l < top; l++) {
-        const Fe rl = ldfe(r, l);
-        acc = f_mul<Fr>(acc, ((x >> (l - s)) & 1) ? rl : f_sub<Fr>(one, rl));
-    }
-    stfe(dst, 0, acc);
 }
 
 struct CombineArgs {

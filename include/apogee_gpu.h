@@ -216,6 +216,66 @@ int apg_sumcheck_prove_dev(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
                            const uint8_t *eq_point_mont, apg_round_fn round_fn, void *user,
                            uint8_t *rounds_out, uint8_t *point_out, uint8_t *final_evals_out);
 
+/* ---- the general GKR layer.
+ * A gate list of a circuit artifact, flattened as gkr_verify::ResolvedList::new
+ * resolves it (crates/gkr-verify/src/lib.rs:330-395): gates[] holds the
+ * cached entries, then the producing gates, then the enforcing gates. Gate i
+ * has shape = the GateDef wire tag (0 Linear, 1 Product, 2 MaskIntoIdentity,
+ * 3 AffineProduct, 4 TreeProduct, 5 Quadratic, 6 TreeCross;
+ * crates/constraints/src/lib.rs:310-360), reads sources[src_begin, src_end)
+ * in eval_gate's value order, and takes its coefficients from
+ * coeffs_mont[coef_begin ..] in GateDef::coefficients order
+ * (crates/constraints/src/lib.rs:393-430) with every challenge slot
+ * substituted; split is an AffineProduct's left length or a Quadratic's
+ * linear length. A source is (kind << 24) | index with kind 0 = a column of
+ * the layer (child 0 of it for a halving list), 1 = child 1, 2 = virtual
+ * table index (virtual_kinds[], VirtualKind wire order 0 RowIndex ..
+ * 6 Xor8Out), 3 = cached entry index. */
+typedef struct {
+    uint32_t shape, src_begin, src_end, coef_begin, split;
+} apg_gkr_gate;
+typedef struct {
+    const apg_gkr_gate *gates;
+    uint32_t ncached, nproducing, nenforcing;
+    const uint32_t *sources;
+    uint32_t nsources;
+    const uint8_t *coeffs_mont; /* 32 B each */
+    uint32_t ncoeffs;
+    const uint32_t *virtual_kinds;
+    uint32_t nvirtual;
+    uint32_t halving;
+} apg_gkr_list;
+
+/* gkr::produce (crates/gkr/src/lib.rs:300-336): the producing gates of the
+ * list at every row. cols: ncols columns of layer k, 2^num_vars elements
+ * each; out: one column per producing gate, 2^num_vars rows for a row-wise
+ * list, 2^(num_vars - 1) for a halving one (row y reads rows y and
+ * y + 2^(num_vars - 1)). APG_ERR_ARG for a malformed list. */
+int apg_gkr_forward_layer(apg_ctx *ctx, const apg_gkr_list *list, const uint8_t *const *cols_mont, size_t ncols,
+                          size_t num_vars, uint8_t *const *out_mont);
+int apg_gkr_forward_layer_dev(apg_ctx *ctx, const apg_gkr_list *list, const void *const *cols_dev, size_t ncols,
+                              size_t num_vars, void *const *out_dev);
+
+/* gkr::prove_sumcheck (crates/gkr/src/lib.rs:792-899) for LayerSummand
+ * { weights } (crates/gkr/src/lib.rs:687-701): the rounds of
+ * sum_y eq(eq_point, y) * sum_j weights[j] gate_j, weights over the producing
+ * then the enforcing gates. lower: ncols tables of 2^num_vars elements (child
+ * 0 of each column for a halving list); upper: child 1 of each (halving
+ * lists only, else NULL). Outputs: rounds_out num_vars x 4 coefficients,
+ * point_out num_vars challenges, finals_out the tables bound to the point
+ * (lower, then upper). round_fn absorbs each round and returns the challenge
+ * (apg_transcript_round does SUMCHECK_ROUND / SUMCHECK_CHALLENGE).
+ * APG_ERR_ARG for a malformed list or a weighted gate of degree above 2 (the
+ * artifact's degree law). The _dev form consumes its tables. */
+int apg_gkr_prove_sumcheck(apg_ctx *ctx, const apg_gkr_list *list, const uint8_t *weights_mont,
+                           const uint8_t *eq_point_mont, size_t num_vars, const uint8_t *const *lower_mont,
+                           const uint8_t *const *upper_mont, size_t ncols, apg_round_fn round_fn, void *user,
+                           uint8_t *rounds_out, uint8_t *point_out, uint8_t *finals_out);
+int apg_gkr_prove_sumcheck_dev(apg_ctx *ctx, const apg_gkr_list *list, const uint8_t *weights_mont,
+                               const uint8_t *eq_point_mont, size_t num_vars, void *const *lower_dev,
+                               void *const *upper_dev, size_t ncols, apg_round_fn round_fn, void *user,
+                               uint8_t *rounds_out, uint8_t *point_out, uint8_t *finals_out);
+
 /* sumcheck::prove_zerocheck (crates/sumcheck/src/lib.rs:359): n
  * SUMCHECK_CHALLENGE draws for r, the rounds, then final_evals absorbed under
  * SUMCHECK_FINAL_EVALS, all on the given transcript. */

@@ -108,6 +108,14 @@ def _declare(L):
     d("orc_fill_uniform", [U64, P, SZ])
     d("orc_fill_witness", [U64, P, SZ])
     d("orc_fill_ctr", [U64, C.c_int, P, SZ])
+    U32 = C.c_uint32
+    d("orc_gkr_eval_gate", [U32, P, U32, P, U32, P])
+    d("orc_gkr_virtual_at_row", [U32, U64, P])
+    d("orc_gkr_virtual_at_point", [U32, P, SZ, P])
+    d("orc_gkr_gate_values", [P, P, SZ, P, SZ, P, P])
+    d("orc_gkr_summand", [P, P, P, SZ, P, SZ, P, P])
+    d("orc_gkr_forward_layer", [P, P, SZ, SZ, P])
+    d("orc_gkr_prove_sumcheck", [P, P, P, SZ, P, P, SZ, P, P, P])
 
 
 def ptr(a: np.ndarray):
@@ -565,3 +573,123 @@ def fill_witness(seed: int, n: int) -> np.ndarray:
     out = fe(n)
     lib().orc_fill_witness(seed, ptr(out), n)
     return out
+
+
+# ---------------------------------------------------------------- GKR layer (oracle/gkr.c)
+GATE_LINEAR, GATE_PRODUCT, GATE_MASK, GATE_AFFINE_PRODUCT, GATE_TREE_PRODUCT, GATE_QUADRATIC, GATE_TREE_CROSS = range(7)
+VIRT_ROW_INDEX, VIRT_RAM_LIVE, VIRT_RANGE19, VIRT_RANGE16, VIRT_XOR8_A, VIRT_XOR8_B, VIRT_XOR8_OUT = range(7)
+SRC_LOWER, SRC_UPPER, SRC_VIRTUAL, SRC_CACHED = range(4)
+
+
+class OrcGkrList(C.Structure):
+    _fields_ = [
+        ("gates", C.c_void_p),
+        ("ncached", C.c_uint32),
+        ("nproducing", C.c_uint32),
+        ("nenforcing", C.c_uint32),
+        ("sources", C.c_void_p),
+        ("nsources", C.c_uint32),
+        ("coeffs", C.c_void_p),
+        ("ncoeffs", C.c_uint32),
+        ("virtual_kinds", C.c_void_p),
+        ("nvirtual", C.c_uint32),
+        ("halving", C.c_uint32),
+    ]
+
+
+class GkrFlat:
+    """A gate list flattened as ResolvedList::new resolves it: gates (N, 5)
+    uint32 rows (shape, src_begin, src_end, coef_begin, split), sources
+    uint32 words (kind << 24 | index), coeffs (M, 4) Montgomery limbs, the
+    virtual kinds, and the counts."""
+
+    def __init__(self, gates, ncached, nproducing, nenforcing, sources, coeffs, kinds, halving):
+        self.gates = np.ascontiguousarray(np.asarray(gates, dtype=np.uint32).reshape(-1, 5))
+        self.sources = np.ascontiguousarray(np.asarray(sources, dtype=np.uint32).reshape(-1))
+        self.coeffs = np.ascontiguousarray(np.asarray(coeffs, dtype=np.uint64).reshape(-1, 4))
+        self.kinds = np.ascontiguousarray(np.asarray(kinds, dtype=np.uint32).reshape(-1))
+        self.ncached, self.nproducing, self.nenforcing = ncached, nproducing, nenforcing
+        self.halving = bool(halving)
+        self._s = OrcGkrList(
+            self.gates.ctypes.data, ncached, nproducing, nenforcing, self.sources.ctypes.data,
+            self.sources.shape[0], self.coeffs.ctypes.data, self.coeffs.shape[0], self.kinds.ctypes.data,
+            self.kinds.shape[0], 1 if halving else 0)
+
+    def ref(self):
+        return C.byref(self._s)
+
+    @property
+    def nweighted(self):
+        return self.nproducing + self.nenforcing
+
+
+def _arr(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.uint64).reshape(-1, 4))
+    if a.shape[0] == 0:
+        a = fe(1)
+    return a
+
+
+def gkr_eval_gate(shape: int, coeffs, split: int, values):
+    co = _arr(coeffs)
+    v = _arr(values)
+    nv = 0 if values is None else np.asarray(values, dtype=np.uint64).reshape(-1, 4).shape[0]
+    out = fe()
+    lib().orc_gkr_eval_gate(shape, ptr(co), split, ptr(v), nv, ptr(out))
+    return out
+
+
+def gkr_virtual_at_row(kind: int, row: int):
+    out = fe()
+    lib().orc_gkr_virtual_at_row(kind, row, ptr(out))
+    return out
+
+
+def gkr_virtual_at_point(kind: int, point):
+    p = _arr(point)
+    n = np.asarray(point, dtype=np.uint64).reshape(-1, 4).shape[0]
+    out = fe()
+    lib().orc_gkr_virtual_at_point(kind, ptr(p), n, ptr(out))
+    return out
+
+
+def _n(x):
+    return 0 if x is None else np.asarray(x, dtype=np.uint64).reshape(-1, 4).shape[0]
+
+
+def gkr_gate_values(flat: GkrFlat, lower, upper=None, virtuals=None):
+    out = fe(max(flat.nweighted, 1))
+    lib().orc_gkr_gate_values(flat.ref(), ptr(_arr(lower)), _n(lower), ptr(_arr(upper if upper is not None else [])),
+                              _n(upper), ptr(_arr(virtuals if virtuals is not None else [])), ptr(out))
+    return out[:flat.nweighted]
+
+
+def gkr_summand(flat: GkrFlat, weights, lower, upper=None, virtuals=None):
+    out = fe()
+    lib().orc_gkr_summand(flat.ref(), ptr(_arr(weights)), ptr(_arr(lower)), _n(lower),
+                          ptr(_arr(upper if upper is not None else [])), _n(upper),
+                          ptr(_arr(virtuals if virtuals is not None else [])), ptr(out))
+    return out
+
+
+def gkr_forward_layer(flat: GkrFlat, cols, num_vars: int):
+    """gkr::produce for one list: one output column per producing gate."""
+    cols = [np.ascontiguousarray(c) for c in cols]
+    rows = (1 << num_vars) // (2 if flat.halving else 1)
+    outs = [fe(rows) for _ in range(flat.nproducing)]
+    lib().orc_gkr_forward_layer(flat.ref(), _table_ptrs(cols), len(cols), num_vars, _table_ptrs(outs))
+    return outs
+
+
+def gkr_prove_sumcheck(flat: GkrFlat, weights, eq_point, lower, upper, transcript: Transcript):
+    """gkr::prove_sumcheck for the weighted list. Tables are copied (the
+    copies are left bound); returns (rounds[n, 4, 4], point[n, 4], finals:
+    the bound lower then upper values)."""
+    n = _n(eq_point)
+    lo = [np.ascontiguousarray(t.copy()) for t in lower]
+    up = [np.ascontiguousarray(t.copy()) for t in (upper or [])]
+    rounds = np.zeros((max(n, 1), 4, 4), dtype=np.uint64)
+    point = fe(max(n, 1))
+    lib().orc_gkr_prove_sumcheck(flat.ref(), ptr(_arr(weights)), ptr(_arr(eq_point)), n, _table_ptrs(lo),
+                                 _table_ptrs(up) if up else None, len(lo), transcript.ref(), ptr(rounds), ptr(point))
+    return rounds[:n], point[:n], np.stack([t[0] for t in lo + up])

@@ -150,6 +150,65 @@ int orc_mercury_batch_open_stacked(const uint64_t *srs_aff, size_t srs_len, cons
                                    const uint8_t *cms_wire, const uint64_t *u, const uint64_t *r, size_t nr,
                                    orc_transcript *tr, size_t workers, uint64_t *vs_out, uint8_t *proof_out);
 
+/* ---- the general GKR layer, oracle/gkr.c ----
+ * Gate shapes carry the wire tags of constraints::GateDef; virtual kinds the
+ * wire order of constraints::VirtualKind (crates/constraints/src/lib.rs:
+ * 215-250,310-360). A source is (kind << 24) | index: lower[index] (a column
+ * of the layer, or child 0 of one), upper[index] (child 1), virtuals[index],
+ * or cached entry index. Coefficients are Montgomery limbs, per gate in
+ * GateDef::coefficients order with challenges substituted; split is an
+ * AffineProduct's left length or a Quadratic's linear length. */
+enum {
+    ORC_GATE_LINEAR = 0,
+    ORC_GATE_PRODUCT = 1,
+    ORC_GATE_MASK = 2,
+    ORC_GATE_AFFINE_PRODUCT = 3,
+    ORC_GATE_TREE_PRODUCT = 4,
+    ORC_GATE_QUADRATIC = 5,
+    ORC_GATE_TREE_CROSS = 6,
+};
+enum {
+    ORC_VIRT_ROW_INDEX = 0,
+    ORC_VIRT_RAM_LIVE = 1,
+    ORC_VIRT_RANGE19 = 2,
+    ORC_VIRT_RANGE16 = 3,
+    ORC_VIRT_XOR8_A = 4,
+    ORC_VIRT_XOR8_B = 5,
+    ORC_VIRT_XOR8_OUT = 6,
+};
+enum { ORC_SRC_LOWER = 0, ORC_SRC_UPPER = 1, ORC_SRC_VIRTUAL = 2, ORC_SRC_CACHED = 3 };
+#define ORC_SRC_KIND_SHIFT 24
+#define ORC_SRC_INDEX_MASK 0xffffffu
+#define ORC_TAG_SUMCHECK_ROUND 4
+#define ORC_TAG_SUMCHECK_CHALLENGE 5
+typedef struct {
+    uint32_t shape, src_begin, src_end, coef_begin, split;
+} orc_gkr_gate;
+typedef struct {
+    const orc_gkr_gate *gates; /* cached entries, producing, enforcing */
+    uint32_t ncached, nproducing, nenforcing;
+    const uint32_t *sources;
+    uint32_t nsources;
+    const uint64_t *coeffs;
+    uint32_t ncoeffs;
+    const uint32_t *virtual_kinds;
+    uint32_t nvirtual;
+    uint32_t halving;
+} orc_gkr_list;
+void orc_gkr_eval_gate(uint32_t shape, const uint64_t *coeffs, uint32_t split, const uint64_t *values,
+                       uint32_t nvalues, uint64_t *out);
+void orc_gkr_virtual_at_row(uint32_t kind, uint64_t row, uint64_t *out);
+void orc_gkr_virtual_at_point(uint32_t kind, const uint64_t *point, size_t n, uint64_t *out);
+void orc_gkr_gate_values(const orc_gkr_list *l, const uint64_t *lower, size_t nlower, const uint64_t *upper,
+                         size_t nupper, const uint64_t *virtuals, uint64_t *out);
+void orc_gkr_summand(const orc_gkr_list *l, const uint64_t *weights, const uint64_t *lower, size_t nlower,
+                     const uint64_t *upper, size_t nupper, const uint64_t *virtuals, uint64_t *out);
+void orc_gkr_forward_layer(const orc_gkr_list *l, uint64_t *const *cols, size_t ncols, size_t num_vars,
+                           uint64_t **out);
+void orc_gkr_prove_sumcheck(const orc_gkr_list *l, const uint64_t *weights, const uint64_t *eq_point, size_t n,
+                            uint64_t *const *lower, uint64_t *const *upper, size_t ncols, orc_transcript *tr,
+                            uint64_t *rounds_out, uint64_t *point_out);
+
 #ifdef __cplusplus
 }
 #endif
