@@ -73,8 +73,8 @@ void gkr_forward_device(Context &ctx, const GkrList &l, const Fe *const *cols, s
 
 // gkr::prove_sumcheck (crates/gkr/src/lib.rs:792-899) for the summand
 // sum_j weights[j] gate_j (producing then enforcing): lower, and for a
-// halving list upper, hold ncols device tables of 2^n entries each
-// (consumed). rounds_out: n x 4 coefficients; point_out: n; finals_out: the
+// halving list upper, hold ncols device tables of 2^n entries each (read,
+// not modified). rounds_out: n x 4 coefficients; point_out: n; finals_out: the
 // tables bound to the point, lower then upper.
 void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights, Fe *const *lower,
                                Fe *const *upper, size_t ncols, int n, const Fe *eq_point,

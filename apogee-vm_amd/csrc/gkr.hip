@@ -260,6 +260,51 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round(DProg p, const Fe *const *sr
     reduce_nodes<3>(acc, 3, partials, counter, out);
 }
 
+// The fold of a round's tables alone: work item w folds pair w % m of table
+// w / m (rounds too small to fill the device with pairs).
+__global__ void __launch_bounds__(RTPB) k_gkr_fold(const Fe *const *src, Fe *const *dst, uint32_t ntab, size_t m,
+                                                 Fe r) {
+    const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= m * ntab) return;
+    const size_t i = w % m;
+    const Fe *s = src[w / m];
+    Fe *d = dst[w / m];
+    const Fe a0 = ldfe(s, 4 * i), a1 = ldfe(s, 4 * i + 1), a2 = ldfe(s, 4 * i + 2), a3 = ldfe(s, 4 * i + 3);
+    stfe(d, 2 * i, f_add<Fr>(a0, f_mul<Fr>(r, f_sub<Fr>(a1, a0))));
+    stfe(d, 2 * i + 1, f_add<Fr>(a2, f_mul<Fr>(r, f_sub<Fr>(a3, a2))));
+}
+
+// A round split over (pair, gate) work items: item w evaluates weighted gate
+// w / m at pair w % m (after that pair's virtual values and cached entries),
+// so a round with few pairs still spreads its gates over the device; the
+// sums are the same, the node sums being linear in the gates.
+__global__ void __launch_bounds__(RTPB) k_gkr_round_split(DProg p, const Fe *const *tab, uint32_t nlower, size_t m,
+                                                        EqArgs eq, const Fe *rho, int n, int b, Fe *slots,
+                                                        Fe *partials, unsigned *counter, Fe *out) {
+    Fe acc[3];
+#pragma unroll
+    for (int x = 0; x < 3; x++) acc[x] = f_zero<Fr>();
+    const size_t stride = (size_t)gridDim.x * blockDim.x, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t items = m * p.ngates;
+    for (size_t w = tid; w < items; w += stride) {
+        const size_t i = w % m;
+        const uint32_t g = (uint32_t)(w / m);
+        const Fe ew = f_mul<Fr>(eq_factor(eq, i), p.w[g]);
+#pragma unroll 1
+        for (int x = 0; x < 3; x++) {
+            const Reader rd{tab, nlower, i, 0, x, false, slots, stride, tid};
+            for (uint32_t v = 0; v < p.nvirt; v++)
+                rd.set_slot(v, virtual_at_point(p.kinds[v], RoundPoint{rho, n, b, i, x}));
+            fill_cached(p, rd);
+            const Fe es = f_mul<Fr>(ew, eval_gate(p, p.ncached + g, rd));
+            if (x == 0) acc[0] = f_add<Fr>(acc[0], es);
+            else if (x == 1) acc[1] = f_add<Fr>(acc[1], es);
+            else acc[2] = f_add<Fr>(acc[2], es);
+        }
+    }
+    reduce_nodes<3>(acc, 3, partials, counter, out);
+}
+
 // gkr::produce: the producing gates at every row y
 __global__ void __launch_bounds__(RTPB) k_gkr_forward(DProg p, const Fe *const *cols, size_t rows, Fe *slots,
                                                     Fe *const *out, uint32_t nproducing) {
@@ -483,6 +528,19 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
     unsigned *counter = static_cast<unsigned *>(ctx.dev["gkr.counter"].ptr);
     Fe *hvals = ctx.h<Fe>("gkr.hvals", 3);
 
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (ctx.profiling) {
+        HIP_CHECK(hipEventCreate(&ev0));
+        HIP_CHECK(hipEventCreate(&ev1));
+    }
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } evg{ev0, ev1};
+
     const Fe inv2 = f_inv<Fr>(f_from_u64<Fr>(2));
     Fe rho = f_zero<Fr>(), P = f_one<Fr>();
     Fe *const *cur = dp;  // round j's tables before the fold
@@ -509,15 +567,37 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
         }
         const size_t cap = std::min<size_t>(MAX_RBLOCKS, (size_t)4 * ctx.cu_count);
         const dim3 g((unsigned)std::min(cap, std::max<size_t>(1, (m + RTPB - 1) / RTPB))), b(RTPB);
-        if (fold)
+        if (ctx.profiling) HIP_CHECK(hipEventRecord(ev0, st));
+        if (m < cap * RTPB / 4 && s.prog.ngates > 1) {
+            // fewer pairs than a quarter of the resident threads: fold, then
+            // spread the (pair, gate) items over the device
+            if (fold)
+                hipLaunchKernelGGL(k_gkr_fold, dim3(grid_for(m * nt, RTPB)), b, 0, st, src, dst, (uint32_t)nt, m,
+                                   rho);
+            const size_t items = m * s.prog.ngates;
+            const dim3 gs((unsigned)std::min(cap, std::max<size_t>(1, (items + RTPB - 1) / RTPB)));
+            hipLaunchKernelGGL(k_gkr_round_split, gs, b, 0, st, s.prog, fold ? dst : src, (uint32_t)ncols, m, eq,
+                               rho_d, n, j, slots, partials, counter, dout);
+        } else if (fold) {
             hipLaunchKernelGGL(k_gkr_round<true>, g, b, 0, st, s.prog, src, dst, (uint32_t)nt, (uint32_t)ncols, m,
                                rho, eq, rho_d, n, j, slots, partials, counter, dout);
-        else
+        } else {
             hipLaunchKernelGGL(k_gkr_round<false>, g, b, 0, st, s.prog, src, dst, (uint32_t)nt, (uint32_t)ncols, m,
                                rho, eq, rho_d, n, j, slots, partials, counter, dout);
+        }
         HIP_CHECK(hipGetLastError());
+        if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
         HIP_CHECK(hipMemcpyAsync(hvals, dout, 3 * sizeof(Fe), hipMemcpyDeviceToHost, st));
         ctx.sync();
+        if (ctx.profiling) {
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            ctx.stats.sumcheck_kernel_ms += ms;
+            ctx.stats.sumcheck_launches += 1;
+            // every table read once (a pair's 2 entries, or 4 of the
+            // previous round's table and the 2 folded entries written)
+            ctx.stats.sumcheck_bytes += (uint64_t)nt * 32 * (fold ? 6 * m : 2 * m);
+        }
         // S(X) = s0 + s1 X + s2 X^2 from S(0), S(1), S(2); then
         // g(X) = P eq(r_j, X) S(X) with eq(r_j, X) = (1 - r_j) + X (2 r_j - 1)
         const Fe s0 = hvals[0];

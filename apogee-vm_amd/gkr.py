@@ -481,7 +481,7 @@ def prove_sumcheck(eq_point, resolved: ResolvedList, weights, lower: Sequence, u
 def prove_sumcheck_device(eq_point, resolved: ResolvedList, weights, lower: Sequence[DeviceBuffer],
                           upper: Optional[Sequence[DeviceBuffer]], transcript: Transcript,
                           ctx: Optional[Context] = None):
-    """prove_sumcheck on device-resident tables (consumed)."""
+    """prove_sumcheck on device-resident tables (read, not modified)."""
     ctx = ctx or default_context()
     up = list(upper) if upper is not None else []
     eqp = _fe_array(eq_point) if len(eq_point) else np.zeros((1, 4), dtype=np.uint64)

@@ -19,6 +19,11 @@ Workloads (--workload):
   open   pcs::open (the Mercury opening) of one witness-mix column of
          2^--open-vars rows resident in HBM, a step per opening; the oracle
          opens a 2^16 column on the host beside it (parity checked).
+  gkr    one GKR layer transition at 2^--gkr-vars rows in HBM (16 columns,
+         20 weighted gates of the reference's shapes, two virtual tables):
+         the forward pass, the row-wise layer sumcheck and the halving list's
+         sumcheck over the next layer; the oracle runs the row-wise sumcheck
+         at 2^14 rows beside it (parity checked).
 Inputs are generated into HBM before any timed region. One JSON line on
 stdout; progress on stderr. The mixed run reports the bucket-accumulation
 kernel's roofline over the engine's timed passes (HIP events on each launch's
@@ -27,7 +32,7 @@ single 2^20 x 8 shard), the single large shard's numbers, and a CPU baseline:
 the oracle's restatement of the reference algorithms timed on a sample of the
 batch's shards on the host cores, whose outputs are checked against the GPU's.
 
-    python bench.py [--workload mixed|shard|serve|copy|open] [--steps K] [--warmup W]
+    python bench.py [--workload mixed|shard|serve|copy|open|gkr] [--steps K] [--warmup W]
                     [--workers N] [--policy fifo|shortest] [--reserve R] [--lane-cus C]
 """
 from __future__ import annotations
@@ -310,13 +315,175 @@ def run_open(ctx, args):
             "deterministic": same_all, "cpu_baseline": cpu}
 
 
+# ------------------------------------------------------------------ GKR layer
+GKR_COLS = 16
+FR_MUL_IMUL = 136  # one 8 x 32-bit CIOS Montgomery multiply: 64 + 64 + 8
+
+
+def gkr_artifact(g, ncols=GKR_COLS):
+    """A two-list circuit of the reference's shapes over `ncols` committed
+    columns: a row-wise list (8 products, 4 affine products of 3 + 3 terms,
+    2 quadratics, 2 linear copies, 4 enforcing quadratics, the RowIndex and
+    Range16 virtual tables read by two of them), then a halving list (a
+    product and a fraction tree over its output: tree products and crosses)."""
+    lit = lambda x: g.Literal.of(apg.fr_from_u64(x))
+    ch = g.Challenge(1)
+    W = [g.Witness(i) for i in range(ncols)]
+    rowi, r16 = g.Virtual(g.VirtualKind.RowIndex), g.Virtual(g.VirtualKind.Range16)
+    prod = []
+    for j in range(8):
+        prod.append(g.Product(lit(j + 2), W[j], W[(j + 5) % ncols]))
+    for j in range(4):
+        a, b = [W[(3 * j + q) % ncols] for q in range(3)], [W[(3 * j + 7 + q) % ncols] for q in range(3)]
+        prod.append(g.AffineProduct([(lit(q + 2), x) for q, x in enumerate(a)], lit(3),
+                                    [(ch if q == 0 else lit(q + 5), x) for q, x in enumerate(b)], lit(7)))
+    prod.append(g.Quadratic(lit(7), [(lit(2), W[1]), (ch, rowi)],
+                            [(lit(3), W[2], W[3]), (ch, W[4], W[5]), (lit(5), W[6], r16)]))
+    prod.append(g.Quadratic(lit(1), [(lit(9), W[8]), (lit(4), W[9])],
+                            [(lit(3), W[10], W[11]), (lit(6), W[12], W[13]), (lit(5), W[14], W[15])]))
+    prod.append(g.Linear([(lit(1), W[14])], lit(0)))
+    prod.append(g.Linear([(lit(1), W[15]), (lit(2), W[0])], lit(0)))
+    enf = [g.Quadratic(lit(j + 1), [(lit(2), W[j])], [(lit(3), W[j + 4], W[j + 8])]) for j in range(4)]
+    L1 = [g.Inner(1, j) for j in range(len(prod))]
+    row = g.LayerSpec(False, 0, len(prod), producing=[(f"p{j}", L1[j], gt) for j, gt in enumerate(prod)],
+                      enforcing=[(f"e{j}", gt) for j, gt in enumerate(enf)])
+    half = []
+    for j in range(0, len(prod), 2):
+        half.append(g.TreeProduct(L1[j]))
+        half.append(g.TreeCross(L1[j + 1], L1[j]))
+    halving = g.LayerSpec(True, 0, len(half),
+                          producing=[(f"h{j}", g.Inner(2, j), gt) for j, gt in enumerate(half)])
+    art = g.CircuitArtifact([], [f"w{i}" for i in range(ncols)], [], [(g.VirtualKind.RowIndex, "row"),
+                                                                     (g.VirtualKind.Range16, "r16")],
+                            [row, halving])
+    return art, {1: apg.synthetic(99, 1, "uniform")[0]}
+
+
+def gkr_mults_per_pair(res):
+    """Fr multiplies per pair of one round (three nodes; the weight and eq
+    products; the fold of every table), from the flattened list."""
+    per_node = 1  # E(i) S
+    for shape, s0, s1, _, t in res.gates:
+        nv = int(s1 - s0)
+        m = {0: nv, 1: 2, 2: 1, 3: nv + 1, 4: 1, 5: int(t) + 2 * ((nv - int(t)) // 2), 6: 2}[int(shape)]
+        per_node += m
+    per_node += res.nproducing + res.nenforcing  # weights
+    return 3 * per_node + 1  # + the eq factor
+
+
+class _View:
+    def __init__(self, buf, offset_bytes):
+        import ctypes
+        self.ptr = ctypes.c_void_p(buf.ptr.value + offset_bytes)
+
+
+def run_gkr(ctx, args):
+    """One GKR layer transition at 2^gkr_vars rows in HBM: the forward pass of
+    the row-wise list (layer 0 -> layer 1), its layer sumcheck, and the
+    halving list's sumcheck over layer 1's children. A step is the three. The
+    CPU leg runs the oracle's row-wise sumcheck at 2^14 rows (parity checked
+    against the GPU there) and is scaled to the timed size by rows."""
+    g = importlib.import_module("apogee-vm_amd.gkr")
+    nv = args.gkr_vars
+    n = 1 << nv
+    art, chal = gkr_artifact(g)
+    res, resh = g.ResolvedList(art, 0, chal), g.ResolvedList(art, 1, chal)
+    cols = [apg.synthetic_device(ctx, args.seed + c, n, "witness") for c in range(GKR_COLS)]
+    outs = [apg.DeviceBuffer(ctx, 32 * n) for _ in range(res.nproducing)]
+    w = apg.synthetic(args.seed + 100, res.nproducing + res.nenforcing, "uniform")
+    wh = apg.synthetic(args.seed + 101, resh.nproducing, "uniform")
+    r = apg.synthetic(args.seed + 102, nv, "uniform")
+    rh = apg.synthetic(args.seed + 103, nv - 1, "uniform")
+    lower = [_View(b, 0) for b in outs]
+    upper = [_View(b, 32 * (n // 2)) for b in outs]
+
+    def step():
+        t0 = time.perf_counter()
+        g.forward_layer_device(res, cols, nv, outs, ctx)
+        t1 = time.perf_counter()
+        p1 = g.prove_sumcheck_device(r, res, w, cols, None, apg.Transcript(), ctx)
+        t2 = time.perf_counter()
+        p2 = g.prove_sumcheck_device(rh, resh, wh, lower, upper, apg.Transcript(), ctx)
+        t3 = time.perf_counter()
+        return (t1 - t0, t2 - t1, t3 - t2), (p1, p2)
+
+    log(f"gkr: 2^{nv} rows, {GKR_COLS} columns, {res.nproducing} + {res.nenforcing} gates, warmup {args.warmup}")
+    first = None
+    for _ in range(max(args.warmup, 1)):
+        _, first = step()
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    before = ctx.stats()
+    times = []
+    same_all = True
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tt, proofs = step()
+        times.append(tt)
+        same_all = same_all and all(np.array_equal(a, b) for p, q in zip(proofs, first) for a, b in zip(p, q))
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    st = stats_delta(ctx.stats(), before)
+    ctx.set_profiling(False)
+    fw, sc, hv = (float(np.mean([t[i] for t in times])) for i in range(3))
+    log(f"gkr: forward {fw * 1e3:.2f} ms, row-wise sumcheck {sc * 1e3:.2f} ms, halving sumcheck {hv * 1e3:.2f} ms")
+    # roofline of the round kernels of the row-wise sumcheck: Fr multiplies x 136 IMUL
+    mults = gkr_mults_per_pair(res)
+    pairs_row = n - 1  # sum over rounds of the pairs
+    multsh = gkr_mults_per_pair(resh)
+    pairs_half = n // 2 - 1
+    # the fold: 2 multiplies per table per pair from round 1 on
+    folds = 2 * GKR_COLS * (n // 2 - 1) + 2 * 2 * GKR_COLS * (n // 4 - 1)
+    imul = FR_MUL_IMUL * (mults * pairs_row + multsh * pairs_half + folds) * args.steps
+    kern_s = st["sumcheck_kernel_ms"] * 1e-3
+    achieved = imul / kern_s if kern_s else 0.0
+    roof = {"kernel": "k_gkr_round (layer sumcheck rounds, row-wise and halving lists)", "bound": "valu-imul",
+            "achieved": round(achieved / 1e12, 3), "peak": round(PEAK_IMUL / 1e12, 3), "unit": "TIMUL/s",
+            "frac": round(achieved / PEAK_IMUL, 4), "traffic": None,
+            "launches": int(st["sumcheck_launches"]),
+            "avg_launch_ms": round(st["sumcheck_kernel_ms"] / max(1, st["sumcheck_launches"]), 4),
+            "fr_mults_per_pair": {"row_wise": mults, "halving": multsh},
+            "table_gbs": round(st["sumcheck_bytes"] / kern_s / 1e9, 1) if kern_s else None}
+    cpu = None
+    if args.cpu_budget > 0:
+        import oracle
+        snv = 14
+        sres = g.ResolvedList(art, 0, chal)
+        scol = [apg.synthetic(args.seed + 500 + c, 1 << snv, "witness") for c in range(GKR_COLS)]
+        sr = apg.synthetic(args.seed + 600, snv, "uniform")
+        a = sres.arrays()
+        flat = oracle.GkrFlat(a["gates"], a["ncached"], a["nproducing"], a["nenforcing"], a["sources"],
+                              a["coeffs"], a["kinds"], a["halving"])
+        t1 = time.perf_counter()
+        rounds_o, point_o, fin_o = oracle.gkr_prove_sumcheck(flat, w, sr, scol, None, oracle.Transcript())
+        cpu_s = time.perf_counter() - t1
+        rounds_d, point_d, fin_d = g.prove_sumcheck(sr, sres, w, scol, None, apg.Transcript(), ctx)
+        parity = bool(np.array_equal(rounds_o, rounds_d) and np.array_equal(point_o, point_d)
+                      and np.array_equal(fin_o, fin_d))
+        cpu = {"value": round((1 << snv) / cpu_s, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"the row-wise layer sumcheck at 2^{snv} rows in {cpu_s:.2f} s (one thread)",
+               "parity": parity}
+    return {"metric": f"GKR layer sumcheck rows/s (row-wise list, {GKR_COLS} columns, "
+                      f"{res.nproducing} + {res.nenforcing} gates, 2^{nv} rows in HBM)",
+            "value": round(n / sc, 1), "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u32x8 (BN254 Fr Montgomery, exact)",
+            "data": "synthetic (counter-mode ChaCha20 witness mix in HBM)",
+            "config": {"workload": f"GKR layer: forward + row-wise sumcheck + halving sumcheck, 2^{nv} rows",
+                       "parallelism": "1 GPU"},
+            "forward_ms": round(fw * 1e3, 3), "rowwise_sumcheck_ms": round(sc * 1e3, 3),
+            "halving_sumcheck_ms": round(hv * 1e3, 3), "deterministic": same_all, "roofline": roof,
+            "cpu_baseline": cpu}
+
+
 def main():
     global apg, wl
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve", "copy", "open"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve", "copy", "open", "gkr"])
+    ap.add_argument("--gkr-vars", type=int, default=20, help="log2 rows of the GKR layer (--workload gkr)")
     ap.add_argument("--open-vars", type=int, default=20, help="log2 rows of the opened table (--workload open)")
     # engine defaults: throughput (mixed) 8 workers in arrival order; serving
     # 4 workers (within the hardware queues the device services at once), one
@@ -349,6 +516,10 @@ def main():
 
     if args.workload == "open":
         print(json.dumps(run_open(ctx, args)), flush=True)
+        return
+
+    if args.workload == "gkr":
+        print(json.dumps(run_gkr(ctx, args)), flush=True)
         return
 
     if args.workload == "shard":

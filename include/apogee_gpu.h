@@ -266,7 +266,8 @@ int apg_gkr_forward_layer_dev(apg_ctx *ctx, const apg_gkr_list *list, const void
  * (lower, then upper). round_fn absorbs each round and returns the challenge
  * (apg_transcript_round does SUMCHECK_ROUND / SUMCHECK_CHALLENGE).
  * APG_ERR_ARG for a malformed list or a weighted gate of degree above 2 (the
- * artifact's degree law). The _dev form consumes its tables. */
+ * artifact's degree law). The _dev form takes device tables and leaves them
+ * unmodified (the folds go to the context's scratch). */
 int apg_gkr_prove_sumcheck(apg_ctx *ctx, const apg_gkr_list *list, const uint8_t *weights_mont,
                            const uint8_t *eq_point_mont, size_t num_vars, const uint8_t *const *lower_mont,
                            const uint8_t *const *upper_mont, size_t ncols, apg_round_fn round_fn, void *user,
