@@ -117,6 +117,12 @@ def _declare(L):
     d("apg_dev_copy", [VP, VP, VP, SZ])
     d("apg_srs_upload", [VP, VP, SZ, C.POINTER(VP)])
     d("apg_srs_generate", [VP, VP, SZ, C.POINTER(VP)])
+    d("apg_srs_load", [VP, C.c_char_p, C.POINTER(VP), VP, C.POINTER(I32), C.POINTER(U64)])
+    d("apg_srs_archive_check", [C.c_char_p, C.POINTER(C.c_uint32), VP, C.POINTER(I32), C.POINTER(U64)])
+    d("apg_srs_save", [VP, VP, VP, C.c_char_p])
+    d("apg_srs_upload_wire", [VP, VP, SZ, C.POINTER(VP), C.POINTER(U64)])
+    d("apg_g2_wire_valid", [VP])
+    d("apg_g1_batch_to_affine", [VP, VP, SZ, VP])
     d("apg_srs_download", [VP, VP, SZ, SZ, VP])
     d("apg_srs_len", [VP], SZ)
     d("apg_srs_prepare", [VP, VP, SZ])
@@ -429,6 +435,76 @@ def check_num_vars(num_vars: int) -> int:
     return 1 << num_vars
 
 
+class SrsError(Exception):
+    """crates/srs/src/lib.rs SrsError: the archive's refusals."""
+
+
+class SrsIo(SrsError):
+    pass
+
+
+class SrsTruncated(SrsError):
+    pass
+
+
+class SrsBadMagic(SrsError):
+    pass
+
+
+class SrsBadVersion(SrsError):
+    def __init__(self, version: int):
+        super().__init__(f"bad archive version {version}")
+        self.version = version
+
+
+class SrsBadSection(SrsError):
+    pass
+
+
+class SrsInvalidPoint(SrsError):
+    def __init__(self, index: int):
+        super().__init__(f"invalid point at index {index}")
+        self.index = index
+
+
+def _srs_error(code: int, detail: int) -> SrsError:
+    return {1: lambda: SrsIo("i/o error"), 2: lambda: SrsTruncated("truncated archive"),
+            3: lambda: SrsBadMagic("bad magic"), 4: lambda: SrsBadVersion(detail),
+            5: lambda: SrsBadSection("archive G1 count is not 2^power"),
+            6: lambda: SrsInvalidPoint(detail)}[code]()
+
+
+def srs_archive_check(path: str):
+    """The archive's header rules and its two G2 points, on the host alone:
+    returns (power, g2 bytes) or raises the SrsError."""
+    pw, code, detail = C.c_uint32(), I32(), U64()
+    g2 = np.zeros(256, dtype=np.uint8)
+    rc = lib().apg_srs_archive_check(os.fsencode(path), C.byref(pw), _p(g2), C.byref(code), C.byref(detail))
+    if rc != 0:
+        if code.value:
+            raise _srs_error(code.value, detail.value)
+        _check(rc)
+    return pw.value, g2.tobytes()
+
+
+def g2_wire_valid(wire: bytes) -> bool:
+    """G2Affine::from_bytes's validity of a 128-byte point (on the host)."""
+    b = np.frombuffer(bytes(wire), dtype=np.uint8).copy()
+    if b.shape[0] != 128:
+        raise ValueError("a G2 point is 128 bytes")
+    return lib().apg_g2_wire_valid(_p(b)) == 1
+
+
+def batch_to_affine(jac, ctx: Optional[Context] = None) -> np.ndarray:
+    """G1Projective::batch_to_affine on the device: (n, 12) Jacobian limbs to
+    (n, 8) affine limbs (all-zero for the identity)."""
+    ctx = ctx or default_context()
+    j = np.ascontiguousarray(np.asarray(jac, dtype=np.uint64).reshape(-1, 12))
+    out = np.zeros((j.shape[0], 8), dtype=np.uint64)
+    _check(lib().apg_g1_batch_to_affine(ctx.h, _p(j), j.shape[0], _p(out)))
+    return out
+
+
 class Srs:
     """The powers [tau^i]_1, resident in HBM (crates/srs/src/lib.rs:96)."""
 
@@ -453,6 +529,42 @@ class Srs:
         h = VP()
         _check(lib().apg_srs_generate(ctx.h, _p(t), n, C.byref(h)))
         return cls(h, ctx)
+
+    @classmethod
+    def load(cls, path: str, ctx: Optional[Context] = None):
+        """Srs::load (crates/srs/src/lib.rs:200-240): returns (srs, g2 bytes),
+        every G1 power validated on the device; raises the SrsError."""
+        ctx = ctx or default_context()
+        h, code, detail = VP(), I32(), U64()
+        g2 = np.zeros(256, dtype=np.uint8)
+        rc = lib().apg_srs_load(ctx.h, os.fsencode(path), C.byref(h), _p(g2), C.byref(code), C.byref(detail))
+        if rc != 0:
+            if code.value:
+                raise _srs_error(code.value, detail.value)
+            _check(rc)
+        return cls(h, ctx), g2.tobytes()
+
+    @classmethod
+    def from_wire(cls, wire: bytes, ctx: Optional[Context] = None) -> "Srs":
+        """Canonical 64-byte G1 points decoded and validated on the device
+        (crates/srs/src/ptau.rs:307-328); SrsInvalidPoint for a bad one."""
+        ctx = ctx or default_context()
+        b = np.frombuffer(bytes(wire) + b"\0" * 64, dtype=np.uint8).copy()
+        n = len(wire) // 64
+        h, bad = VP(), U64()
+        rc = lib().apg_srs_upload_wire(ctx.h, _p(b), n, C.byref(h), C.byref(bad))
+        if rc != 0:
+            if rc == -2 and bad.value < n:
+                raise SrsInvalidPoint(bad.value)
+            _check(rc)
+        return cls(h, ctx)
+
+    def save(self, path: str, g2: bytes):
+        """Srs::save (crates/srs/src/lib.rs:175-192) with the given G2 points."""
+        b = np.frombuffer(bytes(g2), dtype=np.uint8).copy()
+        if b.shape[0] != 256:
+            raise ValueError("the two G2 points are 256 bytes")
+        _check(lib().apg_srs_save(self.ctx.h, self.h, _p(b), os.fsencode(path)))
 
     def __len__(self):
         return lib().apg_srs_len(self.h)

@@ -12,6 +12,7 @@
 #include "../../include/apogee_gpu.h"
 #include "engine.h"
 #include "gkr.h"
+#include "srs_io.h"
 #include "mercury.h"
 #include "curve29.cuh"
 #include "msm.h"
@@ -397,6 +398,113 @@ int apg_srs_upload(apg_ctx *ctx, const uint8_t *bases_mont, size_t n, apg_srs **
         if (n) HIP_CHECK(hipMemcpyAsync(s->base, bases_mont, n * 64, hipMemcpyHostToDevice, ctx->c.stream));
         ctx->c.sync();
         *out = s.release();
+    });
+}
+
+static const char *srs_error_text(int e) {
+    switch (e) {
+        case SRS_IO: return "srs: i/o error";
+        case SRS_TRUNCATED: return "srs: truncated archive";
+        case SRS_BAD_MAGIC: return "srs: bad magic";
+        case SRS_BAD_VERSION: return "srs: bad version";
+        case SRS_BAD_SECTION: return "srs: archive G1 count is not 2^power";
+        default: return "srs: invalid point";
+    }
+}
+
+int apg_srs_load(apg_ctx *ctx, const char *path, apg_srs **out, uint8_t g2_out[256], int *srs_error,
+                 uint64_t *detail) {
+    return guarded([&] {
+        need(ctx);
+        if (!path || !out) throw ApgError(APG_ERR_ARG, "srs load: null argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Affine *base = nullptr;
+        size_t count = 0;
+        uint8_t g2[256];
+        uint64_t det = 0;
+        const int e = srs_load_archive(ctx->c, path, &base, &count, g2, &det);
+        if (srs_error) *srs_error = e;
+        if (detail) *detail = det;
+        if (e != SRS_OK) throw ApgError(APG_ERR_ARG, srs_error_text(e));
+        auto s = std::make_unique<apg_srs>();
+        s->device = ctx->c.device;
+        s->n = count;
+        s->base = base;
+        if (g2_out) std::memcpy(g2_out, g2, 256);
+        *out = s.release();
+    });
+}
+
+int apg_srs_archive_check(const char *path, uint32_t *power, uint8_t g2_out[256], int *srs_error,
+                          uint64_t *detail) {
+    return guarded([&] {
+        if (!path) throw ApgError(APG_ERR_ARG, "srs check: null path");
+        uint64_t det = 0;
+        uint32_t pw = 0;
+        uint8_t g2[256];
+        const int e = srs_check_archive_header(path, &pw, g2, &det);
+        if (srs_error) *srs_error = e;
+        if (detail) *detail = det;
+        if (e != SRS_OK) throw ApgError(APG_ERR_ARG, srs_error_text(e));
+        if (power) *power = pw;
+        if (g2_out) std::memcpy(g2_out, g2, 256);
+    });
+}
+
+int apg_srs_save(apg_ctx *ctx, const apg_srs *srs, const uint8_t g2[256], const char *path) {
+    return guarded([&] {
+        need(ctx);
+        if (!srs || !g2 || !path) throw ApgError(APG_ERR_ARG, "srs save: null argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        srs_save_archive(ctx->c, srs->base, srs->n, g2, path);
+    });
+}
+
+int apg_srs_upload_wire(apg_ctx *ctx, const uint8_t *wire, size_t n, apg_srs **out, uint64_t *bad_index) {
+    return guarded([&] {
+        need(ctx);
+        if (!out || (n && !wire)) throw ApgError(APG_ERR_ARG, "srs upload: null argument");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        auto s = std::make_unique<apg_srs>();
+        s->device = ctx->c.device;
+        s->n = n;
+        if (hipMalloc(&s->base, std::max<size_t>(n, 1) * sizeof(Affine)) != hipSuccess)
+            throw ApgError(APG_ERR_OOM, "device out of memory for the SRS");
+        if (n) HIP_CHECK(hipMemcpyAsync(s->base, wire, n * 64, hipMemcpyHostToDevice, ctx->c.stream));
+        const uint64_t bad = g1_decode_wire_device(ctx->c, s->base, n);
+        if (bad != UINT64_MAX) {
+            if (bad_index) *bad_index = bad;
+            throw ApgError(APG_ERR_ARG, "srs upload: invalid point");
+        }
+        *out = s.release();
+    });
+}
+
+int apg_g2_wire_valid(const uint8_t wire[128]) {
+    if (!wire) return 0;
+    try {
+        return g2_wire_valid(wire) ? 1 : 0;
+    } catch (...) {
+        return 0;
+    }
+}
+
+int apg_g1_batch_to_affine(apg_ctx *ctx, const uint8_t *jac_mont, size_t n, uint8_t *out_affine) {
+    return guarded([&] {
+        need(ctx);
+        if (n && (!jac_mont || !out_affine)) throw ApgError(APG_ERR_ARG, "batch_to_affine: null argument");
+        if (!n) return;
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Fe *jac = ctx->c.d<Fe>("b2a.jac", 3 * n);
+        Affine *aff = ctx->c.d<Affine>("b2a.aff", n);
+        HIP_CHECK(hipMemcpyAsync(jac, jac_mont, n * 96, hipMemcpyHostToDevice, ctx->c.stream));
+        g1_jac_to_affine_device(ctx->c, jac, n, aff);
+        HIP_CHECK(hipMemcpyAsync(out_affine, aff, n * 64, hipMemcpyDeviceToHost, ctx->c.stream));
+        ctx->c.sync();
     });
 }
 

@@ -129,6 +129,42 @@ int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a_mont, const uint
 int apg_debug_accumulate29(apg_ctx *ctx, int form, const uint8_t *points, const uint8_t *neg,
                            const uint64_t *offsets, size_t ncases, uint8_t *out_affine);
 
+/* ---- SRS ingestion (crates/srs/src/lib.rs:175-240, docs/spec/srs.md §4).
+ * The archive: magic "APOGESRS", version u32 = 1, power u32, count u64 =
+ * 2^power, the two G2 points (128 B each, canonical x.c0 x.c1 y.c0 y.c1),
+ * then count canonical G1 points (64 B each, x || y, all-zero = infinity).
+ * srs_error takes SrsError's variant: */
+#define APG_SRS_IO 1
+#define APG_SRS_TRUNCATED 2
+#define APG_SRS_BAD_MAGIC 3
+#define APG_SRS_BAD_VERSION 4       /* detail = the version read */
+#define APG_SRS_BAD_SECTION 5       /* count is not 2^power, or power > 30 */
+#define APG_SRS_INVALID_POINT 6     /* detail = index; count, count + 1 = the G2 points */
+/* Srs::load: every G1 power decoded and validated on the device
+ * (G1Affine::from_bytes: canonical coordinates, on the curve), the header
+ * rules and both G2 points (on the twist, r P = O) on the host, in the
+ * reference's order. On an SrsError, APG_ERR_ARG with *srs_error and
+ * *detail set; with several invalid G1 points, detail is the lowest index.
+ * g2_out receives the two G2 points as stored. */
+int apg_srs_load(apg_ctx *ctx, const char *path, apg_srs **out, uint8_t g2_out[256], int *srs_error,
+                 uint64_t *detail);
+/* The header rules and the two G2 points alone (host only, no device). */
+int apg_srs_archive_check(const char *path, uint32_t *power, uint8_t g2_out[256], int *srs_error,
+                          uint64_t *detail);
+/* Srs::save of a resident SRS whose length is a power of two, with the
+ * caller's G2 points (stored as given). */
+int apg_srs_save(apg_ctx *ctx, const apg_srs *srs, const uint8_t g2[256], const char *path);
+/* n canonical 64-byte G1 points decoded and validated on the device into a
+ * resident SRS (ptau::read_canonical_g1, crates/srs/src/ptau.rs:307-328).
+ * APG_ERR_ARG and *bad_index (the lowest) for an invalid point. */
+int apg_srs_upload_wire(apg_ctx *ctx, const uint8_t *wire, size_t n, apg_srs **out, uint64_t *bad_index);
+/* G2Affine::from_bytes validity of one 128-byte point (host): 1 or 0. */
+int apg_g2_wire_valid(const uint8_t wire[128]);
+/* G1Projective::batch_to_affine (crates/curve/src/g1.rs:328-347) on the
+ * device: n Jacobian points (96 B: x, y, z Montgomery) to affine (64 B,
+ * all-zero for z = 0). */
+int apg_g1_batch_to_affine(apg_ctx *ctx, const uint8_t *jac_mont, size_t n, uint8_t *out_affine);
+
 /* ---- device memory for callers without their own allocator ---- */
 int apg_dev_alloc(apg_ctx *ctx, size_t bytes, void **out);
 int apg_dev_free(apg_ctx *ctx, void *p);
