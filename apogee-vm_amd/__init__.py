@@ -549,8 +549,8 @@ class Srs:
         """Canonical 64-byte G1 points decoded and validated on the device
         (crates/srs/src/ptau.rs:307-328); SrsInvalidPoint for a bad one."""
         ctx = ctx or default_context()
-        b = np.frombuffer(bytes(wire) + b"\0" * 64, dtype=np.uint8).copy()
         n = len(wire) // 64
+        b = np.frombuffer(wire, dtype=np.uint8) if n else np.zeros(64, dtype=np.uint8)
         h, bad = VP(), U64()
         rc = lib().apg_srs_upload_wire(ctx.h, _p(b), n, C.byref(h), C.byref(bad))
         if rc != 0:

@@ -124,6 +124,11 @@ struct Reader {
     bool row_mode;
     Fe *slots;
     size_t nthr, tid;
+    // set: virtual tables are evaluated where a gate reads them (the round's
+    // point from rho, n, b) instead of from the slots
+    const Fe *rho;
+    int n, b;
+    bool lazy_virtual;
     __device__ __forceinline__ Fe line(uint32_t t) const {
         const Fe *p = tab[t];
         if (x == 0) return ldfe(p, 2 * i);
@@ -198,7 +203,10 @@ __device__ Fe eval_gate(const DProg &p, uint32_t g, const Reader &rd) {
         switch (s >> SRC_SHIFT) {
             case SRC_LOWER: v = rd.lower(idx); break;
             case SRC_UPPER: v = rd.upper(idx); break;
-            case SRC_VIRTUAL: v = rd.slot(idx); break;
+            case SRC_VIRTUAL:
+                v = rd.lazy_virtual ? virtual_at_point(p.kinds[idx], RoundPoint{rd.rho, rd.n, rd.b, rd.i, rd.x})
+                                    : rd.slot(idx);
+                break;
             default: v = rd.slot(p.nvirt + idx); break;  // cached entry
         }
         Fe x = v;
@@ -244,7 +252,7 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round(DProg p, const Fe *const *sr
         const Fe e = eq_factor(eq, i);
 #pragma unroll 1
         for (int x = 0; x < 3; x++) {
-            const Reader rd{FOLD ? dst : src, nlower, i, 0, x, false, slots, stride, tid};
+            const Reader rd{FOLD ? dst : src, nlower, i, 0, x, false, slots, stride, tid, rho, n, b, false};
             for (uint32_t v = 0; v < p.nvirt; v++)
                 rd.set_slot(v, virtual_at_point(p.kinds[v], RoundPoint{rho, n, b, i, x}));
             fill_cached(p, rd);
@@ -292,9 +300,8 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round_split(DProg p, const Fe *con
         const Fe ew = f_mul<Fr>(eq_factor(eq, i), p.w[g]);
 #pragma unroll 1
         for (int x = 0; x < 3; x++) {
-            const Reader rd{tab, nlower, i, 0, x, false, slots, stride, tid};
-            for (uint32_t v = 0; v < p.nvirt; v++)
-                rd.set_slot(v, virtual_at_point(p.kinds[v], RoundPoint{rho, n, b, i, x}));
+            // one gate reads few virtual tables: evaluate them where read
+            const Reader rd{tab, nlower, i, 0, x, false, slots, stride, tid, rho, n, b, true};
             fill_cached(p, rd);
             const Fe es = f_mul<Fr>(ew, eval_gate(p, p.ncached + g, rd));
             if (x == 0) acc[0] = f_add<Fr>(acc[0], es);
@@ -310,7 +317,7 @@ __global__ void __launch_bounds__(RTPB) k_gkr_forward(DProg p, const Fe *const *
                                                     Fe *const *out, uint32_t nproducing) {
     const size_t stride = (size_t)gridDim.x * blockDim.x, tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (size_t y = tid; y < rows; y += stride) {
-        const Reader rd{cols, 0, y, rows, 0, true, slots, stride, tid};
+        const Reader rd{cols, 0, y, rows, 0, true, slots, stride, tid, nullptr, 0, 0, false};
         for (uint32_t v = 0; v < p.nvirt; v++) rd.set_slot(v, virtual_at_row(p.kinds[v], y));
         fill_cached(p, rd);
         for (uint32_t g = 0; g < nproducing; g++) stfe(out[g], y, eval_gate(p, p.ncached + g, rd));
@@ -568,8 +575,8 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
         const size_t cap = std::min<size_t>(MAX_RBLOCKS, (size_t)4 * ctx.cu_count);
         const dim3 g((unsigned)std::min(cap, std::max<size_t>(1, (m + RTPB - 1) / RTPB))), b(RTPB);
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev0, st));
-        if (m < cap * RTPB / 4 && s.prog.ngates > 1) {
-            // fewer pairs than a quarter of the resident threads: fold, then
+        if (m < cap * RTPB / 8 && s.prog.ngates > 1) {
+            // fewer pairs than an eighth of the resident threads: fold, then
             // spread the (pair, gate) items over the device
             if (fold)
                 hipLaunchKernelGGL(k_gkr_fold, dim3(grid_for(m * nt, RTPB)), b, 0, st, src, dst, (uint32_t)nt, m,

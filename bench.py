@@ -24,6 +24,9 @@ Workloads (--workload):
          the forward pass, the row-wise layer sumcheck and the halving list's
          sumcheck over the next layer; the oracle runs the row-wise sumcheck
          at 2^14 rows beside it (parity checked).
+  srs    Srs::load of a 2^--srs-power archive (default 2^22) with every G1
+         point validated on the device, and the decode of the same block from
+         host memory; the oracle's decoder runs on a 2^14-point sample.
 Inputs are generated into HBM before any timed region. One JSON line on
 stdout; progress on stderr. The mixed run reports the bucket-accumulation
 kernel's roofline over the engine's timed passes (HIP events on each launch's
@@ -32,7 +35,7 @@ single 2^20 x 8 shard), the single large shard's numbers, and a CPU baseline:
 the oracle's restatement of the reference algorithms timed on a sample of the
 batch's shards on the host cores, whose outputs are checked against the GPU's.
 
-    python bench.py [--workload mixed|shard|serve|copy|open|gkr] [--steps K] [--warmup W]
+    python bench.py [--workload mixed|shard|serve|copy|open|gkr|srs] [--steps K] [--warmup W]
                     [--workers N] [--policy fifo|shortest] [--reserve R] [--lane-cus C]
 """
 from __future__ import annotations
@@ -476,13 +479,75 @@ def run_gkr(ctx, args):
             "cpu_baseline": cpu}
 
 
+# ------------------------------------------------------------------ SRS ingestion
+def run_srs(ctx, args):
+    """Srs::load of a 2^srs_power archive (written first from a device-built
+    SRS, read back from the page cache) and the canonical-point decode of the
+    same block from host memory: points/s, each load's points compared with
+    the source. The CPU leg decodes a 2^14-point sample with the oracle's
+    validating decoder (one thread), scaled by points."""
+    import tempfile
+    pw = args.srs_power
+    n = 1 << pw
+    tau = apg.synthetic(args.seed, 1, "uniform")[0]
+    src = apg.Srs.generate(tau, n, ctx)
+    g2b = bytes(256)  # infinity: valid, and the G1 block is what is timed
+    tmpdir = tempfile.mkdtemp(prefix="apg_srs_")
+    path = os.path.join(tmpdir, f"srs-{pw}.srs")
+    src.save(path, g2b)
+    with open(path, "rb") as f:
+        f.seek(280)
+        wire = f.read()
+    ref = src.g1()
+    log(f"srs: 2^{pw} points, archive {os.path.getsize(path) / 2**20:.0f} MiB")
+    ok = True
+    for _ in range(max(args.warmup, 1)):
+        s, _ = apg.Srs.load(path, ctx)
+        s.release()
+    t_load, t_wire = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        s, _ = apg.Srs.load(path, ctx)
+        t_load.append(time.perf_counter() - t0)
+        ok = ok and np.array_equal(s.g1(), ref)
+        s.release()
+        t0 = time.perf_counter()
+        s = apg.Srs.from_wire(wire, ctx)
+        t_wire.append(time.perf_counter() - t0)
+        s.release()
+    os.remove(path)
+    os.rmdir(tmpdir)
+    tl, tw = float(np.median(t_load)), float(np.median(t_wire))
+    log(f"srs: load {tl * 1e3:.1f} ms, decode from host memory {tw * 1e3:.1f} ms")
+    cpu = None
+    if args.cpu_budget > 0:
+        import oracle
+        m = 1 << 14
+        t0 = time.perf_counter()
+        dec = [oracle.affine_from_wire(wire[64 * i:64 * i + 64]) for i in range(m)]
+        cpu_s = time.perf_counter() - t0
+        parity = all(d is not None and np.array_equal(d, ref[i]) for i, d in enumerate(dec))
+        cpu = {"value": round(m / cpu_s, 1), "unit": "points/s", "cores": 1, "kind": "port",
+               "sample": f"2^14 points through the oracle's validating decoder in {cpu_s:.2f} s (one thread)",
+               "parity": parity}
+    return {"metric": f"SRS ingestion points/s (Srs::load of a 2^{pw}-point archive, validated on the device)",
+            "value": round(n / tl, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(tl * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u32x8 (BN254 Fq Montgomery, exact)",
+            "data": "synthetic (an SRS built on the device, saved and read back)",
+            "config": {"workload": f"Srs::load, 2^{pw} G1 points", "parallelism": "1 GPU"},
+            "load_gbs": round(64 * n / tl / 1e9, 2), "decode_from_host_ms": round(tw * 1e3, 3),
+            "decode_from_host_gbs": round(64 * n / tw / 1e9, 2), "points_equal": ok, "cpu_baseline": cpu}
+
+
 def main():
     global apg, wl
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve", "copy", "open", "gkr"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "shard", "serve", "copy", "open", "gkr", "srs"])
+    ap.add_argument("--srs-power", type=int, default=22, help="log2 points of the archive (--workload srs)")
     ap.add_argument("--gkr-vars", type=int, default=20, help="log2 rows of the GKR layer (--workload gkr)")
     ap.add_argument("--open-vars", type=int, default=20, help="log2 rows of the opened table (--workload open)")
     # engine defaults: throughput (mixed) 8 workers in arrival order; serving
@@ -520,6 +585,10 @@ def main():
 
     if args.workload == "gkr":
         print(json.dumps(run_gkr(ctx, args)), flush=True)
+        return
+
+    if args.workload == "srs":
+        print(json.dumps(run_srs(ctx, args)), flush=True)
         return
 
     if args.workload == "shard":
