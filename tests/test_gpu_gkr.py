@@ -182,3 +182,43 @@ def test_refusals(apg, g, orc):
                                      a["coeffs"][:2], a["kinds"], False)
     with pytest.raises(apg.ApgError):
         g.forward_layer(bad, cols, 3)
+
+
+def test_full_size_layer_properties(apg, g, orc):
+    """2^20 rows (the bench layer's size) with the every-shape list: the
+    forward pass equal to the oracle's, and the sumcheck's size-independent
+    identities: g_j(0) + g_j(1) = g_(j-1)(rho_(j-1)) for every round, the
+    transcript replayed from the rounds gives the point, and the last round
+    at its challenge equals eq(r, rho) * S(rho), S evaluated by the oracle on
+    the bound tables and the virtual tables' closed forms at rho."""
+    art, ncols = gc.rowwise_artifact()
+    ch = gc.challenges()
+    res = g.ResolvedList(art, 0, ch)
+    flat = _flat(orc, res)
+    n = 20
+    cols = [orc.fill_ctr(900 + t, 1 << n, "witness") for t in range(ncols)]
+    fwd = g.forward_layer(res, cols, n)
+    want = orc.gkr_forward_layer(flat, cols, n)
+    for a, b in zip(fwd, want):
+        assert np.array_equal(a, b)
+    rng = np.random.default_rng(20)
+    r = Fs([int(x) for x in rng.integers(0, 2**62, n)])
+    w = Fs([int(x) for x in rng.integers(1, 2**62, flat.nweighted)])
+    t = apg.Transcript()
+    rounds, point, finals = g.prove_sumcheck(r, res, w, cols, None, t)
+    print("gkr full size n=20", flush=True)
+    R = gc.R
+    rs, rho = [gc.Is(x) for x in rounds], gc.Is(point)
+    replay = orc.Transcript()
+    claim = None
+    for j, c in enumerate(rs):
+        if claim is not None:
+            assert (2 * c[0] + c[1] + c[2] + c[3]) % R == claim
+        replay.append_scalars(orc.TAG_SUMCHECK_ROUND, rounds[j])
+        assert gc.I(replay.challenge(orc.TAG_SUMCHECK_CHALLENGE)) == rho[j]
+        x = rho[j]
+        claim = (c[0] + x * (c[1] + x * (c[2] + x * c[3]))) % R
+    virt = np.stack([orc.gkr_virtual_at_point(int(k), point) for k in flat.kinds])
+    s = gc.I(orc.gkr_summand(flat, w, finals, None, virt))
+    e = gc.eq_int(gc.Is(r), rho)
+    assert claim == e * s % R
