@@ -554,7 +554,7 @@ void msm_set_window_bits(int log2n, int c) {
 // Built-in widths for floor(log2 n) = 8..22, the fastest of c in default +- 3
 // for 8 witness-like columns on MI355X (apogee-vm_amd/autotune.py); outside
 // that range, lg n - 2 clamped to [4, 17].
-static const int kTunedC[23] = {0, 0, 0, 0, 0, 0, 0, 0, 7, 8, 8, 9, 9, 11, 12, 14, 14, 14, 17, 17, 17, 17, 17};
+static const int kTunedC[23] = {0, 0, 0, 0, 0, 0, 0, 0, 9, 9, 8, 8, 9, 12, 12, 14, 14, 17, 17, 17, 17, 17, 17};
 
 int msm_pick_c(size_t n, size_t columns) {
     (void)columns;
