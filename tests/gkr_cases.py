@@ -272,3 +272,37 @@ def brute_rounds(art, k, tables_lo, tables_up, r, rho, weights, challenges_):
         c1 = (d1 - d2 * inv2 + 2 * c3) % R
         out.append([vals[0], c1, c2, c3])
     return out
+
+
+def wide_artifact(ncols=24, nterms=40):
+    """A row-wise list with wide gates: a Linear of nterms terms, an
+    AffineProduct of 10 + 10 terms, a Quadratic of 8 linear terms and 12
+    products, three cached entries (each read by several gates), every
+    column read, and two virtual tables."""
+    g = gkr_mod()
+    ch = g.Challenge(CH_SLOT)
+    W = [g.Witness(i) for i in range(ncols)]
+    V0, V1 = g.Virtual(g.VirtualKind.Range16), g.Virtual(g.VirtualKind.Xor8Out)
+    cached = [
+        ("c0", g.Cached(0, 0), g.Linear([(lit(3 + q), W[q]) for q in range(6)], lit(11))),
+        ("c1", g.Cached(0, 1), g.Linear([(ch, W[7]), (lit(2), V0)], lit(0))),
+        ("c2", g.Cached(0, 2), g.Quadratic(lit(5), [(lit(1), W[8])], [(lit(2), W[9], W[10])])),
+    ]
+    C0, C1, C2 = g.Cached(0, 0), g.Cached(0, 1), g.Cached(0, 2)
+    producing = [
+        ("lin", g.Inner(1, 0), g.Linear([(lit(q + 1) if q % 3 else ch, W[q % ncols]) for q in range(nterms)]
+                                        + [(lit(7), C0), (lit(9), C2)], lit(13))),
+        ("aff", g.Inner(1, 1), g.AffineProduct([(lit(q + 2), W[(q + 3) % ncols]) for q in range(9)] + [(ch, C1)],
+                                               lit(4), [(lit(q + 5), W[(q + 11) % ncols]) for q in range(10)],
+                                               lit(6))),
+        ("quad", g.Inner(1, 2), g.Quadratic(lit(2), [(lit(q + 1), W[(q + 12) % ncols]) for q in range(7)] + [(ch, C2)],
+                                            [(lit(q + 3), W[(2 * q) % ncols], W[(2 * q + 5) % ncols])
+                                             for q in range(11)] + [(ch, C0, C1)])),
+        ("mask", g.Inner(1, 3), g.MaskIntoIdentity(C0, W[ncols - 1])),
+        ("prod", g.Inner(1, 4), g.Product(ch, V1, C1)),
+    ]
+    enf = [("e0", g.Linear([(lit(1), C0), (lit(2), C1), (lit(3), C2), (ch, V1)], lit(1)))]
+    layer = g.LayerSpec(False, 0, len(producing), cached=cached, producing=producing, enforcing=enf)
+    art = g.CircuitArtifact([], [f"w{i}" for i in range(ncols)], [], [(g.VirtualKind.Range16, "r16"),
+                                                                     (g.VirtualKind.Xor8Out, "x8")], [layer])
+    return art, ncols

@@ -222,3 +222,17 @@ def test_full_size_layer_properties(apg, g, orc):
     s = gc.I(orc.gkr_summand(flat, w, finals, None, virt))
     e = gc.eq_int(gc.Is(r), rho)
     assert claim == e * s % R
+
+
+@pytest.mark.parametrize("n", [3, 10, 14])
+def test_wide_gates_and_cached_entries(apg, g, orc, n):
+    """Gates of 40 operands, three cached entries, two virtual tables: the
+    sumcheck (its small rounds split over (pair, gate) items) and the forward
+    pass against the oracle."""
+    art, ncols = gc.wide_artifact()
+    cols = _tables(orc, ncols, n, 3000 + n)
+    _run_both(apg, g, orc, art, 0, cols, None, n, 40 + n)
+    res = g.ResolvedList(art, 0, gc.challenges())
+    want = orc.gkr_forward_layer(_flat(orc, res), cols, n)
+    for a, b in zip(g.forward_layer(res, cols, n), want):
+        assert np.array_equal(a, b)

@@ -223,3 +223,19 @@ def test_layer_sumcheck_halving_by_definition(orc, g, n):
     lower = [c[:half] for c in full]
     upper = [c[half:] for c in full]
     _check_sumcheck(orc, g, art, 1, lower, upper, n, 400 + n)
+
+
+def test_wide_gates_gate_values_and_sumcheck(orc, g):
+    """Wide gates, three cached entries read by several gates: gate values
+    against the interpreter, and the sumcheck against the definition at n = 2."""
+    art, ncols = gc.wide_artifact()
+    ch = gc.challenges()
+    flat = _flat(orc, g.ResolvedList(art, 0, ch))
+    rng = np.random.default_rng(21)
+    n = 2
+    cols = _int_cols(rng, ncols, 1 << n)
+    for y in range(1 << n):
+        lower = Fs([c[y] for c in cols])
+        virtuals = Fs([gc.virtual_row(int(kd), y) for kd, _ in art.virtuals])
+        assert Is(orc.gkr_gate_values(flat, lower, None, virtuals)) == gc.row_values(art, 0, cols, y, n, ch)
+    _check_sumcheck(orc, g, art, 0, cols, None, n, 77)
