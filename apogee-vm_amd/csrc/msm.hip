@@ -712,10 +712,11 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     HIP_CHECK(hipGetLastError());
 
     // 3. accumulate over equal slices of the sorted entries: about ACC_ROUNDS
-    // slices per resident thread (4 waves per SIMD), at least MIN_SLICE entries
-    // and an eighth of the mean bucket, so a typical bucket crosses about eight
-    // slice boundaries and stays light (16 partials of at most LIGHT_ITEMS)
-    const size_t resident = (size_t)ctx.cu_count * 4 * 4 * 64;
+    // slices per resident thread (k_accumulate holds 3 waves per SIMD), at
+    // least MIN_SLICE entries and an eighth of the mean bucket, so a typical
+    // bucket crosses about eight slice boundaries and stays light (16 partials
+    // of at most LIGHT_ITEMS)
+    const size_t resident = (size_t)ctx.cu_count * 4 * 3 * 64;
     const size_t by_grid = (total + ACC_ROUNDS * resident - 1) / (ACC_ROUNDS * resident);
     const size_t by_bucket = total / (8 * (size_t)std::max<uint32_t>(nkeys, 1));
     const uint32_t L = (uint32_t)(std::min<size_t>(MAX_SLICE, std::max({MIN_SLICE, by_grid, by_bucket})) + 3) & ~3u;
