@@ -266,6 +266,40 @@ APG_HD Fe f29_to32(const F29 &x) {
     return r;
 }
 
+// x mod p for x < 32p, packed into eight 32-bit words: subtracts 16p, 8p,
+// 4p, 2p and p where they fit (the words of x are below 2^29)
+template <class P, uint32_t K>
+APG_HD void f29_sub_if_ge(uint32_t w[9]) {
+    uint32_t k[9], d[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) k[j] = modk29<P, K>(j);
+    const uint32_t br = sub29_words(w, k, d);
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[j] = br ? w[j] : d[j];
+}
+
+template <class P>
+APG_HD Fe f29_pack_reduced(const F29 &x) {
+    uint32_t w[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[j] = x.v[j];
+    f29_sub_if_ge<P, 16>(w);
+    f29_sub_if_ge<P, 8>(w);
+    f29_sub_if_ge<P, 4>(w);
+    f29_sub_if_ge<P, 2>(w);
+    f29_sub_if_ge<P, 1>(w);
+    Fe r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int bit = 32 * i, j = bit / 29, sh = bit % 29;
+        uint64_t v = (uint64_t)w[j] >> sh;
+        v |= (uint64_t)w[j + 1] << (29 - sh);
+        if (j + 2 < 9) v |= (uint64_t)w[j + 2] << (58 - sh);
+        r.l[i] = (uint32_t)v;
+    }
+    return r;
+}
+
 // enters the domain from a fully reduced 32-bit Montgomery value
 template <class P>
 APG_HD F29 f29_from_mont32(const Fe &a) {
