@@ -300,6 +300,32 @@ APG_HD Fe f29_pack_reduced(const F29 &x) {
     return r;
 }
 
+// x mod p for x < 2p, packed into eight 32-bit words
+template <class P>
+APG_HD Fe f29_pack_small(const F29 &x) {
+    uint32_t w[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[j] = x.v[j];
+    f29_sub_if_ge<P, 1>(w);
+    Fe r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int bit = 32 * i, j = bit / 29, sh = bit % 29;
+        uint64_t v = (uint64_t)w[j] >> sh;
+        v |= (uint64_t)w[j + 1] << (29 - sh);
+        if (j + 2 < 9) v |= (uint64_t)w[j + 2] << (58 - sh);
+        r.l[i] = (uint32_t)v;
+    }
+    return r;
+}
+
+// a b / 2^261 mod p of two fully reduced 32-bit words values, fully reduced:
+// for 32-bit Montgomery images a = x R, b = y R it is the image of x y / 2^5
+template <class P>
+APG_HD Fe f_mul_r261(const Fe &a, const Fe &b) {
+    return f29_pack_small<P>(f29_mul<P>(f29_from32(a), f29_from32(b)));
+}
+
 // enters the domain from a fully reduced 32-bit Montgomery value
 template <class P>
 APG_HD F29 f29_from_mont32(const Fe &a) {
