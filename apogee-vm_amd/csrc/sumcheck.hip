@@ -100,7 +100,10 @@ __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, 
             cur[t] = lo;
             step[t] = f_sub<Fr>(hi, lo);
         }
-        const Fe e = eq_factor(eq, i);
+        // E(i): with two factor tables, their 29-bit product (one more 2^-5)
+        const Fe e = eq.lo ? f_mul_r261<Fr>(ldfe(eq.lo, i & ((size_t(1) << eq.lo_bits) - 1)),
+                                            ldfe(eq.hi, i >> eq.lo_bits))
+                           : ldfe(eq.hi, i);
 #pragma unroll
         for (int x = 0; x < NN; x++) {
             Fe g = cur[0];
@@ -478,8 +481,12 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
         const int nn = full ? D + 1 : D;
         HIP_CHECK(hipMemcpyAsync(hvals, dout, nn * sizeof(Fe), hipMemcpyDeviceToHost, st));
         ctx.sync();
-        if (product)  // the product kernel's node sums carry 2^(-5 k)
-            for (int x = 0; x < nn; x++) hvals[x] = f_mul<Fr>(hvals[x], prod_scale);
+        if (product) {  // the product kernel's node sums carry 2^(-5 k), and 2^-5 more for E = lo hi
+            Fe scale = prod_scale;
+            if (eq.lo)
+                for (int q = 0; q < 5; q++) scale = f_add<Fr>(scale, scale);
+            for (int x = 0; x < nn; x++) hvals[x] = f_mul<Fr>(hvals[x], scale);
+        }
         if (ctx.profiling) {
             float ms = 0;
             HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
