@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "field29.cuh"
 #include "gkr.h"
 #include "sumcheck_dev.cuh"
 
@@ -38,12 +39,22 @@ namespace apg {
 
 namespace {
 
+// The multiplies of a gate are the lazy 29-bit ones of field29.cuh on 32-bit
+// images: a value times a multiplier held in the R' = 2^261 domain is the
+// product's exact image, and a value times a value is the image of their
+// product over 2^5. So the host stores each coefficient twice (coefm as a
+// multiplier, coefc as a constant), times 2^5 where its product is later
+// multiplied by another value (a Product's coefficient, an AffineProduct's
+// left side, a Quadratic's product coefficients), and an operand that is
+// multiplied later without a coefficient is doubled five times: every gate
+// value comes out exact.
 struct DProg {
     const GkrGate *gates;
     const uint32_t *src;
-    const Fe *coef;
+    const F29 *coefm;
+    const Fe *coefc;
     const uint32_t *kinds;
-    const Fe *w;  // weights of the producing then enforcing gates
+    const F29 *w;  // weights of the producing then enforcing gates, R' domain
     uint32_t ncached, ngates, nvirt;
 };
 
@@ -157,16 +168,17 @@ enum : int { BY_ONE = 0, BY_K = 1, BY_A = 2, BY_B = 3 };
 __device__ Fe eval_gate(const DProg &p, uint32_t g, const Reader &rd) {
     const GkrGate G = p.gates[g];
     const uint32_t nv = G.src_end - G.src_begin, t = G.split;
-    const Fe *co = p.coef + G.coef_begin;
+    const F29 *cm = p.coefm + G.coef_begin;
+    const Fe *cc = p.coefc + G.coef_begin;
     Fe acc = f_zero<Fr>(), A = acc, B = acc;
     switch (G.shape) {
-        case GS_LINEAR: acc = co[nv]; break;       // c_1..c_t, c_0
+        case GS_LINEAR: acc = cc[nv]; break;       // c_1..c_t, c_0
         case GS_MASK: acc = f_one<Fr>(); break;    // x m + (1 - m)
         case GS_AFFINE_PRODUCT:                    // a_1..a_t, a_0, b_1..b_u, b_0
-            A = co[t];
-            B = co[nv + 1];
+            A = cc[t];
+            B = cc[nv + 1];
             break;
-        case GS_QUADRATIC: acc = co[0]; break;     // c_0, a_1..a_t, b_1..b_u
+        case GS_QUADRATIC: acc = cc[0]; break;     // c_0, a_1..a_t, b_1..b_u
         default: break;
     }
     for (uint32_t q = 0; q < nv; q++) {
@@ -209,8 +221,16 @@ __device__ Fe eval_gate(const DProg &p, uint32_t g, const Reader &rd) {
                 break;
             default: v = rd.slot(p.nvirt + idx); break;  // cached entry
         }
-        Fe x = v;
-        if (by != BY_ONE) x = f_mul<Fr>(by == BY_K ? co[kq] : by == BY_A ? A : B, v);
+        Fe x;
+        if (by == BY_ONE) {  // multiplied later without a coefficient: times 2^5
+            x = v;
+#pragma unroll
+            for (int d = 0; d < 5; d++) x = f_add<Fr>(x, x);
+        } else if (by == BY_K) {
+            x = f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(v), cm[kq]));
+        } else {
+            x = f_mul_r261<Fr>(by == BY_A ? A : B, v);
+        }
         if (minus_v) x = f_sub<Fr>(x, v);
         switch (to) {
             case TO_ACC: acc = f_add<Fr>(acc, x); break;
@@ -220,7 +240,7 @@ __device__ Fe eval_gate(const DProg &p, uint32_t g, const Reader &rd) {
             default: B = f_add<Fr>(B, x); break;
         }
     }
-    if (G.shape == GS_AFFINE_PRODUCT) acc = f_mul<Fr>(A, B);
+    if (G.shape == GS_AFFINE_PRODUCT) acc = f_mul_r261<Fr>(A, B);
     return acc;
 }
 
@@ -233,7 +253,7 @@ __device__ __forceinline__ void fill_cached(const DProg &p, const Reader &rd) {
 // X = 0, 1, 2, and sum_i E(i) S_i(X) into out[0..3).
 template <bool FOLD>
 __global__ void __launch_bounds__(RTPB) k_gkr_round(DProg p, const Fe *const *src, Fe *const *dst, uint32_t ntab,
-                                                  uint32_t nlower, size_t m, Fe r, EqArgs eq, const Fe *rho, int n,
+                                                  uint32_t nlower, size_t m, F29 r, EqArgs eq, const Fe *rho, int n,
                                                   int b, Fe *slots, Fe *partials, unsigned *counter, Fe *out) {
     Fe acc[3];
 #pragma unroll
@@ -245,8 +265,8 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round(DProg p, const Fe *const *sr
                 const Fe *s = src[t];
                 const Fe a0 = ldfe(s, 4 * i), a1 = ldfe(s, 4 * i + 1), a2 = ldfe(s, 4 * i + 2),
                          a3 = ldfe(s, 4 * i + 3);
-                stfe(dst[t], 2 * i, f_add<Fr>(a0, f_mul<Fr>(r, f_sub<Fr>(a1, a0))));
-                stfe(dst[t], 2 * i + 1, f_add<Fr>(a2, f_mul<Fr>(r, f_sub<Fr>(a3, a2))));
+                stfe(dst[t], 2 * i, f_add<Fr>(a0, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(f_sub<Fr>(a1, a0)), r))));
+                stfe(dst[t], 2 * i + 1, f_add<Fr>(a2, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(f_sub<Fr>(a3, a2)), r))));
             }
         }
         const Fe e = eq_factor(eq, i);
@@ -258,7 +278,7 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round(DProg p, const Fe *const *sr
             fill_cached(p, rd);
             Fe s = f_zero<Fr>();
             for (uint32_t g = 0; g < p.ngates; g++)
-                s = f_add<Fr>(s, f_mul<Fr>(eval_gate(p, p.ncached + g, rd), p.w[g]));
+                s = f_add<Fr>(s, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(eval_gate(p, p.ncached + g, rd)), p.w[g])));
             const Fe es = f_mul<Fr>(e, s);
             if (x == 0) acc[0] = f_add<Fr>(acc[0], es);
             else if (x == 1) acc[1] = f_add<Fr>(acc[1], es);
@@ -271,15 +291,15 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round(DProg p, const Fe *const *sr
 // The fold of a round's tables alone: work item w folds pair w % m of table
 // w / m (rounds too small to fill the device with pairs).
 __global__ void __launch_bounds__(RTPB) k_gkr_fold(const Fe *const *src, Fe *const *dst, uint32_t ntab, size_t m,
-                                                 Fe r) {
+                                                 F29 r) {
     const size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= m * ntab) return;
     const size_t i = w % m;
     const Fe *s = src[w / m];
     Fe *d = dst[w / m];
     const Fe a0 = ldfe(s, 4 * i), a1 = ldfe(s, 4 * i + 1), a2 = ldfe(s, 4 * i + 2), a3 = ldfe(s, 4 * i + 3);
-    stfe(d, 2 * i, f_add<Fr>(a0, f_mul<Fr>(r, f_sub<Fr>(a1, a0))));
-    stfe(d, 2 * i + 1, f_add<Fr>(a2, f_mul<Fr>(r, f_sub<Fr>(a3, a2))));
+    stfe(d, 2 * i, f_add<Fr>(a0, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(f_sub<Fr>(a1, a0)), r))));
+    stfe(d, 2 * i + 1, f_add<Fr>(a2, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(f_sub<Fr>(a3, a2)), r))));
 }
 
 // A round split over (pair, gate) work items: item w evaluates weighted gate
@@ -297,7 +317,7 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round_split(DProg p, const Fe *con
     for (size_t w = tid; w < items; w += stride) {
         const size_t i = w % m;
         const uint32_t g = (uint32_t)(w / m);
-        const Fe ew = f_mul<Fr>(eq_factor(eq, i), p.w[g]);
+        const Fe ew = f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(eq_factor(eq, i)), p.w[g]));
 #pragma unroll 1
         for (int x = 0; x < 3; x++) {
             // one gate reads few virtual tables: evaluate them where read
@@ -330,28 +350,73 @@ struct Staged {
     DProg prog;
 };
 
+// 2^5 where a coefficient's product is multiplied by another value later
+// (the DProg comment): a Product's coefficient, an AffineProduct's left side
+// and constant, a Quadratic's product coefficients.
+bool coef_scaled(const GkrGate &g, uint32_t j) {
+    const uint32_t nv = g.src_end - g.src_begin, t = g.split;
+    switch (g.shape) {
+        case GS_PRODUCT: return true;
+        case GS_AFFINE_PRODUCT: return j <= t;        // a_1..a_t, a_0
+        case GS_QUADRATIC: return j >= 1 + t;         // b_1..b_u after c_0, a_1..a_t
+        default: (void)nv; return false;
+    }
+}
+
 Staged stage_program(Context &ctx, const GkrList &l, const Fe *weights, uint32_t nweights) {
+    // coefficients as multipliers (R' domain) and as constants, role-scaled
+    std::vector<F29> cm(l.coeffs.size());
+    std::vector<Fe> cc(l.coeffs.size());
+    std::vector<bool> done(l.coeffs.size(), false);
+    for (const GkrGate &g : l.gates) {
+        const uint32_t nv = g.src_end - g.src_begin;
+        uint32_t ncoef = 0;
+        switch (g.shape) {
+            case GS_LINEAR: ncoef = nv + 1; break;
+            case GS_PRODUCT: ncoef = 1; break;
+            case GS_AFFINE_PRODUCT: ncoef = nv + 2; break;
+            case GS_QUADRATIC: ncoef = 1 + g.split + (nv - g.split) / 2; break;
+            default: break;
+        }
+        for (uint32_t j = 0; j < ncoef; j++) {
+            const size_t at = (size_t)g.coef_begin + j;
+            Fe c = l.coeffs[at];
+            if (coef_scaled(g, j))
+                for (int d = 0; d < 5; d++) c = f_add<Fr>(c, c);
+            // a slot shared by two gates must have one meaning for both
+            if (done[at] && !f_eq(cc[at], c)) throw ApgError(-2, "gkr: a coefficient slot shared across roles");
+            cc[at] = c;
+            cm[at] = f29_from_mont32<Fr>(c);
+            done[at] = true;
+        }
+    }
+    std::vector<F29> w29(nweights);
+    for (uint32_t j = 0; j < nweights; j++) w29[j] = f29_from_mont32<Fr>(weights[j]);
+
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_gates = 0;
     const size_t o_src = al(o_gates + l.gates.size() * sizeof(GkrGate));
     const size_t o_kinds = al(o_src + l.sources.size() * 4);
-    const size_t o_coef = al(o_kinds + l.virtual_kinds.size() * 4);
-    const size_t o_w = al(o_coef + l.coeffs.size() * sizeof(Fe));
-    const size_t total = al(o_w + (size_t)nweights * sizeof(Fe)) + 256;
+    const size_t o_cm = al(o_kinds + l.virtual_kinds.size() * 4);
+    const size_t o_cc = al(o_cm + cm.size() * sizeof(F29));
+    const size_t o_w = al(o_cc + cc.size() * sizeof(Fe));
+    const size_t total = al(o_w + (size_t)nweights * sizeof(F29)) + 256;
     uint8_t *h = ctx.h<uint8_t>("gkr.prog", total);
     uint8_t *d = ctx.d<uint8_t>("gkr.prog", total);
     if (!l.gates.empty()) std::memcpy(h + o_gates, l.gates.data(), l.gates.size() * sizeof(GkrGate));
     if (!l.sources.empty()) std::memcpy(h + o_src, l.sources.data(), l.sources.size() * 4);
     if (!l.virtual_kinds.empty()) std::memcpy(h + o_kinds, l.virtual_kinds.data(), l.virtual_kinds.size() * 4);
-    if (!l.coeffs.empty()) std::memcpy(h + o_coef, l.coeffs.data(), l.coeffs.size() * sizeof(Fe));
-    if (nweights) std::memcpy(h + o_w, weights, (size_t)nweights * sizeof(Fe));
+    if (!cm.empty()) std::memcpy(h + o_cm, cm.data(), cm.size() * sizeof(F29));
+    if (!cc.empty()) std::memcpy(h + o_cc, cc.data(), cc.size() * sizeof(Fe));
+    if (nweights) std::memcpy(h + o_w, w29.data(), (size_t)nweights * sizeof(F29));
     HIP_CHECK(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, ctx.stream));
     Staged s;
     s.prog.gates = reinterpret_cast<const GkrGate *>(d + o_gates);
     s.prog.src = reinterpret_cast<const uint32_t *>(d + o_src);
     s.prog.kinds = reinterpret_cast<const uint32_t *>(d + o_kinds);
-    s.prog.coef = reinterpret_cast<const Fe *>(d + o_coef);
-    s.prog.w = reinterpret_cast<const Fe *>(d + o_w);
+    s.prog.coefm = reinterpret_cast<const F29 *>(d + o_cm);
+    s.prog.coefc = reinterpret_cast<const Fe *>(d + o_cc);
+    s.prog.w = reinterpret_cast<const F29 *>(d + o_w);
     s.prog.ncached = l.ncached;
     s.prog.ngates = l.nproducing + l.nenforcing;
     s.prog.nvirt = l.halving ? 0 : (uint32_t)l.virtual_kinds.size();  // a halving list reads none
@@ -575,22 +640,23 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
         const size_t cap = std::min<size_t>(MAX_RBLOCKS, (size_t)4 * ctx.cu_count);
         const dim3 g((unsigned)std::min(cap, std::max<size_t>(1, (m + RTPB - 1) / RTPB))), b(RTPB);
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev0, st));
+        const F29 rho29 = f29_from_mont32<Fr>(rho);  // the fold's challenge in the R' domain
         if (m < cap * RTPB / 8 && s.prog.ngates > 1) {
             // fewer pairs than an eighth of the resident threads: fold, then
             // spread the (pair, gate) items over the device
             if (fold)
                 hipLaunchKernelGGL(k_gkr_fold, dim3(grid_for(m * nt, RTPB)), b, 0, st, src, dst, (uint32_t)nt, m,
-                                   rho);
+                                   rho29);
             const size_t items = m * s.prog.ngates;
             const dim3 gs((unsigned)std::min(cap, std::max<size_t>(1, (items + RTPB - 1) / RTPB)));
             hipLaunchKernelGGL(k_gkr_round_split, gs, b, 0, st, s.prog, fold ? dst : src, (uint32_t)ncols, m, eq,
                                rho_d, n, j, slots, partials, counter, dout);
         } else if (fold) {
             hipLaunchKernelGGL(k_gkr_round<true>, g, b, 0, st, s.prog, src, dst, (uint32_t)nt, (uint32_t)ncols, m,
-                               rho, eq, rho_d, n, j, slots, partials, counter, dout);
+                               rho29, eq, rho_d, n, j, slots, partials, counter, dout);
         } else {
             hipLaunchKernelGGL(k_gkr_round<false>, g, b, 0, st, s.prog, src, dst, (uint32_t)nt, (uint32_t)ncols, m,
-                               rho, eq, rho_d, n, j, slots, partials, counter, dout);
+                               rho29, eq, rho_d, n, j, slots, partials, counter, dout);
         }
         HIP_CHECK(hipGetLastError());
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));

@@ -42,9 +42,14 @@ struct TablePtrs {
     Fe *dst[SC_MAX_TABLES];
 };
 
+// A gate's terms for the kernel: factor indices, and the coefficient in the
+// R' = 2^261 domain times 2^(5 (factors - 1)), so that the chain of lazy
+// 29-bit multiplies (each further factor contributes 2^-5) gives the term's
+// exact 32-bit image.
 struct GateArgs {
     int nterms;
-    Term terms[SC_MAX_TERMS];
+    int32_t f[SC_MAX_TERMS][3];
+    F29 coef[SC_MAX_TERMS];
 };
 
 // lo/hi of pair i of table t, folding the previous round's table if asked.
@@ -147,10 +152,9 @@ __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateAr
             if (x < nn) {
                 Fe g = f_zero<Fr>();
                 for (int q = 0; q < gate.nterms; q++) {
-                    const Term &tm = gate.terms[q];
-                    Fe v = f_mul<Fr>(tm.coef, cur[tm.f[0]]);
-                    if (tm.f[1] >= 0) v = f_mul<Fr>(v, cur[tm.f[1]]);
-                    if (tm.f[2] >= 0) v = f_mul<Fr>(v, cur[tm.f[2]]);
+                    Fe v = f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(cur[gate.f[q][0]]), gate.coef[q]));
+                    if (gate.f[q][1] >= 0) v = f_mul_r261<Fr>(v, cur[gate.f[q][1]]);
+                    if (gate.f[q][2] >= 0) v = f_mul_r261<Fr>(v, cur[gate.f[q][2]]);
                     g = f_add<Fr>(g, v);
                 }
                 acc[x] = f_add<Fr>(acc[x], f_mul<Fr>(e, g));
@@ -344,7 +348,15 @@ static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bo
     }
     GateArgs ga;
     ga.nterms = gate.nterms;
-    for (int q = 0; q < gate.nterms; q++) ga.terms[q] = gate.terms[q];
+    for (int q = 0; q < gate.nterms; q++) {
+        const Term &tm = gate.terms[q];
+        Fe c = tm.coef;
+        for (int f = 1; f < 3; f++)
+            if (tm.f[f] >= 0)
+                for (int d = 0; d < 5; d++) c = f_add<Fr>(c, c);  // times 2^5 per further factor
+        ga.coef[q] = f29_from_mont32<Fr>(c);
+        for (int f = 0; f < 3; f++) ga.f[q][f] = tm.f[f];
+    }
     const int nn = full ? D + 1 : D;
     const dim3 g(round_blocks(ctx, m)), b(RTPB);
 #define APG_GATE(NN)                                                                                  \
