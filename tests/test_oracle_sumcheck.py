@@ -138,6 +138,33 @@ def test_witness_digest_is_a_sponge_over_cells(orc):
     assert not np.array_equal(d1, orc.witness_digest(tables))
 
 
+@pytest.mark.parametrize("k,n", [(1, 0), (2, 3), (5, 4)])
+def test_witness_digest_composes_pinned_transcript_calls(orc, k, n):
+    """witness_digest (crates/sumcheck/src/lib.rs:228-256) is a fresh sponge
+    that absorbs [k, n] as one WITNESS_DIGEST message, then one message per
+    column, then squeezes raw. The transcript's absorb/squeeze and the tag
+    value are pinned by the transcript known answers, so this pins the digest
+    up to the order and framing it states; a dropped header, a swapped count
+    or one message for all cells fails here."""
+    tables = _rand_tables(orc, 40 + k, k, n)
+    t = orc.Transcript()
+    t.append_scalars(orc.TAG_WITNESS_DIGEST,
+                     np.stack([orc.from_u64(orc.FR, k), orc.from_u64(orc.FR, n)]))
+    for c in tables:
+        t.append_scalars(orc.TAG_WITNESS_DIGEST, c)
+    assert np.array_equal(orc.witness_digest(tables), t.sample())
+
+
+def test_witness_digest_header_pins_the_shape(orc):
+    """The same cells read as one column of 2^(n+1) or two of 2^n digest
+    differently: the header carries both counts (crates/sumcheck/src/lib.rs:233-236)."""
+    two = _rand_tables(orc, 9, 2, 3)
+    one = [np.concatenate(two)]
+    assert not np.array_equal(orc.witness_digest(two), orc.witness_digest(one))
+    # and swapping the columns moves the digest
+    assert not np.array_equal(orc.witness_digest(two), orc.witness_digest(two[::-1]))
+
+
 def test_mercury_commitment_kat(orc):
     """pcs::commit over the toy SRS (tau = 0xabcdef) equals the committed
     commitment line of mercury_proof.txt (crates/pcs/src/lib.rs:237-247)."""
