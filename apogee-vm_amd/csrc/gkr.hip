@@ -591,14 +591,17 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
     const size_t nslots = (size_t)s.prog.nvirt + l.ncached;
     Fe *slots = ctx.d<Fe>("gkr.slots", std::max<size_t>(nslots * MAX_RBLOCKS * RTPB, 1));
     Fe *partials = ctx.d<Fe>("gkr.partials", (size_t)MAX_RBLOCKS * 3);
-    Fe *dout = ctx.d<Fe>("gkr.out", 3);
     if (!ctx.dev["gkr.counter"].ptr) {
         HIP_CHECK(hipMalloc(&ctx.dev["gkr.counter"].ptr, 256));
         ctx.dev["gkr.counter"].cap = 256;
         HIP_CHECK(hipMemsetAsync(ctx.dev["gkr.counter"].ptr, 0, 256, st));
     }
     unsigned *counter = static_cast<unsigned *>(ctx.dev["gkr.counter"].ptr);
+    // the round kernels' last block writes the node sums straight into
+    // pinned host memory; the stream sync after each round makes them visible
     Fe *hvals = ctx.h<Fe>("gkr.hvals", 3);
+    Fe *hvals_dev = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&hvals_dev), hvals, 0));
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (ctx.profiling) {
@@ -650,17 +653,16 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
             const size_t items = m * s.prog.ngates;
             const dim3 gs((unsigned)std::min(cap, std::max<size_t>(1, (items + RTPB - 1) / RTPB)));
             hipLaunchKernelGGL(k_gkr_round_split, gs, b, 0, st, s.prog, fold ? dst : src, (uint32_t)ncols, m, eq,
-                               rho_d, n, j, slots, partials, counter, dout);
+                               rho_d, n, j, slots, partials, counter, hvals_dev);
         } else if (fold) {
             hipLaunchKernelGGL(k_gkr_round<true>, g, b, 0, st, s.prog, src, dst, (uint32_t)nt, (uint32_t)ncols, m,
-                               rho29, eq, rho_d, n, j, slots, partials, counter, dout);
+                               rho29, eq, rho_d, n, j, slots, partials, counter, hvals_dev);
         } else {
             hipLaunchKernelGGL(k_gkr_round<false>, g, b, 0, st, s.prog, src, dst, (uint32_t)nt, (uint32_t)ncols, m,
-                               rho29, eq, rho_d, n, j, slots, partials, counter, dout);
+                               rho29, eq, rho_d, n, j, slots, partials, counter, hvals_dev);
         }
         HIP_CHECK(hipGetLastError());
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
-        HIP_CHECK(hipMemcpyAsync(hvals, dout, 3 * sizeof(Fe), hipMemcpyDeviceToHost, st));
         ctx.sync();
         if (ctx.profiling) {
             float ms = 0;

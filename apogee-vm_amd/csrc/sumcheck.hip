@@ -434,14 +434,17 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
     for (int t = 0; t < k; t++) cur[t] = tables[t];
 
     Fe *partials = ctx.d<Fe>("sc.partials", (size_t)MAX_RBLOCKS * SC_MAX_NODES);
-    Fe *dout = ctx.d<Fe>("sc.out", SC_MAX_NODES);
     if (!ctx.dev["sc.counter"].ptr) {
         HIP_CHECK(hipMalloc(&ctx.dev["sc.counter"].ptr, 256));
         ctx.dev["sc.counter"].cap = 256;
         HIP_CHECK(hipMemsetAsync(ctx.dev["sc.counter"].ptr, 0, 256, st));
     }
     unsigned *counter = static_cast<unsigned *>(ctx.dev["sc.counter"].ptr);
+    // the round kernels' last block writes the node sums straight into
+    // pinned host memory; the stream sync after each round makes them visible
     Fe *hvals = ctx.h<Fe>("sc.hvals", SC_MAX_NODES);
+    Fe *hvals_dev = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&hvals_dev), hvals, 0));
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (ctx.profiling) {
@@ -487,11 +490,10 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
             eq.hi = eqB + (size_t)(s - h0) * bstride;
         }
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev0, st));
-        launch_round(ctx, gate, k, product, full, D, tp, m, fold, rho, eq, partials, counter, dout);
+        launch_round(ctx, gate, k, product, full, D, tp, m, fold, rho, eq, partials, counter, hvals_dev);
         HIP_CHECK(hipGetLastError());
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
         const int nn = full ? D + 1 : D;
-        HIP_CHECK(hipMemcpyAsync(hvals, dout, nn * sizeof(Fe), hipMemcpyDeviceToHost, st));
         ctx.sync();
         if (product) {  // the product kernel's node sums carry 2^(-5 k), and 2^-5 more for E = lo hi
             Fe scale = prod_scale;
