@@ -151,11 +151,14 @@ __global__ void __launch_bounds__(TPB) k_emit(const Fe *__restrict__ fr,
                                             uint32_t B, uint32_t tab_stride, int stage,
                                             const uint32_t *__restrict__ offsets,
                                             uint32_t *__restrict__ keys,
-                                            uint32_t *__restrict__ vals) {
+                                            uint32_t *__restrict__ vals,
+                                            uint32_t *__restrict__ total_out) {
     extern __shared__ uint32_t staged[];  // [TPB * windows] keys, then as many values
     __shared__ uint32_t wsum[TPB / 64];
     __shared__ uint32_t base;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the scan's last element is the pair count (to pinned host memory)
+    if (blockIdx.x == 0 && tid == 0) *total_out = offsets[gridDim.x];
     const size_t idx = (size_t)blockIdx.x * blockDim.x + tid;
     const bool live = idx < (size_t)n * K;
     uint32_t l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -655,7 +658,7 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
         HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, counts, offs, (int)eblocks + 1, st));  \
         hipLaunchKernelGGL((k_emit<CT, WT>), dim3(eblocks), dim3(TPB), stage_lds, st, fr_dev, u32_dev,           \
                            (uint32_t)n, (uint32_t)K, sh.c, sh.windows, sh.P, sh.G, sh.B, (uint32_t)tab.stride,  \
-                           stage, offs, keys_in, vals_in);                                                      \
+                           stage, offs, keys_in, vals_in, h_total_dev);                                         \
     } while (0)
 #define APG_EMIT_C(CT)                                                  \
     case CT:                                                            \
@@ -664,7 +667,11 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
         else                                                            \
             APG_EMIT(CT, msm_window_count(32, CT));                     \
         break;
-    // counts[eblocks] = 0, so the scan's last element is the total
+    // counts[eblocks] = 0, so the scan's last element is the total; k_emit
+    // writes it into pinned host memory, visible after the stream sync
+    uint32_t *h_total = ctx.h<uint32_t>("msm.total", 1);
+    uint32_t *h_total_dev = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&h_total_dev), h_total, 0));
     HIP_CHECK(hipMemsetAsync(counts + eblocks, 0, sizeof(uint32_t), st));
     switch (sh.c) {
         APG_EMIT_C(7)
@@ -679,8 +686,6 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
 #undef APG_EMIT_C
 #undef APG_EMIT
     HIP_CHECK(hipGetLastError());
-    uint32_t *h_total = ctx.h<uint32_t>("msm.total", 1);
-    HIP_CHECK(hipMemcpyAsync(h_total, offs + eblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     ctx.sync();
     const uint32_t total = *h_total;
 
@@ -764,8 +769,10 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
     // 5. reduce: per problem (column, group): W = sum_b (b + 1) S_b
     const uint32_t nprob = (uint32_t)(K * sh.G);
     const uint32_t B = sh.B;
-    Xyzz *wout = ctx.d<Xyzz>("msm.wout", (size_t)nprob * 4);
+    // k_weighted writes the window sums straight into pinned host memory
     Xyzz *h_w = ctx.h<Xyzz>("msm.hw", (size_t)nprob * 4);
+    Xyzz *wout = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&wout), h_w, 0));
     if (B > (uint32_t)TPB) {
         const uint32_t R = B / TPB;
         Xyzz29 *Z = ctx.d<Xyzz29>("msm.Z", (size_t)nprob * R);
@@ -781,7 +788,6 @@ static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K
                            wout);
     }
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(h_w, wout, (size_t)nprob * 4 * sizeof(Xyzz), hipMemcpyDeviceToHost, st));
     ctx.sync();
 
     // 6. host tail
