@@ -315,17 +315,20 @@ void combine_columns_device(Context &ctx, const Fe *cols, size_t n, int K, const
     HIP_CHECK(hipGetLastError());
 }
 
-// Blocks for m pairs: one pair per thread up to 4 resident blocks per CU,
-// beyond that a grid-stride loop.
-static unsigned round_blocks(const Context &ctx, size_t m) {
-    const size_t cap = std::min<size_t>(MAX_RBLOCKS, (size_t)4 * ctx.cu_count);
+// Blocks for m pairs: one pair per thread up to per_cu blocks per CU, beyond
+// that a grid-stride loop. The product rounds take one block per CU: on the
+// 2^20 x 8 shard's rounds 4, 2 and 1 blocks per CU measured 1.47, 1.39 and
+// 1.33 ms (fewer partial sums for the last block to add). The heavier gate
+// rounds keep 4, as the GKR layer's rounds, which lose 16% at one.
+static unsigned round_blocks(const Context &ctx, size_t m, int per_cu) {
+    const size_t cap = std::min<size_t>(MAX_RBLOCKS, (size_t)per_cu * ctx.cu_count);
     return (unsigned)std::min(cap, std::max<size_t>(1, (m + RTPB - 1) / RTPB));
 }
 
 template <int KT>
 static void launch_product(Context &ctx, bool full, const TablePtrs &tp, size_t m, bool fold,
                            const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out) {
-    const dim3 g(round_blocks(ctx, m)), b(RTPB);
+    const dim3 g(round_blocks(ctx, m, 1)), b(RTPB);
     const F29 r29 = f29_from_mont32<Fr>(r);  // rho R' mod r
     if (full)
         hipLaunchKernelGGL((k_round_product<KT, true>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
@@ -358,7 +361,7 @@ static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bo
         for (int f = 0; f < 3; f++) ga.f[q][f] = tm.f[f];
     }
     const int nn = full ? D + 1 : D;
-    const dim3 g(round_blocks(ctx, m)), b(RTPB);
+    const dim3 g(round_blocks(ctx, m, 4)), b(RTPB);
 #define APG_GATE(NN)                                                                                  \
     hipLaunchKernelGGL((k_round_gate<NN>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r, eq,      \
                        (int)full, nn, partials, counter, out)
