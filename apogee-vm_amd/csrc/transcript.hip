@@ -62,7 +62,17 @@ inline W4 w4_reduce(uint64_t t0, uint64_t t1, uint64_t t2, uint64_t t3) {
     return lt ? W4{{t0, t1, t2, t3}} : W4{{d0, d1, d2, d3}};
 }
 
-inline W4 w4_mul(const W4 &a, const W4 &b) {
+// Lazy forms for the permutation: values stay below 2p. With p < 2^254 the
+// CIOS product of two values below 2p is below (4p^2 + 2^256 p) / 2^256 < 2p
+// without the final subtraction, and the running sum stays below 3p < 2^256;
+// a sum of two values below 2p is below 4p < 2^256 and one conditional
+// subtraction of 2p brings it back below 2p. The state is made canonical
+// (below p) when the permutation ends.
+constexpr uint64_t R2P0 = RP0 << 1, R2P1 = (RP1 << 1) | (RP0 >> 63), R2P2 = (RP2 << 1) | (RP1 >> 63),
+                   R2P3 = (RP3 << 1) | (RP2 >> 63);
+static_assert((RP3 >> 62) == 0, "p < 2^254");
+
+inline W4 w4_mul_lazy(const W4 &a, const W4 &b) {
     uint64_t t0 = 0, t1 = 0, t2 = 0, t3 = 0;
     for (int i = 0; i < 4; i++) {
         const uint64_t bi = b.v[i];
@@ -85,10 +95,10 @@ inline W4 w4_mul(const W4 &a, const W4 &b) {
         t2 = (uint64_t)x;
         t3 = t4 + (uint64_t)(x >> 64);
     }
-    return w4_reduce(t0, t1, t2, t3);
+    return W4{{t0, t1, t2, t3}};
 }
 
-inline W4 w4_add(const W4 &a, const W4 &b) {
+inline W4 w4_add_lazy(const W4 &a, const W4 &b) {
     u128 x = (u128)a.v[0] + b.v[0];
     const uint64_t s0 = (uint64_t)x;
     x = (u128)a.v[1] + b.v[1] + (uint64_t)(x >> 64);
@@ -96,7 +106,18 @@ inline W4 w4_add(const W4 &a, const W4 &b) {
     x = (u128)a.v[2] + b.v[2] + (uint64_t)(x >> 64);
     const uint64_t s2 = (uint64_t)x;
     x = (u128)a.v[3] + b.v[3] + (uint64_t)(x >> 64);
-    return w4_reduce(s0, s1, s2, (uint64_t)x);  // a + b < 2p < 2^256
+    const uint64_t s3 = (uint64_t)x;
+    // s - 2p if s >= 2p
+    u128 y = (u128)s0 - R2P0;
+    const uint64_t d0 = (uint64_t)y;
+    y = (u128)s1 - R2P1 - (uint64_t)((y >> 64) & 1);
+    const uint64_t d1 = (uint64_t)y;
+    y = (u128)s2 - R2P2 - (uint64_t)((y >> 64) & 1);
+    const uint64_t d2 = (uint64_t)y;
+    y = (u128)s3 - R2P3 - (uint64_t)((y >> 64) & 1);
+    const uint64_t d3 = (uint64_t)y;
+    const bool lt = (y >> 64) & 1;
+    return lt ? W4{{s0, s1, s2, s3}} : W4{{d0, d1, d2, d3}};
 }
 
 struct RoundConstants {
@@ -117,21 +138,21 @@ const RoundConstants &rc() {
 }
 
 inline W4 sbox(const W4 &x) {
-    const W4 x2 = w4_mul(x, x);
-    const W4 x4 = w4_mul(x2, x2);
-    return w4_mul(x4, x);
+    const W4 x2 = w4_mul_lazy(x, x);
+    const W4 x4 = w4_mul_lazy(x2, x2);
+    return w4_mul_lazy(x4, x);
 }
 
 inline void ext_matrix(W4 s[3]) {
-    const W4 sum = w4_add(w4_add(s[0], s[1]), s[2]);
-    for (int i = 0; i < 3; i++) s[i] = w4_add(s[i], sum);
+    const W4 sum = w4_add_lazy(w4_add_lazy(s[0], s[1]), s[2]);
+    for (int i = 0; i < 3; i++) s[i] = w4_add_lazy(s[i], sum);
 }
 
 inline void int_matrix(W4 s[3]) {
-    const W4 sum = w4_add(w4_add(s[0], s[1]), s[2]);
-    s[0] = w4_add(s[0], sum);
-    s[1] = w4_add(s[1], sum);
-    s[2] = w4_add(w4_add(s[2], s[2]), sum);
+    const W4 sum = w4_add_lazy(w4_add_lazy(s[0], s[1]), s[2]);
+    s[0] = w4_add_lazy(s[0], sum);
+    s[1] = w4_add_lazy(s[1], sum);
+    s[2] = w4_add_lazy(w4_add_lazy(s[2], s[2]), sum);
 }
 
 void to_bytes(const Fe &a, uint8_t out[32], bool fq) {
@@ -164,18 +185,19 @@ void poseidon2_permute_host(Fe st[3]) {
     W4 s[3] = {w4_of(st[0]), w4_of(st[1]), w4_of(st[2])};
     ext_matrix(s);
     for (int r = 0; r < 4; r++) {
-        for (int l = 0; l < 3; l++) s[l] = sbox(w4_add(s[l], k.ini[r][l]));
+        for (int l = 0; l < 3; l++) s[l] = sbox(w4_add_lazy(s[l], k.ini[r][l]));
         ext_matrix(s);
     }
     for (int i = 0; i < 56; i++) {
-        s[0] = sbox(w4_add(s[0], k.mid[i]));
+        s[0] = sbox(w4_add_lazy(s[0], k.mid[i]));
         int_matrix(s);
     }
     for (int r = 0; r < 4; r++) {
-        for (int l = 0; l < 3; l++) s[l] = sbox(w4_add(s[l], k.ter[r][l]));
+        for (int l = 0; l < 3; l++) s[l] = sbox(w4_add_lazy(s[l], k.ter[r][l]));
         ext_matrix(s);
     }
-    for (int l = 0; l < 3; l++) st[l] = fe_of(s[l]);
+    // below 2p: one subtraction of p makes each word canonical
+    for (int l = 0; l < 3; l++) st[l] = fe_of(w4_reduce(s[l].v[0], s[l].v[1], s[l].v[2], s[l].v[3]));
 }
 
 Transcript::Transcript() {
