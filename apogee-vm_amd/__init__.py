@@ -1060,10 +1060,17 @@ class Engine:
         return jid.value
 
     def done(self, job: int) -> bool:
-        return lib().apg_engine_poll(self.h, job) == 1
+        """Whether the job has finished; ApgError for an id never issued or
+        already waited for."""
+        rc = lib().apg_engine_poll(self.h, job)
+        _check(min(rc, 0))
+        return rc == 1
 
     def wait(self, job: int):
-        """-> (ShardProof, final Transcript, stats dict)"""
+        """-> (ShardProof, final Transcript, stats dict); ApgError for an id
+        never issued or already waited for."""
+        if job not in self._jobs:
+            raise ApgError(-2, "engine: unknown or already collected job id")
         num_vars, k, degree = self._jobs.pop(job)
         cms = np.zeros(64 * k, dtype=np.uint8)
         rounds = np.zeros((num_vars, degree + 1, 4), dtype=np.uint64)

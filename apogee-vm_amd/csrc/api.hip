@@ -29,8 +29,11 @@ struct apg_srs {
     int device = 0;
     Affine *base = nullptr;  // copy 0: [tau^i]G, i < n
     size_t n = 0;
-    std::mutex mu;
-    std::map<int, std::unique_ptr<BaseTable>> tables;  // window width -> shifted copies
+    mutable std::mutex mu;
+    // window width -> shifted copies. Shared: an MSM holds its table until it
+    // returns, so a table outgrown by a longer prefix of the same width (and
+    // replaced here) is freed only when its last user is done.
+    std::map<int, std::shared_ptr<BaseTable>> tables;
     ~apg_srs() {
         tables.clear();
         if (base) (void)hipFree(base);
@@ -103,18 +106,19 @@ __global__ void k_srs_gen(Fe tau, size_t n, Affine *out) {
     out[i] = xyzz_to_affine(acc);
 }
 
-BaseTable &srs_table(Context &ctx, apg_srs *srs, size_t n) {
+std::shared_ptr<BaseTable> srs_table(Context &ctx, apg_srs *srs, size_t n) {
     if (n > srs->n) throw ApgError(APG_ERR_ARG, "SRS too small for this commit");
     const int c = msm_pick_c(n, 1);
     std::lock_guard<std::mutex> lk(srs->mu);
     auto it = srs->tables.find(c);
     if (it == srs->tables.end() || it->second->stride < n) {
         const int copies = msm_window_count(254, c);
-        srs->tables[c] = msm_expand_bases(ctx, srs->base, n, c, copies);
+        std::shared_ptr<BaseTable> t = msm_expand_bases(ctx, srs->base, n, c, copies);
         ctx.sync();
-        it = srs->tables.find(c);
+        srs->tables[c] = t;
+        return t;
     }
-    return *it->second;
+    return it->second;
 }
 
 double now_ms() {
@@ -129,9 +133,10 @@ void commit_device(Context &ctx, apg_srs *srs, const Fe *fr_dev, const uint32_t 
         std::memset(out_mont, 0, 64 * k);
         return;
     }
-    BaseTable &tab = srs_table(ctx, srs, n);
+    need(srs);
+    const std::shared_ptr<BaseTable> tab = srs_table(ctx, srs, n);
     const double t0 = now_ms();
-    msm_run(ctx, tab, n, k, fr_dev, u32_dev, res.data());
+    msm_run(ctx, *tab, n, k, fr_dev, u32_dev, res.data());
     if (ctx.profiling) {
         ctx.stats.msm_ms += now_ms() - t0;
         ctx.stats.msm_calls += 1;
@@ -192,9 +197,11 @@ std::vector<Fe *> upload_tables(Context &ctx, const uint8_t *const *tables, size
     return p;
 }
 
+// A shard's summand is eq times degree - 1 combined tables, one gate term of
+// at most three factors, so degree 3 or 4.
 void check_shard_args(size_t num_vars, size_t k, int degree) {
     if (num_vars > 30 || num_vars < 1) throw ApgError(APG_ERR_ARG, "num_vars out of range");
-    if (degree < 3 || degree > 5) throw ApgError(APG_ERR_ARG, "degree must be 3..5");
+    if (degree < 3 || degree > 4) throw ApgError(APG_ERR_ARG, "degree must be 3 or 4");
     if (k < 1 || k > (size_t)SC_MAX_TABLES) throw ApgError(APG_ERR_ARG, "column count out of range");
 }
 
@@ -285,8 +292,10 @@ void apg_ctx_destroy(apg_ctx *ctx) {
 }
 
 int apg_ctx_synchronize(apg_ctx *ctx) {
-    return guarded([&] { ctx->c.sync(); });
+    return guarded([&] {
         need(ctx);
+        ctx->c.sync();
+    });
 }
 
 int apg_ctx_set_profiling(apg_ctx *ctx, int on) {
@@ -542,6 +551,7 @@ size_t apg_srs_len(const apg_srs *srs) { return srs ? srs->n : 0; }
 
 size_t apg_srs_device_bytes(const apg_srs *srs) {
     if (!srs) return 0;
+    std::lock_guard<std::mutex> lk(srs->mu);
     size_t b = srs->n * 64;
     for (auto &kv : srs->tables) b += kv.second->stride * kv.second->copies * 64;
     return b;
@@ -997,8 +1007,11 @@ int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, siz
                         size_t k, int degree, apg_transcript *t, uint8_t *commitments_wire,
                         uint8_t *rounds_out, uint8_t *final_evals_out) {
     return guarded([&] {
-        need(ctx);
         check_shard_args(num_vars, k, degree);
+        need(ctx);
+        need(srs);
+        need(t);
+        need(columns_dev);
         std::lock_guard<std::mutex> lk(ctx->c.mu);
         HIP_CHECK(hipSetDevice(ctx->c.device));
         prove_shard_impl(ctx->c, srs, static_cast<const Fe *>(columns_dev), num_vars, k, degree, t->t,
@@ -1040,6 +1053,10 @@ int apg_engine_submit_shard(apg_engine *e, apg_srs *srs, const void *columns_dev
                             size_t k, int degree, const apg_transcript *seed, uint64_t *job_id) {
     return guarded([&] {
         check_shard_args(num_vars, k, degree);
+        need(e);
+        need(srs);
+        need(columns_dev);
+        need(job_id);
         auto job = std::make_unique<Job>();
         job->mem_bytes = shard_hbm_estimate(num_vars, k, degree);
         job->work = std::ldexp((double)k, (int)num_vars);
@@ -1060,11 +1077,20 @@ int apg_engine_submit_shard(apg_engine *e, apg_srs *srs, const void *columns_dev
     });
 }
 
-int apg_engine_poll(apg_engine *e, uint64_t job_id) { return e->e->poll(job_id) ? 1 : 0; }
+int apg_engine_poll(apg_engine *e, uint64_t job_id) {
+    if (!e) return APG_ERR_ARG;
+    const int r = e->e->poll(job_id);
+    if (r < 0) {
+        set_last_error("engine: unknown or already collected job id");
+        return APG_ERR_ARG;
+    }
+    return r;
+}
 
 int apg_engine_wait(apg_engine *e, uint64_t job_id, uint8_t *commitments_wire, uint8_t *rounds_out,
                     uint8_t *final_evals_out, apg_transcript *transcript_out, apg_job_stats *stats) {
     return guarded([&] {
+        need(e);
         auto j = e->e->wait(job_id);
         if (stats) {
             stats->queue_ms = j->t_start - j->t_submit;
@@ -1082,6 +1108,7 @@ int apg_engine_wait(apg_engine *e, uint64_t job_id, uint8_t *commitments_wire, u
 
 int apg_engine_stats(apg_engine *e, apg_engine_info *out) {
     return guarded([&] {
+        need(e);
         EngineStats s = e->e->stats();
         out->queued = s.queued;
         out->running = s.running;
@@ -1096,6 +1123,7 @@ int apg_engine_stats(apg_engine *e, apg_engine_info *out) {
 
 int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_scalars) {
     return guarded([&] {
+        need(e);
         if (policy != APG_POLICY_FIFO && policy != APG_POLICY_SHORTEST)
             throw ApgError(APG_ERR_ARG, "unknown engine policy");
         if (reserve < 0) throw ApgError(APG_ERR_ARG, "reserve must be >= 0");
@@ -1104,15 +1132,25 @@ int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_s
 }
 
 int apg_engine_set_profiling(apg_engine *e, int on) {
-    return guarded([&] { e->e->set_profiling(on != 0); });
+    return guarded([&] {
+        need(e);
+        e->e->set_profiling(on != 0);
+    });
 }
 
 int apg_engine_kernel_stats(apg_engine *e, apg_stats *out) {
-    return guarded([&] { export_stats(e->e->kernel_stats(), out); });
+    return guarded([&] {
+        need(e);
+        need(out);
+        export_stats(e->e->kernel_stats(), out);
+    });
 }
 
 int apg_engine_reset_kernel_stats(apg_engine *e) {
-    return guarded([&] { e->e->reset_kernel_stats(); });
+    return guarded([&] {
+        need(e);
+        e->e->reset_kernel_stats();
+    });
 }
 
 int apg_transcript_new(apg_transcript **out) {

@@ -591,12 +591,7 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
     const size_t nslots = (size_t)s.prog.nvirt + l.ncached;
     Fe *slots = ctx.d<Fe>("gkr.slots", std::max<size_t>(nslots * MAX_RBLOCKS * RTPB, 1));
     Fe *partials = ctx.d<Fe>("gkr.partials", (size_t)MAX_RBLOCKS * 3);
-    if (!ctx.dev["gkr.counter"].ptr) {
-        HIP_CHECK(hipMalloc(&ctx.dev["gkr.counter"].ptr, 256));
-        ctx.dev["gkr.counter"].cap = 256;
-        HIP_CHECK(hipMemsetAsync(ctx.dev["gkr.counter"].ptr, 0, 256, st));
-    }
-    unsigned *counter = static_cast<unsigned *>(ctx.dev["gkr.counter"].ptr);
+    unsigned *counter = ctx.zeroed<unsigned>("gkr.counter", 64);
     // the round kernels' last block writes the node sums straight into
     // pinned host memory; the stream sync after each round makes them visible
     Fe *hvals = ctx.h<Fe>("gkr.hvals", 3);

@@ -12,16 +12,9 @@
 #include <unordered_map>
 #include <vector>
 
+#include "error.h"
+
 namespace apg {
-
-// Raised inside the library; every extern "C" entry catches it and returns a
-// negative code. Nothing is thrown across the C ABI.
-struct ApgError : std::runtime_error {
-    int code;
-    ApgError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-void set_last_error(const std::string &msg);
 
 #define HIP_CHECK(x)                                                                         \
     do {                                                                                     \
@@ -40,13 +33,16 @@ void set_last_error(const std::string &msg);
 struct DevBuf {
     void *ptr = nullptr;
     size_t cap = 0;
-    void *get(size_t bytes, std::vector<void *> &retired) {
+    // held: the owning context's count of device bytes (live and retired),
+    // which the shard engine charges against its HBM budget
+    void *get(size_t bytes, std::vector<void *> &retired, std::atomic<size_t> &held) {
         if (bytes > cap) {
             if (ptr) retired.push_back(ptr);
             ptr = nullptr;
             size_t c = bytes + bytes / 4 + 256;
             HIP_CHECK(hipMalloc(&ptr, c));
             cap = c;
+            held += c;
         }
         return ptr;
     }
@@ -113,11 +109,27 @@ struct Context {
     std::unordered_map<std::string, DevBuf> dev;
     std::unordered_map<std::string, HostBuf> host;
     std::vector<void *> retired_dev, retired_host;  // outgrown buffers (see DevBuf)
+    // device bytes this context holds (arena buffers, live and retired);
+    // written by the thread that owns the context, read by others
+    std::atomic<size_t> held_bytes{0};
     int cu_count = 256;
 
     template <class T>
     T *d(const char *name, size_t count) {
-        return static_cast<T *>(dev[name].get(count * sizeof(T), retired_dev));
+        return static_cast<T *>(dev[name].get(count * sizeof(T), retired_dev, held_bytes));
+    }
+    // A buffer that is zero when first handed out; kernels that use it as a
+    // counter leave it zero again when they finish.
+    template <class T>
+    T *zeroed(const char *name, size_t count) {
+        DevBuf &b = dev[name];
+        const bool fresh = b.cap < count * sizeof(T);
+        T *p = static_cast<T *>(b.get(count * sizeof(T), retired_dev, held_bytes));
+        if (fresh) {
+            HIP_CHECK(hipMemsetAsync(p, 0, b.cap, stream));
+            sync();
+        }
+        return p;
     }
     template <class T>
     T *h(const char *name, size_t count) {
@@ -162,6 +174,7 @@ inline void context_release(Context &c) {
     for (void *p : c.retired_host) (void)hipHostFree(p);
     c.retired_dev.clear();
     c.retired_host.clear();
+    c.held_bytes = 0;
     if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
     if (c.ev_join) (void)hipEventDestroy(c.ev_join);
     if (c.side && c.side != c.stream) (void)hipStreamDestroy(c.side);

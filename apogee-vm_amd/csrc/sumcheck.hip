@@ -437,12 +437,7 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
     for (int t = 0; t < k; t++) cur[t] = tables[t];
 
     Fe *partials = ctx.d<Fe>("sc.partials", (size_t)MAX_RBLOCKS * SC_MAX_NODES);
-    if (!ctx.dev["sc.counter"].ptr) {
-        HIP_CHECK(hipMalloc(&ctx.dev["sc.counter"].ptr, 256));
-        ctx.dev["sc.counter"].cap = 256;
-        HIP_CHECK(hipMemsetAsync(ctx.dev["sc.counter"].ptr, 0, 256, st));
-    }
-    unsigned *counter = static_cast<unsigned *>(ctx.dev["sc.counter"].ptr);
+    unsigned *counter = ctx.zeroed<unsigned>("sc.counter", 64);
     // the round kernels' last block writes the node sums straight into
     // pinned host memory; the stream sync after each round makes them visible
     Fe *hvals = ctx.h<Fe>("sc.hvals", SC_MAX_NODES);

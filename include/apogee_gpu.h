@@ -339,7 +339,9 @@ int apg_witness_digest_dev(apg_ctx *ctx, const void *columns_dev, size_t k, size
  * f = degree - 1 combined tables t_o = sum_c rho^(o k + c) col_c; eq point =
  * num_vars SUMCHECK_CHALLENGE draws; rounds of eq * t_0 * ... * t_(f-1)
  * (SUMCHECK_ROUND / SUMCHECK_CHALLENGE per round); final evals absorbed under
- * SUMCHECK_FINAL_EVALS. columns_dev: k * 2^num_vars elements, device. */
+ * SUMCHECK_FINAL_EVALS. columns_dev: k * 2^num_vars elements, device.
+ * degree is 3 or 4 (one gate term of at most three factors); anything else,
+ * num_vars outside 1..30 or k outside 1..24 is APG_ERR_ARG. */
 int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, size_t num_vars,
                         size_t k, int degree, apg_transcript *t, uint8_t *commitments_wire,
                         uint8_t *rounds_out, uint8_t *final_evals_out);
@@ -349,7 +351,10 @@ int apg_prove_shard_dev(apg_ctx *ctx, apg_srs *srs, const void *columns_dev, siz
  * claiming shards, crates/prover/src/streaming.rs:601-680): `workers` HIP
  * streams, each with its own scratch; jobs wait in one queue (arrival order
  * unless apg_engine_set_policy says otherwise) and start when a worker is
- * free and their HBM estimate fits hbm_budget_bytes beside the running ones.
+ * free and their HBM charge fits hbm_budget_bytes: each worker is charged
+ * the device memory its scratch arena holds (live and outgrown buffers), or
+ * while it runs a job the larger of that and the job's estimate; with nothing
+ * running the next job is admitted whatever the arenas hold.
  * Each job runs apg_prove_shard_dev on its own copy of the
  * seed transcript; its bytes do not depend on what else runs. Columns stay
  * caller-owned device memory until the job has been waited for. */
@@ -362,17 +367,19 @@ typedef struct {
 } apg_job_stats;
 typedef struct {
     size_t queued, running, completed;
-    size_t hbm_in_use, hbm_peak, hbm_budget; /* admitted job estimates */
-    size_t arena_bytes;                      /* scratch the workers hold */
+    size_t hbm_in_use, hbm_peak, hbm_budget; /* the workers' charges (see above) */
+    size_t arena_bytes;                      /* device scratch the workers hold */
     size_t workers;
 } apg_engine_info;
 int apg_engine_create(int device, int workers, size_t hbm_budget_bytes, apg_engine **out);
 void apg_engine_destroy(apg_engine *e);
 int apg_engine_submit_shard(apg_engine *e, apg_srs *srs, const void *columns_dev, size_t num_vars,
                             size_t k, int degree, const apg_transcript *seed, uint64_t *job_id);
-int apg_engine_poll(apg_engine *e, uint64_t job_id); /* 1 done, 0 pending */
+/* 1 done, 0 pending, APG_ERR_ARG for an id never issued or already waited for */
+int apg_engine_poll(apg_engine *e, uint64_t job_id);
 /* blocks until the job is done; outputs as apg_prove_shard_dev, plus the final
- * transcript (nullable) and the job's timing (nullable) */
+ * transcript (nullable) and the job's timing (nullable); APG_ERR_ARG at once
+ * for an id never issued or already waited for */
 int apg_engine_wait(apg_engine *e, uint64_t job_id, uint8_t *commitments_wire, uint8_t *rounds_out,
                     uint8_t *final_evals_out, apg_transcript *transcript_out, apg_job_stats *stats);
 int apg_engine_stats(apg_engine *e, apg_engine_info *out);

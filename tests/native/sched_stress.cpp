@@ -1,0 +1,248 @@
+// Host-only stress test of the shard engine's scheduler (jobqueue.h) over mock
+// workers, built with -fsanitize=thread or -fsanitize=address by
+// tests/test_engine_sched.py. A mock worker's context holds a byte count that
+// a job raises to its estimate, as a scratch arena grows; each job's output
+// depends only on its own input. Checks, printing one line per check:
+//   * every job completes with its own output, from several submitting
+//     threads at once, under both policies and with a fast lane;
+//   * with two or more jobs running, the charged HBM never exceeds the budget,
+//     even though idle workers keep their arenas;
+//   * fast-lane workers never run a long job;
+//   * wait and poll refuse ids never issued or already collected;
+//   * a worker that fails to start fails its jobs with that error while the
+//     others serve;
+//   * destroying the scheduler with jobs queued runs them first.
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "jobqueue.h"
+
+namespace {
+
+struct MockCtx {
+    std::atomic<size_t> held{0};
+    int index = 0;
+};
+
+using Sched = apg::Scheduler<MockCtx>;
+using Job = apg::BasicJob<MockCtx>;
+
+int failures = 0;
+#define CHECK(cond, what)                                      \
+    do {                                                       \
+        if (!(cond)) {                                         \
+            std::printf("FAIL %s (line %d)\n", what, __LINE__); \
+            failures++;                                        \
+        }                                                      \
+    } while (0)
+
+uint64_t mix(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+std::unique_ptr<Job> make_job(uint64_t input, size_t mem, double work, int reserve, double small,
+                              std::atomic<int> *lane_violations) {
+    auto j = std::make_unique<Job>();
+    j->mem_bytes = mem;
+    j->work = work;
+    j->fn = [input, mem, work, reserve, small, lane_violations](MockCtx &c, Job &self) {
+        if (c.index < reserve && work > small) (*lane_violations)++;
+        // the arena grows to what the job needs
+        size_t h = c.held.load();
+        while (h < mem && !c.held.compare_exchange_weak(h, mem)) {
+        }
+        uint64_t v = input;
+        const int spins = (int)(mix(input) % 2000);
+        for (int i = 0; i < spins; i++) v = mix(v);
+        std::this_thread::sleep_for(std::chrono::microseconds(mix(input) % 300));
+        self.rounds.resize(8);
+        for (int i = 0; i < 8; i++) self.rounds[i] = (uint8_t)(v >> (8 * i));
+    };
+    return j;
+}
+
+std::vector<uint8_t> expected(uint64_t input) {
+    uint64_t v = input;
+    const int spins = (int)(mix(input) % 2000);
+    for (int i = 0; i < spins; i++) v = mix(v);
+    std::vector<uint8_t> r(8);
+    for (int i = 0; i < 8; i++) r[i] = (uint8_t)(v >> (8 * i));
+    return r;
+}
+
+void run_mix(apg::Policy policy, int workers, int reserve, size_t budget, int submitters, int per_thread) {
+    std::vector<std::unique_ptr<MockCtx>> ctxs;
+    std::vector<MockCtx *> raw;
+    for (int w = 0; w < workers; w++) {
+        ctxs.push_back(std::make_unique<MockCtx>());
+        ctxs.back()->index = w;
+        raw.push_back(ctxs.back().get());
+    }
+    apg::WorkerHooks<MockCtx> hooks;
+    hooks.held = [](const MockCtx &c) { return c.held.load(); };
+    std::atomic<int> collected{0};
+    hooks.collect = [&](MockCtx &) { collected++; };
+    const double small = 50;
+    std::atomic<int> lane_violations{0};
+    std::atomic<bool> over_budget{false}, stop_monitor{false};
+    {
+        Sched s(raw, budget, reserve, hooks);
+        s.set_policy(policy, reserve, small);
+        std::thread monitor([&] {
+            while (!stop_monitor) {
+                const apg::EngineStats st = s.stats();
+                if (st.running >= 2 && st.hbm_in_use > budget) over_budget = true;
+                std::this_thread::yield();
+            }
+        });
+        std::vector<std::thread> subs;
+        std::atomic<int> wrong{0}, errors{0};
+        for (int t = 0; t < submitters; t++) {
+            subs.emplace_back([&, t] {
+                std::mt19937_64 rng(1000 + t);
+                std::vector<std::pair<uint64_t, uint64_t>> ids;
+                for (int i = 0; i < per_thread; i++) {
+                    const uint64_t input = rng();
+                    const size_t mem = 10 + rng() % 51;  // 10..60 units
+                    const double work = (double)(1 + rng() % 100);
+                    ids.push_back({s.submit(make_job(input, mem, work, reserve, small, &lane_violations)), input});
+                    if (i % 7 == 3) {  // collect some early, out of order
+                        auto p = ids.back();
+                        ids.pop_back();
+                        auto j = s.wait(p.first);
+                        if (j->status != 0) errors++;
+                        if (j->rounds != expected(p.second)) wrong++;
+                    }
+                }
+                for (auto &p : ids) {
+                    auto j = s.wait(p.first);
+                    if (j->status != 0) errors++;
+                    if (j->rounds != expected(p.second)) wrong++;
+                    // a second wait on a collected id is refused at once
+                    bool threw = false;
+                    try {
+                        s.wait(p.first);
+                    } catch (const apg::ApgError &e) {
+                        threw = e.code == -2;
+                    }
+                    if (!threw) wrong++;
+                    if (s.poll(p.first) != -1) wrong++;
+                }
+            });
+        }
+        for (auto &t : subs) t.join();
+        stop_monitor = true;
+        monitor.join();
+        CHECK(wrong == 0, "every job's output is its own, and collected ids are refused");
+        CHECK(errors == 0, "no job failed");
+        const apg::EngineStats st = s.stats();
+        CHECK(st.completed == (size_t)(submitters * per_thread), "every job completed");
+        CHECK(st.queued == 0 && st.running == 0, "nothing left queued or running");
+        CHECK(st.hbm_peak >= st.arena_bytes || st.hbm_peak > 0, "peak recorded");
+    }
+    CHECK(!over_budget, "charged HBM within the budget whenever two or more jobs run");
+    CHECK(lane_violations == 0, "fast-lane workers ran short jobs only");
+    CHECK(collected == submitters * per_thread, "collect hook ran once per job");
+    std::printf("ok mix policy=%d workers=%d reserve=%d budget=%zu\n", (int)policy, workers, reserve, budget);
+}
+
+void unknown_ids() {
+    MockCtx c;
+    std::vector<MockCtx *> raw{&c};
+    apg::WorkerHooks<MockCtx> hooks;
+    hooks.held = [](const MockCtx &x) { return x.held.load(); };
+    Sched s(raw, 100, 0, hooks);
+    bool threw = false;
+    try {
+        s.wait(12345);
+    } catch (const apg::ApgError &e) {
+        threw = e.code == -2;
+    }
+    CHECK(threw, "wait on an id never issued is refused");
+    CHECK(s.poll(12345) == -1, "poll on an id never issued is refused");
+    std::atomic<int> lv{0};
+    auto big = make_job(1, 1000, 1, 0, 0, &lv);
+    const uint64_t id = s.submit(std::move(big));
+    auto j = s.wait(id);
+    CHECK(j->status == -2, "a job above the budget is refused at submission");
+    std::printf("ok unknown ids\n");
+}
+
+void failing_worker() {
+    std::vector<std::unique_ptr<MockCtx>> ctxs;
+    std::vector<MockCtx *> raw;
+    for (int w = 0; w < 3; w++) {
+        ctxs.push_back(std::make_unique<MockCtx>());
+        ctxs.back()->index = w;
+        raw.push_back(ctxs.back().get());
+    }
+    apg::WorkerHooks<MockCtx> hooks;
+    hooks.held = [](const MockCtx &x) { return x.held.load(); };
+    hooks.start = [](int w, MockCtx &) {
+        if (w == 0) throw apg::ApgError(-3, "device unavailable");
+    };
+    std::atomic<int> lv{0};
+    Sched s(raw, 1000, 0, hooks);
+    std::vector<std::pair<uint64_t, uint64_t>> ids;
+    for (int i = 0; i < 60; i++) ids.push_back({s.submit(make_job(100 + i, 10, 1, 0, 0, &lv)), 100 + i});
+    int failed = 0, good = 0;
+    for (auto &p : ids) {
+        auto j = s.wait(p.first);
+        if (j->status == -3 && j->worker == 0) failed++;
+        if (j->status == 0 && j->worker != 0 && j->rounds == expected(p.second)) good++;
+    }
+    CHECK(failed + good == 60, "jobs on the failed worker carry its error, the others their outputs");
+    std::printf("ok failing worker (%d failed on worker 0, %d served)\n", failed, good);
+}
+
+void drain_on_destroy() {
+    MockCtx c;
+    std::vector<MockCtx *> raw{&c};
+    apg::WorkerHooks<MockCtx> hooks;
+    hooks.held = [](const MockCtx &x) { return x.held.load(); };
+    std::atomic<int> ran{0};
+    {
+        Sched s(raw, 100, 0, hooks);
+        for (int i = 0; i < 20; i++) {
+            auto j = std::make_unique<Job>();
+            j->mem_bytes = 5;
+            j->fn = [&ran](MockCtx &, Job &) {
+                std::this_thread::sleep_for(std::chrono::microseconds(200));
+                ran++;
+            };
+            s.submit(std::move(j));
+        }
+    }
+    CHECK(ran == 20, "queued jobs run before the scheduler is destroyed");
+    std::printf("ok drain on destroy\n");
+}
+
+}  // namespace
+
+namespace apg {
+void set_last_error(const std::string &) {}
+}  // namespace apg
+
+int main() {
+    unknown_ids();
+    failing_worker();
+    drain_on_destroy();
+    // budgets below, at and above what the arenas reach (workers x 60 units)
+    run_mix(apg::Policy::kFifo, 4, 0, 100, 3, 40);
+    run_mix(apg::Policy::kShortest, 4, 0, 130, 3, 40);
+    run_mix(apg::Policy::kShortest, 5, 2, 150, 4, 30);
+    run_mix(apg::Policy::kFifo, 6, 1, 1000, 4, 30);
+    if (failures) {
+        std::printf("%d checks failed\n", failures);
+        return 1;
+    }
+    std::printf("all checks passed\n");
+    return 0;
+}

@@ -125,6 +125,33 @@ def test_interface_errors(apg):
     assert apg.Gate([0], [apg.GateTerm(one, 0)]).degree() == 3
 
 
+def test_abi_refusals_without_a_device(apg):
+    """Argument errors come back as APG_ERR_ARG before anything touches a
+    device: a shard of degree 5 (a gate term holds three factors, so a
+    shard's summand is of degree 3 or 4), null handles, and engine ids never
+    issued."""
+    L = apg.lib()
+    err = lambda: L.apg_last_error().decode()
+    buf = np.zeros(16, dtype=np.uint64)
+    for deg in (2, 5, 6):
+        rc = L.apg_prove_shard_dev(None, None, None, 4, 2, deg, None, buf.ctypes.data, buf.ctypes.data,
+                                   buf.ctypes.data)
+        assert rc == -2 and "degree" in err()
+    for deg in (3, 4):  # a valid degree gets as far as the null context
+        rc = L.apg_prove_shard_dev(None, None, None, 4, 2, deg, None, buf.ctypes.data, buf.ctypes.data,
+                                   buf.ctypes.data)
+        assert rc == -2 and "null handle" in err()
+    jid = ctypes.c_uint64()
+    assert L.apg_engine_submit_shard(None, None, None, 4, 2, 5, None, ctypes.byref(jid)) == -2
+    assert "degree" in err()
+    assert L.apg_engine_submit_shard(None, None, None, 4, 2, 3, None, ctypes.byref(jid)) == -2
+    assert L.apg_ctx_synchronize(None) == -2
+    assert L.apg_engine_poll(None, 1) == -2
+    assert L.apg_engine_wait(None, 1, None, None, None, None, None) == -2
+    assert L.apg_engine_stats(None, None) == -2
+    assert L.apg_engine_set_policy(None, 0, 0, ctypes.c_double(0.0)) == -2
+
+
 def _mul32(apg, which, a, b):
     a = np.ascontiguousarray(a, dtype=np.uint64)
     b = np.ascontiguousarray(b, dtype=np.uint64)
