@@ -102,6 +102,8 @@ def _declare(L):
     d("orc_prove_sumcheck", [P, SZ, P, SZ, P, SZ, P, SZ, P, P])
     d("orc_witness_digest", [P, SZ, SZ, P])
     d("orc_combine", [P, SZ, SZ, P, SZ, P])
+    d("orc_axpy", [SZ, P, P, P])
+    d("orc_product_claim", [P, SZ, P, SZ, P])
     d("orc_splitmix_next", [P], U64)
     d("orc_splitmix_next_fr", [P, P])
     d("orc_chacha_block", [P, C.c_uint32, P, P])
@@ -505,6 +507,20 @@ def combine(cols: np.ndarray, weights: np.ndarray, outs: int) -> np.ndarray:
     k, n = cols.shape[0], cols.shape[1]
     out = np.zeros((outs, n, 4), dtype=np.uint64)
     lib().orc_combine(ptr(np.ascontiguousarray(cols)), n, k, ptr(np.ascontiguousarray(weights)), outs, ptr(out))
+    return out
+
+
+def axpy(a, x: np.ndarray, y: np.ndarray) -> None:
+    """y += a * x in place (Fr, (n, 4) arrays)."""
+    assert y.flags["C_CONTIGUOUS"] and x.shape == y.shape
+    lib().orc_axpy(x.shape[0], ptr(np.ascontiguousarray(a)), ptr(np.ascontiguousarray(x)), ptr(y))
+
+
+def product_claim(r: np.ndarray, tables) -> np.ndarray:
+    """sum_x eq(r, x) prod_o t_o(x) over the hypercube of r's length."""
+    work = [np.ascontiguousarray(t) for t in tables]
+    out = fe()
+    lib().orc_product_claim(ptr(np.ascontiguousarray(r)), r.shape[0], _table_ptrs(work), len(work), ptr(out))
     return out
 
 

@@ -1149,6 +1149,34 @@ void orc_combine(const u64 *cols, size_t n, size_t k, const u64 *w, size_t outs,
         }
 }
 
+/* y[i] += a * x[i] over n elements of Fr: one column's share of a
+ * random-linear-combination fold (orc_combine), one column at a time. */
+void orc_axpy(size_t n, const u64 *a, const u64 *x, u64 *y) {
+    for (size_t i = 0; i < n; i++) {
+        u64 t[4];
+        f_mul(FR, a, x + 4 * i, t);
+        f_add(FR, y + 4 * i, t, y + 4 * i);
+    }
+}
+
+/* The claim of a product sumcheck, sum_x eq(r, x) prod_{o<f} t_o(x) over the
+ * 2^n hypercube: what g_0(0) + g_0(1) must equal (the verifier's first check,
+ * crates/sumcheck/src/lib.rs:441), with eq by eq_table (lib.rs:233-248). */
+void orc_product_claim(const u64 *r, size_t n, const u64 *const *tables, size_t f, u64 *out) {
+    const size_t len = (size_t)1 << n;
+    u64 *eq = (u64 *)malloc(len * 32);
+    orc_eq_table(r, n, eq);
+    u64 acc[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < len; i++) {
+        u64 p[4];
+        memcpy(p, eq + 4 * i, 32);
+        for (size_t o = 0; o < f; o++) f_mul(FR, p, tables[o] + 4 * i, p);
+        f_add(FR, acc, p, acc);
+    }
+    memcpy(out, acc, 32);
+    free(eq);
+}
+
 /* sumcheck/src/lib.rs:228-255 witness_digest */
 void orc_witness_digest(u64 **tables, size_t k, size_t n, u64 *out) {
     orc_transcript s;

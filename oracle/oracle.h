@@ -103,6 +103,8 @@ void orc_prove_sumcheck(const uint64_t *eq_point, size_t num_vars, const orc_ter
                         size_t workers, uint64_t *rounds_out, uint64_t *point_out);
 void orc_witness_digest(uint64_t **tables, size_t k, size_t num_vars, uint64_t *out);
 /* out_o[i] = sum_c w[o k + c] * cols[c n + i] for o < outs */
+void orc_axpy(size_t n, const uint64_t *a, const uint64_t *x, uint64_t *y);
+void orc_product_claim(const uint64_t *r, size_t n, const uint64_t *const *tables, size_t f, uint64_t *out);
 void orc_combine(const uint64_t *cols, size_t n, size_t k, const uint64_t *w, size_t outs,
                  uint64_t *out);
 

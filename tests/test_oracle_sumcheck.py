@@ -267,3 +267,61 @@ def test_mercury_batch_open_stacked_properties(orc):
         orc.mercury_batch_open_stacked(srs, [], [], u, r, orc.Transcript())
     with pytest.raises(ValueError):
         orc.mercury_batch_open_stacked(srs, [[cols[0], cols[1], cols[2]]], cms[:1], u, r, orc.Transcript())
+
+
+@pytest.mark.parametrize("n", [1, 3, 6, 10])
+def test_product_claim_and_axpy(orc, n):
+    """The claim the full-size shard checks compare with: with one table it
+    is the multilinear extension at r (evaluate, pinned by poly_kats); with a
+    constant second table it scales by that constant; and it is g_0(0) +
+    g_0(1) of the oracle's own product sumcheck (pinned by the brute-force
+    definition). axpy accumulates the same fold as combine."""
+    rng_seed = 300 + n
+    a, b, c = (orc.fill_uniform(rng_seed + i, 1 << n) for i in range(3))
+    r = orc.fill_uniform(rng_seed + 9, n)
+    assert np.array_equal(orc.product_claim(r, [a]), orc.evaluate(a, r))
+    k7 = orc.from_u64(orc.FR, 7)
+    seven = np.tile(k7, (1 << n, 1))
+    assert np.array_equal(orc.product_claim(r, [a, seven]), orc.mul(orc.FR, orc.evaluate(a, r), k7))
+    one = orc.from_u64(orc.FR, 1)
+    rounds, _, _ = orc.prove_sumcheck(r, [(one, [0, 1, 2])], [a, b, c], orc.Transcript())
+    g0 = rounds[0]
+    s = orc.add(orc.FR, g0[0], g0[0])  # g(0) + g(1) = 2 c0 + c1 + c2 + c3 (+ c4)
+    for coef in g0[1:]:
+        s = orc.add(orc.FR, s, coef)
+    assert np.array_equal(orc.product_claim(r, [a, b, c]), s)
+    cols = np.stack([a, b, c])
+    w = orc.fill_uniform(rng_seed + 11, 3)
+    y = orc.fe(1 << n)
+    for i in range(3):
+        orc.axpy(w[i], cols[i], y)
+    assert np.array_equal(y, orc.combine(cols, w, 1)[0])
+
+
+def test_large_shard_identities_accept_the_oracle_and_reject_changes(orc):
+    """The verifier identities the full-size GPU test applies to the large
+    shards (tests/test_gpu_mixed_batch.py) hold for the oracle's own shard
+    script, and fail when a round coefficient, a final evaluation or an input
+    column differs."""
+    from types import SimpleNamespace
+
+    from test_gpu_mixed_batch import _large_shard_identities
+
+    nv, k, seed = 6, 3, 8800
+    spec = SimpleNamespace(num_vars=nv, columns=k, degree=3, seed=seed)
+    cols = np.stack([orc.fill_ctr(seed + c, 1 << nv, "witness") for c in range(k)])
+    srs = orc.srs_powers(orc.fill_uniform(5, 1)[0], 1 << nv)
+    t = orc.Transcript()
+    cms, rounds, fin = orc.shard_script(srs, cols, 3, t)
+    proof = SimpleNamespace(commitments=cms, rounds=rounds, final_evals=fin)
+    assert _large_shard_identities(orc, spec, proof, t.snapshot()) == []
+    bad = rounds.copy()
+    bad[2][1] = orc.add(orc.FR, bad[2][1], orc.from_u64(orc.FR, 1))
+    assert _large_shard_identities(orc, spec, SimpleNamespace(commitments=cms, rounds=bad, final_evals=fin),
+                                   t.snapshot())
+    badf = fin.copy()
+    badf[1] = orc.add(orc.FR, badf[1], orc.from_u64(orc.FR, 1))
+    assert "final evaluation 1" in _large_shard_identities(
+        orc, spec, SimpleNamespace(commitments=cms, rounds=rounds, final_evals=badf), t.snapshot())
+    other = SimpleNamespace(num_vars=nv, columns=k, degree=3, seed=seed + 1)  # other inputs
+    assert "claim" in _large_shard_identities(orc, other, proof, t.snapshot())
