@@ -137,6 +137,7 @@ def _declare(L):
     d("apg_commit_fr_dev", [VP, VP, VP, SZ, SZ, VP])
     d("apg_sumcheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_sumcheck_prove_dev", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
+    d("apg_sumcheck_prove_dev_noalloc", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_zerocheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, VP, VP])
     d("apg_witness_digest", [VP, SZ, SZ, VP])
     d("apg_witness_digest_dev", [VP, VP, SZ, SZ, VP])

@@ -43,6 +43,19 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables_dev
                            int num_vars, const Fe *eq_point_host, const RoundFn &round_fn,
                            Fe *rounds_out, Fe *point_out, Fe *finals_out);
 
+// The same proof with all device scratch in one file-scope array per device
+// (sumcheck_noalloc_limit() bytes), so the call itself allocates no device
+// memory. sumcheck_noalloc_init(device) takes the array's address on that
+// device (once, with the device current; the library does it when a context
+// is created). Sizes whose scratch (sumcheck_noalloc_bytes) exceeds the
+// limit are refused. Calls on one device are serialised by a lock.
+void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *tables_dev, int k,
+                                   int num_vars, const Fe *eq_point_host, const RoundFn &round_fn,
+                                   Fe *rounds_out, Fe *point_out, Fe *finals_out);
+void sumcheck_noalloc_init(int device);
+size_t sumcheck_noalloc_limit();
+size_t sumcheck_noalloc_bytes(int k, int num_vars);
+
 // Host interpolation: the d+1 values at nodes 0..d to ascending coefficients
 // (crates/sumcheck/src/lib.rs:294-305 for d = 3, Newton form beyond).
 void interpolate_host(const Fe *vals, int d, Fe *coeffs);

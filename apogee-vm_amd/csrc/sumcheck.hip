@@ -28,6 +28,8 @@
 //    block to finish.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "sumcheck.h"
 #include "field29.cuh"
@@ -373,36 +375,169 @@ static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bo
 #undef APG_GATE
 }
 
-void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, int k, int n,
-                           const Fe *eq_point, const RoundFn &round_fn, Fe *rounds_out,
-                           Fe *point_out, Fe *finals_out) {
+// Device scratch of one proof, carved from one region in a fixed order (each
+// piece 256-byte aligned): the eq point, the suffix-eq tables, two ping-pong
+// fold pools, the round kernels' per-block partial sums and their counter,
+// and the node sums of a round (read back after each round).
+struct SumcheckScratch {
+    Fe *r, *eqA, *eqB, *poolA, *poolB, *partials, *nodes;
+    unsigned *counter;
+};
+
+struct ScratchLayout {
+    size_t off[8];
+    size_t total;
+};
+
+static ScratchLayout sumcheck_layout(int k, int n) {
+    const size_t len0 = size_t(1) << n;
+    const int h0 = std::max(1, std::min(n, (n + 1) / 2));
+    const size_t astride = size_t(1) << (h0 - 1), bstride = size_t(1) << (n - h0);
+    const size_t asz = (size_t)(h0 - 1) * astride, bsz = (size_t)(n - h0 + 1) * bstride;
+    const size_t bytes[8] = {
+        (size_t)std::max(n, 1) * sizeof(Fe),
+        std::max<size_t>(asz, 1) * sizeof(Fe),
+        std::max<size_t>(bsz, 1) * sizeof(Fe),
+        (size_t)k * std::max<size_t>(len0 / 2, 1) * sizeof(Fe),
+        (size_t)k * std::max<size_t>(len0 / 4, 1) * sizeof(Fe),
+        (size_t)MAX_RBLOCKS * SC_MAX_NODES * sizeof(Fe),
+        SC_MAX_NODES * sizeof(Fe),
+        64 * sizeof(unsigned),
+    };
+    ScratchLayout l;
+    size_t at = 0;
+    for (int i = 0; i < 8; i++) {
+        l.off[i] = at;
+        at += (bytes[i] + 255) & ~size_t(255);
+    }
+    l.total = at;
+    return l;
+}
+
+static SumcheckScratch sumcheck_carve(uint8_t *base, const ScratchLayout &l) {
+    SumcheckScratch s;
+    s.r = reinterpret_cast<Fe *>(base + l.off[0]);
+    s.eqA = reinterpret_cast<Fe *>(base + l.off[1]);
+    s.eqB = reinterpret_cast<Fe *>(base + l.off[2]);
+    s.poolA = reinterpret_cast<Fe *>(base + l.off[3]);
+    s.poolB = reinterpret_cast<Fe *>(base + l.off[4]);
+    s.partials = reinterpret_cast<Fe *>(base + l.off[5]);
+    s.nodes = reinterpret_cast<Fe *>(base + l.off[6]);
+    s.counter = reinterpret_cast<unsigned *>(base + l.off[7]);
+    return s;
+}
+
+static void check_sumcheck_args(const Gate &gate, int k) {
     if (k < 1 || k > SC_MAX_TABLES) throw ApgError(-2, "sumcheck: table count out of range");
     if (gate.nterms < 0 || gate.nterms > SC_MAX_TERMS) throw ApgError(-2, "sumcheck: too many terms");
     for (int q = 0; q < gate.nterms; q++)
         for (int f = 0; f < 3; f++)
             if (gate.terms[q].f[f] >= k || (f == 0 && gate.terms[q].f[0] < 0))
                 throw ApgError(-2, "sumcheck: term factor out of range");
-    const int D = gate.x_degree();  // degree of G along one variable
-    const int d = gate.degree();    // round polynomial degree
-    if (d + 1 > SC_MAX_NODES || D > 3) throw ApgError(-2, "sumcheck: degree too high");
-    const bool product = gate.is_plain_product(k);
-    hipStream_t st = ctx.stream;
-    const size_t len0 = size_t(1) << n;
+    if (gate.degree() + 1 > SC_MAX_NODES || gate.x_degree() > 3) throw ApgError(-2, "sumcheck: degree too high");
+}
 
+// The round loop over given scratch. mapped_nodes: the round kernels write
+// their node sums to `nodes_host` through its device mapping `sc.nodes`
+// (pinned memory, visible after the stream sync); otherwise they write
+// sc.nodes in device memory and each round copies them to nodes_host.
+static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *nodes_host, bool mapped_nodes,
+                            bool timing, const Gate &gate, Fe *const *tables, int k, int n, const Fe *eq_point,
+                            const RoundFn &round_fn, Fe *rounds_out, Fe *point_out, Fe *finals_out);
+
+void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, int k, int n,
+                           const Fe *eq_point, const RoundFn &round_fn, Fe *rounds_out,
+                           Fe *point_out, Fe *finals_out) {
+    check_sumcheck_args(gate, k);
     if (n == 0) {  // nothing to bind: the finals are the single entries
         for (int t = 0; t < k; t++)
-            HIP_CHECK(hipMemcpyAsync(&finals_out[t], tables[t], sizeof(Fe), hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(&finals_out[t], tables[t], sizeof(Fe), hipMemcpyDeviceToHost, ctx.stream));
         ctx.sync();
         return;
     }
+    const ScratchLayout l = sumcheck_layout(k, n);
+    SumcheckScratch sc = sumcheck_carve(ctx.d<uint8_t>("sc.scratch", l.total), l);
+    // the counter lives apart from the carved region, zero once and left zero
+    // by the round kernels
+    sc.counter = ctx.zeroed<unsigned>("sc.counter", 64);
+    Fe *hvals = ctx.h<Fe>("sc.hvals", SC_MAX_NODES);
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&sc.nodes), hvals, 0));
+    sumcheck_rounds(ctx, sc, hvals, true, ctx.profiling, gate, tables, k, n, eq_point, round_fn, rounds_out,
+                    point_out, finals_out);
+}
+
+namespace {
+// Scratch of apg_sumcheck_prove_dev_noalloc: one array per device, held
+// until the process ends; its device address is taken once per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu[NOALLOC_MAX_DEVICES];
+void *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+__global__ void k_zero_words(unsigned *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+}  // namespace
+
+size_t sumcheck_noalloc_limit() { return NOALLOC_SCRATCH_BYTES; }
+
+size_t sumcheck_noalloc_bytes(int k, int n) { return n == 0 ? 0 : sumcheck_layout(k, n).total; }
+
+void sumcheck_noalloc_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) throw ApgError(-2, "no-allocation scratch: device index out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu[device]);
+    if (g_noalloc_ptr[device]) return;
+    void *p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_noalloc_ptr[device] = p;
+}
+
+void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *tables, int k, int n,
+                                   const Fe *eq_point, const RoundFn &round_fn, Fe *rounds_out, Fe *point_out,
+                                   Fe *finals_out) {
+    check_sumcheck_args(gate, k);
+    if (ctx.device < 0 || ctx.device >= NOALLOC_MAX_DEVICES || !g_noalloc_ptr[ctx.device])
+        throw ApgError(-2, "no-allocation scratch: not set up on this device");
+    if (n == 0) {
+        Fe one[SC_MAX_TABLES];
+        for (int t = 0; t < k; t++)
+            HIP_CHECK(hipMemcpyAsync(&one[t], tables[t], sizeof(Fe), hipMemcpyDeviceToHost, ctx.stream));
+        ctx.sync();
+        for (int t = 0; t < k; t++) finals_out[t] = one[t];
+        return;
+    }
+    const ScratchLayout l = sumcheck_layout(k, n);
+    if (l.total > NOALLOC_SCRATCH_BYTES)
+        throw ApgError(-2, "sumcheck (no allocation): " + std::to_string(k) + " tables of 2^" + std::to_string(n) +
+                               " entries need " + std::to_string(l.total) + " bytes of scratch, above the " +
+                               std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte limit");
+    // one call at a time per device: the calls share the device's array
+    std::lock_guard<std::mutex> lk(g_noalloc_mu[ctx.device]);
+    const SumcheckScratch sc = sumcheck_carve(static_cast<uint8_t *>(g_noalloc_ptr[ctx.device]), l);
+    // the array keeps the last call's contents: clear the counter
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx.stream, sc.counter, size_t(64));
+    HIP_CHECK(hipGetLastError());
+    Fe nodes_host[SC_MAX_NODES];
+    sumcheck_rounds(ctx, sc, nodes_host, false, false, gate, tables, k, n, eq_point, round_fn, rounds_out,
+                    point_out, finals_out);
+}
+
+static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, bool mapped_nodes, bool timing,
+                            const Gate &gate, Fe *const *tables, int k, int n, const Fe *eq_point,
+                            const RoundFn &round_fn, Fe *rounds_out, Fe *point_out, Fe *finals_out) {
+    const int D = gate.x_degree();  // degree of G along one variable
+    const int d = gate.degree();    // round polynomial degree
+    const bool product = gate.is_plain_product(k);
+    hipStream_t st = ctx.stream;
+    const size_t len0 = size_t(1) << n;
 
     // suffix-eq tables (k_eq_suffix), split at h0
     const int h0 = std::max(1, std::min(n, (n + 1) / 2));
     const size_t astride = size_t(1) << (h0 - 1), bstride = size_t(1) << (n - h0);
     const size_t asz = (size_t)(h0 - 1) * astride, bsz = (size_t)(n - h0 + 1) * bstride;
-    Fe *rd = ctx.d<Fe>("sc.r", n);
-    Fe *eqA = ctx.d<Fe>("sc.eqA", std::max<size_t>(asz, 1));
-    Fe *eqB = ctx.d<Fe>("sc.eqB", bsz);
+    Fe *rd = sc.r, *eqA = sc.eqA, *eqB = sc.eqB;
     HIP_CHECK(hipMemcpyAsync(rd, eq_point, n * sizeof(Fe), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_eq_suffix, dim3(grid_for(asz + bsz, 256)), dim3(256), 0, st, rd, n, h0, eqA, eqB);
     HIP_CHECK(hipGetLastError());
@@ -427,25 +562,19 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
     // two ping-pong buffers per table (the inputs are read in round 0 and
     // folded into scratch from round 1 on)
     std::vector<Fe *> bufA(k), bufB(k);
-    Fe *poolA = ctx.d<Fe>("sc.poolA", (size_t)k * std::max<size_t>(len0 / 2, 1));
-    Fe *poolB = ctx.d<Fe>("sc.poolB", (size_t)k * std::max<size_t>(len0 / 4, 1));
     for (int t = 0; t < k; t++) {
-        bufA[t] = poolA + (size_t)t * std::max<size_t>(len0 / 2, 1);
-        bufB[t] = poolB + (size_t)t * std::max<size_t>(len0 / 4, 1);
+        bufA[t] = sc.poolA + (size_t)t * std::max<size_t>(len0 / 2, 1);
+        bufB[t] = sc.poolB + (size_t)t * std::max<size_t>(len0 / 4, 1);
     }
     std::vector<const Fe *> cur(k);
     for (int t = 0; t < k; t++) cur[t] = tables[t];
 
-    Fe *partials = ctx.d<Fe>("sc.partials", (size_t)MAX_RBLOCKS * SC_MAX_NODES);
-    unsigned *counter = ctx.zeroed<unsigned>("sc.counter", 64);
-    // the round kernels' last block writes the node sums straight into
-    // pinned host memory; the stream sync after each round makes them visible
-    Fe *hvals = ctx.h<Fe>("sc.hvals", SC_MAX_NODES);
-    Fe *hvals_dev = nullptr;
-    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&hvals_dev), hvals, 0));
+    Fe *partials = sc.partials;
+    unsigned *counter = sc.counter;
+    Fe *hvals_dev = sc.nodes;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (ctx.profiling) {
+    if (timing) {
         HIP_CHECK(hipEventCreate(&ev0));
         HIP_CHECK(hipEventCreate(&ev1));
     }
@@ -487,11 +616,13 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
             eq.lo_bits = 0;
             eq.hi = eqB + (size_t)(s - h0) * bstride;
         }
-        if (ctx.profiling) HIP_CHECK(hipEventRecord(ev0, st));
+        if (timing) HIP_CHECK(hipEventRecord(ev0, st));
         launch_round(ctx, gate, k, product, full, D, tp, m, fold, rho, eq, partials, counter, hvals_dev);
         HIP_CHECK(hipGetLastError());
-        if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
+        if (timing) HIP_CHECK(hipEventRecord(ev1, st));
         const int nn = full ? D + 1 : D;
+        if (!mapped_nodes)
+            HIP_CHECK(hipMemcpyAsync(hvals, hvals_dev, nn * sizeof(Fe), hipMemcpyDeviceToHost, st));
         ctx.sync();
         if (product) {  // the product kernel's node sums carry 2^(-5 k), and 2^-5 more for E = lo hi
             Fe scale = prod_scale;
@@ -499,7 +630,7 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
                 for (int q = 0; q < 5; q++) scale = f_add<Fr>(scale, scale);
             for (int x = 0; x < nn; x++) hvals[x] = f_mul<Fr>(hvals[x], scale);
         }
-        if (ctx.profiling) {
+        if (timing) {
             float ms = 0;
             HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
             ctx.stats.sumcheck_kernel_ms += ms;
@@ -531,7 +662,7 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
             for (int t = 0; t < k; t++) cur[t] = nxt[t];
     }
     // final fold of the two-entry tables with the last challenge
-    std::vector<Fe> last(2 * k);
+    Fe last[2 * SC_MAX_TABLES];
     for (int t = 0; t < k; t++)
         HIP_CHECK(hipMemcpyAsync(&last[2 * t], cur[t], 2 * sizeof(Fe), hipMemcpyDeviceToHost, st));
     ctx.sync();

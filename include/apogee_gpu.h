@@ -313,6 +313,22 @@ int apg_gkr_prove_sumcheck_dev(apg_ctx *ctx, const apg_gkr_list *list, const uin
                                void *const *upper_dev, size_t ncols, apg_round_fn round_fn, void *user,
                                uint8_t *rounds_out, uint8_t *point_out, uint8_t *finals_out);
 
+/* apg_sumcheck_prove_dev with no device allocation in the call. No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Its scratch is one 64 MiB array per device, held from
+ * the first context on that device until the process ends; it needs
+ * 24 B x k x 2^num_vars (the two fold buffers) plus about 0.5 MiB and the
+ * suffix-eq tables, so k x 2^num_vars up to 2^21 entries fit, and a
+ * larger call is APG_ERR_ARG with the sizes in apg_last_error(). Calls on one
+ * device share that array and are serialised by a lock; each starts from
+ * what the last call left, and clears what must start at zero. Results equal
+ * apg_sumcheck_prove_dev's. */
+int apg_sumcheck_prove_dev_noalloc(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
+                                   void *const *tables_dev, size_t k, size_t num_vars,
+                                   const uint8_t *eq_point_mont, apg_round_fn round_fn, void *user,
+                                   uint8_t *rounds_out, uint8_t *point_out, uint8_t *final_evals_out);
+
 /* sumcheck::prove_zerocheck (crates/sumcheck/src/lib.rs:359): n
  * SUMCHECK_CHALLENGE draws for r, the rounds, then final_evals absorbed under
  * SUMCHECK_FINAL_EVALS, all on the given transcript. */
