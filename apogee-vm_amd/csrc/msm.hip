@@ -32,8 +32,6 @@
 //               sum of msm.rs:304-310.
 //  6. host    : groups combine with 2^(c P) doublings (none when the table holds
 //               every window shifted), one batched inversion gives affine.
-#include <hipcub/hipcub.hpp>
-
 #include <algorithm>
 #include <atomic>
 #include <cstdio>
@@ -105,127 +103,446 @@ __device__ __forceinline__ void recode_digits(const uint32_t l[8], int c_rt, int
     }
 }
 
-// Emits one (bucket key, table index | sign) pair per nonzero digit. Each
-// block packs its pairs contiguously at its offset from k_emit_count's scan,
-// so the output is dense (zero digits, 20-70% of all digits for witness-like
-// scalars, never reach the sort) and in scalar order. (One atomic per block
-// on a shared counter instead serialised the ~32K blocks of a 2^20 x 8 shard:
-// 387 us against 157 us without it.) With stage != 0 (windows <=
-// STAGE_WINDOWS) the block first gathers its pairs in LDS, keys then values,
-// and writes them out with consecutive lanes on consecutive words; otherwise
-// each thread writes its own run.
-constexpr int STAGE_WINDOWS = 31;  // 256 x 31 x 8 B + the scan: within 64 KiB of LDS
+// ---- digit sort: a two-level counting sort over the known key range ----
+//
+// A digit's local key is lk = group * B + (|digit| - 1) < G B = lkeys, its key
+// column * lkeys + lk. The sort leaves every key's pairs (key, table index |
+// sign) in one contiguous run, keys ascending, and writes each key's range
+// [off[2k], off[2k + 1]); within a key the order is arbitrary, since the
+// bucket's sum does not depend on it.
+//  1. k_sort_count: blocks of SORT_SPB scalars, never straddling a column,
+//     recode and count their digits per coarse bin (lk >> fine_bits, at most
+//     256 bins per column) in LDS.
+//  2. k_sort_rowscan: for each (column, coarse bin) row, the exclusive scan of
+//     its blocks' counts. k_sort_base: the scan of the rows' totals, the pair
+//     total, the accumulation's slice length (the plan) and the fine pass's
+//     extra chunks, on the device.
+//  3. k_sort_scatter: the blocks recode again, stage their pairs in LDS grouped
+//     by coarse bin and write each bin's segment contiguously.
+//  4. k_sort_fine: each coarse bin is ordered by the key's low fine_bits, in
+//     chunks of at most `cap` pairs, one chunk per workgroup, staged in LDS and
+//     written contiguously per fine bucket; a row of several chunks (the
+//     witness mix puts every scalar equal to one in one bucket per column) is
+//     counted first (k_sort_fine_count) and its chunks claim their places per
+//     bucket with one global atomic each. The ranges of the keys come out of
+//     the same counts. When lkeys <= 256 the coarse bins are the keys and this
+//     pass is not needed (k_sort_base writes the ranges).
+// Nothing is read back to the host before the MSM's end.
+constexpr uint32_t LIGHT_ITEMS = 32;  // more partials than this: a heavy bucket
+constexpr size_t ACC_ROUNDS = 4, MIN_SLICE = 16, MAX_SLICE = 4096;
 
-// Digits per block of scalars (the first pass of the emission): with the
-// blocks' exclusive scan as output offsets, the pairs land densely in scalar
-// order (column-major) with no atomics.
-template <int CT, int WT>
-__global__ void __launch_bounds__(TPB) k_emit_count(const Fe *__restrict__ fr, const uint32_t *__restrict__ u32,
-                                                  uint32_t n, uint32_t K, int c, int windows, int P,
-                                                  uint32_t *__restrict__ counts) {
-    __shared__ uint32_t wsum[TPB / 64];
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + tid;
-    uint32_t cnt = 0;
-    if (idx < (size_t)n * K) {
-        uint32_t l[8];
-        load_scalar(fr, u32, idx, l);
-        recode_digits<CT, WT>(l, c, windows, P, [&](int, int, uint32_t, bool) { cnt++; });
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if (lane == 0) wsum[wave] = cnt;
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t t = 0;
-#pragma unroll
-        for (int w = 0; w < TPB / 64; w++) t += wsum[w];
-        counts[blockIdx.x] = t;
-    }
-}
+constexpr int SORT_TPB = 256;
+constexpr int SORT_SPT = 4;                                // scalars per thread
+constexpr uint32_t SORT_SPB = (uint32_t)SORT_TPB * SORT_SPT;  // scalars per block
+constexpr uint32_t MAX_COARSE = 256;
+constexpr int MAX_FINE_BITS = 12;
+// LDS: a scatter block stages up to SCATTER_CAP pairs (40 KiB); a fine
+// workgroup holds a chunk of at most FINE_CAP pairs, FINE_PER_THREAD to a
+// thread, and stages it (32 KiB, less when its counters need more room, so
+// that it stays within 64 KiB)
+constexpr uint32_t SCATTER_CAP = 5120;
+constexpr uint32_t FINE_CAP = 4096;
+constexpr int FINE_PER_THREAD = FINE_CAP / SORT_TPB;
+constexpr size_t LDS_LIMIT = 65536;
+constexpr size_t MAX_GROUP_COLUMNS = 64;  // keeps k_sort_base's rows <= 64 x 256
 
-template <int CT, int WT>
-__global__ void __launch_bounds__(TPB) k_emit(const Fe *__restrict__ fr,
-                                            const uint32_t *__restrict__ u32, uint32_t n,
-                                            uint32_t K, int c, int windows, int P, int G,
-                                            uint32_t B, uint32_t tab_stride, int stage,
-                                            const uint32_t *__restrict__ offsets,
-                                            uint32_t *__restrict__ keys,
-                                            uint32_t *__restrict__ vals,
-                                            uint32_t *__restrict__ total_out) {
-    extern __shared__ uint32_t staged[];  // [TPB * windows] keys, then as many values
-    __shared__ uint32_t wsum[TPB / 64];
-    __shared__ uint32_t base;
-    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // the scan's last element is the pair count (to pinned host memory)
-    if (blockIdx.x == 0 && tid == 0) *total_out = offsets[gridDim.x];
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + tid;
-    const bool live = idx < (size_t)n * K;
-    uint32_t l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (live) load_scalar(fr, u32, idx, l);
-    uint32_t cnt = 0;
-    if (live) recode_digits<CT, WT>(l, c, windows, P, [&](int, int, uint32_t, bool) { cnt++; });
-    // inclusive scan of the counts: within each wave by shuffles, then across
-    // the block's waves (one barrier)
-    uint32_t incl = cnt;
+struct SortGeom {
+    uint32_t n, K, nbc;  // scalars per column, columns, blocks per column
+    int c, windows, P;
+    uint32_t B, lkeys;   // buckets per group, local keys per column (G B)
+    uint32_t CB;         // coarse bins per column: ceil(lkeys / 2^fine_bits) <= 256
+    int fine_bits;
+    uint32_t tab_stride;
+};
+
+// Device-side plan of one MSM group, written by k_sort_base: the pair total,
+// the accumulation's slice length and slice count, the fine pass's extra
+// chunks, and the heavy-bucket allocation k_classify fills.
+struct HeavyPlan {
+    uint32_t nheavy, npieces;
+};
+struct MsmPlan {
+    uint32_t total, L, nslices, nextra;
+    HeavyPlan heavy;
+};
+
+// Exclusive scan of one value per thread over a SORT_TPB-thread block;
+// `total` receives the block's sum. Every thread of the block calls it.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &total) {
+    __shared__ uint32_t ws[SORT_TPB / 64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         const uint32_t o = __shfl_up(incl, off, 64);
         if ((int)lane >= off) incl += o;
     }
-    if (lane == 63) wsum[wave] = incl;
+    if (lane == 63) ws[wave] = incl;
     __syncthreads();
-    uint32_t btotal = 0;
+    uint32_t before = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < TPB / 64; w++) {
-        if (w < (int)wave) incl += wsum[w];
-        btotal += wsum[w];
+    for (int w = 0; w < SORT_TPB / 64; w++) {
+        if (w < (int)wave) before += ws[w];
+        tot += ws[w];
     }
-    if (tid == 0) base = offsets[blockIdx.x];
-    const uint32_t col = (uint32_t)(idx / n);
-    const uint32_t i = (uint32_t)(idx - (size_t)col * n);
-    const uint32_t key0 = col * (uint32_t)G;
-    if (stage) {
-        const uint32_t cap = TPB * (uint32_t)windows;
-        uint32_t pos = incl - cnt;
-        if (live && cnt)
-            recode_digits<CT, WT>(l, c, windows, P, [&](int copy, int group, uint32_t mag, bool neg) {
-                staged[pos] = (key0 + (uint32_t)group) * B + (mag - 1);
-                staged[cap + pos] = ((uint32_t)copy * tab_stride + i) | (neg ? SIGN : 0u);
-                pos++;
-            });
-        __syncthreads();
-        const uint32_t b0 = base;
-        for (uint32_t e = tid; e < btotal; e += TPB) {
-            keys[b0 + e] = staged[e];
-            vals[b0 + e] = staged[cap + e];
+    __syncthreads();  // ws is reused by the next call
+    total = tot;
+    return before + incl - v;
+}
+
+// A block's scalars, loaded before any is recoded so the loads overlap.
+__device__ __forceinline__ void load_block_scalars(const Fe *fr, const uint32_t *u32, const SortGeom &g,
+                                                   uint32_t col, uint32_t i0, uint32_t (&l)[SORT_SPT][8]) {
+#pragma unroll
+    for (int s = 0; s < SORT_SPT; s++) {
+        const uint32_t i = i0 + (uint32_t)s * SORT_TPB;
+#pragma unroll
+        for (int w = 0; w < 8; w++) l[s][w] = 0;
+        if (i < g.n) load_scalar(fr, u32, (size_t)col * g.n + i, l[s]);
+    }
+}
+
+template <int CT, int WT>
+__global__ void __launch_bounds__(SORT_TPB) k_sort_count(const Fe *__restrict__ fr, const uint32_t *__restrict__ u32,
+                                                       SortGeom g, uint32_t *__restrict__ counts) {
+    __shared__ uint32_t hist[MAX_COARSE];
+    const uint32_t col = blockIdx.x / g.nbc, b = blockIdx.x - col * g.nbc;
+    for (uint32_t i = threadIdx.x; i < g.CB; i += SORT_TPB) hist[i] = 0;
+    const uint32_t i0 = b * SORT_SPB + threadIdx.x;
+    uint32_t l[SORT_SPT][8];
+    load_block_scalars(fr, u32, g, col, i0, l);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SORT_SPT; s++) {
+        if (i0 + (uint32_t)s * SORT_TPB >= g.n) break;
+        recode_digits<CT, WT>(l[s], g.c, g.windows, g.P, [&](int, int group, uint32_t mag, bool) {
+            atomicAdd(&hist[((uint32_t)group * g.B + (mag - 1)) >> g.fine_bits], 1u);
+        });
+    }
+    __syncthreads();
+    for (uint32_t bin = threadIdx.x; bin < g.CB; bin += SORT_TPB)
+        counts[((size_t)col * g.CB + bin) * g.nbc + b] = hist[bin];
+}
+
+// Row r = column * CB + bin: counts[r][0..nbc) becomes its exclusive scan,
+// rowtot[r] its sum. A row of more than cap pairs (fine pass, several chunks)
+// gets its fine counters and claims cleared.
+__global__ void __launch_bounds__(SORT_TPB) k_sort_rowscan(uint32_t *__restrict__ counts, uint32_t nbc,
+                                                         uint32_t *__restrict__ rowtot, int fine_bits, uint32_t cap,
+                                                         uint32_t *__restrict__ fcount, uint32_t *__restrict__ fclaim) {
+    uint32_t *row = counts + (size_t)blockIdx.x * nbc;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nbc; base += SORT_TPB) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nbc ? row[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(v, tot);
+        if (i < nbc) row[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) rowtot[blockIdx.x] = carry;
+    if (fine_bits > 0 && carry > cap) {
+        const size_t FB = size_t(1) << fine_bits, f0 = (size_t)blockIdx.x * FB;
+        for (size_t f = threadIdx.x; f < FB; f += SORT_TPB) {
+            fcount[f0 + f] = 0;
+            fclaim[f0 + f] = 0;
         }
+    }
+}
+
+// One block: rowbase[r] = the pairs of the rows before r (every row of the
+// group in column-major, bin-ascending order: the final key order), the plan,
+// the fine pass's extra chunks (chunks 1.. of every row of more than cap
+// pairs, as (row, chunk)), and, when rows are keys, the key ranges.
+// `resident` is the number of threads k_accumulate keeps resident; slices are
+// at least MIN_SLICE long, about ACC_ROUNDS per resident thread and an eighth
+// of the mean bucket, a multiple of 4, and never more than slice_cap.
+__global__ void __launch_bounds__(SORT_TPB) k_sort_base(const uint32_t *__restrict__ rowtot, uint32_t rows,
+                                                      uint32_t nkeys, uint32_t resident, uint32_t slice_cap,
+                                                      int rows_are_keys, uint32_t cap,
+                                                      uint32_t *__restrict__ rowbase, uint32_t *__restrict__ off,
+                                                      uint2 *__restrict__ extra, MsmPlan *__restrict__ plan,
+                                                      uint32_t *__restrict__ total_host) {
+    uint32_t carry = 0, xcarry = 0;
+    for (uint32_t base = 0; base < rows; base += SORT_TPB) {
+        const uint32_t r = base + threadIdx.x;
+        const uint32_t v = r < rows ? rowtot[r] : 0u;
+        const uint32_t xc = (!rows_are_keys && v > cap) ? (v + cap - 1) / cap - 1 : 0u;
+        uint32_t tot, xtot;
+        const uint32_t ex = carry + block_exclusive_scan(v, tot);
+        const uint32_t xex = xcarry + block_exclusive_scan(xc, xtot);
+        if (r < rows) {
+            rowbase[r] = ex;
+            if (rows_are_keys) {
+                off[2 * (size_t)r] = ex;
+                off[2 * (size_t)r + 1] = ex + v;
+            }
+            for (uint32_t c = 0; c < xc; c++) extra[xex + c] = make_uint2(r, c + 1);
+        }
+        carry += tot;
+        xcarry += xtot;
+    }
+    if (threadIdx.x == 0) {
+        const uint64_t total = carry;
+        uint64_t L = MIN_SLICE;
+        L = max(L, (total + (uint64_t)ACC_ROUNDS * resident - 1) / ((uint64_t)ACC_ROUNDS * resident));
+        L = max(L, total / (8 * (uint64_t)max(nkeys, 1u)));
+        L = min<uint64_t>(L, MAX_SLICE);
+        L = max(L, (total + slice_cap - 1) / slice_cap);
+        L = (L + 3) & ~uint64_t(3);
+        plan->total = (uint32_t)total;
+        plan->L = (uint32_t)L;
+        plan->nslices = (uint32_t)((total + L - 1) / L);
+        plan->nextra = xcarry;
+        plan->heavy.nheavy = 0;
+        plan->heavy.npieces = 0;
+        *total_host = (uint32_t)total;
+    }
+}
+
+// The block's pairs are staged in LDS grouped by coarse bin (each bin's
+// region at its exclusive prefix of the block's counts) and then written out
+// with consecutive lanes on consecutive pairs of a bin, so every bin's
+// segment of the block is one contiguous write. Bins are taken in ranges
+// whose pairs fit the stage, one recode pass per range (one pass unless the
+// block holds more than SCATTER_CAP pairs); a single bin above the stage is
+// written directly.
+template <int CT, int WT>
+__global__ void __launch_bounds__(SORT_TPB) k_sort_scatter(const Fe *__restrict__ fr,
+                                                         const uint32_t *__restrict__ u32, SortGeom g,
+                                                         const uint32_t *__restrict__ counts,
+                                                         const uint32_t *__restrict__ rowtot,
+                                                         const uint32_t *__restrict__ rowbase,
+                                                         uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    __shared__ uint2 stage[SCATTER_CAP];
+    __shared__ uint32_t cnt[MAX_COARSE], lbase[MAX_COARSE], gbase[MAX_COARSE], cur[MAX_COARSE];
+    __shared__ uint32_t range_count;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t col = blockIdx.x / g.nbc, b = blockIdx.x - col * g.nbc;
+    const uint32_t i0 = b * SORT_SPB + tid;
+    for (uint32_t bin = tid; bin < g.CB; bin += SORT_TPB) {
+        const size_t r = (size_t)col * g.CB + bin;
+        const uint32_t p = counts[r * g.nbc + b];
+        const uint32_t nx = b + 1 < g.nbc ? counts[r * g.nbc + b + 1] : rowtot[r];
+        cnt[bin] = nx - p;
+        gbase[bin] = rowbase[r] + p;
+    }
+    __syncthreads();
+    const uint32_t key0 = col * g.lkeys;
+    uint32_t lo = 0;
+    while (lo < g.CB) {  // uniform: every thread derives the same ranges
+        // bins [lo, hi): as many as fit the stage, at least one (CB <= SORT_TPB:
+        // one bin per thread)
+        const uint32_t bin = tid;
+        const bool live = bin >= lo && bin < g.CB;
+        const uint32_t v = live ? cnt[bin] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(v, tot);
+        const int nfit = __syncthreads_count(live && ex + v <= SCATTER_CAP);
+        const uint32_t hi = lo + (uint32_t)max(nfit, 1);
+        if (bin == hi - 1) range_count = ex + v;
+        if (live && bin < hi) {
+            lbase[bin] = ex;
+            cur[bin] = ex;
+        }
+        __syncthreads();
+        const uint32_t staged = range_count;
+        const bool direct = staged > SCATTER_CAP;  // one bin larger than the stage
+        if (direct && tid == 0) cur[lo] = gbase[lo];
+        __syncthreads();
+        // one scalar at a time (unrolled, the recode's registers quadruple)
+#pragma unroll 1
+        for (int sc = 0; sc < SORT_SPT; sc++) {
+            const uint32_t i = i0 + (uint32_t)sc * SORT_TPB;
+            if (i >= g.n) break;
+            uint32_t l[8];
+            load_scalar(fr, u32, (size_t)col * g.n + i, l);
+            recode_digits<CT, WT>(l, g.c, g.windows, g.P, [&](int copy, int group, uint32_t mag, bool neg) {
+                const uint32_t lk = (uint32_t)group * g.B + (mag - 1);
+                const uint32_t bn = lk >> g.fine_bits;
+                if (bn < lo || bn >= hi) return;
+                const uint32_t val = ((uint32_t)copy * g.tab_stride + i) | (neg ? SIGN : 0u);
+                const uint32_t slot = atomicAdd(&cur[bn], 1u);
+                if (direct) {
+                    keys[slot] = key0 + lk;
+                    vals[slot] = val;
+                } else {
+                    stage[slot] = make_uint2(key0 + lk, val);
+                }
+            });
+        }
+        __syncthreads();
+        if (!direct)
+            for (uint32_t e = tid; e < staged; e += SORT_TPB) {
+                const uint2 kv = stage[e];
+                const uint32_t bn = (kv.x - key0) >> g.fine_bits;
+                const uint32_t dest = gbase[bn] + (e - lbase[bn]);
+                keys[dest] = kv.x;
+                vals[dest] = kv.y;
+            }
+        __syncthreads();
+        lo = hi;
+    }
+}
+
+// The fine pass's work items: workgroup w < rows takes chunk 0 of row w; the
+// others take the plan's extra chunks, striding over them. Calls f(row,
+// chunk) per item.
+template <class F>
+__device__ __forceinline__ void fine_items(uint32_t rows, const MsmPlan *plan, const uint2 *extra, F &&f) {
+    const uint32_t w = blockIdx.x;
+    if (w < rows) {
+        f(w, 0u);
         return;
     }
-    __syncthreads();
-    if (!live || !cnt) return;
-    uint32_t pos = base + incl - cnt;
-    recode_digits<CT, WT>(l, c, windows, P, [&](int copy, int group, uint32_t mag, bool neg) {
-        keys[pos] = (key0 + (uint32_t)group) * B + (mag - 1);
-        vals[pos] = ((uint32_t)copy * tab_stride + i) | (neg ? SIGN : 0u);
-        pos++;
+    const uint32_t nx = plan->nextra, stride = gridDim.x - rows;
+    for (uint32_t x = w - rows; x < nx; x += stride) {
+        const uint2 it = extra[x];
+        f(it.x, it.y);
+    }
+}
+
+// A chunk of pairs into registers (FINE_PER_THREAD to a thread, m of them
+// live).
+__device__ __forceinline__ void load_chunk(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t at,
+                                           uint32_t m, uint2 (&v)[FINE_PER_THREAD]) {
+#pragma unroll
+    for (int q = 0; q < FINE_PER_THREAD; q++) {
+        const uint32_t e = threadIdx.x + (uint32_t)q * SORT_TPB;
+        v[q] = make_uint2(0u, 0u);
+        if (e < m) v[q] = make_uint2(keys_in[at + e], vals_in[at + e]);
+    }
+}
+
+// Fine counts of every row of more than cap pairs: each chunk counts in LDS
+// and adds its counts to the row's (global atomics, one per bucket).
+__global__ void __launch_bounds__(SORT_TPB) k_sort_fine_count(const uint32_t *__restrict__ keys_in, SortGeom g,
+                                                            uint32_t cap, uint32_t rows,
+                                                            const uint32_t *__restrict__ rowbase,
+                                                            const uint32_t *__restrict__ rowtot,
+                                                            const MsmPlan *__restrict__ plan,
+                                                            const uint2 *__restrict__ extra,
+                                                            uint32_t *__restrict__ fcount) {
+    extern __shared__ uint32_t sm[];
+    const uint32_t FB = 1u << g.fine_bits, mask = FB - 1;
+    fine_items(rows, plan, extra, [&](uint32_t r, uint32_t chunk) {
+        const uint32_t E = rowtot[r];
+        if (E <= cap) return;  // uniform per workgroup
+        const uint32_t c0 = chunk * cap, m = min(cap, E - c0);
+        __syncthreads();
+        for (uint32_t f = threadIdx.x; f < FB; f += SORT_TPB) sm[f] = 0;
+        __syncthreads();
+        uint32_t k[FINE_PER_THREAD];
+#pragma unroll
+        for (int q = 0; q < FINE_PER_THREAD; q++) {  // every load in flight before the counts
+            const uint32_t e = threadIdx.x + (uint32_t)q * SORT_TPB;
+            k[q] = 0;
+            if (e < m) k[q] = keys_in[rowbase[r] + c0 + e];
+        }
+#pragma unroll
+        for (int q = 0; q < FINE_PER_THREAD; q++)
+            if (threadIdx.x + (uint32_t)q * SORT_TPB < m) atomicAdd(&sm[k[q] & mask], 1u);
+        __syncthreads();
+        for (uint32_t f = threadIdx.x; f < FB; f += SORT_TPB)
+            if (sm[f]) atomicAdd(&fcount[(size_t)r * FB + f], sm[f]);
     });
 }
 
-// off[2k], off[2k + 1] = the range of key k in the sorted key array, which
-// holds each key's entries contiguously (sorted fully, or by the bucket bits
-// alone with the columns in emission order within a bucket). One thread per
-// entry marks where the key changes; keys without entries keep the (0, 0)
-// the array is cleared to.
-__global__ void k_bounds(const uint32_t *__restrict__ skeys, uint32_t total, uint32_t *__restrict__ off) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const uint32_t k = skeys[i];
-    if (i == 0 || skeys[i - 1] != k) {
-        off[2 * (size_t)k] = i;
-        if (i > 0) off[2 * (size_t)skeys[i - 1] + 1] = i;
-    }
-    if (i == total - 1) off[2 * (size_t)k + 1] = total;
+// Dynamic LDS: 3 x 2^fine_bits words of counters, then cap <= FINE_CAP pairs
+// (cap a multiple of SORT_TPB).
+__global__ void __launch_bounds__(SORT_TPB) k_sort_fine(const uint32_t *__restrict__ keys_in,
+                                                      const uint32_t *__restrict__ vals_in, SortGeom g, uint32_t cap,
+                                                      uint32_t rows, const uint32_t *__restrict__ rowbase,
+                                                      const uint32_t *__restrict__ rowtot,
+                                                      const MsmPlan *__restrict__ plan,
+                                                      const uint2 *__restrict__ extra,
+                                                      const uint32_t *__restrict__ fcount,
+                                                      uint32_t *__restrict__ fclaim, uint32_t *__restrict__ skeys,
+                                                      uint32_t *__restrict__ entries, uint32_t *__restrict__ off) {
+    extern __shared__ uint32_t sm[];
+    const uint32_t FB = 1u << g.fine_bits, mask = FB - 1;
+    uint32_t *rstart = sm;                                 // the row's bucket starts (several chunks)
+    uint32_t *lcur = sm + FB;                              // the chunk's counts, then its cursors
+    uint32_t *dl = sm + 2 * FB;                            // stage index -> destination, per bucket
+    uint2 *stage = reinterpret_cast<uint2 *>(sm + 3 * FB);  // cap pairs
+    const uint32_t tid = threadIdx.x;
+    fine_items(rows, plan, extra, [&](uint32_t r, uint32_t chunk) {
+        const uint32_t col = r / g.CB, bin = r - col * g.CB;
+        const uint32_t start = rowbase[r], E = rowtot[r];
+        const uint32_t lk0 = bin << g.fine_bits;
+        const bool single = E <= cap;
+        const uint32_t c0 = chunk * cap, m = min(cap, E - c0);
+        __syncthreads();
+        for (uint32_t f = tid; f < FB; f += SORT_TPB) lcur[f] = 0;
+        uint2 v[FINE_PER_THREAD];
+        load_chunk(keys_in, vals_in, start + c0, m, v);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < FINE_PER_THREAD; q++)
+            if (tid + (uint32_t)q * SORT_TPB < m) atomicAdd(&lcur[v[q].x & mask], 1u);
+        __syncthreads();
+        // the keys' ranges: from the chunk's counts (a one-chunk row) or the
+        // row's (written by chunk 0); the row's bucket starts in rstart
+        if (single || chunk == 0) {
+            uint32_t carry = start;
+            for (uint32_t b0 = 0; b0 < FB; b0 += SORT_TPB) {
+                const uint32_t f = b0 + tid;
+                const uint32_t cnt = f < FB ? (single ? lcur[f] : fcount[(size_t)r * FB + f]) : 0u;
+                uint32_t tot;
+                const uint32_t at = carry + block_exclusive_scan(cnt, tot);
+                if (f < FB) {
+                    rstart[f] = at;
+                    if (lk0 + f < g.lkeys) {
+                        const size_t key = (size_t)col * g.lkeys + lk0 + f;
+                        off[2 * key] = at;
+                        off[2 * key + 1] = at + cnt;
+                    }
+                }
+                carry += tot;
+            }
+        } else {
+            uint32_t carry = start;
+            for (uint32_t b0 = 0; b0 < FB; b0 += SORT_TPB) {
+                const uint32_t f = b0 + tid;
+                const uint32_t cnt = f < FB ? fcount[(size_t)r * FB + f] : 0u;
+                uint32_t tot;
+                const uint32_t at = carry + block_exclusive_scan(cnt, tot);
+                if (f < FB) rstart[f] = at;
+                carry += tot;
+            }
+        }
+        __syncthreads();
+        // chunk cursors (exclusive prefix of the chunk's counts) and where each
+        // bucket's staged pairs go: right after the row's earlier pairs of the
+        // bucket (one chunk: at once; several: claimed per bucket)
+        {
+            uint32_t carry = 0;
+            for (uint32_t b0 = 0; b0 < FB; b0 += SORT_TPB) {
+                const uint32_t f = b0 + tid;
+                const uint32_t cnt = f < FB ? lcur[f] : 0u;
+                uint32_t tot;
+                const uint32_t lb = carry + block_exclusive_scan(cnt, tot);
+                if (f < FB) {
+                    uint32_t at = rstart[f];
+                    if (!single && cnt) at += atomicAdd(&fclaim[(size_t)r * FB + f], cnt);
+                    dl[f] = at - lb;
+                    lcur[f] = lb;
+                }
+                carry += tot;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < FINE_PER_THREAD; q++)
+            if (tid + (uint32_t)q * SORT_TPB < m) stage[atomicAdd(&lcur[v[q].x & mask], 1u)] = v[q];
+        __syncthreads();
+        for (uint32_t e = tid; e < m; e += SORT_TPB) {
+            const uint2 kv = stage[e];
+            const uint32_t dest = e + dl[kv.x & mask];
+            skeys[dest] = kv.x;
+            entries[dest] = kv.y;
+        }
+    });
 }
 
 __device__ __forceinline__ Affine load_affine(const Affine *__restrict__ p, uint32_t idx) {
@@ -249,10 +566,13 @@ __device__ __forceinline__ Affine load_affine(const Affine *__restrict__ p, uint
 // slice; unused slots hold the identity. Bucket k's partials are then the
 // contiguous part[2 t0 + 1 .. 2 t1], t0 and t1 the slices of its first and last
 // entries.
+// The launch covers the plan's largest possible slice count; threads past
+// the pair total (plan->total, plan->L: k_sort_base) return at once.
 __global__ void __launch_bounds__(TPB) k_accumulate(const uint32_t *__restrict__ skeys,
-                                                  const uint32_t *__restrict__ entries, uint32_t total,
-                                                  uint32_t L, const Affine *__restrict__ bases,
+                                                  const uint32_t *__restrict__ entries,
+                                                  const MsmPlan *__restrict__ plan, const Affine *__restrict__ bases,
                                                   Xyzz29 *__restrict__ buckets, Xyzz29 *__restrict__ part) {
+    const uint32_t total = plan->total, L = plan->L;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if ((size_t)t * L >= total) return;
     const uint32_t lo = t * L, hi = lo + min(total - lo, L);
@@ -317,14 +637,14 @@ __device__ __forceinline__ Span merge_span(const uint32_t *__restrict__ off, uin
     return {2 * t0 + 1, 2 * (t1 - t0)};
 }
 
-constexpr uint32_t LIGHT_ITEMS = 32;  // more partials than this: a heavy bucket
-constexpr size_t ACC_ROUNDS = 4, MIN_SLICE = 16, MAX_SLICE = 4096;
 
 // One thread per bucket: empty buckets become the identity, buckets with at
 // most LIGHT_ITEMS partials are summed here, heavier ones by the pieces below.
 __global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restrict__ off, uint32_t nkeys,
-                                                       uint32_t L, const Xyzz29 *__restrict__ part,
+                                                       const MsmPlan *__restrict__ plan,
+                                                       const Xyzz29 *__restrict__ part,
                                                        Xyzz29 *__restrict__ buckets) {
+    const uint32_t L = plan->L;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
     if (off[2 * (size_t)k] == off[2 * (size_t)k + 1]) {
@@ -359,14 +679,12 @@ __device__ __forceinline__ Xyzz29 shfl_xor_xyzz(const Xyzz29 &p, int mask) {
 // the buckets.
 constexpr uint32_t PIECE = 4 * TPB;
 
-struct HeavyPlan {
-    uint32_t nheavy, npieces;
-};
-
 // heavy[h] = (bucket, first piece, pieces) of heavy bucket h; piece_of[p] =
 // (heavy index, piece within the bucket).
-__global__ void k_classify(const uint32_t *__restrict__ off, uint32_t nkeys, uint32_t L, HeavyPlan *plan,
+__global__ void k_classify(const uint32_t *__restrict__ off, uint32_t nkeys, MsmPlan *mplan,
                            uint4 *__restrict__ heavy, uint2 *__restrict__ piece_of) {
+    const uint32_t L = mplan->L;
+    HeavyPlan *plan = &mplan->heavy;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
     const uint32_t q = merge_span(off, k, L).count;
@@ -391,14 +709,15 @@ __device__ __forceinline__ Xyzz29 block_sum256(Xyzz29 acc, Xyzz29 *lds) {
     return r;
 }
 
-__global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restrict__ off, uint32_t L,
+__global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restrict__ off,
                                                        const Xyzz29 *__restrict__ part,
-                                                       const HeavyPlan *plan,
+                                                       const MsmPlan *mplan,
                                                        const uint4 *__restrict__ heavy,
                                                        const uint2 *__restrict__ piece_of,
                                                        Xyzz29 *__restrict__ piece_sums) {
     __shared__ Xyzz29 lds[TPB / 64];
-    const uint32_t np = plan->npieces, tid = threadIdx.x;
+    const uint32_t L = mplan->L;
+    const uint32_t np = mplan->heavy.npieces, tid = threadIdx.x;
     for (uint32_t pc = blockIdx.x; pc < np; pc += gridDim.x) {
         const uint2 po = piece_of[pc];
         const Span sp = merge_span(off, heavy[po.x].x, L);
@@ -411,12 +730,12 @@ __global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restr
     }
 }
 
-__global__ void __launch_bounds__(TPB, 2) k_heavy_final(const HeavyPlan *plan,
+__global__ void __launch_bounds__(TPB, 2) k_heavy_final(const MsmPlan *mplan,
                                                       const uint4 *__restrict__ heavy,
                                                       const Xyzz29 *__restrict__ piece_sums,
                                                       Xyzz29 *__restrict__ buckets) {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= plan->nheavy) return;
+    if (h >= mplan->heavy.nheavy) return;
     const uint4 e = heavy[h];
     Xyzz29 acc = xyzz29_identity();
     for (uint32_t j = 0; j < e.z; j++) acc = xyzz29_add(acc, ld29(piece_sums + e.y + j));
@@ -594,11 +913,211 @@ std::unique_ptr<BaseTable> msm_expand_bases(Context &ctx, const Affine *src, siz
     return t;
 }
 
-static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *fr_dev,
-                          const uint32_t *u32_dev, Affine *out_host);
+namespace {
 
-// Columns are processed in groups of at most MAX_PAIRS digit slots, which
-// bounds the scratch one call holds (keys, entries and their sorted copies).
+// Host-side record of one column group, kept until the MSM's single sync.
+struct GroupRun {
+    size_t k0, K;
+    MsmShape sh;
+    size_t w_off;              // first window sum of this group in the pinned array
+    uint32_t *total_host;      // the group's pair count (pinned)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+int ceil_log2(uint64_t x) {
+    int b = 0;
+    while ((uint64_t(1) << b) < x) b++;
+    return b;
+}
+
+// Enqueues one column group: digits, sort, accumulation, bucket sums and the
+// window sums (written by k_weighted into pinned host memory at w_dev).
+void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g, const Fe *fr_dev,
+                       const uint32_t *u32_dev, Xyzz *w_dev, uint32_t *total_dev) {
+    const size_t K = g.K;
+    MsmShape &sh = g.sh;
+    sh.c = tab.c;
+    sh.windows = msm_window_count(fr_dev ? 254 : 32, sh.c);
+    sh.P = std::min(tab.copies, sh.windows);
+    sh.G = (sh.windows + sh.P - 1) / sh.P;
+    sh.B = 1u << (sh.c - 1);
+    const uint64_t lkeys = (uint64_t)sh.G * sh.B;
+    const size_t nkeys64 = K * lkeys;
+    if (nkeys64 >= (size_t(1) << 31)) throw ApgError(-2, "msm: too many buckets");
+    const uint32_t nkeys = (uint32_t)nkeys64;
+    hipStream_t st = ctx.stream;
+
+    // 1-2. the digit sort (k_sort_*), geometry first
+    SortGeom geo;
+    geo.n = (uint32_t)n;
+    geo.K = (uint32_t)K;
+    geo.nbc = (uint32_t)((n + SORT_SPB - 1) / SORT_SPB);
+    geo.c = sh.c;
+    geo.windows = sh.windows;
+    geo.P = sh.P;
+    geo.B = sh.B;
+    geo.lkeys = (uint32_t)lkeys;
+    geo.fine_bits = std::max(0, ceil_log2(lkeys) - 8);
+    if (geo.fine_bits > MAX_FINE_BITS) throw ApgError(-2, "msm: window geometry outside the sort's range");
+    geo.CB = (uint32_t)((lkeys + (uint64_t(1) << geo.fine_bits) - 1) >> geo.fine_bits);
+    geo.tab_stride = (uint32_t)tab.stride;
+    const bool fine = geo.fine_bits > 0;
+    const uint32_t rows = (uint32_t)(K * geo.CB);
+    const size_t max_pairs = n * K * (size_t)sh.windows;
+
+    uint32_t *counts = ctx.d<uint32_t>("msm.counts", (size_t)rows * geo.nbc);
+    uint32_t *rowtot = ctx.d<uint32_t>("msm.rowtot", rows);
+    uint32_t *rowbase = ctx.d<uint32_t>("msm.rowbase", rows);
+    MsmPlan *plan = ctx.d<MsmPlan>("msm.plan", 1);
+    // 4 words of slack: the accumulation reads keys and entries as 16-byte vectors
+    uint32_t *skeys = ctx.d<uint32_t>("msm.skeys", max_pairs + 4);
+    uint32_t *entries = ctx.d<uint32_t>("msm.entries", max_pairs + 4);
+    uint32_t *keys_c = fine ? ctx.d<uint32_t>("msm.keys_c", max_pairs) : skeys;
+    uint32_t *vals_c = fine ? ctx.d<uint32_t>("msm.vals_c", max_pairs) : entries;
+    uint32_t *off = ctx.d<uint32_t>("msm.off", 2 * (size_t)nkeys);
+    // the fine pass: chunk capacity FINE_CAP, or what 64 KiB leaves beside the
+    // counters (256 bytes kept for the scans' static words); per-row fine
+    // counts and claims, and the list of extra chunks
+    const size_t counters = 3 * (size_t(4) << geo.fine_bits);
+    const uint32_t cap =
+        (uint32_t)std::min<size_t>(FINE_CAP, (LDS_LIMIT - 256 - counters) / sizeof(uint2) / SORT_TPB * SORT_TPB);
+    const size_t xmax = max_pairs / cap + 1;
+    const uint32_t xgroups = (uint32_t)std::min<size_t>(xmax, 4 * (size_t)ctx.cu_count);
+    uint32_t *fcount = fine ? ctx.d<uint32_t>("msm.fcount", (size_t)rows << geo.fine_bits) : nullptr;
+    uint32_t *fclaim = fine ? ctx.d<uint32_t>("msm.fclaim", (size_t)rows << geo.fine_bits) : nullptr;
+    uint2 *extra = ctx.d<uint2>("msm.extra", xmax);
+
+    // k_accumulate holds 3 waves per SIMD; slices follow k_sort_base's rule,
+    // so at most ACC_ROUNDS per resident thread (+1) and at most one per
+    // MIN_SLICE pairs
+    const size_t resident = (size_t)ctx.cu_count * 4 * 3 * 64;
+    const size_t slice_cap = std::min(ACC_ROUNDS * resident + 1, max_pairs / MIN_SLICE + 1);
+
+    const unsigned sblocks = (unsigned)(K * geo.nbc);
+#define APG_SORT(CT, WT)                                                                                        \
+    do {                                                                                                        \
+        hipLaunchKernelGGL((k_sort_count<CT, WT>), dim3(sblocks), dim3(SORT_TPB), 0, st, fr_dev, u32_dev, geo,   \
+                           counts);                                                                             \
+        hipLaunchKernelGGL(k_sort_rowscan, dim3(rows), dim3(SORT_TPB), 0, st, counts, geo.nbc, rowtot,           \
+                           geo.fine_bits, cap, fcount, fclaim);                                                 \
+        hipLaunchKernelGGL(k_sort_base, dim3(1), dim3(SORT_TPB), 0, st, rowtot, rows, nkeys, (uint32_t)resident, \
+                           (uint32_t)slice_cap, fine ? 0 : 1, cap, rowbase, off, extra, plan, total_dev);       \
+        hipLaunchKernelGGL((k_sort_scatter<CT, WT>), dim3(sblocks), dim3(SORT_TPB), 0, st, fr_dev, u32_dev, geo, \
+                           counts, rowtot, rowbase, keys_c, vals_c);                                            \
+    } while (0)
+#define APG_SORT_C(CT)                                                  \
+    case CT:                                                            \
+        if (fr_dev)                                                     \
+            APG_SORT(CT, msm_window_count(254, CT));                    \
+        else                                                            \
+            APG_SORT(CT, msm_window_count(32, CT));                     \
+        break;
+    // compile-time window widths for the built-in size classes, Fr or u32 scalars
+    switch (sh.c) {
+        APG_SORT_C(8)
+        APG_SORT_C(9)
+        APG_SORT_C(12)
+        APG_SORT_C(14)
+        APG_SORT_C(17)
+        default: APG_SORT(0, 0); break;
+    }
+#undef APG_SORT_C
+#undef APG_SORT
+    if (fine) {
+        const dim3 fg(rows + xgroups);
+        hipLaunchKernelGGL(k_sort_fine_count, fg, dim3(SORT_TPB), size_t(4) << geo.fine_bits, st, keys_c, geo, cap,
+                           rows, rowbase, rowtot, plan, extra, fcount);
+        hipLaunchKernelGGL(k_sort_fine, fg, dim3(SORT_TPB), counters + (size_t)cap * sizeof(uint2), st, keys_c,
+                           vals_c, geo, cap, rows, rowbase, rowtot, plan, extra, fcount, fclaim, skeys, entries,
+                           off);
+    }
+    HIP_CHECK(hipGetLastError());
+
+    // 3. accumulate over equal slices of the sorted entries
+    Xyzz29 *buckets = ctx.d<Xyzz29>("msm.buckets", nkeys);
+    Xyzz29 *part = ctx.d<Xyzz29>("msm.part", 2 * slice_cap);
+    if (ctx.profiling) {
+        HIP_CHECK(hipEventCreate(&g.ev0));
+        HIP_CHECK(hipEventCreate(&g.ev1));
+        HIP_CHECK(hipEventRecord(g.ev0, st));
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(slice_cap, TPB)), dim3(TPB), 0, st, skeys, entries, plan,
+                       tab.points, buckets, part);
+    HIP_CHECK(hipGetLastError());
+    if (ctx.profiling) HIP_CHECK(hipEventRecord(g.ev1, st));
+
+    // 4. buckets: empty ones set, crossing ones merged from their partials. A
+    // heavy bucket spans more than LIGHT_ITEMS / 2 slices, each slice belongs
+    // to at most one heavy span, and spans hold 2 partials per slice.
+    const uint32_t max_heavy = (uint32_t)std::min<size_t>(nkeys, slice_cap / (LIGHT_ITEMS / 2) + 1);
+    const uint32_t max_pieces = (uint32_t)(2 * slice_cap / PIECE + max_heavy + 1);
+    uint4 *heavy = ctx.d<uint4>("msm.heavy", max_heavy);
+    uint2 *piece_of = ctx.d<uint2>("msm.piece_of", max_pieces);
+    Xyzz29 *piece_sums = ctx.d<Xyzz29>("msm.piece_sums", max_pieces);
+    hipLaunchKernelGGL(k_classify, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, off, nkeys, plan, heavy, piece_of);
+    // heavy pieces (a few latency-bound blocks) on the side stream, queued
+    // first, beside the light buckets (throughput-bound) here
+    HIP_CHECK(hipEventRecord(ctx.ev_fork, st));
+    HIP_CHECK(hipStreamWaitEvent(ctx.side, ctx.ev_fork, 0));
+    hipLaunchKernelGGL(k_heavy_pieces, dim3(std::min<uint32_t>(max_pieces, 4u * ctx.cu_count)), dim3(TPB), 0,
+                       ctx.side, off, part, plan, heavy, piece_of, piece_sums);
+    hipLaunchKernelGGL(k_heavy_final, dim3(grid_for(max_heavy, TPB)), dim3(TPB), 0, ctx.side, plan, heavy,
+                       piece_sums, buckets);
+    HIP_CHECK(hipEventRecord(ctx.ev_join, ctx.side));
+    hipLaunchKernelGGL(k_bucket_light, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, off, nkeys, plan, part,
+                       buckets);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamWaitEvent(st, ctx.ev_join, 0));
+
+    // 5. reduce: per problem (column, group): W = sum_b (b + 1) S_b, written
+    // by k_weighted straight into pinned host memory
+    const uint32_t nprob = (uint32_t)(K * sh.G);
+    const uint32_t B = sh.B;
+    if (B > (uint32_t)TPB) {
+        const uint32_t R = B / TPB;
+        Xyzz29 *Z = ctx.d<Xyzz29>("msm.Z", (size_t)nprob * R);
+        Xyzz29 *U = ctx.d<Xyzz29>("msm.U", (size_t)nprob * TPB);
+        const size_t threads = (size_t)nprob * (R + TPB) * RGS;
+        hipLaunchKernelGGL(k_reduce_rows, dim3(grid_for(threads, TPB)), dim3(TPB), 0, st, buckets, R, nprob, Z, U);
+        // WS/TOT of the Z_r (blocks 0..nprob) and of the U_t (nprob..2 nprob)
+        hipLaunchKernelGGL(k_weighted, dim3(2 * nprob), dim3(TPB), 0, st, Z, R, nprob, U, (uint32_t)TPB, w_dev);
+    } else {
+        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, buckets, B, nprob, buckets, B, w_dev);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+// Groups combine their windows with 2^(c P) doublings (none when the table
+// holds every window shifted); one batched inversion gives affine.
+void msm_group_tail(const GroupRun &g, const Xyzz *h_w, Xyzz *res) {
+    const MsmShape &sh = g.sh;
+    const uint32_t nprob = (uint32_t)(g.K * sh.G);
+    const Xyzz *w = h_w + g.w_off;
+    for (size_t col = 0; col < g.K; col++) {
+        Xyzz acc = xyzz_identity();
+        for (int grp = sh.G - 1; grp >= 0; grp--) {
+            acc = host_xyzz_mul_pow2(acc, sh.c * sh.P);
+            const uint32_t p = (uint32_t)(col * sh.G + grp);
+            Xyzz W;
+            if (sh.B > (uint32_t)TPB) {
+                // 256 * WS(Z) + WS(U) + TOT(Z)
+                W = xyzz_add(host_xyzz_mul_pow2(w[2 * p], 8), w[2 * nprob + 2 * p]);
+                W = xyzz_add(W, w[2 * p + 1]);
+            } else {
+                W = xyzz_add(w[2 * p], w[2 * p + 1]);
+            }
+            acc = xyzz_add(acc, W);
+        }
+        res[g.k0 + col] = acc;
+    }
+}
+
+}  // namespace
+
+// Columns are processed in groups of at most MAX_PAIRS digit slots (and
+// MAX_GROUP_COLUMNS columns), which bounds the scratch one call holds. All
+// groups are enqueued on the context's stream with nothing read back until
+// the one synchronisation at the end.
 void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *fr_dev,
              const uint32_t *u32_dev, Affine *out_host) {
     if (K == 0) return;
@@ -609,215 +1128,57 @@ void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *f
         }
         return;
     }
-    constexpr size_t MAX_PAIRS = size_t(1) << 28;
-    const size_t windows = (size_t)msm_window_count(fr_dev ? 254 : 32, tab.c);
-    size_t per = std::max<size_t>(1, MAX_PAIRS / (n * windows));
-    for (size_t k0 = 0; k0 < K; k0 += per) {
-        const size_t kg = std::min(per, K - k0);
-        msm_run_group(ctx, tab, n, kg, fr_dev ? fr_dev + k0 * n : nullptr,
-                      u32_dev ? u32_dev + k0 * n : nullptr, out_host + k0);
-    }
-}
-
-static void msm_run_group(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *fr_dev,
-                          const uint32_t *u32_dev, Affine *out_host) {
     if (n > tab.stride) throw ApgError(-2, "msm: more scalars than table points");
-    if (n * K >= (size_t(1) << 31)) throw ApgError(-2, "msm: batch too large");
-    MsmShape sh;
-    sh.c = tab.c;
-    sh.windows = msm_window_count(fr_dev ? 254 : 32, sh.c);
-    sh.P = std::min(tab.copies, sh.windows);
-    sh.G = (sh.windows + sh.P - 1) / sh.P;
-    sh.B = 1u << (sh.c - 1);
     if ((size_t)tab.copies * tab.stride >= (size_t(1) << 31))
         throw ApgError(-2, "msm: table too large for 31-bit entry indices");
-    const size_t nkeys64 = K * (size_t)sh.G * sh.B;
-    if (nkeys64 >= (size_t(1) << 31)) throw ApgError(-2, "msm: too many buckets");
-    const uint32_t nkeys = (uint32_t)nkeys64;
-    hipStream_t st = ctx.stream;
-
-    // 1. digits -> dense (key, entry) pairs
-    const size_t nthreads = n * K;
-    const size_t max_pairs = nthreads * (size_t)sh.windows;
-    if (max_pairs >= (size_t(1) << 32)) throw ApgError(-2, "msm: too many digits");
-    uint32_t *keys_in = ctx.d<uint32_t>("msm.keys_in", max_pairs);
-    uint32_t *vals_in = ctx.d<uint32_t>("msm.vals_in", max_pairs);
-    const unsigned eblocks = grid_for(nthreads, TPB);
-    uint32_t *counts = ctx.d<uint32_t>("msm.counts", (size_t)eblocks + 1);
-    uint32_t *offs = ctx.d<uint32_t>("msm.offs", (size_t)eblocks + 1);
-    size_t scan_bytes = 0;
-    HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, counts, offs, (int)eblocks + 1, st));
-    void *scan_tmp = ctx.d<uint8_t>("msm.scantmp", std::max<size_t>(scan_bytes, 1));
-    const int stage = sh.windows <= STAGE_WINDOWS ? 1 : 0;
-    const size_t stage_lds = stage ? (size_t)TPB * sh.windows * 2 * sizeof(uint32_t) : 0;
-    // compile-time window widths for the built-in size classes, Fr or u32 scalars
-#define APG_EMIT(CT, WT)                                                                                        \
-    do {                                                                                                        \
-        hipLaunchKernelGGL((k_emit_count<CT, WT>), dim3(eblocks), dim3(TPB), 0, st, fr_dev, u32_dev, (uint32_t)n, \
-                           (uint32_t)K, sh.c, sh.windows, sh.P, counts);                                        \
-        HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, counts, offs, (int)eblocks + 1, st));  \
-        hipLaunchKernelGGL((k_emit<CT, WT>), dim3(eblocks), dim3(TPB), stage_lds, st, fr_dev, u32_dev,           \
-                           (uint32_t)n, (uint32_t)K, sh.c, sh.windows, sh.P, sh.G, sh.B, (uint32_t)tab.stride,  \
-                           stage, offs, keys_in, vals_in, h_total_dev);                                         \
-    } while (0)
-#define APG_EMIT_C(CT)                                                  \
-    case CT:                                                            \
-        if (fr_dev)                                                     \
-            APG_EMIT(CT, msm_window_count(254, CT));                    \
-        else                                                            \
-            APG_EMIT(CT, msm_window_count(32, CT));                     \
-        break;
-    // counts[eblocks] = 0, so the scan's last element is the total; k_emit
-    // writes it into pinned host memory, visible after the stream sync
-    uint32_t *h_total = ctx.h<uint32_t>("msm.total", 1);
-    uint32_t *h_total_dev = nullptr;
-    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&h_total_dev), h_total, 0));
-    HIP_CHECK(hipMemsetAsync(counts + eblocks, 0, sizeof(uint32_t), st));
-    switch (sh.c) {
-        APG_EMIT_C(7)
-        APG_EMIT_C(8)
-        APG_EMIT_C(9)
-        APG_EMIT_C(11)
-        APG_EMIT_C(12)
-        APG_EMIT_C(14)
-        APG_EMIT_C(17)
-        default: APG_EMIT(0, 0); break;
+    constexpr size_t MAX_PAIRS = size_t(1) << 28;
+    const int windows = msm_window_count(fr_dev ? 254 : 32, tab.c);
+    if (n * (size_t)windows >= MAX_PAIRS * 2) throw ApgError(-2, "msm: too many digits for one column");
+    const size_t per = std::min(MAX_GROUP_COLUMNS, std::max<size_t>(1, MAX_PAIRS / (n * windows)));
+    std::vector<GroupRun> groups;
+    size_t wtot = 0;
+    for (size_t k0 = 0; k0 < K; k0 += per) {
+        GroupRun g;
+        g.k0 = k0;
+        g.K = std::min(per, K - k0);
+        g.w_off = wtot;
+        const int P = std::min(tab.copies, windows);
+        wtot += 4 * g.K * (size_t)((windows + P - 1) / P);
+        groups.push_back(g);
     }
-#undef APG_EMIT_C
-#undef APG_EMIT
-    HIP_CHECK(hipGetLastError());
-    ctx.sync();
-    const uint32_t total = *h_total;
-
-    // 2. one radix sort of all pairs by bucket key (per-column sorts of the
-    // low key bits measured slower: small onesweep passes lose more than the
-    // saved pass), then bucket boundaries
-    int key_bits = 1;
-    while ((1ull << key_bits) < (uint64_t)nkeys) key_bits++;
-    // With one bucket group per column (every window has its table copy) the
-    // sort keys only the bucket's c - 1 bits: it is stable and the emission is
-    // column-major, so each (column, bucket) stays contiguous, in bucket-major
-    // order. 16 bits at c = 17 is two onesweep passes instead of three.
-    const int bbits = sh.c - 1;
-    const bool bucket_major = sh.G == 1 && K > 1 && bbits < key_bits;
-    const int sort_bits = bucket_major ? bbits : key_bits;
-    // 4 words of slack: the accumulation reads keys and entries as 16-byte vectors
-    uint32_t *skeys = ctx.d<uint32_t>("msm.skeys", (size_t)total + 4);
-    uint32_t *entries = ctx.d<uint32_t>("msm.entries", (size_t)total + 4);
-    size_t tmp_bytes = 0;
-    HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys_in, skeys, vals_in, entries,
-                                                 (int)total, 0, sort_bits, st));
-    void *tmp = ctx.d<uint8_t>("msm.cubtmp", std::max<size_t>(tmp_bytes, 1));
-    if (total)
-        HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys_in, skeys, vals_in, entries,
-                                                     (int)total, 0, sort_bits, st));
-    uint32_t *off = ctx.d<uint32_t>("msm.off", 2 * (size_t)nkeys);
-    HIP_CHECK(hipMemsetAsync(off, 0, 2 * (size_t)nkeys * sizeof(uint32_t), st));
-    if (total) hipLaunchKernelGGL(k_bounds, dim3(grid_for(total, TPB)), dim3(TPB), 0, st, skeys, total, off);
-    HIP_CHECK(hipGetLastError());
-
-    // 3. accumulate over equal slices of the sorted entries: about ACC_ROUNDS
-    // slices per resident thread (k_accumulate holds 3 waves per SIMD), at
-    // least MIN_SLICE entries and an eighth of the mean bucket, so a typical
-    // bucket crosses about eight slice boundaries and stays light (16 partials
-    // of at most LIGHT_ITEMS)
-    const size_t resident = (size_t)ctx.cu_count * 4 * 3 * 64;
-    const size_t by_grid = (total + ACC_ROUNDS * resident - 1) / (ACC_ROUNDS * resident);
-    const size_t by_bucket = total / (8 * (size_t)std::max<uint32_t>(nkeys, 1));
-    const uint32_t L = (uint32_t)(std::min<size_t>(MAX_SLICE, std::max({MIN_SLICE, by_grid, by_bucket})) + 3) & ~3u;
-    const uint32_t nslices = (total + L - 1) / L;
-    Xyzz29 *buckets = ctx.d<Xyzz29>("msm.buckets", nkeys);
-    Xyzz29 *part = ctx.d<Xyzz29>("msm.part", 2 * (size_t)std::max<uint32_t>(nslices, 1));
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (ctx.profiling) {
-        HIP_CHECK(hipEventCreate(&ev0));
-        HIP_CHECK(hipEventCreate(&ev1));
-        HIP_CHECK(hipEventRecord(ev0, st));
-    }
-    if (nslices)
-        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(nslices, TPB)), dim3(TPB), 0, st, skeys, entries,
-                           total, L, tab.points, buckets, part);
-    HIP_CHECK(hipGetLastError());
-    if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
-
-    // 4. buckets: empty ones set, crossing ones merged from their partials. A
-    // heavy bucket spans more than LIGHT_ITEMS / 2 slices, each slice belongs
-    // to at most one heavy span, and spans hold 2 partials per slice.
-    const uint32_t max_heavy = std::min<uint32_t>(nkeys, nslices / (LIGHT_ITEMS / 2) + 1);
-    const uint32_t max_pieces = 2 * nslices / PIECE + max_heavy + 1;
-    HeavyPlan *plan = ctx.d<HeavyPlan>("msm.plan", 1);
-    uint4 *heavy = ctx.d<uint4>("msm.heavy", max_heavy);
-    uint2 *piece_of = ctx.d<uint2>("msm.piece_of", max_pieces);
-    Xyzz29 *piece_sums = ctx.d<Xyzz29>("msm.piece_sums", max_pieces);
-    HIP_CHECK(hipMemsetAsync(plan, 0, sizeof(HeavyPlan), st));
-    hipLaunchKernelGGL(k_classify, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, off, nkeys, L, plan, heavy,
-                       piece_of);
-    // heavy pieces (a few latency-bound blocks) on the side stream, queued
-    // first, beside the light buckets (throughput-bound) here
-    HIP_CHECK(hipEventRecord(ctx.ev_fork, st));
-    HIP_CHECK(hipStreamWaitEvent(ctx.side, ctx.ev_fork, 0));
-    hipLaunchKernelGGL(k_heavy_pieces, dim3(std::min<uint32_t>(max_pieces, 4u * ctx.cu_count)), dim3(TPB), 0,
-                       ctx.side, off, L, part, plan, heavy, piece_of, piece_sums);
-    hipLaunchKernelGGL(k_heavy_final, dim3(grid_for(max_heavy, TPB)), dim3(TPB), 0, ctx.side, plan, heavy,
-                       piece_sums, buckets);
-    HIP_CHECK(hipEventRecord(ctx.ev_join, ctx.side));
-    hipLaunchKernelGGL(k_bucket_light, dim3(grid_for(nkeys, TPB)), dim3(TPB), 0, st, off, nkeys, L, part,
-                       buckets);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipStreamWaitEvent(st, ctx.ev_join, 0));
-
-    // 5. reduce: per problem (column, group): W = sum_b (b + 1) S_b
-    const uint32_t nprob = (uint32_t)(K * sh.G);
-    const uint32_t B = sh.B;
-    // k_weighted writes the window sums straight into pinned host memory
-    Xyzz *h_w = ctx.h<Xyzz>("msm.hw", (size_t)nprob * 4);
-    Xyzz *wout = nullptr;
-    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&wout), h_w, 0));
-    if (B > (uint32_t)TPB) {
-        const uint32_t R = B / TPB;
-        Xyzz29 *Z = ctx.d<Xyzz29>("msm.Z", (size_t)nprob * R);
-        Xyzz29 *U = ctx.d<Xyzz29>("msm.U", (size_t)nprob * TPB);
-        const size_t threads = (size_t)nprob * (R + TPB) * RGS;
-        hipLaunchKernelGGL(k_reduce_rows, dim3(grid_for(threads, TPB)), dim3(TPB), 0, st, buckets, R,
-                           nprob, Z, U);
-        // WS/TOT of the Z_r (blocks 0..nprob) and of the U_t (nprob..2 nprob)
-        hipLaunchKernelGGL(k_weighted, dim3(2 * nprob), dim3(TPB), 0, st, Z, R, nprob, U,
-                           (uint32_t)TPB, wout);
-    } else {
-        hipLaunchKernelGGL(k_weighted, dim3(nprob), dim3(TPB), 0, st, buckets, B, nprob, buckets, B,
-                           wout);
-    }
-    HIP_CHECK(hipGetLastError());
-    ctx.sync();
-
-    // 6. host tail
-    std::vector<Xyzz> res(K);
-    for (size_t col = 0; col < K; col++) {
-        Xyzz acc = xyzz_identity();
-        for (int g = sh.G - 1; g >= 0; g--) {
-            acc = host_xyzz_mul_pow2(acc, sh.c * sh.P);
-            const uint32_t p = (uint32_t)(col * sh.G + g);
-            Xyzz W;
-            if (B > (uint32_t)TPB) {
-                // 256 * WS(Z) + WS(U) + TOT(Z)
-                W = xyzz_add(host_xyzz_mul_pow2(h_w[2 * p], 8), h_w[2 * nprob + 2 * p]);
-                W = xyzz_add(W, h_w[2 * p + 1]);
-            } else {
-                W = xyzz_add(h_w[2 * p], h_w[2 * p + 1]);
+    Xyzz *h_w = ctx.h<Xyzz>("msm.hw", wtot);
+    uint32_t *h_tot = ctx.h<uint32_t>("msm.total", groups.size());
+    Xyzz *w_dev = nullptr;
+    uint32_t *tot_dev = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&w_dev), h_w, 0));
+    HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&tot_dev), h_tot, 0));
+    struct EvGuard {
+        std::vector<GroupRun> &gs;
+        ~EvGuard() {
+            for (auto &g : gs) {
+                if (g.ev0) (void)hipEventDestroy(g.ev0);
+                if (g.ev1) (void)hipEventDestroy(g.ev1);
             }
-            acc = xyzz_add(acc, W);
         }
-        res[col] = acc;
+    } evg{groups};
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        GroupRun &g = groups[gi];
+        g.total_host = h_tot + gi;
+        msm_enqueue_group(ctx, tab, n, g, fr_dev ? fr_dev + g.k0 * n : nullptr,
+                          u32_dev ? u32_dev + g.k0 * n : nullptr, w_dev + g.w_off, tot_dev + gi);
     }
+    ctx.sync();
+    std::vector<Xyzz> res(K);
+    for (const GroupRun &g : groups) msm_group_tail(g, h_w, res.data());
     host_batch_to_affine(res.data(), out_host, K);
     if (ctx.profiling) {
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
-        ctx.stats.accumulate_ms += ms;
-        ctx.stats.accumulate_launches += nslices ? 1 : 0;
-        ctx.stats.accumulate_entries += total;
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
+        for (const GroupRun &g : groups) {
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+            ctx.stats.accumulate_ms += ms;
+            ctx.stats.accumulate_launches += 1;
+            ctx.stats.accumulate_entries += *g.total_host;
+        }
     }
 }
 
