@@ -138,6 +138,7 @@ def _declare(L):
     d("apg_sumcheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_sumcheck_prove_dev", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_sumcheck_prove_dev_noalloc", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
+    d("apg_debug_imul_peak", [VP, I32, I32, C.POINTER(C.c_double), C.POINTER(C.c_double)])
     d("apg_zerocheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, VP, VP])
     d("apg_witness_digest", [VP, SZ, SZ, VP])
     d("apg_witness_digest_dev", [VP, VP, SZ, SZ, VP])
@@ -1015,6 +1016,14 @@ def synthetic_device(ctx: Context, seed: int, n: int, kind: str = "witness", out
     ptr = C.c_void_p(buf.ptr.value + 32 * offset_elems)
     _check(lib().apg_synthetic_fill_dev(ctx.h, seed, {"uniform": 0, "witness": 1}[kind], n, ptr))
     return buf
+
+
+def imul_peak(ctx: Optional[Context] = None, iters: int = 4096, reps: int = 5) -> dict:
+    """The measured issue rate of v_mad_u64_u32 (apg_debug_imul_peak)."""
+    ctx = ctx or default_context()
+    ms, rate = C.c_double(), C.c_double()
+    _check(lib().apg_debug_imul_peak(ctx.h, iters, reps, C.byref(ms), C.byref(rate)))
+    return {"best_ms": ms.value, "imul_per_s": rate.value}
 
 
 # ------------------------------------------------------------------ engine

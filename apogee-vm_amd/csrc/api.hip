@@ -1313,6 +1313,25 @@ __global__ void k_debug_accumulate(int form, const Affine *pts, const uint8_t *n
     if (c < ncases) out[c] = debug_chain(form, pts, neg, off[c], off[c + 1]);
 }
 
+// Issue-rate probe of the 32 x 32 -> 64 multiply-add the Fq/Fr multiplies
+// are built from: every lane runs 32 independent chains acc = lo(acc) * m +
+// acc (one v_mad_u64_u32 each), so the rate measured is the instruction's
+// issue rate, not its latency.
+__global__ void __launch_bounds__(256) k_imul_peak(uint32_t seed, int iters, uint64_t *out) {
+    uint64_t acc[32];
+    const uint32_t m = seed * 2654435761u + threadIdx.x + 1u;
+#pragma unroll
+    for (int c = 0; c < 32; c++) acc[c] = ((uint64_t)(m + (uint32_t)c) << 7) | (uint64_t)c;
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int c = 0; c < 32; c++) acc[c] = (uint64_t)(uint32_t)acc[c] * m + acc[c];
+    }
+    uint64_t x = 0;
+#pragma unroll
+    for (int c = 0; c < 32; c++) x ^= acc[c];
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = x;
+}
+
 // Device scratch for the debug entries: copies in, one launch, copies out.
 struct DebugBufs {
     std::vector<void *> ptrs;
@@ -1328,6 +1347,35 @@ struct DebugBufs {
 };
 
 }  // namespace
+
+int apg_debug_imul_peak(apg_ctx *ctx, int iters, int reps, double *best_ms, double *imul_per_s) {
+    return guarded([&] {
+        need(ctx);
+        if (iters < 1 || reps < 1 || !best_ms || !imul_per_s) throw ApgError(APG_ERR_ARG, "imul peak: bad arguments");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        const unsigned blocks = 8u * (unsigned)ctx->c.cu_count;
+        uint64_t *out = ctx->c.d<uint64_t>("peak.out", (size_t)blocks * 256);
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int r = 0; r <= reps; r++) {  // the first launch warms up
+            HIP_CHECK(hipEventRecord(e0, ctx->c.stream));
+            hipLaunchKernelGGL(k_imul_peak, dim3(blocks), dim3(256), 0, ctx->c.stream, 12345u + (uint32_t)r, iters, out);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipEventRecord(e1, ctx->c.stream));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (r > 0) best = std::min(best, ms);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *best_ms = best;
+        *imul_per_s = (double)blocks * 256.0 * 32.0 * (double)iters / (best * 1e-3);
+    });
+}
 
 int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
     if (op < 0 || op > 3) return APG_ERR_ARG;

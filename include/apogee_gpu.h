@@ -128,6 +128,13 @@ int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a_mont, const uint
  * scaled to Z = i + 2; ctx NULL runs on the host. */
 int apg_debug_accumulate29(apg_ctx *ctx, int form, const uint8_t *points, const uint8_t *neg,
                            const uint64_t *offsets, size_t ncases, uint8_t *out_affine);
+/* The device's issue rate of the 32 x 32 -> 64 multiply-add (v_mad_u64_u32)
+ * that the field multiplies are built from, the denominator of the MSM and
+ * sumcheck rooflines: 8 blocks of 256 threads per CU, each thread running 32
+ * independent chains of `iters` multiply-adds; the best of `reps` timed
+ * launches (HIP events) after one untimed launch. No reference counterpart
+ * (a measurement aid). */
+int apg_debug_imul_peak(apg_ctx *ctx, int iters, int reps, double *best_ms, double *imul_per_s);
 
 /* ---- SRS ingestion (crates/srs/src/lib.rs:175-240, docs/spec/srs.md §4).
  * The archive: magic "APOGESRS", version u32 = 1, power u32, count u64 =

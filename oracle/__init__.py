@@ -103,6 +103,8 @@ def _declare(L):
     d("orc_witness_digest", [P, SZ, SZ, P])
     d("orc_combine", [P, SZ, SZ, P, SZ, P])
     d("orc_axpy", [SZ, P, P, P])
+    d("orc_msm_recode", [P, SZ, P], SZ)
+    d("orc_msm_nonzero_digits", [P, SZ, SZ], C.c_uint64)
     d("orc_product_claim", [P, SZ, P, SZ, P])
     d("orc_splitmix_next", [P], U64)
     d("orc_splitmix_next_fr", [P, P])
@@ -281,6 +283,19 @@ def msm(bases_aff: np.ndarray, scalars: np.ndarray, threads: int = 8, workers: i
     out = np.zeros(12, dtype=np.uint64)
     lib().orc_msm(ptr(np.ascontiguousarray(bases_aff)), ptr(np.ascontiguousarray(scalars)), n, threads, workers, ptr(out))
     return out
+
+
+def msm_recode(scalar, w: int) -> np.ndarray:
+    """One scalar's signed digits at window width w (msm.rs:180-198)."""
+    out = np.zeros(256, dtype=np.int32)
+    nd = lib().orc_msm_recode(ptr(np.ascontiguousarray(scalar)), w, ptr(out))
+    return out[:nd].copy()
+
+
+def msm_nonzero_digits(scalars: np.ndarray, w: int = 0) -> int:
+    """Nonzero signed digits of the scalars at width w (0: the reference's rule)."""
+    s = np.ascontiguousarray(scalars)
+    return int(lib().orc_msm_nonzero_digits(ptr(s), s.shape[0], w))
 
 
 def msm_small_u32(bases_aff, scalars_u32, threads: int = 8, workers: int = 1):

@@ -616,6 +616,31 @@ static void pippenger(const aff *bases, const int32_t *digits, size_t n, size_t 
     free(tids);
 }
 
+/* One Montgomery Fr scalar's signed digits under the reference's recoding
+ * (msm.rs:149-198) at window width w; returns the window count. */
+size_t orc_msm_recode(const u64 *scalar_mont, size_t w, int32_t *digits) {
+    const size_t windows = window_count(254, w);
+    u64 one[4] = {1, 0, 0, 0}, c[4];
+    f_mul(FR, scalar_mont, one, c);
+    recode(c, w, digits, windows);
+    return windows;
+}
+
+/* Nonzero signed digits of n Montgomery Fr scalars at window width w (0: the
+ * reference's rule for n, msm.rs:112-118): the digit count in the survey's
+ * algorithmic MSM work, one mixed addition per nonzero digit. */
+uint64_t orc_msm_nonzero_digits(const u64 *scalars, size_t n, size_t w) {
+    if (w == 0) w = orc_msm_window_bits(n);
+    const size_t windows = window_count(254, w);
+    int32_t d[256];
+    uint64_t nz = 0;
+    for (size_t i = 0; i < n; i++) {
+        orc_msm_recode(scalars + 4 * i, w, d);
+        for (size_t j = 0; j < windows; j++) nz += d[j] != 0;
+    }
+    return nz;
+}
+
 /* msm.rs:48-67 msm: recode every scalar's canonical limbs, then pippenger */
 int orc_msm(const u64 *bases, const u64 *scalars, size_t n, size_t threads, size_t workers,
             u64 *out) {

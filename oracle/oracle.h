@@ -105,6 +105,8 @@ void orc_witness_digest(uint64_t **tables, size_t k, size_t num_vars, uint64_t *
 /* out_o[i] = sum_c w[o k + c] * cols[c n + i] for o < outs */
 void orc_axpy(size_t n, const uint64_t *a, const uint64_t *x, uint64_t *y);
 void orc_product_claim(const uint64_t *r, size_t n, const uint64_t *const *tables, size_t f, uint64_t *out);
+size_t orc_msm_recode(const uint64_t *scalar_mont, size_t w, int32_t *digits);
+uint64_t orc_msm_nonzero_digits(const uint64_t *scalars, size_t n, size_t w);
 void orc_combine(const uint64_t *cols, size_t n, size_t k, const uint64_t *w, size_t outs,
                  uint64_t *out);
 

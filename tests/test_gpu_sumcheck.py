@@ -134,3 +134,17 @@ def test_config1_end_to_end(apg, orc):
     assert np.array_equal(r_g, r_o)
     assert np.array_equal(p_g, p_o)
     assert np.array_equal(f_g, np.stack([x[0] for x in f_o]))
+
+
+@pytest.mark.parametrize("k,nv", [(1, 0), (3, 6), (5, 12)])
+def test_witness_digest_dev_matches_oracle(apg, orc, k, nv):
+    """apg_witness_digest_dev over device-resident columns (k x 2^nv,
+    column-major) equals the oracle's digest and the host entry's."""
+    ctx = apg.default_context()
+    cols = [orc.fill_witness(600 + c, 1 << nv) for c in range(k)]
+    buf = apg.DeviceBuffer.from_array(ctx, np.concatenate(cols))
+    out = np.zeros(4, dtype=np.uint64)
+    assert apg.lib().apg_witness_digest_dev(ctx.h, buf.ptr, k, nv, out.ctypes.data) == 0
+    assert np.array_equal(out, orc.witness_digest(cols))
+    assert np.array_equal(out, apg.witness_digest(cols))
+    buf.free()

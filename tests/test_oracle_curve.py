@@ -127,3 +127,27 @@ def test_window_rule(orc):
     assert orc.window_bits(1 << 20) == 15
     assert orc.window_bits(1 << 22) == 17
     assert orc.window_bits(33) == 6 * 69 // 100 + 2
+
+
+@pytest.mark.parametrize("w", [3, 13, 15, 17])
+def test_recode_digits_and_nonzero_count(orc, w):
+    """The reference's signed-digit recoding (crates/curve/src/msm.rs:180-198),
+    which the survey's algorithmic MSM work counts: digits reconstruct the
+    canonical scalar (sum d_j 2^(w j)), lie in [-2^(w-1), 2^(w-1)], and the
+    batch count equals the per-scalar nonzero digits; 0 has none, 1 one,
+    2^w - 1 two (-1, then the carry)."""
+    R = 2**256
+    p = 0x30644E72E131A029B85045B68181585D2833E84879B9709143E1F593F0000001
+    xs = [orc.fe(), orc.from_u64(orc.FR, 1), orc.from_u64(orc.FR, (1 << w) - 1)]
+    xs += list(orc.fill_uniform(41 + w, 40)) + list(orc.fill_witness(43 + w, 40))
+    total = 0
+    for x in xs:
+        d = orc.msm_recode(x, w)
+        half = 1 << (w - 1)
+        assert all(-half <= int(v) <= half for v in d)
+        val = sum(int(v) << (w * j) for j, v in enumerate(d))
+        mont = sum(int(x[i]) << (64 * i) for i in range(4))
+        assert val == mont * pow(R, -1, p) % p
+        total += int(np.count_nonzero(d))
+    assert [int(np.count_nonzero(orc.msm_recode(x, w))) for x in xs[:3]] == [0, 1, 2]
+    assert orc.msm_nonzero_digits(np.stack(xs), w) == total
