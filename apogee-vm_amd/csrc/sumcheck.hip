@@ -33,6 +33,7 @@
 
 #include "sumcheck.h"
 #include "field29.cuh"
+#include "host_fr.h"
 #include "sumcheck_dev.cuh"
 
 namespace apg {
@@ -524,6 +525,90 @@ void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *ta
                     point_out, finals_out);
 }
 
+// Rounds j0.. on the host. A device round over small tables is bound by
+// latency (one block, a launch and a synchronisation: about 20 us), while the
+// host evaluates a round of m pairs with about ten multiplies per pair at
+// ~30 ns each, so once the k tables hold at most TAIL_ELEMS entries in all
+// they are read back once and the remaining rounds run here: the node sums
+// S(X) at X = 0..D directly (no node from the claim), then the same
+// extrapolation, interpolation and transcript step as a device round, then
+// the fold. The tables read are T^(j0-1) (len = 2^(n-j0+1) entries, folded
+// here with rho_(j0-1) first) or, when j0 == 0, the inputs.
+constexpr size_t TAIL_ELEMS = 256;
+
+static void sumcheck_host_tail(Context &ctx, const Gate &gate, int k, int n, int j0, const Fe *eq_point,
+                               const Fe *const *dev_tables, size_t len, bool fold_first, const Fe &rho_prev, Fe P,
+                               const RoundFn &round_fn, Fe *rounds_out, Fe *point_out, Fe *finals_out) {
+    using namespace hfr;
+    const int D = gate.x_degree(), d = gate.degree();
+    std::vector<W4> tab((size_t)k * len);
+    for (int t = 0; t < k; t++)
+        HIP_CHECK(hipMemcpyAsync(&tab[(size_t)t * len], dev_tables[t], len * sizeof(Fe), hipMemcpyDeviceToHost,
+                                 ctx.stream));
+    ctx.sync();
+    size_t L = len;
+    auto fold = [&](const W4 &r) {  // f'(i) = f(2i) + r (f(2i+1) - f(2i)), crates/poly/src/lib.rs:156-185
+        for (int t = 0; t < k; t++) {
+            W4 *f = &tab[(size_t)t * len];
+            for (size_t i = 0; i < L / 2; i++) f[i] = w4_add(f[2 * i], w4_mul(r, w4_sub(f[2 * i + 1], f[2 * i])));
+        }
+        L /= 2;
+    };
+    if (fold_first) fold(w4_of(rho_prev));
+    std::vector<W4> coef(gate.nterms);
+    for (int q = 0; q < gate.nterms; q++) coef[q] = w4_of(gate.terms[q].coef);
+    const W4 one = w4_of(f_one<Fr>()), zero = w4_of(f_zero<Fr>());
+    std::vector<W4> E;
+    for (int j = j0; j < n; j++) {
+        const size_t m = L / 2;
+        // E_(j+1)(i) = eq(r_(j+1).., i) by doubling (crates/poly/src/lib.rs:233-248)
+        E.assign(m, zero);
+        E[0] = one;
+        for (size_t h = 1, b = 0; h < m; h *= 2, b++) {
+            const W4 r = w4_of(eq_point[j + 1 + b]), omr = w4_sub(one, r);
+            for (size_t i = 0; i < h; i++) {
+                E[i + h] = w4_mul(E[i], r);
+                E[i] = w4_mul(E[i], omr);
+            }
+        }
+        W4 S[SC_MAX_NODES];
+        for (int x = 0; x <= D; x++) S[x] = zero;
+        for (size_t i = 0; i < m; i++) {
+            W4 cur[SC_MAX_TABLES], step[SC_MAX_TABLES];
+            for (int t = 0; t < k; t++) {
+                const W4 *f = &tab[(size_t)t * len];
+                cur[t] = f[2 * i];
+                step[t] = w4_sub(f[2 * i + 1], f[2 * i]);
+            }
+            for (int x = 0; x <= D; x++) {
+                W4 g = zero;
+                for (int q = 0; q < gate.nterms; q++) {
+                    const Term &tm = gate.terms[q];
+                    W4 v = w4_mul(coef[q], cur[tm.f[0]]);
+                    if (tm.f[1] >= 0) v = w4_mul(v, cur[tm.f[1]]);
+                    if (tm.f[2] >= 0) v = w4_mul(v, cur[tm.f[2]]);
+                    g = w4_add(g, v);
+                }
+                S[x] = w4_add(S[x], w4_mul(E[i], g));
+                if (x < D)
+                    for (int t = 0; t < k; t++) cur[t] = w4_add(cur[t], step[t]);
+            }
+        }
+        const Fe &rj = eq_point[j];
+        Fe T[SC_MAX_NODES], vals[SC_MAX_NODES];
+        for (int x = 0; x <= D; x++) T[x] = f_mul<Fr>(P, fe_of(S[x]));
+        extrapolate(T, D, d);
+        for (int x = 0; x <= d; x++) vals[x] = f_mul<Fr>(eq_at(rj, x), T[x]);
+        Fe *coeffs = rounds_out + (size_t)j * (d + 1);
+        interpolate_host(vals, d, coeffs);
+        const Fe rho = round_fn(coeffs, d + 1);
+        point_out[j] = rho;
+        P = f_mul<Fr>(P, eq_fe(rj, rho));
+        fold(w4_of(rho));
+    }
+    for (int t = 0; t < k; t++) finals_out[t] = fe_of(tab[(size_t)t * len]);
+}
+
 static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, bool mapped_nodes, bool timing,
                             const Gate &gate, Fe *const *tables, int k, int n, const Fe *eq_point,
                             const RoundFn &round_fn, Fe *rounds_out, Fe *point_out, Fe *finals_out) {
@@ -532,6 +617,12 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
     const bool product = gate.is_plain_product(k);
     hipStream_t st = ctx.stream;
     const size_t len0 = size_t(1) << n;
+
+    if ((size_t)k * len0 <= TAIL_ELEMS) {  // small enough for the host from the start
+        sumcheck_host_tail(ctx, gate, k, n, 0, eq_point, tables, len0, false, f_zero<Fr>(), f_one<Fr>(), round_fn,
+                           rounds_out, point_out, finals_out);
+        return;
+    }
 
     // suffix-eq tables (k_eq_suffix), split at h0
     const int h0 = std::max(1, std::min(n, (n + 1) / 2));
@@ -594,6 +685,13 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
     Fe claim = f_zero<Fr>();
     Fe P = f_one<Fr>();  // prod_{l<j} eq(r_l, rho_l)
     for (int j = 0; j < n; j++) {
+        if (j > 0 && (size_t)k * (len0 >> j) <= TAIL_ELEMS) {  // T^j is small: the rest on the host
+            std::vector<Fe *> src(k);
+            for (int t = 0; t < k; t++) src[t] = const_cast<Fe *>(cur[t]);
+            sumcheck_host_tail(ctx, gate, k, n, j, eq_point, src.data(), len0 >> (j - 1), true, rho, P, round_fn,
+                               rounds_out, point_out, finals_out);
+            return;
+        }
         const bool fold = j > 0;
         const size_t m = len0 >> (j + 1);  // pairs of T^j
         const Fe &rj = eq_point[j];
@@ -661,15 +759,7 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
         if (fold)
             for (int t = 0; t < k; t++) cur[t] = nxt[t];
     }
-    // final fold of the two-entry tables with the last challenge
-    Fe last[2 * SC_MAX_TABLES];
-    for (int t = 0; t < k; t++)
-        HIP_CHECK(hipMemcpyAsync(&last[2 * t], cur[t], 2 * sizeof(Fe), hipMemcpyDeviceToHost, st));
-    ctx.sync();
-    for (int t = 0; t < k; t++) {
-        const Fe lo = last[2 * t], hi = last[2 * t + 1];
-        finals_out[t] = f_add<Fr>(lo, f_mul<Fr>(rho, f_sub<Fr>(hi, lo)));
-    }
+    throw ApgError(-5, "sumcheck: the host tail was not reached");  // k * 2 <= TAIL_ELEMS always holds
 }
 
 }  // namespace apg
