@@ -138,6 +138,7 @@ def _declare(L):
     d("apg_sumcheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_sumcheck_prove_dev", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_sumcheck_prove_dev_noalloc", [VP, VP, SZ, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
+    d("apg_sumcheck_prove_dev_lifted", [VP, VP, SZ, VP, VP, SZ, SZ, VP, ROUND_FN, VP, VP, VP, VP])
     d("apg_debug_imul_peak", [VP, I32, I32, C.POINTER(C.c_double), C.POINTER(C.c_double)])
     d("apg_zerocheck_prove", [VP, VP, SZ, VP, SZ, SZ, VP, VP, VP])
     d("apg_witness_digest", [VP, SZ, SZ, VP])
@@ -946,6 +947,58 @@ def prove_sumcheck(eq_point, gate: Gate, tables, transcript: Transcript, ctx: Op
     cb = C.cast(lib().apg_transcript_round, ROUND_FN)
     _check(lib().apg_sumcheck_prove(ctx.h, terms, len(gate.terms), ptrs, len(arrs), n, _p(eqp), cb,
                                     transcript.h, _p(rounds), _p(point), _p(finals)))
+    return rounds[:n], point[:n], finals
+
+
+BACKINGS = {"fr": 0, "u1": 1, "u8": 2, "u16": 3, "u32": 4}
+
+
+def backing_of(column: np.ndarray) -> str:
+    """The PolyBacking a host column maps to: bool -> u1, uint8/16/32 -> u8/16/32,
+    (n, 4) uint64 Montgomery -> fr (crates/poly/src/lib.rs:46-58)."""
+    if column.dtype == np.bool_:
+        return "u1"
+    if column.dtype == np.uint8:
+        return "u8"
+    if column.dtype == np.uint16:
+        return "u16"
+    if column.dtype == np.uint32:
+        return "u32"
+    if column.dtype == np.uint64 and column.ndim == 2 and column.shape[1] == 4:
+        return "fr"
+    raise ValueError(f"no table backing for dtype {column.dtype} shape {column.shape}")
+
+
+def upload_backed(ctx: Context, column: np.ndarray) -> tuple:
+    """A host column to device memory at its own width: (DeviceBuffer, backing).
+    Bits are packed into u64 limbs, entry i at bit i % 64 of limb i / 64."""
+    kind = backing_of(column)
+    if kind == "u1":
+        n = column.shape[0]
+        bits = np.zeros(((n + 63) // 64) * 64, dtype=np.uint8)
+        bits[:n] = column.astype(np.uint8)
+        data = np.packbits(bits, bitorder="little").view(np.uint64)
+    else:
+        data = np.ascontiguousarray(column)
+    return DeviceBuffer.from_array(ctx, data), kind
+
+
+def prove_sumcheck_device(eq_point, gate: Gate, tables, transcript: Transcript, ctx: Optional[Context] = None):
+    """prove_sumcheck over device tables given as (DeviceBuffer, backing)
+    pairs, narrow backings lifted on the device (apg_sumcheck_prove_dev_lifted);
+    every table holds 2^n entries. Returns (rounds, point, final_evals)."""
+    ctx = ctx or default_context()
+    n = len(eq_point)
+    ptrs = (C.c_void_p * len(tables))(*[b.ptr.value for b, _ in tables])
+    kinds = np.array([BACKINGS[k] for _, k in tables], dtype=np.int32)
+    eqp = _fe_array(eq_point) if n else np.zeros((1, 4), dtype=np.uint64)
+    d = gate.degree()
+    rounds = np.zeros((max(n, 1), d + 1, 4), dtype=np.uint64)
+    point = np.zeros((max(n, 1), 4), dtype=np.uint64)
+    finals = np.zeros((len(tables), 4), dtype=np.uint64)
+    cb = C.cast(lib().apg_transcript_round, ROUND_FN)
+    _check(lib().apg_sumcheck_prove_dev_lifted(ctx.h, gate.c_terms(), len(gate.terms), ptrs, _p(kinds), len(tables),
+                                               n, _p(eqp), cb, transcript.h, _p(rounds), _p(point), _p(finals)))
     return rounds[:n], point[:n], finals
 
 

@@ -171,7 +171,8 @@ RoundFn wrap_round(apg_round_fn fn, void *user) {
 
 void sumcheck_common(Context &ctx, const Gate &gate, Fe *const *tabs, size_t k, size_t num_vars,
                      const uint8_t *eq_point_mont, const RoundFn &fn, uint8_t *rounds_out,
-                     uint8_t *point_out, uint8_t *final_evals_out, bool noalloc = false) {
+                     uint8_t *point_out, uint8_t *final_evals_out, bool noalloc = false,
+                     const uint8_t *kinds = nullptr) {
     const int d = gate.degree();
     std::vector<Fe> eqp(num_vars + 1), rounds((size_t)num_vars * (d + 1) + 1), point(num_vars + 1),
         finals(k);
@@ -182,7 +183,7 @@ void sumcheck_common(Context &ctx, const Gate &gate, Fe *const *tabs, size_t k, 
                                       point.data(), finals.data());
     else
         sumcheck_prove_device(ctx, gate, tabs, (int)k, (int)num_vars, eqp.data(), fn, rounds.data(),
-                              point.data(), finals.data());
+                              point.data(), finals.data(), kinds);
     if (ctx.profiling) ctx.stats.sumcheck_ms += now_ms() - t0;
     for (size_t i = 0; i < (size_t)num_vars * (d + 1); i++) store_fe(rounds_out + 32 * i, rounds[i]);
     if (point_out)
@@ -806,6 +807,34 @@ int apg_sumcheck_prove_dev(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
         for (size_t t = 0; t < k; t++) tabs[t] = static_cast<Fe *>(tables_dev[t]);
         sumcheck_common(ctx->c, g, tabs.data(), k, num_vars, eq_point_mont, wrap_round(round_fn, user),
                         rounds_out, point_out, final_evals_out);
+    });
+}
+
+int apg_sumcheck_prove_dev_lifted(apg_ctx *ctx, const apg_term *terms, size_t n_terms, void *const *tables_dev,
+                                  const int *backings, size_t k, size_t num_vars, const uint8_t *eq_point_mont,
+                                  apg_round_fn round_fn, void *user, uint8_t *rounds_out, uint8_t *point_out,
+                                  uint8_t *final_evals_out) {
+    return guarded([&] {
+        need(ctx);
+        if (num_vars > 40) throw ApgError(APG_ERR_ARG, "num_vars out of range");
+        if (k < 1 || k > (size_t)SC_MAX_TABLES) throw ApgError(APG_ERR_ARG, "sumcheck: table count out of range");
+        if (!tables_dev || !backings) throw ApgError(APG_ERR_ARG, "sumcheck: no tables or backings");
+        uint8_t kinds[SC_MAX_TABLES];
+        for (size_t t = 0; t < k; t++) {
+            if (backings[t] < APG_BACKING_FR || backings[t] > APG_BACKING_U32)
+                throw ApgError(APG_ERR_ARG, "sumcheck: unknown table backing");
+            kinds[t] = (uint8_t)backings[t];
+        }
+        static_assert(APG_BACKING_FR == SC_FR && APG_BACKING_U1 == SC_U1 && APG_BACKING_U8 == SC_U8 &&
+                          APG_BACKING_U16 == SC_U16 && APG_BACKING_U32 == SC_U32,
+                      "backing codes");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        Gate g = make_gate(terms, n_terms);
+        Fe *tabs[SC_MAX_TABLES];
+        for (size_t t = 0; t < k; t++) tabs[t] = static_cast<Fe *>(tables_dev[t]);
+        sumcheck_common(ctx->c, g, tabs, k, num_vars, eq_point_mont, wrap_round(round_fn, user), rounds_out,
+                        point_out, final_evals_out, false, kinds);
     });
 }
 

@@ -31,6 +31,11 @@ struct Gate {
     bool is_plain_product(int k) const;  // one term, coef 1, factors 0..k-1
 };
 
+// Backings of a table (crates/poly/src/lib.rs:46-58): Fr entries (32 B), or
+// narrow entries lifted to Fr on read: one bit (u64 limbs, entry i = bit i %
+// 64 of limb i / 64), u8, u16, u32.
+enum : uint8_t { SC_FR = 0, SC_U1 = 1, SC_U8 = 2, SC_U16 = 3, SC_U32 = 4 };
+
 // Per-round hook: receives the d+1 ascending coefficients, returns the challenge.
 using RoundFn = std::function<Fe(const Fe *coeffs, int ncoeffs)>;
 
@@ -39,9 +44,11 @@ using RoundFn = std::function<Fe(const Fe *coeffs, int ncoeffs)>;
 // (ping-pong scratch is internal). rounds_out receives num_vars * (d + 1)
 // coefficients, point_out num_vars challenges, finals_out the k fully bound
 // values.
+// kinds (nullable: all Fr) gives each table's backing; narrow tables are read
+// through the lift in the first round and folded into Fr in the second.
 void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables_dev, int k,
                            int num_vars, const Fe *eq_point_host, const RoundFn &round_fn,
-                           Fe *rounds_out, Fe *point_out, Fe *finals_out);
+                           Fe *rounds_out, Fe *point_out, Fe *finals_out, const uint8_t *kinds = nullptr);
 
 // The same proof with all device scratch in one file-scope array per device
 // (sumcheck_noalloc_limit() bytes), so the call itself allocates no device
@@ -51,7 +58,7 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables_dev
 // limit are refused. Calls on one device are serialised by a lock.
 void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *tables_dev, int k,
                                    int num_vars, const Fe *eq_point_host, const RoundFn &round_fn,
-                                   Fe *rounds_out, Fe *point_out, Fe *finals_out);
+                                   Fe *rounds_out, Fe *point_out, Fe *finals_out, const uint8_t *kinds = nullptr);
 void sumcheck_noalloc_init(int device);
 size_t sumcheck_noalloc_limit();
 size_t sumcheck_noalloc_bytes(int k, int num_vars);

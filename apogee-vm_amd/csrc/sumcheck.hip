@@ -43,7 +43,30 @@ namespace {
 struct TablePtrs {
     const Fe *src[SC_MAX_TABLES];
     Fe *dst[SC_MAX_TABLES];
+    uint8_t kind[SC_MAX_TABLES];  // the backing of src (SC_FR unless LIFT)
 };
+
+// Entry i of a source table of the given backing (crates/poly/src/lib.rs:
+// 46-97): Fr as stored; a narrow integer v lifted on read to v R mod p (its
+// Montgomery image, Fr::from_u64); one bit to 0 or 1. The first round reads
+// the caller's tables through this and the second folds them through it into
+// Fr scratch, so no full-size lifted copy is made.
+__device__ __forceinline__ Fe ld_entry(const Fe *src, uint8_t kind, size_t i) {
+    switch (kind) {
+        case SC_U1:
+            return ((reinterpret_cast<const uint64_t *>(src)[i >> 6] >> (i & 63)) & 1) ? f_one<Fr>() : f_zero<Fr>();
+        case SC_U8: return f_from_u64<Fr>(reinterpret_cast<const uint8_t *>(src)[i]);
+        case SC_U16: return f_from_u64<Fr>(reinterpret_cast<const uint16_t *>(src)[i]);
+        case SC_U32: return f_from_u64<Fr>(reinterpret_cast<const uint32_t *>(src)[i]);
+        default: return ldfe(src, i);
+    }
+}
+
+template <bool LIFT>
+__device__ __forceinline__ Fe ld_src(const TablePtrs &tp, int t, size_t i) {
+    if (LIFT) return ld_entry(tp.src[t], tp.kind[t], i);
+    return ldfe(tp.src[t], i);
+}
 
 // A gate's terms for the kernel: factor indices, and the coefficient in the
 // R' = 2^261 domain times 2^(5 (factors - 1)), so that the chain of lazy
@@ -56,19 +79,19 @@ struct GateArgs {
 };
 
 // lo/hi of pair i of table t, folding the previous round's table if asked.
+template <bool LIFT>
 __device__ __forceinline__ void pair_of(const TablePtrs &tp, int t, size_t i, bool fold,
                                         const Fe &r, Fe &lo, Fe &hi) {
     if (fold) {
-        const Fe *s = tp.src[t];
-        Fe a0 = ldfe(s, 4 * i), a1 = ldfe(s, 4 * i + 1), a2 = ldfe(s, 4 * i + 2),
-           a3 = ldfe(s, 4 * i + 3);
+        Fe a0 = ld_src<LIFT>(tp, t, 4 * i), a1 = ld_src<LIFT>(tp, t, 4 * i + 1), a2 = ld_src<LIFT>(tp, t, 4 * i + 2),
+           a3 = ld_src<LIFT>(tp, t, 4 * i + 3);
         lo = f_add<Fr>(a0, f_mul<Fr>(r, f_sub<Fr>(a1, a0)));
         hi = f_add<Fr>(a2, f_mul<Fr>(r, f_sub<Fr>(a3, a2)));
         stfe(tp.dst[t], 2 * i, lo);
         stfe(tp.dst[t], 2 * i + 1, hi);
     } else {
-        lo = ldfe(tp.src[t], 2 * i);
-        hi = ldfe(tp.src[t], 2 * i + 1);
+        lo = ld_src<LIFT>(tp, t, 2 * i);
+        hi = ld_src<LIFT>(tp, t, 2 * i + 1);
     }
 }
 
@@ -78,7 +101,7 @@ __device__ __forceinline__ void pair_of(const TablePtrs &tp, int t, size_t i, bo
 // (f_mul_r261: the image of x y / 2^5, ~40% fewer instructions than the
 // 32-bit CIOS): the fold takes the challenge in the R' = 2^261 domain and so
 // stays exact, and each node sum carries 2^(-5 KT), which the host removes.
-template <int KT, bool FULL>
+template <int KT, bool FULL, bool LIFT>
 __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, int fold, F29 r29,
                                                       EqArgs eq, Fe *partials, unsigned *counter,
                                                       Fe *out) {
@@ -94,16 +117,15 @@ __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, 
             Fe lo, hi;
             if (fold) {
                 // (a1 - a0) rho: the image times rho R' is the image of the product
-                const Fe *s = tp.src[t];
-                const Fe a0 = ldfe(s, 4 * i), a1 = ldfe(s, 4 * i + 1), a2 = ldfe(s, 4 * i + 2),
-                         a3 = ldfe(s, 4 * i + 3);
+                const Fe a0 = ld_src<LIFT>(tp, t, 4 * i), a1 = ld_src<LIFT>(tp, t, 4 * i + 1),
+                         a2 = ld_src<LIFT>(tp, t, 4 * i + 2), a3 = ld_src<LIFT>(tp, t, 4 * i + 3);
                 lo = f_add<Fr>(a0, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(f_sub<Fr>(a1, a0)), r29)));
                 hi = f_add<Fr>(a2, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(f_sub<Fr>(a3, a2)), r29)));
                 stfe(tp.dst[t], 2 * i, lo);
                 stfe(tp.dst[t], 2 * i + 1, hi);
             } else {
-                lo = ldfe(tp.src[t], 2 * i);
-                hi = ldfe(tp.src[t], 2 * i + 1);
+                lo = ld_src<LIFT>(tp, t, 2 * i);
+                hi = ld_src<LIFT>(tp, t, 2 * i + 1);
             }
             cur[t] = lo;
             step[t] = f_sub<Fr>(hi, lo);
@@ -133,7 +155,7 @@ __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, 
 // Any gate: terms read from the kernel argument, table lines kept in a local
 // array (dynamic factor indices). nn of NN nodes: X = 0..nn-1 (full) or
 // X = 0, 2..nn.
-template <int NN>
+template <int NN, bool LIFT>
 __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateArgs gate, size_t m,
                                                    int fold, Fe r, EqArgs eq, int full, int nn,
                                                    Fe *partials, unsigned *counter, Fe *out) {
@@ -145,7 +167,7 @@ __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateAr
         Fe cur[SC_MAX_TABLES], step[SC_MAX_TABLES];
         for (int t = 0; t < k; t++) {
             Fe lo, hi;
-            pair_of(tp, t, i, fold, r, lo, hi);
+            pair_of<LIFT>(tp, t, i, fold, r, lo, hi);
             cur[t] = lo;
             step[t] = f_sub<Fr>(hi, lo);
         }
@@ -328,27 +350,37 @@ static unsigned round_blocks(const Context &ctx, size_t m, int per_cu) {
     return (unsigned)std::min(cap, std::max<size_t>(1, (m + RTPB - 1) / RTPB));
 }
 
-template <int KT>
+template <int KT, bool LIFT>
 static void launch_product(Context &ctx, bool full, const TablePtrs &tp, size_t m, bool fold,
                            const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out) {
     const dim3 g(round_blocks(ctx, m, 1)), b(RTPB);
     const F29 r29 = f29_from_mont32<Fr>(r);  // rho R' mod r
     if (full)
-        hipLaunchKernelGGL((k_round_product<KT, true>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
+        hipLaunchKernelGGL((k_round_product<KT, true, LIFT>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
                            partials, counter, out);
     else
-        hipLaunchKernelGGL((k_round_product<KT, false>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
+        hipLaunchKernelGGL((k_round_product<KT, false, LIFT>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
                            partials, counter, out);
 }
 
-static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bool full, int D,
+template <int KT>
+static void launch_product(Context &ctx, bool lift, bool full, const TablePtrs &tp, size_t m, bool fold,
+                           const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out) {
+    if (lift)
+        launch_product<KT, true>(ctx, full, tp, m, fold, r, eq, partials, counter, out);
+    else
+        launch_product<KT, false>(ctx, full, tp, m, fold, r, eq, partials, counter, out);
+}
+
+// lift: some source table is a narrow backing (the first two rounds)
+static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bool full, int D, bool lift,
                          const TablePtrs &tp, size_t m, bool fold, const Fe &r, const EqArgs &eq,
                          Fe *partials, unsigned *counter, Fe *out) {
     if (product) {
         switch (k) {
-            case 1: launch_product<1>(ctx, full, tp, m, fold, r, eq, partials, counter, out); return;
-            case 2: launch_product<2>(ctx, full, tp, m, fold, r, eq, partials, counter, out); return;
-            case 3: launch_product<3>(ctx, full, tp, m, fold, r, eq, partials, counter, out); return;
+            case 1: launch_product<1>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out); return;
+            case 2: launch_product<2>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out); return;
+            case 3: launch_product<3>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out); return;
             default: break;
         }
     }
@@ -365,9 +397,15 @@ static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bo
     }
     const int nn = full ? D + 1 : D;
     const dim3 g(round_blocks(ctx, m, 4)), b(RTPB);
-#define APG_GATE(NN)                                                                                  \
-    hipLaunchKernelGGL((k_round_gate<NN>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r, eq,      \
-                       (int)full, nn, partials, counter, out)
+#define APG_GATE(NN)                                                                                          \
+    do {                                                                                                      \
+        if (lift)                                                                                             \
+            hipLaunchKernelGGL((k_round_gate<NN, true>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r, eq, \
+                               (int)full, nn, partials, counter, out);                                        \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_round_gate<NN, false>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r,    \
+                               eq, (int)full, nn, partials, counter, out);                                    \
+    } while (0)
     switch (D + 1) {
         case 2: APG_GATE(2); break;
         case 3: APG_GATE(3); break;
@@ -443,17 +481,19 @@ static void check_sumcheck_args(const Gate &gate, int k) {
 // (pinned memory, visible after the stream sync); otherwise they write
 // sc.nodes in device memory and each round copies them to nodes_host.
 static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *nodes_host, bool mapped_nodes,
-                            bool timing, const Gate &gate, Fe *const *tables, int k, int n, const Fe *eq_point,
-                            const RoundFn &round_fn, Fe *rounds_out, Fe *point_out, Fe *finals_out);
+                            bool timing, const Gate &gate, Fe *const *tables, const uint8_t *kinds, int k, int n,
+                            const Fe *eq_point, const RoundFn &round_fn, Fe *rounds_out, Fe *point_out,
+                            Fe *finals_out);
+
+// The single entry of each table (no variables to bind), lifted.
+static void sumcheck_finals_n0(Context &ctx, Fe *const *tables, const uint8_t *kinds, int k, Fe *finals_out);
 
 void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, int k, int n,
                            const Fe *eq_point, const RoundFn &round_fn, Fe *rounds_out,
-                           Fe *point_out, Fe *finals_out) {
+                           Fe *point_out, Fe *finals_out, const uint8_t *kinds) {
     check_sumcheck_args(gate, k);
-    if (n == 0) {  // nothing to bind: the finals are the single entries
-        for (int t = 0; t < k; t++)
-            HIP_CHECK(hipMemcpyAsync(&finals_out[t], tables[t], sizeof(Fe), hipMemcpyDeviceToHost, ctx.stream));
-        ctx.sync();
+    if (n == 0) {
+        sumcheck_finals_n0(ctx, tables, kinds, k, finals_out);
         return;
     }
     const ScratchLayout l = sumcheck_layout(k, n);
@@ -463,7 +503,7 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
     sc.counter = ctx.zeroed<unsigned>("sc.counter", 64);
     Fe *hvals = ctx.h<Fe>("sc.hvals", SC_MAX_NODES);
     HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&sc.nodes), hvals, 0));
-    sumcheck_rounds(ctx, sc, hvals, true, ctx.profiling, gate, tables, k, n, eq_point, round_fn, rounds_out,
+    sumcheck_rounds(ctx, sc, hvals, true, ctx.profiling, gate, tables, kinds, k, n, eq_point, round_fn, rounds_out,
                     point_out, finals_out);
 }
 
@@ -497,16 +537,12 @@ void sumcheck_noalloc_init(int device) {
 
 void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *tables, int k, int n,
                                    const Fe *eq_point, const RoundFn &round_fn, Fe *rounds_out, Fe *point_out,
-                                   Fe *finals_out) {
+                                   Fe *finals_out, const uint8_t *kinds) {
     check_sumcheck_args(gate, k);
     if (ctx.device < 0 || ctx.device >= NOALLOC_MAX_DEVICES || !g_noalloc_ptr[ctx.device])
         throw ApgError(-2, "no-allocation scratch: not set up on this device");
     if (n == 0) {
-        Fe one[SC_MAX_TABLES];
-        for (int t = 0; t < k; t++)
-            HIP_CHECK(hipMemcpyAsync(&one[t], tables[t], sizeof(Fe), hipMemcpyDeviceToHost, ctx.stream));
-        ctx.sync();
-        for (int t = 0; t < k; t++) finals_out[t] = one[t];
+        sumcheck_finals_n0(ctx, tables, kinds, k, finals_out);
         return;
     }
     const ScratchLayout l = sumcheck_layout(k, n);
@@ -521,7 +557,7 @@ void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *ta
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx.stream, sc.counter, size_t(64));
     HIP_CHECK(hipGetLastError());
     Fe nodes_host[SC_MAX_NODES];
-    sumcheck_rounds(ctx, sc, nodes_host, false, false, gate, tables, k, n, eq_point, round_fn, rounds_out,
+    sumcheck_rounds(ctx, sc, nodes_host, false, false, gate, tables, kinds, k, n, eq_point, round_fn, rounds_out,
                     point_out, finals_out);
 }
 
@@ -536,16 +572,76 @@ void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *ta
 // here with rho_(j0-1) first) or, when j0 == 0, the inputs.
 constexpr size_t TAIL_ELEMS = 256;
 
+// Bytes a table of `len` entries of backing `kind` occupies.
+static size_t backing_bytes(uint8_t kind, size_t len) {
+    switch (kind) {
+        case SC_U1: return (len + 63) / 64 * 8;
+        case SC_U8: return len;
+        case SC_U16: return 2 * len;
+        case SC_U32: return 4 * len;
+        default: return len * sizeof(Fe);
+    }
+}
+
+// Entry i of a host copy of a table of backing `kind`, lifted (as ld_entry).
+static Fe host_entry(const uint8_t *b, uint8_t kind, size_t i) {
+    switch (kind) {
+        case SC_U1: {
+            uint64_t w;
+            std::memcpy(&w, b + 8 * (i >> 6), 8);
+            return ((w >> (i & 63)) & 1) ? f_one<Fr>() : f_zero<Fr>();
+        }
+        case SC_U8: return f_from_u64<Fr>(b[i]);
+        case SC_U16: {
+            uint16_t v;
+            std::memcpy(&v, b + 2 * i, 2);
+            return f_from_u64<Fr>(v);
+        }
+        case SC_U32: {
+            uint32_t v;
+            std::memcpy(&v, b + 4 * i, 4);
+            return f_from_u64<Fr>(v);
+        }
+        default: {
+            Fe v;
+            std::memcpy(&v, b + sizeof(Fe) * i, sizeof(Fe));
+            return v;
+        }
+    }
+}
+
+static void sumcheck_finals_n0(Context &ctx, Fe *const *tables, const uint8_t *kinds, int k, Fe *finals_out) {
+    uint8_t one[SC_MAX_TABLES][sizeof(Fe)];
+    for (int t = 0; t < k; t++)
+        HIP_CHECK(hipMemcpyAsync(one[t], tables[t], backing_bytes(kinds ? kinds[t] : SC_FR, 1), hipMemcpyDeviceToHost,
+                                 ctx.stream));
+    ctx.sync();
+    for (int t = 0; t < k; t++) finals_out[t] = host_entry(one[t], kinds ? kinds[t] : SC_FR, 0);
+}
+
 static void sumcheck_host_tail(Context &ctx, const Gate &gate, int k, int n, int j0, const Fe *eq_point,
-                               const Fe *const *dev_tables, size_t len, bool fold_first, const Fe &rho_prev, Fe P,
-                               const RoundFn &round_fn, Fe *rounds_out, Fe *point_out, Fe *finals_out) {
+                               const Fe *const *dev_tables, const uint8_t *kinds, size_t len, bool fold_first,
+                               const Fe &rho_prev, Fe P, const RoundFn &round_fn, Fe *rounds_out, Fe *point_out,
+                               Fe *finals_out) {
     using namespace hfr;
     const int D = gate.x_degree(), d = gate.degree();
     std::vector<W4> tab((size_t)k * len);
-    for (int t = 0; t < k; t++)
-        HIP_CHECK(hipMemcpyAsync(&tab[(size_t)t * len], dev_tables[t], len * sizeof(Fe), hipMemcpyDeviceToHost,
-                                 ctx.stream));
+    std::vector<std::vector<uint8_t>> narrow(k);
+    for (int t = 0; t < k; t++) {
+        const uint8_t kind = kinds ? kinds[t] : SC_FR;
+        if (kind == SC_FR) {
+            HIP_CHECK(hipMemcpyAsync(&tab[(size_t)t * len], dev_tables[t], len * sizeof(Fe), hipMemcpyDeviceToHost,
+                                     ctx.stream));
+        } else {
+            narrow[t].resize(backing_bytes(kind, len));
+            HIP_CHECK(hipMemcpyAsync(narrow[t].data(), dev_tables[t], narrow[t].size(), hipMemcpyDeviceToHost,
+                                     ctx.stream));
+        }
+    }
     ctx.sync();
+    for (int t = 0; t < k; t++)  // narrow tables lifted on the host, as ld_entry lifts them
+        if (!narrow[t].empty())
+            for (size_t i = 0; i < len; i++) tab[(size_t)t * len + i] = w4_of(host_entry(narrow[t].data(), kinds[t], i));
     size_t L = len;
     auto fold = [&](const W4 &r) {  // f'(i) = f(2i) + r (f(2i+1) - f(2i)), crates/poly/src/lib.rs:156-185
         for (int t = 0; t < k; t++) {
@@ -610,8 +706,11 @@ static void sumcheck_host_tail(Context &ctx, const Gate &gate, int k, int n, int
 }
 
 static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, bool mapped_nodes, bool timing,
-                            const Gate &gate, Fe *const *tables, int k, int n, const Fe *eq_point,
-                            const RoundFn &round_fn, Fe *rounds_out, Fe *point_out, Fe *finals_out) {
+                            const Gate &gate, Fe *const *tables, const uint8_t *kinds, int k, int n,
+                            const Fe *eq_point, const RoundFn &round_fn, Fe *rounds_out, Fe *point_out,
+                            Fe *finals_out) {
+    bool narrow = false;  // a narrow backing: the first two rounds read through the lift
+    for (int t = 0; t < k && kinds; t++) narrow = narrow || kinds[t] != SC_FR;
     const int D = gate.x_degree();  // degree of G along one variable
     const int d = gate.degree();    // round polynomial degree
     const bool product = gate.is_plain_product(k);
@@ -619,8 +718,8 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
     const size_t len0 = size_t(1) << n;
 
     if ((size_t)k * len0 <= TAIL_ELEMS) {  // small enough for the host from the start
-        sumcheck_host_tail(ctx, gate, k, n, 0, eq_point, tables, len0, false, f_zero<Fr>(), f_one<Fr>(), round_fn,
-                           rounds_out, point_out, finals_out);
+        sumcheck_host_tail(ctx, gate, k, n, 0, eq_point, tables, kinds, len0, false, f_zero<Fr>(), f_one<Fr>(),
+                           round_fn, rounds_out, point_out, finals_out);
         return;
     }
 
@@ -688,8 +787,9 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
         if (j > 0 && (size_t)k * (len0 >> j) <= TAIL_ELEMS) {  // T^j is small: the rest on the host
             std::vector<Fe *> src(k);
             for (int t = 0; t < k; t++) src[t] = const_cast<Fe *>(cur[t]);
-            sumcheck_host_tail(ctx, gate, k, n, j, eq_point, src.data(), len0 >> (j - 1), true, rho, P, round_fn,
-                               rounds_out, point_out, finals_out);
+            // T^(j-1) is still the caller's table when j == 1
+            sumcheck_host_tail(ctx, gate, k, n, j, eq_point, src.data(), j == 1 ? kinds : nullptr, len0 >> (j - 1),
+                               true, rho, P, round_fn, rounds_out, point_out, finals_out);
             return;
         }
         const bool fold = j > 0;
@@ -701,6 +801,7 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
         for (int t = 0; t < k; t++) {
             nxt[t] = (j % 2 == 1) ? bufA[t] : bufB[t];
             tp.src[t] = cur[t];
+            tp.kind[t] = (j <= 1 && kinds) ? kinds[t] : (uint8_t)SC_FR;
             tp.dst[t] = nxt[t];
         }
         const int s = j + 1;
@@ -715,7 +816,8 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
             eq.hi = eqB + (size_t)(s - h0) * bstride;
         }
         if (timing) HIP_CHECK(hipEventRecord(ev0, st));
-        launch_round(ctx, gate, k, product, full, D, tp, m, fold, rho, eq, partials, counter, hvals_dev);
+        launch_round(ctx, gate, k, product, full, D, narrow && j <= 1, tp, m, fold, rho, eq, partials, counter,
+                     hvals_dev);
         HIP_CHECK(hipGetLastError());
         if (timing) HIP_CHECK(hipEventRecord(ev1, st));
         const int nn = full ? D + 1 : D;

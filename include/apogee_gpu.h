@@ -320,6 +320,22 @@ int apg_gkr_prove_sumcheck_dev(apg_ctx *ctx, const apg_gkr_list *list, const uin
                                void *const *upper_dev, size_t ncols, apg_round_fn round_fn, void *user,
                                uint8_t *rounds_out, uint8_t *point_out, uint8_t *finals_out);
 
+/* Table backings (crates/poly/src/lib.rs:46-58, PolyBacking): Fr entries
+ * (32 B Montgomery), or narrow entries kept at their width on the device and
+ * lifted to Fr as they are read: U1 (u64 limbs, entry i = bit i % 64 of limb
+ * i / 64), U8, U16, U32 (an entry v is v in Fr). */
+enum { APG_BACKING_FR = 0, APG_BACKING_U1 = 1, APG_BACKING_U8 = 2, APG_BACKING_U16 = 3, APG_BACKING_U32 = 4 };
+/* apg_sumcheck_prove_dev over tables of the given backings (one per table):
+ * the first round reads narrow tables through the lift and the second folds
+ * them into Fr scratch, as MultilinearPoly::bind lifts on its first call
+ * (crates/poly/src/lib.rs:156-185), so a narrow table crosses PCIe and sits
+ * in HBM at its own width and no lifted copy is made. APG_ERR_ARG for an
+ * unknown backing. */
+int apg_sumcheck_prove_dev_lifted(apg_ctx *ctx, const apg_term *terms, size_t n_terms, void *const *tables_dev,
+                                  const int *backings, size_t k, size_t num_vars, const uint8_t *eq_point_mont,
+                                  apg_round_fn round_fn, void *user, uint8_t *rounds_out, uint8_t *point_out,
+                                  uint8_t *final_evals_out);
+
 /* apg_sumcheck_prove_dev with no device allocation in the call. No
  * counterpart in the reference: it is for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

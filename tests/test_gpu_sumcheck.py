@@ -148,3 +148,55 @@ def test_witness_digest_dev_matches_oracle(apg, orc, k, nv):
     assert np.array_equal(out, orc.witness_digest(cols))
     assert np.array_equal(out, apg.witness_digest(cols))
     buf.free()
+
+
+def _lift(orc, col):
+    if col.dtype == np.bool_:
+        col = col.astype(np.uint32)
+    return np.stack([orc.from_u64(orc.FR, int(v)) for v in col])
+
+
+@pytest.mark.parametrize("n,kinds,shape", [
+    (1, ("u1", "u8", "fr"), "product3"),   # every round on the host, from the caller's tables
+    (5, ("u8", "u16"), "product2"),        # likewise
+    (7, ("u1", "u16", "u32"), "product3"),  # device round 0 through the lift, the host folds T^0 next
+    (13, ("u8", "u32", "fr"), "product3"),  # device rounds 0 (lift) and 1 (fold through the lift)
+    (12, ("u16", "u1"), "gate"),           # the general gate kernel through the lift
+])
+def test_lifted_backings_match_oracle(apg, orc, n, kinds, shape):
+    """Narrow tables (PolyBacking U1/U8/U16/U32, crates/poly/src/lib.rs:46-97)
+    kept at their width on the device and lifted in the first rounds give the
+    oracle's proof over the lifted Fr tables."""
+    ctx = apg.default_context()
+    rng = np.random.default_rng(700 + n)
+    cols = []
+    for i, kd in enumerate(kinds):
+        m = 1 << n
+        if kd == "u1":
+            cols.append(rng.integers(0, 2, m).astype(np.bool_))
+        elif kd == "u8":
+            cols.append(rng.integers(0, 256, m, dtype=np.uint8))
+        elif kd == "u16":
+            cols.append(rng.integers(0, 1 << 16, m, dtype=np.uint16))
+        elif kd == "u32":
+            cols.append(rng.integers(0, 1 << 32, m, dtype=np.uint32))
+        else:
+            cols.append(orc.fill_uniform(710 + i, m))
+    lifted = [c if c.ndim == 2 else _lift(orc, c) for c in cols]
+    one = orc.from_u64(orc.FR, 1)
+    if shape == "gate":
+        gate = apg.Gate([0, 1], [apg.GateTerm(apg.fr_from_u64(3), 0, 1), apg.GateTerm(one, 1)])
+        oterms = [(orc.from_u64(orc.FR, 3), [0, 1]), (one, [1])]
+    else:
+        k = len(kinds)
+        gate = apg.Gate.product(k)
+        oterms = [(one, list(range(k)))]
+    r = orc.fill_uniform(720 + n, n)
+    dev = [apg.upload_backed(ctx, c) for c in cols]
+    assert [kd for _, kd in dev] == list(kinds)
+    rg, pg, fg = apg.prove_sumcheck_device(r, gate, dev, apg.Transcript())
+    ro, po, fo = orc.prove_sumcheck(r, oterms, lifted, orc.Transcript())
+    assert np.array_equal(rg, ro) and np.array_equal(pg, po)
+    assert np.array_equal(fg, np.stack([x[0] for x in fo]) if isinstance(fo, list) else fo.reshape(fg.shape))
+    for b, _ in dev:
+        b.free()
