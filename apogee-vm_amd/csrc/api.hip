@@ -1407,7 +1407,7 @@ int apg_debug_imul_peak(apg_ctx *ctx, int iters, int reps, double *best_ms, doub
 }
 
 int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
-    if (op < 0 || op > 3) return APG_ERR_ARG;
+    if (op < 0 || op > 5) return APG_ERR_ARG;
     if (!ctx) {
         for (size_t i = 0; i < n; i++)
             store_fe(out + 32 * i, f29_test_chain<Fq>(load_fe(a + 32 * i), load_fe(b + 32 * i), op));

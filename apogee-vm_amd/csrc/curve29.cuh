@@ -53,14 +53,15 @@ APG_HD bool xyzz29_is_id(const Xyzz29 &p) { return f29_is_zero<Fq, 2>(p.zz); }
 // Y3 = M (S - X3) - W y, ZZ3 = V, ZZZ3 = W.
 APG_HD Xyzz29 xyzz29_mdbl(const F29 &x, const F29 &y) {
     const F29 u = f29_add(y, y);                 // < 2p
-    const F29 v = f29_mul<Fq>(u, u);             // < 1.03p
+    const F29 v = f29_sqr<Fq>(u);                // < 1.03p
     const F29 w = f29_mul<Fq>(u, v);             // < 1.02p
     const F29 s = f29_mul<Fq>(x, v);             // < 1.01p
-    const F29 x2 = f29_mul<Fq>(x, x);            // < 1.01p
+    const F29 x2 = f29_sqr<Fq>(x);               // < 1.01p
     const F29 m = f29_add(f29_add(x2, x2), x2);  // < 3.03p
     Xyzz29 r;
-    r.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(m, m), s), s);                     // < 5.06p
-    r.y = f29_sub<Fq, 2>(f29_mul<Fq>(m, f29_sub<Fq, 8>(s, r.x)), f29_mul<Fq>(w, y));  // < 3.17p
+    r.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sqr<Fq>(m), s), s);  // < 5.06p
+    // M (S - X3) - W y as one reduction: M (S - X3 + 8p) + W (2p - y)
+    r.y = f29_mul2<Fq>(m, f29_sub<Fq, 8>(s, r.x), w, f29_sub<Fq, 2>(F29{}, y));  // < 1.18p
     r.zz = v;
     r.zzz = w;
     return r;
@@ -82,14 +83,15 @@ APG_HD void xyzz29_add_affine(Xyzz29 &acc, bool &id, const F29 &x, const F29 &y)
     }
     // ordered so that each coordinate of acc is replaced as soon as its last
     // use is done (fewer live values)
-    const F29 pp = f29_mul<Fq>(pd, pd);      // < 1.49p
+    const F29 pp = f29_sqr<Fq>(pd);          // < 1.49p
     acc.zz = f29_mul<Fq>(acc.zz, pp);        // < 1.01p
     const F29 ppp = f29_mul<Fq>(pd, pp);     // < 1.08p
     acc.zzz = f29_mul<Fq>(acc.zzz, ppp);     // < 1.01p
     const F29 q = f29_mul<Fq>(acc.x, pp);    // < 1.07p
-    const F29 yp = f29_mul<Fq>(acc.y, ppp);  // < 1.02p
-    const F29 x3 = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(r, r), ppp), q), q);  // < 7.15p
-    acc.y = f29_sub<Fq, 2>(f29_mul<Fq>(r, f29_sub<Fq, 8>(q, x3)), yp);  // < 3.27p
+    const F29 x3 = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sqr<Fq>(r), ppp), q), q);  // < 7.15p
+    // R (Q - X3) - Y1 PPP as one reduction: R (Q - X3 + 8p) + Y1 (2p - PPP);
+    // (5.01 x 9.07 + 3.3 x 2) p^2 / R' + p with R' / p ~ 169: < 1.31p
+    acc.y = f29_mul2<Fq>(r, f29_sub<Fq, 8>(q, x3), acc.y, f29_sub<Fq, 2>(F29{}, ppp));
     acc.x = x3;
 }
 
@@ -112,14 +114,15 @@ APG_HD void xyzz29_accumulate(Xyzz29 &acc, bool &id, const Affine &a, bool neg) 
 APG_HD Xyzz29 xyzz29_dbl(const Xyzz29 &p) {
     if (xyzz29_is_id(p)) return p;
     const F29 u = f29_add(p.y, p.y);             // < 6.6p
-    const F29 v = f29_mul<Fq>(u, u);             // < 1.26p
+    const F29 v = f29_sqr<Fq>(u);                // < 1.26p
     const F29 w = f29_mul<Fq>(u, v);             // < 1.05p
     const F29 s = f29_mul<Fq>(p.x, v);           // < 1.06p
-    const F29 x2 = f29_mul<Fq>(p.x, p.x);        // < 1.31p
+    const F29 x2 = f29_sqr<Fq>(p.x);             // < 1.31p
     const F29 m = f29_add(f29_add(x2, x2), x2);  // < 3.93p
     Xyzz29 r;
-    r.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(m, m), s), s);                       // < 5.1p
-    r.y = f29_sub<Fq, 2>(f29_mul<Fq>(m, f29_sub<Fq, 8>(s, r.x)), f29_mul<Fq>(w, p.y));  // < 3.22p
+    r.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sqr<Fq>(m), s), s);  // < 5.1p
+    // M (S - X3) - W Y as one reduction: M (S - X3 + 8p) + W (4p - Y), Y < 3.3p
+    r.y = f29_mul2<Fq>(m, f29_sub<Fq, 8>(s, r.x), w, f29_sub<Fq, 4>(F29{}, p.y));  // < 1.24p
     r.zz = f29_mul<Fq>(v, p.zz);                 // < 1.01p
     r.zzz = f29_mul<Fq>(w, p.zzz);               // < 1.01p
     return r;
@@ -139,12 +142,13 @@ APG_HD Xyzz29 xyzz29_add(const Xyzz29 &p, const Xyzz29 &q) {
         if (f29_is_zero<Fq, 3>(r)) return xyzz29_dbl(p);
         return xyzz29_identity();
     }
-    const F29 pp = f29_mul<Fq>(pd, pd);      // < 1.06p
+    const F29 pp = f29_sqr<Fq>(pd);          // < 1.06p
     const F29 ppp = f29_mul<Fq>(pd, pp);     // < 1.02p
     const F29 qq = f29_mul<Fq>(u1, pp);      // < 1.01p
     Xyzz29 o;
-    o.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_mul<Fq>(r, r), ppp), qq), qq);  // < 7.06p
-    o.y = f29_sub<Fq, 2>(f29_mul<Fq>(r, f29_sub<Fq, 8>(qq, o.x)), f29_mul<Fq>(s1, ppp));  // < 3.17p
+    o.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sqr<Fq>(r), ppp), qq), qq);  // < 7.06p
+    // R (QQ - X3) - S1 PPP as one reduction: R (QQ - X3 + 8p) + S1 (2p - PPP)
+    o.y = f29_mul2<Fq>(r, f29_sub<Fq, 8>(qq, o.x), s1, f29_sub<Fq, 2>(F29{}, ppp));  // < 1.18p
     o.zz = f29_mul<Fq>(f29_mul<Fq>(p.zz, q.zz), pp);      // < 1.01p
     o.zzz = f29_mul<Fq>(f29_mul<Fq>(p.zzz, q.zzz), ppp);  // < 1.01p
     return o;

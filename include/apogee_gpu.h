@@ -117,7 +117,8 @@ int apg_pcs_batch_open_stacked_dev(apg_ctx *ctx, apg_srs *srs, const void *cols_
 /* The MSM accumulation's Fq arithmetic in nine 29-bit words with lazy
  * reduction (field29.cuh), on the host (ctx NULL) or in a kernel on ctx's
  * device (host buffers in and out): op 0 = a b, 1 = a + b, 2 = a - b,
- * 3 = ((a + b)(a - b) + a^2) - b; outputs fully reduced Montgomery Fq. */
+ * 3 = ((a + b)(a - b) + a^2) - b, 4 = a^2 (the squaring), 5 = a b + a a (two
+ * products under one reduction); outputs fully reduced Montgomery Fq. */
 int apg_debug_fq29_batch(apg_ctx *ctx, int op, const uint8_t *a_mont, const uint8_t *b_mont, size_t n,
                          uint8_t *out_mont);
 /* The MSM bucket loop's chains: for each case c, the sum over points

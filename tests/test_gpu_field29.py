@@ -25,7 +25,7 @@ def ctx(apg):
 
 def test_fq29_on_device_matches_oracle(apg, orc, ctx):
     a, b, want = fq29_cases(orc)
-    for op in range(4):
+    for op in range(6):
         got = fq29_run(apg, ctx.h, op, a, b)
         bad = np.nonzero((got != want[op]).any(axis=1))[0]
         assert bad.size == 0, (op, bad[:8].tolist())

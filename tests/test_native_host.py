@@ -200,7 +200,7 @@ def _limbs(xs):
 
 def fq29_cases(orc, n=40_000):
     """Operand pairs for the lazy 29-bit Fq and the oracle's results for ops
-    0..3: random values, the edge grid (0, 1, p - 1, 2^k, p - 2^k, R mod p),
+    0..5: random values, the edge grid (0, 1, p - 1, 2^k, p - 2^k, R mod p),
     and non-canonical inputs x + p (below 2p), which the lazy form accepts."""
     p = FQ_MODULUS
     rng = np.random.default_rng(2029)
@@ -214,7 +214,7 @@ def fq29_cases(orc, n=40_000):
     assert max(ra + rb) < 2 * p
     a, b = _limbs(xa), _limbs(xb)
     L = orc.lib()
-    want = {op: np.zeros_like(a) for op in range(4)}
+    want = {op: np.zeros_like(a) for op in range(6)}
     s, d, q = (np.zeros(4, dtype=np.uint64) for _ in range(3))
     for i in range(a.shape[0]):
         pa, pb = orc.ptr(a[i]), orc.ptr(b[i])
@@ -225,6 +225,9 @@ def fq29_cases(orc, n=40_000):
         L.orc_mul(1, pa, pa, orc.ptr(q))
         L.orc_add(1, orc.ptr(d), orc.ptr(q), orc.ptr(s))
         L.orc_sub(1, orc.ptr(s), pb, orc.ptr(want[3][i]))
+        L.orc_mul(1, pa, pa, orc.ptr(want[4][i]))
+        L.orc_mul(1, pa, pb, orc.ptr(d))
+        L.orc_add(1, orc.ptr(d), orc.ptr(want[4][i]), orc.ptr(want[5][i]))
     return _limbs(ra), _limbs(rb), want
 
 
@@ -238,12 +241,13 @@ def fq29_run(apg, ctx_handle, op, a, b):
 def test_fq29_lazy_arithmetic_matches_oracle(apg, orc):
     """The MSM accumulation's Fq in nine 29-bit words with lazy reduction
     (results kept in [0, 2p)), compiled for the host, against the oracle's
-    u64 Montgomery Fq: multiply, add, subtract and a chain through the whole
-    lazy range, with non-canonical inputs and the edge grid."""
+    u64 Montgomery Fq: multiply, add, subtract, a chain through the whole
+    lazy range, the squaring and two products under one reduction, with
+    non-canonical inputs and the edge grid."""
     a, b, want = fq29_cases(orc)
-    for op in range(4):
+    for op in range(6):
         assert np.array_equal(fq29_run(apg, None, op, a, b), want[op]), op
-    assert apg.lib().apg_debug_fq29_batch(None, 4, a.ctypes.data, b.ctypes.data, 1, a.ctypes.data) != 0
+    assert apg.lib().apg_debug_fq29_batch(None, 6, a.ctypes.data, b.ctypes.data, 1, a.ctypes.data) != 0
     off = np.zeros(2, dtype=np.uint64)
     assert apg.lib().apg_debug_accumulate29(None, 3, a.ctypes.data, a.ctypes.data, off.ctypes.data, 1,
                                             a.ctypes.data) != 0
