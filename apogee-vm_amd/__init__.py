@@ -172,6 +172,7 @@ def _declare(L):
     d("apg_ctx_reset_stats", [VP])
     d("apg_synthetic_fill", [U64, I32, SZ, VP])
     d("apg_debug_mul32_batch", [I32, VP, VP, SZ, VP])
+    d("apg_debug_poseidon2_batch", [I32, VP, SZ, VP])
     d("apg_pcs_open", [VP, VP, VP, SZ, VP, VP, VP, VP, VP])
     d("apg_pcs_open_dev", [VP, VP, VP, SZ, VP, VP, VP, VP, VP])
     d("apg_pcs_batch_open_stacked", [VP, VP, VP, VP, SZ, SZ, VP, VP, VP, SZ, VP, VP, VP])

@@ -158,7 +158,24 @@ Gate make_gate(const apg_term *terms, size_t n_terms) {
 
 struct CbError {};
 
+// append_scalars(SUMCHECK_ROUND, coeffs) then challenge(SUMCHECK_CHALLENGE)
+// (crates/sumcheck/src/lib.rs:392-395), with the message's tag and length
+// split off as the prefix the round loops run while a round's kernel runs.
+RoundFn transcript_round(Transcript &t) {
+    return RoundFn(
+        [&t](const Fe *coeffs, int nc) -> Fe {
+            for (int i = 0; i < nc; i++) t.observe(coeffs[i]);
+            return t.challenge(TAG_SUMCHECK_CHALLENGE);
+        },
+        [&t](int nc) {
+            t.observe(f_from_u64<Fr>(TAG_SUMCHECK_ROUND));
+            t.observe(f_from_u64<Fr>((uint64_t)nc));
+        });
+}
+
 RoundFn wrap_round(apg_round_fn fn, void *user) {
+    // the library's own transcript as the callback: call it natively
+    if (fn == &apg_transcript_round && user) return transcript_round(static_cast<apg_transcript *>(user)->t);
     return [fn, user](const Fe *coeffs, int nc) -> Fe {
         std::vector<uint8_t> buf(32 * (size_t)nc);
         for (int i = 0; i < nc; i++) store_fe(&buf[32 * (size_t)i], coeffs[i]);
@@ -238,28 +255,27 @@ void prove_shard_impl(Context &c, apg_srs *srs, const Fe *cols, size_t num_vars,
     for (size_t i = 0; i < k; i++) apg_g1_to_wire(&cm_mont[64 * i], commitments_wire + 64 * i);
     t.append_g1_wire(TAG_COMMITMENT, commitments_wire, k);
     const Fe rho = t.challenge(TAG_MERCURY_BATCH);
+    // table o has weights rho^(o k + c): rho^(o k) times sum_c rho^c col_c
     const int f = degree - 1;
-    std::vector<Fe> w((size_t)f * k);
+    std::vector<Fe> w(k), scale(f);
     Fe pw = f_one<Fr>();
-    for (int o = 0; o < f; o++)
-        for (size_t col = 0; col < k; col++) {
-            w[o * k + col] = pw;
-            pw = f_mul<Fr>(pw, rho);
-        }
+    for (size_t col = 0; col < k; col++) {
+        w[col] = pw;
+        pw = f_mul<Fr>(pw, rho);
+    }
+    scale[0] = f_one<Fr>();
+    for (int o = 1; o < f; o++) scale[o] = f_mul<Fr>(scale[o - 1], pw);
     Fe *comb = c.d<Fe>("shard.comb", (size_t)f * n);
     std::vector<Fe *> outs(f);
     for (int o = 0; o < f; o++) outs[o] = comb + (size_t)o * n;
-    combine_columns_device(c, cols, n, (int)k, w.data(), f, outs.data());
+    combine_columns_device(c, cols, n, (int)k, w.data(), scale.data(), f, outs.data());
     std::vector<uint8_t> r(32 * num_vars);
     for (size_t i = 0; i < num_vars; i++) store_fe(&r[32 * i], t.challenge(TAG_SUMCHECK_CHALLENGE));
     Gate g;
     g.nterms = 1;
     g.terms[0].coef = f_one<Fr>();
     for (int q = 0; q < 3; q++) g.terms[0].f[q] = q < f ? q : -1;
-    RoundFn fn = [&t](const Fe *coeffs, int nc) -> Fe {
-        t.append_scalars(TAG_SUMCHECK_ROUND, coeffs, nc);
-        return t.challenge(TAG_SUMCHECK_CHALLENGE);
-    };
+    const RoundFn fn = transcript_round(t);
     sumcheck_common(c, g, outs.data(), (size_t)f, num_vars, r.data(), fn, rounds_out, nullptr,
                     final_evals_out);
     std::vector<Fe> fin(f);
@@ -1044,10 +1060,7 @@ int apg_zerocheck_prove(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
         std::vector<uint8_t> r(32 * (num_vars + 1));
         for (size_t i = 0; i < num_vars; i++) store_fe(&r[32 * i], t->t.challenge(TAG_SUMCHECK_CHALLENGE));
         auto tabs = upload_tables(ctx->c, tables_mont, k, size_t(1) << num_vars);
-        RoundFn fn = [t](const Fe *coeffs, int nc) -> Fe {
-            t->t.append_scalars(TAG_SUMCHECK_ROUND, coeffs, nc);
-            return t->t.challenge(TAG_SUMCHECK_CHALLENGE);
-        };
+        const RoundFn fn = transcript_round(t->t);
         sumcheck_common(ctx->c, g, tabs.data(), k, num_vars, r.data(), fn, rounds_out, nullptr,
                         final_evals_out);
         std::vector<Fe> fin(k);
@@ -1473,6 +1486,19 @@ int apg_debug_mul32_batch(int which, const uint8_t *a, const uint8_t *b, size_t 
         store_fe(out + 32 * i, which ? f_mul32<Fq>(x, y) : f_mul32<Fr>(x, y));
     }
     return APG_OK;
+}
+
+int apg_debug_poseidon2_batch(int form, const uint8_t *states, size_t n, uint8_t *out) {
+    return guarded([&] {
+        if (form < 0 || form > 2 || (n && (!states || !out))) throw ApgError(APG_ERR_ARG, "bad arguments");
+        for (size_t i = 0; i < n; i++) {
+            Fe st[3];
+            for (int l = 0; l < 3; l++) st[l] = load_fe(states + 96 * i + 32 * l);
+            if (!poseidon2_permute_form(form, st))
+                throw ApgError(APG_ERR_ARG, "this CPU lacks BMI2/ADX (form 2)");
+            for (int l = 0; l < 3; l++) store_fe(out + 96 * i + 32 * l, st[l]);
+        }
+    });
 }
 
 int apg_g1_to_wire(const uint8_t mont[64], uint8_t wire[64]) {

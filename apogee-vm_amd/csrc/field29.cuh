@@ -245,6 +245,40 @@ APG_HD F29 f29_mul2(const F29 &a, const F29 &b, const F29 &c, const F29 &d) {
     return r;
 }
 
+// sum_g a_g b_g over G <= 6 pairs with one reduction: (sum a_g b_g + M p) /
+// 2^261, below sum a_g b_g / R' + p. A column holds at most 9 (G + 1)
+// products below 2^58: 63 of them at G = 6, below 2^64.
+template <class P, int G>
+APG_HD F29 f29_dot(const F29 *a, const F29 *b) {
+    static_assert(G >= 1 && G <= 6, "at most six products per reduction");
+    uint64_t t[10];
+#pragma unroll
+    for (int j = 0; j < 10; j++) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int j = 0; j < 9; j++) t[j] += (uint64_t)a[g].v[j] * b[g].v[i];
+        const uint32_t m = ((uint32_t)t[0] * P::inv) & W29;
+#pragma unroll
+        for (int j = 0; j < 9; j++) t[j] += (uint64_t)m * mod29<P>(j);
+        const uint64_t cy = t[0] >> 29;
+#pragma unroll
+        for (int j = 0; j < 9; j++) t[j] = t[j + 1];
+        t[0] += cy;
+    }
+    F29 r;
+    uint64_t cy = 0;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        const uint64_t s = t[j] + cy;
+        r.v[j] = (uint32_t)s & W29;
+        cy = s >> 29;
+    }
+    return r;
+}
+
 // a + b, not reduced (the bound is the sum of the bounds)
 APG_HD F29 f29_add(const F29 &a, const F29 &b) {
     F29 r;

@@ -658,6 +658,7 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
         }
         HIP_CHECK(hipGetLastError());
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));
+        round_fn.begin(4);  // the transcript's message prefix, while the kernel runs
         ctx.sync();
         if (ctx.profiling) {
             float ms = 0;

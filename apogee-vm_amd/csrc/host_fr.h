@@ -4,6 +4,10 @@
 // permutation of the transcript and the last sumcheck rounds.
 #pragma once
 #include <stdint.h>
+#if defined(__x86_64__) && !defined(__HIP_DEVICE_COMPILE__)
+#include <cpuid.h>
+#define APG_HOST_ADX 1
+#endif
 
 #include "field.cuh"
 
@@ -88,6 +92,159 @@ inline W4 w4_mul_lazy(const W4 &a, const W4 &b) {
     }
     return W4{{t0, t1, t2, t3}};
 }
+
+#ifdef APG_HOST_ADX
+// The same CIOS product on x86-64 with BMI2 and ADX: mulx leaves the flags
+// alone, so each row runs two carry chains at once (adox adds the low halves
+// of the partial products, adcx the high halves one word up) and the
+// running sum rotates through five registers instead of being shifted. The
+// bounds are those of w4_mul_lazy: inputs below 2p, output below 2p, every
+// intermediate sum below 2^320. Callers check w4_have_adx() first.
+alignas(64) inline const uint64_t W4_MOD_INV[5] = {RP0, RP1, RP2, RP3, FrP::inv64};
+static_assert((uint64_t)(RP0 * FrP::inv64) == ~uint64_t(0), "-1/p mod 2^64");
+
+inline W4 w4_mul_lazy_adx(const W4 &a, const W4 &b) {
+    uint64_t t0, t1, t2, t3, t4, lo, hi, zero;
+    asm(
+        "movq %[b0], %%rdx\n\t"
+        "mulxq %[a0], %[t0], %[t1]\n\t"
+        "mulxq %[a1], %[lo], %[t2]\n\t"
+        "addq %[lo], %[t1]\n\t"
+        "mulxq %[a2], %[lo], %[t3]\n\t"
+        "adcq %[lo], %[t2]\n\t"
+        "mulxq %[a3], %[lo], %[t4]\n\t"
+        "adcq %[lo], %[t3]\n\t"
+        "adcq $0, %[t4]\n\t"
+        "movq %[t0], %%rdx\n\t"
+        "imulq %[inv], %%rdx\n\t"
+        "xorl %k[zero], %k[zero]\n\t"
+        "mulxq %[p0], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t0]\n\t"
+        "adcxq %[hi], %[t1]\n\t"
+        "mulxq %[p1], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t1]\n\t"
+        "adcxq %[hi], %[t2]\n\t"
+        "mulxq %[p2], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t2]\n\t"
+        "adcxq %[hi], %[t3]\n\t"
+        "mulxq %[p3], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t3]\n\t"
+        "adcxq %[hi], %[t4]\n\t"
+        "adoxq %[zero], %[t4]\n\t"
+        "movq %[b1], %%rdx\n\t"
+        "xorl %k[zero], %k[zero]\n\t"
+        "movq %[zero], %[t0]\n\t"
+        "mulxq %[a0], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t1]\n\t"
+        "adcxq %[hi], %[t2]\n\t"
+        "mulxq %[a1], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t2]\n\t"
+        "adcxq %[hi], %[t3]\n\t"
+        "mulxq %[a2], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t3]\n\t"
+        "adcxq %[hi], %[t4]\n\t"
+        "mulxq %[a3], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t4]\n\t"
+        "adcxq %[hi], %[t0]\n\t"
+        "adoxq %[zero], %[t0]\n\t"
+        "movq %[t1], %%rdx\n\t"
+        "imulq %[inv], %%rdx\n\t"
+        "xorl %k[zero], %k[zero]\n\t"
+        "mulxq %[p0], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t1]\n\t"
+        "adcxq %[hi], %[t2]\n\t"
+        "mulxq %[p1], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t2]\n\t"
+        "adcxq %[hi], %[t3]\n\t"
+        "mulxq %[p2], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t3]\n\t"
+        "adcxq %[hi], %[t4]\n\t"
+        "mulxq %[p3], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t4]\n\t"
+        "adcxq %[hi], %[t0]\n\t"
+        "adoxq %[zero], %[t0]\n\t"
+        "movq %[b2], %%rdx\n\t"
+        "xorl %k[zero], %k[zero]\n\t"
+        "movq %[zero], %[t1]\n\t"
+        "mulxq %[a0], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t2]\n\t"
+        "adcxq %[hi], %[t3]\n\t"
+        "mulxq %[a1], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t3]\n\t"
+        "adcxq %[hi], %[t4]\n\t"
+        "mulxq %[a2], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t4]\n\t"
+        "adcxq %[hi], %[t0]\n\t"
+        "mulxq %[a3], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t0]\n\t"
+        "adcxq %[hi], %[t1]\n\t"
+        "adoxq %[zero], %[t1]\n\t"
+        "movq %[t2], %%rdx\n\t"
+        "imulq %[inv], %%rdx\n\t"
+        "xorl %k[zero], %k[zero]\n\t"
+        "mulxq %[p0], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t2]\n\t"
+        "adcxq %[hi], %[t3]\n\t"
+        "mulxq %[p1], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t3]\n\t"
+        "adcxq %[hi], %[t4]\n\t"
+        "mulxq %[p2], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t4]\n\t"
+        "adcxq %[hi], %[t0]\n\t"
+        "mulxq %[p3], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t0]\n\t"
+        "adcxq %[hi], %[t1]\n\t"
+        "adoxq %[zero], %[t1]\n\t"
+        "movq %[b3], %%rdx\n\t"
+        "xorl %k[zero], %k[zero]\n\t"
+        "movq %[zero], %[t2]\n\t"
+        "mulxq %[a0], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t3]\n\t"
+        "adcxq %[hi], %[t4]\n\t"
+        "mulxq %[a1], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t4]\n\t"
+        "adcxq %[hi], %[t0]\n\t"
+        "mulxq %[a2], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t0]\n\t"
+        "adcxq %[hi], %[t1]\n\t"
+        "mulxq %[a3], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t1]\n\t"
+        "adcxq %[hi], %[t2]\n\t"
+        "adoxq %[zero], %[t2]\n\t"
+        "movq %[t3], %%rdx\n\t"
+        "imulq %[inv], %%rdx\n\t"
+        "xorl %k[zero], %k[zero]\n\t"
+        "mulxq %[p0], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t3]\n\t"
+        "adcxq %[hi], %[t4]\n\t"
+        "mulxq %[p1], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t4]\n\t"
+        "adcxq %[hi], %[t0]\n\t"
+        "mulxq %[p2], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t0]\n\t"
+        "adcxq %[hi], %[t1]\n\t"
+        "mulxq %[p3], %[lo], %[hi]\n\t"
+        "adoxq %[lo], %[t1]\n\t"
+        "adcxq %[hi], %[t2]\n\t"
+        "adoxq %[zero], %[t2]\n\t"
+        : [t0] "=&r"(t0), [t1] "=&r"(t1), [t2] "=&r"(t2), [t3] "=&r"(t3), [t4] "=&r"(t4), [lo] "=&r"(lo),
+          [hi] "=&r"(hi), [zero] "=&r"(zero)
+        : [a0] "m"(a.v[0]), [a1] "m"(a.v[1]), [a2] "m"(a.v[2]), [a3] "m"(a.v[3]), [b0] "m"(b.v[0]),
+          [b1] "m"(b.v[1]), [b2] "m"(b.v[2]), [b3] "m"(b.v[3]), [p0] "m"(W4_MOD_INV[0]),
+          [p1] "m"(W4_MOD_INV[1]), [p2] "m"(W4_MOD_INV[2]), [p3] "m"(W4_MOD_INV[3]), [inv] "m"(W4_MOD_INV[4])
+        : "rdx", "cc");
+    return W4{{t4, t0, t1, t2}};
+}
+
+// BMI2 (mulx) and ADX (adcx, adox): CPUID leaf 7, EBX bits 8 and 19.
+inline bool w4_have_adx() {
+    unsigned a = 0, b = 0, c = 0, d = 0;
+    if (!__get_cpuid_count(7, 0, &a, &b, &c, &d)) return false;
+    return ((b >> 8) & 1) && ((b >> 19) & 1);
+}
+#else
+inline bool w4_have_adx() { return false; }
+#endif
 
 inline W4 w4_add_lazy(const W4 &a, const W4 &b) {
     u128 x = (u128)a.v[0] + b.v[0];

@@ -1,6 +1,7 @@
 // Sumcheck round evaluation and fold over multilinear Fr tables on gfx950.
 #pragma once
 #include <functional>
+#include <type_traits>
 #include <vector>
 
 #include "field.cuh"
@@ -36,8 +37,33 @@ struct Gate {
 // 64 of limb i / 64), u8, u16, u32.
 enum : uint8_t { SC_FR = 0, SC_U1 = 1, SC_U8 = 2, SC_U16 = 3, SC_U32 = 4 };
 
-// Per-round hook: receives the d+1 ascending coefficients, returns the challenge.
-using RoundFn = std::function<Fe(const Fe *coeffs, int ncoeffs)>;
+// Per-round hook: receives the d+1 ascending coefficients, returns the
+// challenge. A hook may split off a prefix, the part of its absorption that
+// comes before the coefficients (a transcript's message tag and length): the
+// round loops call begin() once the round's kernel is queued, so the prefix
+// runs on the host while the kernel runs on the device; a call without a
+// preceding begin() runs the prefix itself first.
+struct RoundFn {
+    std::function<Fe(const Fe *coeffs, int ncoeffs)> fn;
+    std::function<void(int ncoeffs)> prefix;
+    mutable bool begun = false;
+
+    RoundFn() = default;
+    template <class F, class = std::enable_if_t<std::is_invocable_r_v<Fe, F &, const Fe *, int>>>
+    RoundFn(F f) : fn(std::move(f)) {}
+    RoundFn(std::function<Fe(const Fe *, int)> f, std::function<void(int)> p)
+        : fn(std::move(f)), prefix(std::move(p)) {}
+
+    void begin(int ncoeffs) const {
+        if (prefix && !begun) prefix(ncoeffs);
+        begun = true;
+    }
+    Fe operator()(const Fe *coeffs, int ncoeffs) const {
+        begin(ncoeffs);
+        begun = false;
+        return fn(coeffs, ncoeffs);
+    }
+};
 
 // gkr::prove_sumcheck (crates/gkr/src/lib.rs:792-899) with a gate summand.
 // Tables are device buffers of 2^num_vars elements each; they are consumed
@@ -67,9 +93,9 @@ size_t sumcheck_noalloc_bytes(int k, int num_vars);
 // (crates/sumcheck/src/lib.rs:294-305 for d = 3, Newton form beyond).
 void interpolate_host(const Fe *vals, int d, Fe *coeffs);
 
-// out[i] = sum_c w[c] * cols[c][i] for each of `outs` weight rows; cols are K
-// device columns of n elements, w is outs x K host weights.
+// out[o][i] = scale[o] * sum_c w[c] * cols[c][i] for o < outs; cols are K
+// device columns of n elements, w (K) and scale (outs) host values.
 void combine_columns_device(Context &ctx, const Fe *cols_dev, size_t n, int K, const Fe *w_host,
-                            int outs, Fe *const *out_dev);
+                            const Fe *scale_host, int outs, Fe *const *out_dev);
 
 }  // namespace apg

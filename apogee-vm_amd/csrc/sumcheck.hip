@@ -195,34 +195,47 @@ __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateAr
     reduce_nodes<NN>(acc, nn, partials, counter, out);
 }
 
-// Weights in the Fq-style lazy 29-bit form over Fr, in the domain R' = 2^261
-// (field29.cuh): a column value c R (its 32-bit Montgomery image, re-sliced)
-// times w R' gives c w R, the 32-bit Montgomery image of c w, below 1.01 r;
-// the products sum with no carries, and one reduction and
-// a repack finish the element (24 columns at most: the sum stays below
-// 32 r). A 29-bit multiply is ~230 instructions against ~430 for the 32-bit
-// CIOS one.
+// out_o = scale_o * sum_c w_c col_c, o < OUTS (the shard's tables are
+// rho^(o K) * sum_c rho^c col_c). Weights and scales are in the Fq-style lazy
+// 29-bit form over Fr, in the domain R' = 2^261 (field29.cuh): a column value
+// c R (its 32-bit Montgomery image, re-sliced) times w R' gives c w R. The
+// sum takes up to six columns per reduction (f29_dot: ~90 reduction
+// multiply-adds per six columns instead of per column), each group below
+// 1.04 r; at most four groups (24 columns) stay below 4.2 r, and each scale
+// multiply brings that below 1.03 r before the repack.
 struct CombineArgs {
-    F29 w[4][SC_MAX_TABLES];
+    F29 w[SC_MAX_TABLES];
+    F29 scale[4];
     Fe *out[4];
 };
 
+template <int G>
+__device__ __forceinline__ F29 combine_group(const Fe *__restrict__ cols, size_t n, size_t i, int c0,
+                                             const CombineArgs &a) {
+    F29 v[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) v[g] = f29_from32(ldfe(cols, (size_t)(c0 + g) * n + i));
+    return f29_dot<Fr, G>(v, &a.w[c0]);
+}
+
 template <int OUTS>
-__global__ void k_combine(const Fe *__restrict__ cols, size_t n, int K, CombineArgs a) {
+__global__ void __launch_bounds__(256) k_combine(const Fe *__restrict__ cols, size_t n, int K, CombineArgs a) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    F29 s[OUTS];
-#pragma unroll
-    for (int o = 0; o < OUTS; o++)
-#pragma unroll
-        for (int j = 0; j < 9; j++) s[o].v[j] = 0;
-    for (int c = 0; c < K; c++) {
-        const F29 v = f29_from32(ldfe(cols, (size_t)c * n + i));
-#pragma unroll
-        for (int o = 0; o < OUTS; o++) s[o] = f29_add(s[o], f29_mul<Fr>(v, a.w[o][c]));
+    F29 s{};
+    int c = 0;
+    for (; c + 6 <= K; c += 6) s = f29_add(s, combine_group<6>(cols, n, i, c, a));
+    switch (K - c) {
+        case 1: s = f29_add(s, combine_group<1>(cols, n, i, c, a)); break;
+        case 2: s = f29_add(s, combine_group<2>(cols, n, i, c, a)); break;
+        case 3: s = f29_add(s, combine_group<3>(cols, n, i, c, a)); break;
+        case 4: s = f29_add(s, combine_group<4>(cols, n, i, c, a)); break;
+        case 5: s = f29_add(s, combine_group<5>(cols, n, i, c, a)); break;
+        default: break;
     }
+    stfe(a.out[0], i, f29_pack_reduced<Fr>(s));
 #pragma unroll
-    for (int o = 0; o < OUTS; o++) stfe(a.out[o], i, f29_pack_reduced<Fr>(s[o]));
+    for (int o = 1; o < OUTS; o++) stfe(a.out[o], i, f29_pack_reduced<Fr>(f29_mul<Fr>(s, a.scale[o])));
 }
 
 }  // namespace
@@ -322,12 +335,14 @@ static Fe eq_fe(const Fe &r, const Fe &x) {
 
 static Fe eq_at(const Fe &r, int x) { return eq_fe(r, f_from_u64<Fr>((uint64_t)x)); }
 
-void combine_columns_device(Context &ctx, const Fe *cols, size_t n, int K, const Fe *w, int outs,
-                            Fe *const *out) {
-    if (K > SC_MAX_TABLES || outs > 4) throw ApgError(-2, "combine: too many columns/outputs");
+void combine_columns_device(Context &ctx, const Fe *cols, size_t n, int K, const Fe *w, const Fe *scale,
+                            int outs, Fe *const *out) {
+    if (K < 1 || K > SC_MAX_TABLES || outs < 1 || outs > 4)
+        throw ApgError(-2, "combine: too many columns/outputs");
     CombineArgs a;
+    for (int c = 0; c < K; c++) a.w[c] = f29_from_mont32<Fr>(w[c]);
     for (int o = 0; o < outs; o++) {
-        for (int c = 0; c < K; c++) a.w[o][c] = f29_from_mont32<Fr>(w[o * K + c]);
+        a.scale[o] = f29_from_mont32<Fr>(scale[o]);
         a.out[o] = out[o];
     }
     const dim3 g(grid_for(n, 256)), b(256);
@@ -823,6 +838,7 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
         const int nn = full ? D + 1 : D;
         if (!mapped_nodes)
             HIP_CHECK(hipMemcpyAsync(hvals, hvals_dev, nn * sizeof(Fe), hipMemcpyDeviceToHost, st));
+        round_fn.begin(d + 1);  // the transcript's message prefix, while the kernel runs
         ctx.sync();
         if (product) {  // the product kernel's node sums carry 2^(-5 k), and 2^-5 more for E = lo hi
             Fe scale = prod_scale;

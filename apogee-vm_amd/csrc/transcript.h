@@ -14,6 +14,9 @@
 namespace apg {
 
 void poseidon2_permute_host(Fe s[3]);
+// form 0 as poseidon2_permute_host, 1 with the portable multiply, 2 with the
+// mulx/adcx/adox one; false if the form is unknown or the CPU lacks it
+bool poseidon2_permute_form(int form, Fe s[3]);
 
 struct Transcript {
     Fe state[3];

@@ -39,10 +39,11 @@ const RoundConstants &rc() {
     return k;
 }
 
+template <W4 (*MUL)(const W4 &, const W4 &)>
 inline W4 sbox(const W4 &x) {
-    const W4 x2 = w4_mul_lazy(x, x);
-    const W4 x4 = w4_mul_lazy(x2, x2);
-    return w4_mul_lazy(x4, x);
+    const W4 x2 = MUL(x, x);
+    const W4 x4 = MUL(x2, x2);
+    return MUL(x4, x);
 }
 
 inline void ext_matrix(W4 s[3]) {
@@ -50,8 +51,10 @@ inline void ext_matrix(W4 s[3]) {
     for (int i = 0; i < 3; i++) s[i] = w4_add_lazy(s[i], sum);
 }
 
-inline void int_matrix(W4 s[3]) {
-    const W4 sum = w4_add_lazy(w4_add_lazy(s[0], s[1]), s[2]);
+// s12 = s[1] + s[2], formed before s[0]'s S-box so that only two additions
+// follow the S-box on the chain from one partial round to the next
+inline void int_matrix(W4 s[3], const W4 &s12) {
+    const W4 sum = w4_add_lazy(s[0], s12);
     s[0] = w4_add_lazy(s[0], sum);
     s[1] = w4_add_lazy(s[1], sum);
     s[2] = w4_add_lazy(w4_add_lazy(s[2], s[2]), sum);
@@ -82,24 +85,54 @@ void fe_to_bytes_fq(const Fe &a, uint8_t out[32]) { to_bytes(a, out, true); }
 bool fe_from_bytes_fr(const uint8_t in[32], Fe &out) { return from_bytes<Fr>(in, out); }
 bool fe_from_bytes_fq(const uint8_t in[32], Fe &out) { return from_bytes<Fq>(in, out); }
 
-void poseidon2_permute_host(Fe st[3]) {
+namespace {
+
+template <W4 (*MUL)(const W4 &, const W4 &)>
+void permute(Fe st[3]) {
     const RoundConstants &k = rc();
     W4 s[3] = {w4_of(st[0]), w4_of(st[1]), w4_of(st[2])};
     ext_matrix(s);
     for (int r = 0; r < 4; r++) {
-        for (int l = 0; l < 3; l++) s[l] = sbox(w4_add_lazy(s[l], k.ini[r][l]));
+        for (int l = 0; l < 3; l++) s[l] = sbox<MUL>(w4_add_lazy(s[l], k.ini[r][l]));
         ext_matrix(s);
     }
     for (int i = 0; i < 56; i++) {
-        s[0] = sbox(w4_add_lazy(s[0], k.mid[i]));
-        int_matrix(s);
+        const W4 s12 = w4_add_lazy(s[1], s[2]);
+        s[0] = sbox<MUL>(w4_add_lazy(s[0], k.mid[i]));
+        int_matrix(s, s12);
     }
     for (int r = 0; r < 4; r++) {
-        for (int l = 0; l < 3; l++) s[l] = sbox(w4_add_lazy(s[l], k.ter[r][l]));
+        for (int l = 0; l < 3; l++) s[l] = sbox<MUL>(w4_add_lazy(s[l], k.ter[r][l]));
         ext_matrix(s);
     }
     // below 2p: one subtraction of p makes each word canonical
     for (int l = 0; l < 3; l++) st[l] = fe_of(w4_reduce(s[l].v[0], s[l].v[1], s[l].v[2], s[l].v[3]));
+}
+
+}  // namespace
+
+// The permutation is a chain of 240 dependent multiplies, so the multiply's
+// latency sets its speed: the mulx/adcx/adox form where the CPU has it.
+void poseidon2_permute_host(Fe st[3]) {
+#ifdef APG_HOST_ADX
+    static const bool adx = w4_have_adx();
+    if (adx) return permute<w4_mul_lazy_adx>(st);
+#endif
+    permute<w4_mul_lazy>(st);
+}
+
+bool poseidon2_permute_form(int form, Fe st[3]) {
+    switch (form) {
+        case 0: poseidon2_permute_host(st); return true;
+        case 1: permute<w4_mul_lazy>(st); return true;
+#ifdef APG_HOST_ADX
+        case 2:
+            if (!w4_have_adx()) return false;
+            permute<w4_mul_lazy_adx>(st);
+            return true;
+#endif
+        default: return false;
+    }
 }
 
 Transcript::Transcript() {

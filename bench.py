@@ -117,6 +117,7 @@ def run_single(ctx, srs, args, nv=20, k=8, deg=3, steps=10, warmup=2):
         "g1_msm_points_per_s": round(st["msm_points"] / (st["msm_ms"] * 1e-3), 1) if st["msm_ms"] else None,
         "msm_ms_per_shard": round(st["msm_ms"] / steps, 3),
         "sumcheck_ms_per_shard": round(st["sumcheck_ms"] / steps, 3),
+        "host_poseidon2_us": host_permutation_us(),
         "sumcheck_kernel_gbs": round(sc_gbs, 1),
         "sumcheck_hbm_frac": round(sc_gbs * 1e9 / PEAK_HBM, 4),
         "latency_ms": {"p50": round(float(np.percentile(np.array(lat) * 1e3, 50)), 3),
@@ -127,6 +128,25 @@ def run_single(ctx, srs, args, nv=20, k=8, deg=3, steps=10, warmup=2):
                               "frac_of_peak": round(ff_rate / PEAK_IMUL, 4)},
     }
     return res, roof, (spec, buf, proof)
+
+
+def host_permutation_us(n=2000):
+    """Microseconds per Poseidon2 permutation on this host, the transcript's
+    dispatched multiply (form 0) and the portable one (form 1): every round of
+    a sumcheck waits on a few of these."""
+    st = np.zeros((n, 3, 4), dtype=np.uint64)
+    st[:, :, 0] = np.arange(3 * n, dtype=np.uint64).reshape(n, 3)
+    out = np.zeros_like(st)
+    res = {}
+    for form, name in ((0, "dispatched"), (1, "portable")):
+        best = float("inf")
+        for _ in range(5):
+            t0 = time.perf_counter()
+            rc = apg.lib().apg_debug_poseidon2_batch(form, st.ctypes.data, n, out.ctypes.data)
+            best = min(best, time.perf_counter() - t0)
+            assert rc == 0
+        res[name] = round(best / n * 1e6, 3)
+    return res
 
 
 def stats_delta(after, before):

@@ -79,6 +79,13 @@ int apg_synthetic_fill_dev(apg_ctx *ctx, uint64_t seed, int kind, size_t n, void
  * Fq. No counterpart in the reference. */
 int apg_debug_mul32_batch(int which, const uint8_t *a_mont, const uint8_t *b_mont, size_t n,
                           uint8_t *out_mont);
+/* The transcript's Poseidon2 permutation (crates/transcript/src/lib.rs:
+ * 273-292) on n states of three Montgomery words (96 B each): form 0 the one
+ * the transcript uses, 1 the portable 64-bit multiply, 2 the x86-64
+ * mulx/adcx/adox multiply (APG_ERR_ARG where the CPU lacks BMI2 or ADX). For
+ * CPU-only checks that every form agrees with the reference's vectors; no
+ * counterpart in the reference. */
+int apg_debug_poseidon2_batch(int form, const uint8_t *states_mont, size_t n, uint8_t *out_mont);
 /* pcs::open (crates/pcs/src/lib.rs:313-492): the Mercury opening of the
  * commitment cm_wire (64-byte wire form, absorbed as passed) at u (num_vars
  * Montgomery Fr) for the 2^num_vars evaluation table, read as coefficients.

@@ -76,6 +76,49 @@ def test_native_permutation_edge_values_match_oracle(apg, orc):
         assert np.array_equal(a.sample(), b.sample())
 
 
+def _permute_forms(apg, states):
+    """Runs apg_debug_poseidon2_batch in every form the CPU has; returns
+    {form: out} (form 2, the mulx/adcx/adox multiply, only where supported)."""
+    st = np.ascontiguousarray(states, dtype=np.uint64).reshape(-1, 3, 4)
+    outs = {}
+    for form in (0, 1, 2):
+        out = np.zeros_like(st)
+        rc = apg.lib().apg_debug_poseidon2_batch(form, st.ctypes.data, st.shape[0], out.ctypes.data)
+        if form == 2 and rc != 0:
+            assert "ADX" in apg.lib().apg_last_error().decode()
+            continue
+        assert rc == 0, (form, apg.lib().apg_last_error())
+        outs[form] = out
+    return outs
+
+
+def test_native_permutation_forms_match_reference_vectors(apg, orc):
+    """Each host multiply the transcript may dispatch to (the portable 64-bit
+    CIOS and the x86-64 mulx/adcx/adox one) gives the reference's
+    poseidon2_perm.txt outputs, and agrees with the oracle on states near 0
+    and p and on random ones."""
+    ins, want = [], []
+    for rec in golden_lines("poseidon2_perm.txt"):
+        ins.append(np.stack([orc.from_hex_le(orc.FR, h) for h in rec[1:4]]))
+        want.append(rec[4:7])
+    outs = _permute_forms(apg, np.stack(ins))
+    assert len(ins) >= 100 and 0 in outs and 1 in outs
+    for form, out in outs.items():
+        got = [[orc.to_hex_le(orc.FR, out[i, l]) for l in range(3)] for i in range(len(ins))]
+        assert got == want, form
+    pm1 = orc.neg(orc.FR, orc.from_u64(orc.FR, 1))
+    rng = orc.SplitMix(41)
+    edge = [orc.fe(), orc.from_u64(orc.FR, 1), pm1, orc.neg(orc.FR, orc.from_u64(orc.FR, 2))]
+    states = [np.stack([edge[(i + l) % 4] for l in range(3)]) for i in range(4)]
+    states += [np.stack([rng.next_fr() for _ in range(3)]) for _ in range(200)]
+    outs = _permute_forms(apg, np.stack(states))
+    for i, s in enumerate(states):
+        ref = orc.poseidon2_permute(s)
+        for form, out in outs.items():
+            assert np.array_equal(out[i], ref), (form, i)
+    assert apg.lib().apg_debug_poseidon2_batch(3, None, 0, None) != 0
+
+
 def test_native_transcript_state_matches_oracle_after_g1(apg, orc):
     recs = list(golden_lines("g1_absorb_kats.txt", {"single"}))
     wires = [bytes.fromhex(r[1]) for r in recs]
