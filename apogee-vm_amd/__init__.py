@@ -1083,8 +1083,8 @@ def imul_peak(ctx: Optional[Context] = None, iters: int = 4096, reps: int = 5) -
 # ------------------------------------------------------------------ engine
 class Engine:
     """Continuous batching of shard jobs on one device (apg_engine): `workers`
-    concurrent HIP streams, admission against an HBM budget in arrival order
-    or shortest first (set_policy)."""
+    concurrent HIP streams, admission against an HBM budget in arrival order,
+    shortest first or longest first (set_policy)."""
 
     def __init__(self, device: int = 0, workers: int = 4, hbm_budget: int = 200 << 30, lane_workers: int = 0,
                  lane_cus: int = 0, small_scalars: float = float(1 << 18)):
@@ -1151,10 +1151,10 @@ class Engine:
         _check(lib().apg_engine_stats(self.h, C.byref(s)))
         return {name: getattr(s, name) for name, _ in ApgEngineInfo._fields_}
 
-    POLICIES = {"fifo": 0, "shortest": 1}
+    POLICIES = {"fifo": 0, "shortest": 1, "longest": 2}
 
     def set_policy(self, policy: str = "fifo", reserve: int = 0, small_scalars: float = 0.0):
-        """Queue order: "fifo" or "shortest" (rows x columns); with reserve > 0,
+        """Queue order: "fifo", "shortest" or "longest" (rows x columns); with reserve > 0,
         jobs above small_scalars hold at most workers - reserve workers."""
         if policy not in self.POLICIES:
             raise ValueError(f"unknown policy {policy!r}")

@@ -1190,7 +1190,7 @@ int apg_engine_stats(apg_engine *e, apg_engine_info *out) {
 int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_scalars) {
     return guarded([&] {
         need(e);
-        if (policy != APG_POLICY_FIFO && policy != APG_POLICY_SHORTEST)
+        if (policy != APG_POLICY_FIFO && policy != APG_POLICY_SHORTEST && policy != APG_POLICY_LONGEST)
             throw ApgError(APG_ERR_ARG, "unknown engine policy");
         if (reserve < 0) throw ApgError(APG_ERR_ARG, "reserve must be >= 0");
         e->e->set_policy(static_cast<Policy>(policy), reserve, small_scalars);

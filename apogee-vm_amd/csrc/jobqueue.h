@@ -34,10 +34,14 @@ namespace apg {
 //   kFifo:     arrival order; the head waits until its charge fits.
 //   kShortest: the smallest job (by work) whose charge fits, so short shards
 //              do not queue behind long ones.
+//   kLongest:  the largest job whose charge fits (longest processing time
+//              first): for a batch submitted at once, the long shards start
+//              early and the short ones fill in around them, so the batch
+//              does not end on one long shard running alone.
 // With `reserve` > 0 the first `reserve` workers are a fast lane: they take
 // only jobs of at most `small_work`, and the other workers leave such jobs to
 // an idle fast-lane worker.
-enum class Policy { kFifo = 0, kShortest = 1 };
+enum class Policy { kFifo = 0, kShortest = 1, kLongest = 2 };
 
 struct EngineStats {
     size_t queued = 0, running = 0, completed = 0;
@@ -201,7 +205,8 @@ class Scheduler {
             if (policy_ == Policy::kFifo)  // this lane's oldest job, once it fits
                 return fits_locked(j, w) ? (long)i : -1;
             if (!fits_locked(j, w)) continue;
-            if (best < 0 || j.work < queue_[best]->work) best = (long)i;
+            if (best < 0 || (policy_ == Policy::kLongest ? j.work > queue_[best]->work : j.work < queue_[best]->work))
+                best = (long)i;
         }
         return best;
     }

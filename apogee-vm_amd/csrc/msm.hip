@@ -4,15 +4,16 @@
 // The value is the group element sum_i s_i B_i, so any correct bucket schedule
 // returns the reference's exact affine bytes; this one is built for the MI355X:
 //
-//  1. digits  : one thread per scalar converts Montgomery -> canonical, recodes
-//               into signed c-bit digits exactly as msm.rs:180-224 does (same
-//               carry rule) and emits one (bucket key, table index | sign)
-//               pair per nonzero digit, packed densely in scalar order (a
-//               count pass, a scan of the blocks' counts, a write pass).
-//  2. sort    : an LSD radix sort of the pairs (hipCUB): by the bucket bits
-//               alone when every window has its table copy (stable, so the
-//               columns stay apart within a bucket), else by the whole key;
-//               then a binary search per key for its range.
+//  1. digits  : one thread per scalar converts Montgomery -> canonical and
+//               recodes into signed c-bit digits exactly as msm.rs:180-224
+//               does (same carry rule); every nonzero digit is one (bucket
+//               key, table index | sign) pair. The recode runs twice (count,
+//               then scatter) instead of storing an unsorted pair array.
+//  2. sort    : a two-level counting sort over the known key range (the
+//               "digit sort" section below): per-block coarse-bin counts in
+//               LDS, row scans and the accumulation plan on the device, an
+//               LDS-staged scatter writing each bin contiguously, then a
+//               chunked fine pass writing each key's run and its range.
 //  3. accumulate: the sorted entries are cut into equal slices, one thread
 //               per slice summing its points with mixed XYZZ adds, so the
 //               lanes of a wave finish together whatever the digit skew (the

@@ -36,7 +36,7 @@ the oracle's restatement of the reference algorithms timed on a sample of the
 batch's shards on the host cores, whose outputs are checked against the GPU's.
 
     python bench.py [--workload mixed|shard|serve|copy|open|gkr|srs] [--steps K] [--warmup W]
-                    [--workers N] [--policy fifo|shortest] [--reserve R] [--lane-cus C]
+                    [--workers N] [--policy fifo|shortest|longest] [--reserve R] [--lane-cus C]
 """
 from __future__ import annotations
 
@@ -685,7 +685,7 @@ def main():
     # 4 workers (within the hardware queues the device services at once), one
     # of them a fast lane on 32 CUs, shortest job first
     ap.add_argument("--workers", type=int, default=None)
-    ap.add_argument("--policy", default=None, choices=["fifo", "shortest"])
+    ap.add_argument("--policy", default=None, choices=["fifo", "shortest", "longest"])
     ap.add_argument("--reserve", type=int, default=None,
                     help="fast-lane workers, kept for shards of at most --small-scalars rows x columns")
     ap.add_argument("--lane-cus", type=int, default=None, help="CUs of the fast lane (0: no CU split)")

@@ -430,14 +430,15 @@ int apg_engine_poll(apg_engine *e, uint64_t job_id);
 int apg_engine_wait(apg_engine *e, uint64_t job_id, uint8_t *commitments_wire, uint8_t *rounds_out,
                     uint8_t *final_evals_out, apg_transcript *transcript_out, apg_job_stats *stats);
 int apg_engine_stats(apg_engine *e, apg_engine_info *out);
-/* Queue policy: APG_POLICY_FIFO (arrival order, the default) or
+/* Queue policy: APG_POLICY_FIFO (arrival order, the default),
  * APG_POLICY_SHORTEST (smallest rows x columns first among jobs whose HBM
- * estimate fits). With reserve > 0 the first `reserve` workers are a fast
+ * estimate fits) or APG_POLICY_LONGEST (largest first among those: a batch
+ * submitted at once starts its long shards early). With reserve > 0 the first `reserve` workers are a fast
  * lane that takes only jobs of at most small_scalars rows x columns; the
  * other workers leave such jobs to an idle fast-lane worker. No counterpart
  * in the reference, whose workers claim shards in emission order
  * (crates/prover/src/streaming.rs:601-680). */
-enum { APG_POLICY_FIFO = 0, APG_POLICY_SHORTEST = 1 };
+enum { APG_POLICY_FIFO = 0, APG_POLICY_SHORTEST = 1, APG_POLICY_LONGEST = 2 };
 int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_scalars);
 /* An engine whose first lane_workers workers are a fast lane for jobs of at
  * most small_scalars rows x columns (shortest-first policy), with their

@@ -239,6 +239,8 @@ int main() {
     run_mix(apg::Policy::kShortest, 4, 0, 130, 3, 40);
     run_mix(apg::Policy::kShortest, 5, 2, 150, 4, 30);
     run_mix(apg::Policy::kFifo, 6, 1, 1000, 4, 30);
+    run_mix(apg::Policy::kLongest, 4, 0, 130, 3, 40);
+    run_mix(apg::Policy::kLongest, 5, 2, 150, 4, 30);
     if (failures) {
         std::printf("%d checks failed\n", failures);
         return 1;

@@ -80,6 +80,14 @@ def test_engine_lanes_match_plain_path(apg, setup):
     eng.set_policy("fifo", 1, small)
     q, _, _ = eng.wait(eng.submit_shard(srs, bufs[0], specs[0].num_vars, specs[0].columns, specs[0].degree))
     assert q.commitments == plain[0][0].commitments
+    # longest first, no lane: the same bytes in any completion order
+    eng.set_policy("longest", 0, 0.0)
+    ids = [eng.submit_shard(srs, b, s.num_vars, s.columns, s.degree) for s, b in zip(specs, bufs)]
+    for (p, st), jid in zip(plain, ids):
+        q, t, _ = eng.wait(jid)
+        assert q.commitments == p.commitments and np.array_equal(q.rounds, p.rounds) and t.state() == st
+    with pytest.raises(ValueError):
+        eng.set_policy("widest")
 
 
 def test_engine_matches_oracle(apg, orc, setup):
