@@ -31,13 +31,22 @@ FLAGS = [
 ]
 
 
+# Per-file additions. The sumcheck round kernels keep their per-table and
+# per-node arrays in registers only when their `#pragma unroll` loops unroll
+# fully; at the default pragma threshold the three-table and lookahead rounds
+# stop short and put those arrays in scratch memory (300-500 B per lane).
+FILE_FLAGS = {
+    "sumcheck.hip": ["-mllvm", "-pragma-unroll-threshold=200000"],
+}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
