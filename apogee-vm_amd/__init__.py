@@ -188,6 +188,7 @@ def _declare(L):
     d("apg_engine_wait", [VP, C.c_uint64, VP, VP, VP, VP, C.POINTER(ApgJobStats)])
     d("apg_engine_stats", [VP, C.POINTER(ApgEngineInfo)])
     d("apg_engine_set_policy", [VP, I32, I32, C.c_double])
+    d("apg_engine_set_preemption", [VP, C.c_double, C.POINTER(C.c_uint64)])
     d("apg_engine_set_profiling", [VP, I32])
     d("apg_engine_kernel_stats", [VP, C.POINTER(ApgStats)])
     d("apg_engine_reset_kernel_stats", [VP])
@@ -1101,6 +1102,7 @@ class Engine:
             _check(lib().apg_engine_create(device, workers, hbm_budget, C.byref(h)))
         self.h = h
         self._jobs = {}
+        self._preempt = 0.0
         _live["engine"].add(self)
 
     def close(self):
@@ -1159,6 +1161,19 @@ class Engine:
         if policy not in self.POLICIES:
             raise ValueError(f"unknown policy {policy!r}")
         _check(lib().apg_engine_set_policy(self.h, self.POLICIES[policy], reserve, float(small_scalars)))
+
+    def set_preemption(self, ratio: float = 0.0) -> int:
+        """Phase-boundary preemption: with ratio >= 1 a running job pauses at its next
+        phase boundary while a job ratio times smaller (rows x columns) runs beside it;
+        0 turns it off. Returns the pauses so far."""
+        pauses = C.c_uint64(0)
+        _check(lib().apg_engine_set_preemption(self.h, float(ratio), C.byref(pauses)))
+        self._preempt = float(ratio)
+        return int(pauses.value)
+
+    def pauses(self) -> int:
+        """Pauses of jobs at phase boundaries so far (the preemption setting is kept)."""
+        return self.set_preemption(self._preempt)
 
     def set_profiling(self, on: bool = True):
         _check(lib().apg_engine_set_profiling(self.h, 1 if on else 0))

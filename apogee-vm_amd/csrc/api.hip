@@ -269,6 +269,7 @@ void prove_shard_impl(Context &c, apg_srs *srs, const Fe *cols, size_t num_vars,
     std::vector<Fe *> outs(f);
     for (int o = 0; o < f; o++) outs[o] = comb + (size_t)o * n;
     combine_columns_device(c, cols, n, (int)k, w.data(), scale.data(), f, outs.data());
+    c.yield_point();
     std::vector<uint8_t> r(32 * num_vars);
     for (size_t i = 0; i < num_vars; i++) store_fe(&r[32 * i], t.challenge(TAG_SUMCHECK_CHALLENGE));
     Gate g;
@@ -1194,6 +1195,15 @@ int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_s
             throw ApgError(APG_ERR_ARG, "unknown engine policy");
         if (reserve < 0) throw ApgError(APG_ERR_ARG, "reserve must be >= 0");
         e->e->set_policy(static_cast<Policy>(policy), reserve, small_scalars);
+    });
+}
+
+int apg_engine_set_preemption(apg_engine *e, double ratio, uint64_t *pauses) {
+    return guarded([&] {
+        need(e);
+        if (!(ratio >= 0) || (ratio > 0 && ratio < 1)) throw ApgError(APG_ERR_ARG, "preemption ratio must be 0 or >= 1");
+        e->e->set_preemption(ratio);
+        if (pauses) *pauses = e->e->pauses();
     });
 }
 

@@ -25,6 +25,9 @@ class Engine {
     EngineStats stats() { return sched_->stats(); }
     int workers() const { return (int)ctxs_.size(); }
     void set_policy(Policy p, int reserve, double small_work) { sched_->set_policy(p, reserve, small_work); }
+    // fast-lane jobs count for preemption only when the lane shares its CUs
+    void set_preemption(double ratio) { sched_->set_preemption(ratio, !lane_split_); }
+    uint64_t pauses() { return sched_->pauses(); }
     // Kernel timing on every worker's stream (see Stats); summed over workers
     // as their jobs finish.
     void set_profiling(bool on);
@@ -33,6 +36,7 @@ class Engine {
 
    private:
     int device_;
+    bool lane_split_ = false;  // the fast lane runs on CUs of its own
     Stats kstats_;  // written under the scheduler's lock
     std::vector<std::unique_ptr<Context>> ctxs_;
     std::unique_ptr<Scheduler<Context>> sched_;

@@ -35,6 +35,7 @@ Engine::Engine(int device, int workers, size_t hbm_budget, int lane_workers, int
         fast_mask.assign((ncu + 31) / 32, 0u);
         slow_mask.assign((ncu + 31) / 32, 0u);
         for (int cu = 0; cu < ncu; cu++) (cu < lane_cus ? fast_mask : slow_mask)[cu / 32] |= 1u << (cu % 32);
+        lane_split_ = true;
     }
     ctxs_.resize(workers);
     std::vector<Context *> raw;
@@ -55,6 +56,12 @@ Engine::Engine(int device, int workers, size_t hbm_budget, int lane_workers, int
     hooks.collect = [this](Context &c) {
         kstats_ += c.stats;
         c.stats = Stats();
+    };
+    hooks.bind_yield = [](Context &c, std::function<bool()> on, std::function<bool()> wanted,
+                          std::function<void()> wait) {
+        c.yield_on = std::move(on);
+        c.yield_wanted = std::move(wanted);
+        c.yield_wait = std::move(wait);
     };
     sched_ = std::make_unique<Scheduler<Context>>(raw, hbm_budget, lane_workers, std::move(hooks));
 }

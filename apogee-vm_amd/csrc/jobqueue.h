@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -41,6 +42,19 @@ namespace apg {
 // With `reserve` > 0 the first `reserve` workers are a fast lane: they take
 // only jobs of at most `small_work`, and the other workers leave such jobs to
 // an idle fast-lane worker.
+//
+// Preemption at phase boundaries (set_preemption, ratio > 0): a job running
+// on a worker outside the fast lane pauses at its next yield point (a phase
+// boundary of the shard: between MSM phases and accumulation chunks, between
+// sumcheck rounds) while another worker runs a job whose work times the ratio
+// is below its own, and resumes when none does. Jobs on the fast lane count
+// only when the lane shares its CUs with the other workers (count_lane). The
+// device has no preemption of its own, so the job's host thread drains its
+// stream at each yield point; a smaller job that starts beside it has the
+// CUs to itself once the phase in flight ends. A paused job
+// keeps its worker, so a job only pauses for one that already runs (never
+// for a queued one, which might wait for that very worker): no cycle of
+// waits can form, because each pause is for a strictly smaller job.
 enum class Policy { kFifo = 0, kShortest = 1, kLongest = 2 };
 
 struct EngineStats {
@@ -73,6 +87,10 @@ struct WorkerHooks {
     std::function<size_t(const Ctx &)> held;
     // after each job, under the scheduler's lock (kernel statistics)
     std::function<void(Ctx &)> collect;
+    // on the worker's thread before its first job: hands the context the
+    // worker's preemption calls (whether preemption is on, whether the job
+    // should pause now, and the pause itself)
+    std::function<void(Ctx &, std::function<bool()>, std::function<bool()>, std::function<void()>)> bind_yield;
 };
 
 inline double sched_now_ms() {
@@ -91,7 +109,9 @@ class Scheduler {
         const int nw = (int)ctxs_.size();
         reserve_ = std::max(0, std::min(reserve, nw - 1));
         running_mem_.assign(nw, 0);
+        running_work_.assign(nw, 0.0);
         busy_.assign(nw, false);
+        paused_.assign(nw, false);
         for (int w = 0; w < nw; w++) threads_.emplace_back([this, w] { run(w); });
     }
 
@@ -102,6 +122,7 @@ class Scheduler {
             stop_ = true;
         }
         cv_.notify_all();
+        pause_cv_.notify_all();
         for (auto &t : threads_) t.join();
     }
 
@@ -163,6 +184,40 @@ class Scheduler {
         reserve_ = std::max(0, std::min(reserve, (int)ctxs_.size() - 1));
         small_work_ = small_work;
         cv_.notify_all();
+        pause_cv_.notify_all();
+    }
+
+    // ratio 0 turns preemption off (paused jobs resume); otherwise >= 1
+    void set_preemption(double ratio, bool count_lane = false) {
+        std::lock_guard<std::mutex> lk(mu_);
+        preempt_ratio_ = ratio > 0 ? std::max(1.0, ratio) : 0.0;
+        count_lane_ = count_lane;
+        preempt_on_ = preempt_ratio_ > 0;
+        pause_cv_.notify_all();
+    }
+
+    bool preemption_on() const { return preempt_on_.load(std::memory_order_relaxed); }
+
+    // whether the job on worker w should pause at this yield point
+    bool yield_wanted(int w) {
+        std::lock_guard<std::mutex> lk(mu_);
+        return yield_wanted_locked(w);
+    }
+
+    // pauses the job on worker w until no smaller job runs beside it
+    void yield_wait(int w) {
+        std::unique_lock<std::mutex> lk(mu_);
+        if (!yield_wanted_locked(w)) return;
+        pauses_++;
+        paused_[w] = true;
+        pause_cv_.wait(lk, [&] { return !yield_wanted_locked(w); });
+        paused_[w] = false;
+    }
+
+    // pauses so far (for tests and statistics)
+    uint64_t pauses() {
+        std::lock_guard<std::mutex> lk(mu_);
+        return pauses_;
     }
 
     // runs f under the scheduler's lock (for state the collect hook writes)
@@ -175,6 +230,14 @@ class Scheduler {
     int workers() const { return (int)ctxs_.size(); }
 
    private:
+    bool yield_wanted_locked(int w) const {
+        if (preempt_ratio_ <= 0 || stop_ || w < reserve_ || !busy_[w]) return false;
+        for (int v = count_lane_ ? 0 : reserve_; v < (int)ctxs_.size(); v++)
+            if (v != w && busy_[v] && !paused_[v] && running_work_[v] * preempt_ratio_ < running_work_[w])
+                return true;
+        return false;
+    }
+
     size_t charge_locked(int w) const {
         const size_t h = hooks_.held(*ctxs_[w]);
         return busy_[w] ? std::max(h, running_mem_[w]) : h;
@@ -216,6 +279,10 @@ class Scheduler {
         std::string start_error;
         int start_code = 0;
         try {
+            if (hooks_.bind_yield)
+                hooks_.bind_yield(
+                    ctx, [this] { return preemption_on(); }, [this, w] { return yield_wanted(w); },
+                    [this, w] { yield_wait(w); });
             if (hooks_.start) hooks_.start(w, ctx);
         } catch (const ApgError &e) {
             start_code = e.code;
@@ -243,6 +310,7 @@ class Scheduler {
                 queue_.erase(queue_.begin() + at);
                 busy_[w] = true;
                 running_mem_[w] = job->mem_bytes;
+                running_work_[w] = job->work;
                 running_++;
                 peak_ = std::max(peak_, in_use_locked());
             }
@@ -273,12 +341,14 @@ class Scheduler {
                 if (hooks_.collect) hooks_.collect(ctx);
                 busy_[w] = false;
                 running_mem_[w] = 0;
+                running_work_[w] = 0;
                 running_--;
                 completed_++;
                 done_[job->id] = std::move(job);
             }
             done_cv_.notify_all();
             cv_.notify_all();
+            pause_cv_.notify_all();  // a paused job may resume
         }
     }
 
@@ -290,10 +360,15 @@ class Scheduler {
     double small_work_ = 0;
     int idle_fast_ = 0;  // fast-lane workers waiting for a job
     std::vector<size_t> running_mem_;
-    std::vector<bool> busy_;
+    std::vector<double> running_work_;
+    std::vector<bool> busy_, paused_;
+    double preempt_ratio_ = 0;
+    bool count_lane_ = false;
+    std::atomic<bool> preempt_on_{false};
+    uint64_t pauses_ = 0;
     std::vector<std::thread> threads_;
     std::mutex mu_;
-    std::condition_variable cv_, done_cv_;
+    std::condition_variable cv_, done_cv_, pause_cv_;
     std::deque<std::unique_ptr<Job>> queue_;
     std::unordered_map<uint64_t, std::unique_ptr<Job>> done_;
     std::unordered_set<uint64_t> outstanding_;  // issued and not yet collected

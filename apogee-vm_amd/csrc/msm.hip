@@ -255,11 +255,12 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_rowscan(uint32_t *__restrict_
 // group in column-major, bin-ascending order: the final key order), the plan,
 // the fine pass's extra chunks (chunks 1.. of every row of more than cap
 // pairs, as (row, chunk)), and, when rows are keys, the key ranges.
-// `resident` is the number of threads k_accumulate keeps resident; slices are
-// at least MIN_SLICE long, about ACC_ROUNDS per resident thread and an eighth
-// of the mean bucket, a multiple of 4, and never more than slice_cap.
+// `target` is the slice count wanted (ACC_ROUNDS, or twice that for a job
+// that pauses between accumulation chunks, per thread k_accumulate keeps
+// resident); slices are at least MIN_SLICE long, at most total / target and an
+// eighth of the mean bucket, a multiple of 4, and never more than slice_cap.
 __global__ void __launch_bounds__(SORT_TPB) k_sort_base(const uint32_t *__restrict__ rowtot, uint32_t rows,
-                                                      uint32_t nkeys, uint32_t resident, uint32_t slice_cap,
+                                                      uint32_t nkeys, uint32_t target, uint32_t slice_cap,
                                                       int rows_are_keys, uint32_t cap,
                                                       uint32_t *__restrict__ rowbase, uint32_t *__restrict__ off,
                                                       uint2 *__restrict__ extra, MsmPlan *__restrict__ plan,
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_base(const uint32_t *__restri
     if (threadIdx.x == 0) {
         const uint64_t total = carry;
         uint64_t L = MIN_SLICE;
-        L = max(L, (total + (uint64_t)ACC_ROUNDS * resident - 1) / ((uint64_t)ACC_ROUNDS * resident));
+        L = max(L, (total + (uint64_t)target - 1) / (uint64_t)target);
         L = max(L, total / (8 * (uint64_t)max(nkeys, 1u)));
         L = min<uint64_t>(L, MAX_SLICE);
         L = max(L, (total + slice_cap - 1) / slice_cap);
@@ -567,14 +568,17 @@ __device__ __forceinline__ Affine load_affine(const Affine *__restrict__ p, uint
 // slice; unused slots hold the identity. Bucket k's partials are then the
 // contiguous part[2 t0 + 1 .. 2 t1], t0 and t1 the slices of its first and last
 // entries.
-// The launch covers the plan's largest possible slice count; threads past
-// the pair total (plan->total, plan->L: k_sort_base) return at once.
+// The launches cover the plan's largest possible slice count (one launch, or
+// consecutive chunks of slices starting at t0 where a job must be able to
+// pause between them); threads past the pair total (plan->total, plan->L:
+// k_sort_base) return at once.
 __global__ void __launch_bounds__(TPB) k_accumulate(const uint32_t *__restrict__ skeys,
                                                   const uint32_t *__restrict__ entries,
                                                   const MsmPlan *__restrict__ plan, const Affine *__restrict__ bases,
-                                                  Xyzz29 *__restrict__ buckets, Xyzz29 *__restrict__ part) {
+                                                  Xyzz29 *__restrict__ buckets, Xyzz29 *__restrict__ part,
+                                                  uint32_t t0) {
     const uint32_t total = plan->total, L = plan->L;
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
     if ((size_t)t * L >= total) return;
     const uint32_t lo = t * L, hi = lo + min(total - lo, L);
     const bool head_open = lo > 0 && skeys[lo - 1] == skeys[lo];
@@ -989,10 +993,12 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     uint2 *extra = ctx.d<uint2>("msm.extra", xmax);
 
     // k_accumulate holds 3 waves per SIMD; slices follow k_sort_base's rule,
-    // so at most ACC_ROUNDS per resident thread (+1) and at most one per
-    // MIN_SLICE pairs
+    // so at most `rounds` per resident thread (+1) and at most one per
+    // MIN_SLICE pairs. A preemptible job launches one resident wave of slices
+    // at a time and pauses between launches, so its slices are half as long.
     const size_t resident = (size_t)ctx.cu_count * 4 * 3 * 64;
-    const size_t slice_cap = std::min(ACC_ROUNDS * resident + 1, max_pairs / MIN_SLICE + 1);
+    const size_t rounds = ctx.preemptible() ? 2 * ACC_ROUNDS : ACC_ROUNDS;
+    const size_t slice_cap = std::min(rounds * resident + 1, max_pairs / MIN_SLICE + 1);
 
     const unsigned sblocks = (unsigned)(K * geo.nbc);
 #define APG_SORT(CT, WT)                                                                                        \
@@ -1001,7 +1007,8 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
                            counts);                                                                             \
         hipLaunchKernelGGL(k_sort_rowscan, dim3(rows), dim3(SORT_TPB), 0, st, counts, geo.nbc, rowtot,           \
                            geo.fine_bits, cap, fcount, fclaim);                                                 \
-        hipLaunchKernelGGL(k_sort_base, dim3(1), dim3(SORT_TPB), 0, st, rowtot, rows, nkeys, (uint32_t)resident, \
+        hipLaunchKernelGGL(k_sort_base, dim3(1), dim3(SORT_TPB), 0, st, rowtot, rows, nkeys,                    \
+                           (uint32_t)(rounds * resident),                                                       \
                            (uint32_t)slice_cap, fine ? 0 : 1, cap, rowbase, off, extra, plan, total_dev);       \
         hipLaunchKernelGGL((k_sort_scatter<CT, WT>), dim3(sblocks), dim3(SORT_TPB), 0, st, fr_dev, u32_dev, geo, \
                            counts, rowtot, rowbase, keys_c, vals_c);                                            \
@@ -1025,6 +1032,7 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
 #undef APG_SORT_C
 #undef APG_SORT
     if (fine) {
+        ctx.yield_point();
         const dim3 fg(rows + xgroups);
         hipLaunchKernelGGL(k_sort_fine_count, fg, dim3(SORT_TPB), size_t(4) << geo.fine_bits, st, keys_c, geo, cap,
                            rows, rowbase, rowtot, plan, extra, fcount);
@@ -1034,18 +1042,27 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     }
     HIP_CHECK(hipGetLastError());
 
-    // 3. accumulate over equal slices of the sorted entries
+    // 3. accumulate over equal slices of the sorted entries; a preemptible
+    // job launches one resident wave of slices at a time, with a yield point
+    // between launches (the slices are independent)
     Xyzz29 *buckets = ctx.d<Xyzz29>("msm.buckets", nkeys);
     Xyzz29 *part = ctx.d<Xyzz29>("msm.part", 2 * slice_cap);
+    const bool chunked = ctx.preemptible();
+    if (chunked) ctx.yield_point();
     if (ctx.profiling) {
         HIP_CHECK(hipEventCreate(&g.ev0));
         HIP_CHECK(hipEventCreate(&g.ev1));
         HIP_CHECK(hipEventRecord(g.ev0, st));
     }
-    hipLaunchKernelGGL(k_accumulate, dim3(grid_for(slice_cap, TPB)), dim3(TPB), 0, st, skeys, entries, plan,
-                       tab.points, buckets, part);
-    HIP_CHECK(hipGetLastError());
+    const size_t chunk = chunked ? (resident + TPB - 1) / TPB * TPB : slice_cap;
+    for (size_t t0 = 0; t0 < slice_cap; t0 += chunk) {
+        if (t0 > 0) ctx.yield_point();
+        hipLaunchKernelGGL(k_accumulate, dim3(grid_for(std::min(chunk, slice_cap - t0), TPB)), dim3(TPB), 0, st,
+                           skeys, entries, plan, tab.points, buckets, part, (uint32_t)t0);
+        HIP_CHECK(hipGetLastError());
+    }
     if (ctx.profiling) HIP_CHECK(hipEventRecord(g.ev1, st));
+    if (chunked) ctx.yield_point();
 
     // 4. buckets: empty ones set, crossing ones merged from their partials. A
     // heavy bucket spans more than LIGHT_ITEMS / 2 slices, each slice belongs
@@ -1069,6 +1086,7 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
                        buckets);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamWaitEvent(st, ctx.ev_join, 0));
+    ctx.yield_point();
 
     // 5. reduce: per problem (column, group): W = sum_b (b + 1) S_b, written
     // by k_weighted straight into pinned host memory
@@ -1165,6 +1183,7 @@ void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *f
     for (size_t gi = 0; gi < groups.size(); gi++) {
         GroupRun &g = groups[gi];
         g.total_host = h_tot + gi;
+        if (gi > 0) ctx.yield_point();
         msm_enqueue_group(ctx, tab, n, g, fr_dev ? fr_dev + g.k0 * n : nullptr,
                           u32_dev ? u32_dev + g.k0 * n : nullptr, w_dev + g.w_off, tot_dev + gi);
     }

@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -113,6 +114,21 @@ struct Context {
     // written by the thread that owns the context, read by others
     std::atomic<size_t> held_bytes{0};
     int cu_count = 256;
+    // Phase-boundary preemption, bound by the shard engine to its scheduler
+    // (jobqueue.h); empty for plain contexts, where yield_point does nothing.
+    std::function<bool()> yield_on, yield_wanted;
+    std::function<void()> yield_wait;
+
+    bool preemptible() const { return yield_on && yield_on(); }
+
+    // A phase boundary of a job: the host waits for the stream to drain (so a
+    // smaller job that starts now waits for at most the phase in flight),
+    // then pauses while the scheduler asks it to.
+    void yield_point() {
+        if (!preemptible()) return;
+        sync();
+        if (yield_wanted()) yield_wait();
+    }
 
     template <class T>
     T *d(const char *name, size_t count) {

@@ -799,6 +799,7 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
     Fe claim = f_zero<Fr>();
     Fe P = f_one<Fr>();  // prod_{l<j} eq(r_l, rho_l)
     for (int j = 0; j < n; j++) {
+        if (j > 0) ctx.yield_point();  // between rounds (the last one synchronised the stream)
         if (j > 0 && (size_t)k * (len0 >> j) <= TAIL_ELEMS) {  // T^j is small: the rest on the host
             std::vector<Fe *> src(k);
             for (int t = 0; t < k; t++) src[t] = const_cast<Fe *>(cur[t]);

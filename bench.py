@@ -65,6 +65,11 @@ METRIC = "shards/s (commit MSM+sumcheck) on mixed 2^8–2^22 batch; G1 MSM pts/s
 PEAK_IMUL = 256 * 64 * 2.4e9
 PEAK_HBM = 8.0e12
 IMUL_PER_MADD = 1496  # madd-2007-bl: 7M + 4S Fq multiplies x 136
+# serving engine: workers (one of them the fast lane) and the preemption ratio
+SERVE_WORKERS = 6
+SERVE_RESERVE = 2
+SERVE_LANE_CUS = 0
+SERVE_PREEMPT = 2.0
 
 apg = None
 wl = None
@@ -690,6 +695,8 @@ def main():
                     help="fast-lane workers, kept for shards of at most --small-scalars rows x columns")
     ap.add_argument("--lane-cus", type=int, default=None, help="CUs of the fast lane (0: no CU split)")
     ap.add_argument("--small-scalars", type=float, default=float(1 << 20))
+    ap.add_argument("--preempt", type=float, default=None,
+                    help="phase-boundary preemption ratio (0: off; serving default %g)" % SERVE_PREEMPT)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work (0 = skip)")
     args = ap.parse_args()
@@ -698,8 +705,9 @@ def main():
               f"(the path would shard across GPUs as described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     serving = args.workload == "serve"
-    for name, mixed_default, serve_default in (("workers", 12, 4), ("policy", "fifo", "shortest"),
-                                               ("reserve", 0, 1), ("lane_cus", 0, 32)):
+    for name, mixed_default, serve_default in (("workers", 12, SERVE_WORKERS), ("policy", "fifo", "shortest"),
+                                               ("reserve", 0, SERVE_RESERVE), ("lane_cus", 0, SERVE_LANE_CUS),
+                                               ("preempt", 0.0, SERVE_PREEMPT)):
         if getattr(args, name) is None:
             setattr(args, name, serve_default if serving else mixed_default)
     apg = importlib.import_module("apogee-vm_amd")
@@ -750,6 +758,7 @@ def main():
     eng = apg.Engine(0, workers=args.workers, hbm_budget=200 << 30, lane_workers=args.reserve,
                      lane_cus=args.lane_cus, small_scalars=args.small_scalars)
     eng.set_policy(args.policy, args.reserve, args.small_scalars)
+    eng.set_preemption(args.preempt)
     eng.set_profiling(True)
 
     if args.workload == "serve":
@@ -789,10 +798,12 @@ def main():
     uni = msm_uniform_line(ctx, srs, args.seed)
     log(f"uniform 2^22 MSM: {uni['ms']} ms, {uni['g1_points_per_s'] / 1e6:.1f} M points/s")
     log("served stream: Poisson arrivals at 0.5x, 1x, 1.5x the serving engine's capacity")
-    seng = apg.Engine(0, workers=4, hbm_budget=200 << 30, lane_workers=1, lane_cus=32,
-                      small_scalars=args.small_scalars)
-    seng.set_policy("shortest", 1, args.small_scalars)
-    sargs = argparse.Namespace(**{**vars(args), "workers": 4, "policy": "shortest", "reserve": 1, "lane_cus": 32,
+    seng = apg.Engine(0, workers=SERVE_WORKERS, hbm_budget=200 << 30, lane_workers=SERVE_RESERVE,
+                      lane_cus=SERVE_LANE_CUS, small_scalars=args.small_scalars)
+    seng.set_policy("shortest", SERVE_RESERVE, args.small_scalars)
+    seng.set_preemption(SERVE_PREEMPT)
+    sargs = argparse.Namespace(**{**vars(args), "workers": SERVE_WORKERS, "policy": "shortest",
+                                  "reserve": SERVE_RESERVE, "lane_cus": SERVE_LANE_CUS, "preempt": SERVE_PREEMPT,
                                   "warmup": 1})
     served = serve(seng, srs, specs, bufs, plain, iso, sargs)
     seng.close()
@@ -824,8 +835,11 @@ def main():
         "single_shard": single,
         "g1_msm_points_per_s": single["g1_msm_points_per_s"],
         "msm_uniform_2e22": uni,
-        "serve": {"engine": "4 workers, one a fast lane on 32 CUs for shards of at most 2^20 scalars, "
-                            "shortest first; 200 GB admission budget",
+        "serve": {"engine": f"{SERVE_WORKERS} workers, {SERVE_RESERVE} of them kept for shards of at most 2^20 "
+                            f"scalars" + (f" on {SERVE_LANE_CUS} CUs of their own" if SERVE_LANE_CUS else "") +
+                            f", shortest first; a job pauses at phase boundaries while jobs "
+                            f"{SERVE_PREEMPT:g}x smaller run; 200 GB admission budget",
+                  "pauses": served["pauses"],
                   "capacity_shards_per_s": served["capacity_shards_per_s"], "rows": served["serve"],
                   "bit_exact_vs_plain": served["bit_exact_vs_plain"]},
         "roofline": roof,
@@ -874,16 +888,25 @@ def serve(eng, srs, specs, bufs, plain, iso, args):
             detail.append((ratios[-1], i, st))
         wall = time.perf_counter() - t0
         lats, ratios = np.array(lats), np.array(ratios)
+        nv_of = np.array([specs[i].num_vars for i, _ in ids])
         rows.append({
             "offered_rate": round(rate, 2), "load": mult, "shards_per_s": round(len(specs) / wall, 2),
             "latency_ms": {q: round(float(np.percentile(lats, p)), 3) for q, p in
                            (("p50", 50), ("p99", 99), ("p999", 99.9))},
             "latency_over_isolated": {q: round(float(np.percentile(ratios, p)), 3) for q, p in
                                       (("p50", 50), ("p99", 99))},
+            # the same by size class of the mixed batch: 2^8-2^14, 2^15-2^19, 2^20-2^22 rows
+            "over_isolated_by_class": {
+                name: {"shards": int(sel.sum()),
+                       "p50": round(float(np.percentile(ratios[sel], 50)), 3),
+                       "p99": round(float(np.percentile(ratios[sel], 99)), 3),
+                       "max": round(float(ratios[sel].max()), 3)}
+                for name, sel in (("rows_2e8_2e14", nv_of <= 14), ("rows_2e15_2e19", (nv_of >= 15) & (nv_of <= 19)),
+                                  ("rows_2e20_2e22", nv_of >= 20)) if sel.any()},
             "hbm_admitted_peak_gib": round(peak / 2**30, 3),
             "worst": [{"rows_log2": specs[i].num_vars, "columns": specs[i].columns,
                        "isolated_ms": round(float(iso_ms[i]), 3), "queue_ms": round(st["queue_ms"], 3),
-                       "run_ms": round(st["run_ms"], 3), "over_isolated": round(r, 2)}
+                       "run_ms": round(st["run_ms"], 3), "worker": st.get("worker"), "over_isolated": round(r, 2)}
                       for r, i, st in sorted(detail, key=lambda t: -t[0])[:5]],
         })
         log(f"load {mult}: {rows[-1]['shards_per_s']} shards/s, p99 {rows[-1]['latency_ms']['p99']} ms")
@@ -892,8 +915,9 @@ def serve(eng, srs, specs, bufs, plain, iso, args):
             "steps": 1, "warmup": 1, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u32x8 (BN254 Fr/Fq Montgomery, exact)",
             "data": "synthetic, seeded Poisson arrivals",
+            "pauses": eng.pauses(),
             "config": {"workload": "served stream of the mixed batch's shards", "parallelism": "1 GPU",
-                       "engine_workers": args.workers, "engine_policy": args.policy,
+                       "engine_workers": args.workers, "engine_policy": args.policy, "preempt": args.preempt,
                        "reserve": args.reserve, "lane_cus": args.lane_cus, "small_scalars": args.small_scalars, "hbm_budget_gib": 200},
             "capacity_shards_per_s": round(capacity, 2), "serve": rows, "bit_exact_vs_plain": bool(exact),
             "hbm": {"srs_gib": round(srs.device_bytes() / 2**30, 3),

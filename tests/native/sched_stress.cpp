@@ -11,8 +11,14 @@
 //   * wait and poll refuse ids never issued or already collected;
 //   * a worker that fails to start fails its jobs with that error while the
 //     others serve;
-//   * destroying the scheduler with jobs queued runs them first.
+//   * destroying the scheduler with jobs queued runs them first;
+//   * with preemption on, a long job makes no progress while a much shorter
+//     one runs beside it, resumes after it, and mixes with yield points in
+//     every job still complete with their own outputs (no cycle of pauses);
+//     with preemption off nothing pauses.
 #include <atomic>
+#include <chrono>
+#include <functional>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -26,7 +32,21 @@ namespace {
 struct MockCtx {
     std::atomic<size_t> held{0};
     int index = 0;
+    std::function<bool()> on, wanted;
+    std::function<void()> wait;
+    void yield_point() {
+        if (on && on() && wanted()) wait();
+    }
 };
+
+void bind(apg::WorkerHooks<MockCtx> &hooks) {
+    hooks.bind_yield = [](MockCtx &c, std::function<bool()> on, std::function<bool()> wanted,
+                          std::function<void()> wait) {
+        c.on = std::move(on);
+        c.wanted = std::move(wanted);
+        c.wait = std::move(wait);
+    };
+}
 
 using Sched = apg::Scheduler<MockCtx>;
 using Job = apg::BasicJob<MockCtx>;
@@ -60,8 +80,12 @@ std::unique_ptr<Job> make_job(uint64_t input, size_t mem, double work, int reser
         }
         uint64_t v = input;
         const int spins = (int)(mix(input) % 2000);
-        for (int i = 0; i < spins; i++) v = mix(v);
+        for (int i = 0; i < spins; i++) {
+            v = mix(v);
+            if (i % 500 == 499) c.yield_point();
+        }
         std::this_thread::sleep_for(std::chrono::microseconds(mix(input) % 300));
+        c.yield_point();
         self.rounds.resize(8);
         for (int i = 0; i < 8; i++) self.rounds[i] = (uint8_t)(v >> (8 * i));
     };
@@ -77,7 +101,8 @@ std::vector<uint8_t> expected(uint64_t input) {
     return r;
 }
 
-void run_mix(apg::Policy policy, int workers, int reserve, size_t budget, int submitters, int per_thread) {
+void run_mix(apg::Policy policy, int workers, int reserve, size_t budget, int submitters, int per_thread,
+             double preempt = 0) {
     std::vector<std::unique_ptr<MockCtx>> ctxs;
     std::vector<MockCtx *> raw;
     for (int w = 0; w < workers; w++) {
@@ -89,12 +114,14 @@ void run_mix(apg::Policy policy, int workers, int reserve, size_t budget, int su
     hooks.held = [](const MockCtx &c) { return c.held.load(); };
     std::atomic<int> collected{0};
     hooks.collect = [&](MockCtx &) { collected++; };
+    bind(hooks);
     const double small = 50;
     std::atomic<int> lane_violations{0};
     std::atomic<bool> over_budget{false}, stop_monitor{false};
     {
         Sched s(raw, budget, reserve, hooks);
         s.set_policy(policy, reserve, small);
+        s.set_preemption(preempt);
         std::thread monitor([&] {
             while (!stop_monitor) {
                 const apg::EngineStats st = s.stats();
@@ -146,11 +173,13 @@ void run_mix(apg::Policy policy, int workers, int reserve, size_t budget, int su
         CHECK(st.completed == (size_t)(submitters * per_thread), "every job completed");
         CHECK(st.queued == 0 && st.running == 0, "nothing left queued or running");
         CHECK(st.hbm_peak >= st.arena_bytes || st.hbm_peak > 0, "peak recorded");
+        if (preempt == 0) CHECK(s.pauses() == 0, "nothing pauses with preemption off");
     }
     CHECK(!over_budget, "charged HBM within the budget whenever two or more jobs run");
     CHECK(lane_violations == 0, "fast-lane workers ran short jobs only");
     CHECK(collected == submitters * per_thread, "collect hook ran once per job");
-    std::printf("ok mix policy=%d workers=%d reserve=%d budget=%zu\n", (int)policy, workers, reserve, budget);
+    std::printf("ok mix policy=%d workers=%d reserve=%d budget=%zu preempt=%g\n", (int)policy, workers, reserve,
+                budget, preempt);
 }
 
 void unknown_ids() {
@@ -224,6 +253,86 @@ void drain_on_destroy() {
     std::printf("ok drain on destroy\n");
 }
 
+// A long job (work 1000) steps with a yield point per step; a short job (work
+// 10) submitted while it runs must run with the long job paused: the long
+// job's step count may move by at most one (the step in flight) between the
+// short job's start and end.
+void preemption_order() {
+    std::vector<std::unique_ptr<MockCtx>> ctxs;
+    std::vector<MockCtx *> raw;
+    for (int w = 0; w < 3; w++) {
+        ctxs.push_back(std::make_unique<MockCtx>());
+        ctxs.back()->index = w;
+        raw.push_back(ctxs.back().get());
+    }
+    apg::WorkerHooks<MockCtx> hooks;
+    hooks.held = [](const MockCtx &x) { return x.held.load(); };
+    bind(hooks);
+    std::atomic<int> steps{0}, at_start{-1}, at_end{-1};
+    std::atomic<bool> long_started{false};
+    {
+        Sched s(raw, 1000, 0, hooks);
+        s.set_policy(apg::Policy::kShortest, 0, 0);
+        s.set_preemption(4.0);
+        auto lj = std::make_unique<Job>();
+        lj->mem_bytes = 10;
+        lj->work = 1000;
+        lj->fn = [&](MockCtx &c, Job &) {
+            long_started = true;
+            for (int i = 0; i < 400; i++) {
+                std::this_thread::sleep_for(std::chrono::microseconds(100));
+                steps++;
+                c.yield_point();
+            }
+        };
+        const uint64_t lid = s.submit(std::move(lj));
+        while (!long_started) std::this_thread::yield();
+        std::this_thread::sleep_for(std::chrono::milliseconds(2));
+        auto sj = std::make_unique<Job>();
+        sj->mem_bytes = 10;
+        sj->work = 10;
+        sj->fn = [&](MockCtx &, Job &) {
+            at_start = steps.load();
+            std::this_thread::sleep_for(std::chrono::milliseconds(5));  // ~50 of the long job's steps
+            at_end = steps.load();
+        };
+        const uint64_t sid = s.submit(std::move(sj));
+        auto sr = s.wait(sid);
+        auto lr = s.wait(lid);
+        CHECK(sr->status == 0 && lr->status == 0, "both jobs completed");
+        CHECK(steps == 400, "the long job finished all its steps after resuming");
+        CHECK(at_start >= 0 && at_end - at_start <= 1, "the long job paused while the short one ran");
+        CHECK(s.pauses() >= 1, "the long job paused");
+        // a job of similar size does not pause it (ratio 4)
+        const int before = (int)s.pauses();
+        steps = 0;
+        auto a = std::make_unique<Job>();
+        a->mem_bytes = 10;
+        a->work = 1000;
+        a->fn = [&](MockCtx &c, Job &) {
+            for (int i = 0; i < 100; i++) {
+                std::this_thread::sleep_for(std::chrono::microseconds(100));
+                steps++;
+                c.yield_point();
+            }
+        };
+        auto b = std::make_unique<Job>();
+        b->mem_bytes = 10;
+        b->work = 400;
+        b->fn = [&](MockCtx &c, Job &) {
+            for (int i = 0; i < 100; i++) {
+                std::this_thread::sleep_for(std::chrono::microseconds(100));
+                c.yield_point();
+            }
+        };
+        const uint64_t ia = s.submit(std::move(a)), ib = s.submit(std::move(b));
+        CHECK(s.wait(ia)->status == 0 && s.wait(ib)->status == 0, "similar jobs completed");
+        CHECK((int)s.pauses() == before, "a job less than the ratio smaller causes no pause");
+    }
+    std::printf("ok preemption order (long job steps %d..%d while the short one ran)\n", at_start.load(),
+                at_end.load());
+}
+
 }  // namespace
 
 namespace apg {
@@ -241,6 +350,10 @@ int main() {
     run_mix(apg::Policy::kFifo, 6, 1, 1000, 4, 30);
     run_mix(apg::Policy::kLongest, 4, 0, 130, 3, 40);
     run_mix(apg::Policy::kLongest, 5, 2, 150, 4, 30);
+    preemption_order();
+    run_mix(apg::Policy::kShortest, 5, 1, 1000, 4, 30, 1.0);
+    run_mix(apg::Policy::kShortest, 6, 0, 150, 4, 30, 2.0);
+    run_mix(apg::Policy::kFifo, 4, 1, 130, 3, 40, 4.0);
     if (failures) {
         std::printf("%d checks failed\n", failures);
         return 1;

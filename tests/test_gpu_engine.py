@@ -150,3 +150,49 @@ def test_engine_tables_outgrown_while_in_use(apg, orc):
         cols = bufs[i].download().reshape(k, 1 << nv, 4)
         assert got[i].commitments[0] == orc.jac_to_wire(orc.msm(srs.g1(0, 1 << nv), cols[0], threads=8, workers=8))
     srs.release()
+
+
+def test_engine_preemption_matches_plain_path(apg):
+    """Phase-boundary preemption: a long shard (2^18 rows x 8) whose
+    accumulation runs in chunks with yield points between them and between
+    its sumcheck rounds, and short shards submitted while it runs, which make
+    it pause. Every shard's bytes equal the plain path's, the long one's
+    included, and the long job paused at least once."""
+    import time
+
+    ctx = apg.default_context()
+    tau = apg.synthetic(2468, 1, "uniform")[0]
+    srs = apg.Srs.generate(tau, 1 << 18, ctx)
+    specs = [(18, 8, 70)] + [(10 + (i % 3), 3 + (i % 4), 71 + i) for i in range(12)]
+    bufs = []
+    for nv, k, seed in specs:
+        b = apg.DeviceBuffer(ctx, 32 * k << nv)
+        for c in range(k):
+            apg.synthetic_device(ctx, seed * 10 + c, 1 << nv, "witness", out=b, offset_elems=c << nv)
+        bufs.append(b)
+    plain = []
+    for (nv, k, _), b in zip(specs, bufs):
+        t = apg.Transcript()
+        plain.append((apg.prove_shard(srs, b, nv, k, 3, t), t.state()))
+    eng = apg.Engine(0, workers=4, hbm_budget=16 << 30)
+    eng.set_policy("shortest", 0, 0.0)
+    eng.set_preemption(1.0)
+    ids = [eng.submit_shard(srs, bufs[0], specs[0][0], specs[0][1], 3)]
+    t0 = time.perf_counter()
+    while eng.stats()["running"] == 0 and time.perf_counter() - t0 < 5:
+        time.sleep(0.0002)
+    ids += [eng.submit_shard(srs, b, nv, k, 3) for (nv, k, _), b in zip(specs[1:], bufs[1:])]
+    for (p, st), jid in zip(plain, ids):
+        q, t, _ = eng.wait(jid)
+        assert q.commitments == p.commitments
+        assert np.array_equal(q.rounds, p.rounds) and np.array_equal(q.final_evals, p.final_evals)
+        assert t.state() == st
+    assert eng.pauses() >= 1
+    # preemption off again: nothing more pauses
+    before = eng.set_preemption(0.0)
+    ids = [eng.submit_shard(srs, b, nv, k, 3) for (nv, k, _), b in zip(specs, bufs)]
+    for (p, _), jid in zip(plain, ids):
+        assert eng.wait(jid)[0].commitments == p.commitments
+    assert eng.pauses() == before
+    eng.close()
+    srs.release()
