@@ -285,6 +285,7 @@ class Context:
         h = VP()
         _check(lib().apg_ctx_create(device, C.byref(h)))
         self.h = h
+        self.profiling = False
         _live["ctx"].add(self)
 
     def close(self):
@@ -302,7 +303,10 @@ class Context:
         _check(lib().apg_ctx_synchronize(self.h))
 
     def set_profiling(self, on: bool = True):
+        """Kernel timing with HIP events (apg_stats); the host clocks of MSMs and
+        sumchecks are kept either way."""
         _check(lib().apg_ctx_set_profiling(self.h, 1 if on else 0))
+        self.profiling = bool(on)
 
     def stats(self) -> dict:
         s = ApgStats()

@@ -137,11 +137,9 @@ void commit_device(Context &ctx, apg_srs *srs, const Fe *fr_dev, const uint32_t 
     const std::shared_ptr<BaseTable> tab = srs_table(ctx, srs, n);
     const double t0 = now_ms();
     msm_run(ctx, *tab, n, k, fr_dev, u32_dev, res.data());
-    if (ctx.profiling) {
-        ctx.stats.msm_ms += now_ms() - t0;
-        ctx.stats.msm_calls += 1;
-        ctx.stats.msm_points += n * k;
-    }
+    ctx.stats.msm_ms += now_ms() - t0;
+    ctx.stats.msm_calls += 1;
+    ctx.stats.msm_points += n * k;
     for (size_t i = 0; i < k; i++) std::memcpy(out_mont + 64 * i, &res[i], 64);
 }
 
@@ -201,7 +199,7 @@ void sumcheck_common(Context &ctx, const Gate &gate, Fe *const *tabs, size_t k, 
     else
         sumcheck_prove_device(ctx, gate, tabs, (int)k, (int)num_vars, eqp.data(), fn, rounds.data(),
                               point.data(), finals.data(), kinds);
-    if (ctx.profiling) ctx.stats.sumcheck_ms += now_ms() - t0;
+    ctx.stats.sumcheck_ms += now_ms() - t0;
     for (size_t i = 0; i < (size_t)num_vars * (d + 1); i++) store_fe(rounds_out + 32 * i, rounds[i]);
     if (point_out)
         for (size_t i = 0; i < num_vars; i++) store_fe(point_out + 32 * i, point[i]);
@@ -960,7 +958,7 @@ static void gkr_sumcheck_common(Context &ctx, const apg_gkr_list *list, const ui
     const double t0 = now_ms();
     gkr_prove_sumcheck_device(ctx, l, w.data(), lower, upper, ncols, (int)num_vars, eqp.data(), fn, rounds.data(),
                               point.data(), finals.data());
-    if (ctx.profiling) ctx.stats.sumcheck_ms += now_ms() - t0;
+    ctx.stats.sumcheck_ms += now_ms() - t0;
     for (size_t i = 0; i < 4 * num_vars; i++) store_fe(rounds_out + 32 * i, rounds[i]);
     if (point_out)
         for (size_t i = 0; i < num_vars; i++) store_fe(point_out + 32 * i, point[i]);

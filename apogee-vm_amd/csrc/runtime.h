@@ -55,12 +55,12 @@ struct DevBuf {
 struct HostBuf {
     void *ptr = nullptr;
     size_t cap = 0;
-    void *get(size_t bytes, std::vector<void *> &retired) {
+    void *get(size_t bytes, std::vector<void *> &retired, unsigned flags = hipHostMallocDefault) {
         if (bytes > cap) {
             if (ptr) retired.push_back(ptr);
             ptr = nullptr;
             size_t c = bytes + bytes / 4 + 256;
-            HIP_CHECK(hipHostMalloc(&ptr, c, hipHostMallocDefault));
+            HIP_CHECK(hipHostMalloc(&ptr, c, flags));
             cap = c;
         }
         return ptr;
@@ -70,16 +70,18 @@ struct HostBuf {
     }
 };
 
-// Kernel timings recorded with HIP events on the context's stream when
-// profiling is on (apg_ctx_set_profiling).
+// Per-context statistics: the host wall clock of every commit MSM and
+// sumcheck, always; kernel timings recorded with HIP events on the context's
+// stream only when profiling is on (apg_ctx_set_profiling), since the events
+// and their reads cost a few microseconds per launch.
 struct Stats {
-    double accumulate_ms = 0;        // k_accumulate (MSM bucket accumulation)
+    double accumulate_ms = 0;        // k_accumulate (MSM bucket accumulation), profiling
     uint64_t accumulate_launches = 0;
     uint64_t accumulate_entries = 0;  // nonzero signed digits processed
     double msm_ms = 0;                // whole msm_run, host wall clock
     uint64_t msm_calls = 0;
     uint64_t msm_points = 0;          // scalars (points x columns)
-    double sumcheck_kernel_ms = 0;    // round kernels (fold + evaluate)
+    double sumcheck_kernel_ms = 0;    // round kernels (fold + evaluate), profiling
     uint64_t sumcheck_launches = 0;
     uint64_t sumcheck_bytes = 0;      // algorithmic HBM bytes of those launches
     double sumcheck_ms = 0;           // whole sumcheck, host wall clock
@@ -150,6 +152,12 @@ struct Context {
     template <class T>
     T *h(const char *name, size_t count) {
         return static_cast<T *>(host[name].get(count * sizeof(T), retired_host));
+    }
+    // pinned host memory that device stores reach without a stream
+    // synchronisation (coherent mapping): completion flags and what they guard
+    template <class T>
+    T *hc(const char *name, size_t count) {
+        return static_cast<T *>(host[name].get(count * sizeof(T), retired_host, hipHostMallocCoherent));
     }
     void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
 };

@@ -104,7 +104,7 @@ __device__ __forceinline__ void pair_of(const TablePtrs &tp, int t, size_t i, bo
 template <int KT, bool FULL, bool LIFT>
 __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, int fold, F29 r29,
                                                       EqArgs eq, Fe *partials, unsigned *counter,
-                                                      Fe *out) {
+                                                      Fe *out, unsigned *done) {
     constexpr int NN = FULL ? KT + 1 : KT;
     Fe acc[NN];
 #pragma unroll
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, 
             }
         }
     }
-    reduce_nodes<NN>(acc, NN, partials, counter, out);
+    reduce_nodes<NN>(acc, NN, partials, counter, out, done);
 }
 
 // Any gate: terms read from the kernel argument, table lines kept in a local
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, 
 template <int NN, bool LIFT>
 __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateArgs gate, size_t m,
                                                    int fold, Fe r, EqArgs eq, int full, int nn,
-                                                   Fe *partials, unsigned *counter, Fe *out) {
+                                                   Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
     Fe acc[NN];
 #pragma unroll
     for (int x = 0; x < NN; x++) acc[x] = f_zero<Fr>();
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateAr
             }
         }
     }
-    reduce_nodes<NN>(acc, nn, partials, counter, out);
+    reduce_nodes<NN>(acc, nn, partials, counter, out, done);
 }
 
 // out_o = scale_o * sum_c w_c col_c, o < OUTS (the shard's tables are
@@ -262,78 +262,131 @@ bool Gate::is_plain_product(int k) const {
     return true;
 }
 
-void interpolate_host(const Fe *v, int d, Fe *c) {
+// Host arithmetic of a round (the node sums to the round polynomial, the
+// claim and the running eq product) on four 64-bit words (host_fr.h). The
+// constants 1..SC_MAX_NODES and their inverses are computed once.
+namespace {
+using hfr::W4;
+struct SmallConsts {
+    W4 num[SC_MAX_NODES + 1], inv[SC_MAX_NODES + 1];
+    W4 one, zero, inv2, inv3, inv6;
+    SmallConsts() {
+        for (int x = 0; x <= SC_MAX_NODES; x++) {
+            num[x] = hfr::w4_of(f_from_u64<Fr>((uint64_t)x));
+            inv[x] = x ? hfr::w4_of(f_inv<Fr>(f_from_u64<Fr>((uint64_t)x))) : num[0];
+        }
+        zero = num[0];
+        one = num[1];
+        inv2 = inv[2];
+        inv3 = inv[3];
+        inv6 = hfr::w4_of(f_inv<Fr>(f_from_u64<Fr>(6)));
+    }
+};
+const SmallConsts &smallc() {
+    static const SmallConsts c;
+    return c;
+}
+
+// Coefficients (ascending) of the degree-d polynomial through (x, v[x]),
+// x = 0..d.
+void interpolate_w4(const W4 *v, int d, W4 *c) {
+    using namespace hfr;
+    const SmallConsts &K = smallc();
     if (d == 3) {
         // crates/sumcheck/src/lib.rs:294-305
-        static const Fe inv2 = f_inv<Fr>(f_from_u64<Fr>(2));
-        static const Fe inv3 = f_inv<Fr>(f_from_u64<Fr>(3));
-        static const Fe inv6 = f_inv<Fr>(f_from_u64<Fr>(6));
-        Fe d1 = f_sub<Fr>(v[1], v[0]);
-        Fe t = f_sub<Fr>(v[2], v[1]);
-        Fe d2 = f_sub<Fr>(t, d1);
-        Fe d3 = f_sub<Fr>(f_sub<Fr>(f_sub<Fr>(v[3], v[2]), t), d2);
+        const W4 d1 = w4_sub(v[1], v[0]);
+        const W4 t = w4_sub(v[2], v[1]);
+        const W4 d2 = w4_sub(t, d1);
+        const W4 d3 = w4_sub(w4_sub(w4_sub(v[3], v[2]), t), d2);
         c[0] = v[0];
-        c[1] = f_add<Fr>(f_sub<Fr>(d1, f_mul<Fr>(d2, inv2)), f_mul<Fr>(d3, inv3));
-        c[2] = f_mul<Fr>(f_sub<Fr>(d2, d3), inv2);
-        c[3] = f_mul<Fr>(d3, inv6);
+        c[1] = w4_add(w4_sub(d1, w4_mul(d2, K.inv2)), w4_mul(d3, K.inv3));
+        c[2] = w4_mul(w4_sub(d2, d3), K.inv2);
+        c[3] = w4_mul(d3, K.inv6);
         return;
     }
-    // Newton forward differences on nodes 0..d, expanded into monomials.
-    Fe diff[SC_MAX_NODES], delta[SC_MAX_NODES];
+    // Newton forward differences on nodes 0..d, expanded into monomials:
+    // basis_k(X) = X (X - 1) ... (X - k + 1) / k!
+    W4 diff[SC_MAX_NODES], delta[SC_MAX_NODES];
     for (int i = 0; i <= d; i++) diff[i] = v[i];
     for (int k = 0; k <= d; k++) {
         delta[k] = diff[0];
-        for (int i = 0; i + k < d; i++) diff[i] = f_sub<Fr>(diff[i + 1], diff[i]);
+        for (int i = 0; i + k < d; i++) diff[i] = w4_sub(diff[i + 1], diff[i]);
     }
-    Fe basis[SC_MAX_NODES], acc[SC_MAX_NODES];
-    for (int i = 0; i <= d; i++) basis[i] = acc[i] = f_zero<Fr>();
-    basis[0] = f_one<Fr>();
+    W4 basis[SC_MAX_NODES], acc[SC_MAX_NODES];
+    for (int i = 0; i <= d; i++) basis[i] = acc[i] = K.zero;
+    basis[0] = K.one;
     acc[0] = delta[0];
     for (int k = 1; k <= d; k++) {
-        Fe km1 = f_from_u64<Fr>(k - 1), invk = f_inv<Fr>(f_from_u64<Fr>(k));
-        Fe nb[SC_MAX_NODES];
-        for (int i = 0; i <= d; i++) nb[i] = f_zero<Fr>();
+        W4 nb[SC_MAX_NODES];
+        for (int i = 0; i <= d; i++) nb[i] = K.zero;
         for (int i = 0; i < k; i++) {
-            nb[i + 1] = f_add<Fr>(nb[i + 1], basis[i]);
-            nb[i] = f_sub<Fr>(nb[i], f_mul<Fr>(basis[i], km1));
+            nb[i + 1] = w4_add(nb[i + 1], basis[i]);
+            nb[i] = w4_sub(nb[i], w4_mul(basis[i], K.num[k - 1]));
         }
-        for (int i = 0; i <= k; i++) basis[i] = f_mul<Fr>(nb[i], invk);
-        for (int i = 0; i <= k; i++) acc[i] = f_add<Fr>(acc[i], f_mul<Fr>(basis[i], delta[k]));
+        for (int i = 0; i <= k; i++) basis[i] = w4_mul(nb[i], K.inv[k]);
+        for (int i = 0; i <= k; i++) acc[i] = w4_add(acc[i], w4_mul(basis[i], delta[k]));
     }
     for (int i = 0; i <= d; i++) c[i] = acc[i];
 }
 
-static Fe horner(const Fe *c, int d, const Fe &x) {
-    Fe acc = c[d];
-    for (int i = d - 1; i >= 0; i--) acc = f_add<Fr>(f_mul<Fr>(acc, x), c[i]);
+W4 horner_w4(const W4 *c, int d, const W4 &x) {
+    W4 acc = c[d];
+    for (int i = d - 1; i >= 0; i--) acc = hfr::w4_add(hfr::w4_mul(acc, x), c[i]);
     return acc;
 }
 
 // Values of a degree-D polynomial at 0..D extended to 0..d (d >= D) by
 // forward differences (the D-th difference is constant).
-static void extrapolate(Fe *v, int D, int d) {
-    Fe last[SC_MAX_NODES];  // last[k] = k-th difference ending at v[D]
-    Fe row[SC_MAX_NODES];
+void extrapolate_w4(W4 *v, int D, int d) {
+    W4 last[SC_MAX_NODES];  // last[k] = k-th difference ending at v[D]
+    W4 row[SC_MAX_NODES];
     for (int i = 0; i <= D; i++) row[i] = v[i];
     for (int k = 0; k <= D; k++) {
         last[k] = row[D - k];
-        for (int i = 0; i + k < D; i++) row[i] = f_sub<Fr>(row[i + 1], row[i]);
+        for (int i = 0; i + k < D; i++) row[i] = hfr::w4_sub(row[i + 1], row[i]);
     }
     for (int x = D + 1; x <= d; x++) {
-        for (int k = D - 1; k >= 0; k--) last[k] = f_add<Fr>(last[k], last[k + 1]);
+        for (int k = D - 1; k >= 0; k--) last[k] = hfr::w4_add(last[k], last[k + 1]);
         v[x] = last[0];
     }
 }
 
-// eq(r, x) = (1 - r) + x (2r - 1), the one-variable factor of eq.
-static Fe eq_fe(const Fe &r, const Fe &x) {
-    const Fe one = f_one<Fr>();
-    const Fe base = f_sub<Fr>(one, r);
-    const Fe slope = f_sub<Fr>(f_add<Fr>(r, r), one);
-    return f_add<Fr>(base, f_mul<Fr>(x, slope));
-}
+// The one-variable factor of eq, eq(r, x) = (1 - r) + x (2r - 1): its base
+// and slope.
+struct EqLine {
+    W4 base, slope;
+    explicit EqLine(const Fe &r) {
+        using namespace hfr;
+        const W4 rw = w4_of(r), one = smallc().one;
+        base = w4_sub(one, rw);
+        slope = w4_sub(w4_add(rw, rw), one);
+    }
+    W4 at(const W4 &x) const { return hfr::w4_add(base, hfr::w4_mul(x, slope)); }
+};
 
-static Fe eq_at(const Fe &r, int x) { return eq_fe(r, f_from_u64<Fr>((uint64_t)x)); }
+// g_j from T(X) = P S(X) at X = 0..D (T[1] may come from the claim): the
+// values eq(r_j, X) T(X) at X = 0..d, then the coefficients. With from_claim,
+// g(1) = claim - g(0).
+void round_poly(W4 *T, int D, int d, const EqLine &eq, bool from_claim, const W4 &claim, W4 *coeffs) {
+    using namespace hfr;
+    extrapolate_w4(T, D, d);
+    W4 vals[SC_MAX_NODES];
+    W4 e = eq.base;  // eq at X = 0, then one slope per step
+    for (int x = 0; x <= d; x++) {
+        vals[x] = w4_mul(e, T[x]);
+        e = w4_add(e, eq.slope);
+    }
+    if (from_claim) vals[1] = w4_sub(claim, vals[0]);
+    interpolate_w4(vals, d, coeffs);
+}
+}  // namespace
+
+void interpolate_host(const Fe *v, int d, Fe *c) {
+    W4 vw[SC_MAX_NODES], cw[SC_MAX_NODES];
+    for (int i = 0; i <= d; i++) vw[i] = hfr::w4_of(v[i]);
+    interpolate_w4(vw, d, cw);
+    for (int i = 0; i <= d; i++) c[i] = hfr::fe_of(cw[i]);
+}
 
 void combine_columns_device(Context &ctx, const Fe *cols, size_t n, int K, const Fe *w, const Fe *scale,
                             int outs, Fe *const *out) {
@@ -367,35 +420,38 @@ static unsigned round_blocks(const Context &ctx, size_t m, int per_cu) {
 
 template <int KT, bool LIFT>
 static void launch_product(Context &ctx, bool full, const TablePtrs &tp, size_t m, bool fold,
-                           const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out) {
+                           const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out,
+                           unsigned *done) {
     const dim3 g(round_blocks(ctx, m, 1)), b(RTPB);
     const F29 r29 = f29_from_mont32<Fr>(r);  // rho R' mod r
     if (full)
         hipLaunchKernelGGL((k_round_product<KT, true, LIFT>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
-                           partials, counter, out);
+                           partials, counter, out, done);
     else
         hipLaunchKernelGGL((k_round_product<KT, false, LIFT>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
-                           partials, counter, out);
+                           partials, counter, out, done);
 }
 
 template <int KT>
 static void launch_product(Context &ctx, bool lift, bool full, const TablePtrs &tp, size_t m, bool fold,
-                           const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out) {
+                           const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out,
+                           unsigned *done) {
     if (lift)
-        launch_product<KT, true>(ctx, full, tp, m, fold, r, eq, partials, counter, out);
+        launch_product<KT, true>(ctx, full, tp, m, fold, r, eq, partials, counter, out, done);
     else
-        launch_product<KT, false>(ctx, full, tp, m, fold, r, eq, partials, counter, out);
+        launch_product<KT, false>(ctx, full, tp, m, fold, r, eq, partials, counter, out, done);
 }
 
-// lift: some source table is a narrow backing (the first two rounds)
+// lift: some source table is a narrow backing (the first two rounds); done:
+// the completion flag the round sets after its node sums (or null)
 static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bool full, int D, bool lift,
                          const TablePtrs &tp, size_t m, bool fold, const Fe &r, const EqArgs &eq,
-                         Fe *partials, unsigned *counter, Fe *out) {
+                         Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
     if (product) {
         switch (k) {
-            case 1: launch_product<1>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out); return;
-            case 2: launch_product<2>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out); return;
-            case 3: launch_product<3>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out); return;
+            case 1: launch_product<1>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out, done); return;
+            case 2: launch_product<2>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out, done); return;
+            case 3: launch_product<3>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out, done); return;
             default: break;
         }
     }
@@ -416,10 +472,10 @@ static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bo
     do {                                                                                                      \
         if (lift)                                                                                             \
             hipLaunchKernelGGL((k_round_gate<NN, true>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r, eq, \
-                               (int)full, nn, partials, counter, out);                                        \
+                               (int)full, nn, partials, counter, out, done);                                  \
         else                                                                                                  \
             hipLaunchKernelGGL((k_round_gate<NN, false>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r,    \
-                               eq, (int)full, nn, partials, counter, out);                                    \
+                               eq, (int)full, nn, partials, counter, out, done);                              \
     } while (0)
     switch (D + 1) {
         case 2: APG_GATE(2); break;
@@ -436,7 +492,27 @@ static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bo
 struct SumcheckScratch {
     Fe *r, *eqA, *eqB, *poolA, *poolB, *partials, *nodes;
     unsigned *counter;
+    // completion flag of a round (pinned, coherent; its device mapping), or null
+    unsigned *done_host = nullptr, *done_dev = nullptr;
 };
+
+// Waits until the round kernel in flight has set its completion flag. The
+// stream is queried now and then, so a kernel that failed (or ended without
+// setting the flag) raises an error instead of a wait without end.
+static void wait_round(Context &ctx, volatile unsigned *done) {
+    for (uint32_t spin = 1;; spin++) {
+        if (__atomic_load_n(done, __ATOMIC_ACQUIRE) != 0) return;
+        if ((spin & 255) == 0) {
+            const hipError_t e = hipStreamQuery(ctx.stream);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(done, __ATOMIC_ACQUIRE) != 0) return;
+                throw ApgError(-5, "sumcheck: a round kernel ended without its completion flag");
+            }
+            if (e != hipErrorNotReady) HIP_CHECK(e);
+        }
+        __builtin_ia32_pause();
+    }
+}
 
 struct ScratchLayout {
     size_t off[8];
@@ -516,8 +592,12 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
     // the counter lives apart from the carved region, zero once and left zero
     // by the round kernels
     sc.counter = ctx.zeroed<unsigned>("sc.counter", 64);
-    Fe *hvals = ctx.h<Fe>("sc.hvals", SC_MAX_NODES);
+    // node sums and the round's completion flag in coherent pinned memory:
+    // the host waits on the flag, not on the stream
+    Fe *hvals = ctx.hc<Fe>("sc.hvals", SC_MAX_NODES + 1);
     HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&sc.nodes), hvals, 0));
+    sc.done_host = reinterpret_cast<unsigned *>(hvals + SC_MAX_NODES);
+    sc.done_dev = reinterpret_cast<unsigned *>(sc.nodes + SC_MAX_NODES);
     sumcheck_rounds(ctx, sc, hvals, true, ctx.profiling, gate, tables, kinds, k, n, eq_point, round_fn, rounds_out,
                     point_out, finals_out);
 }
@@ -636,9 +716,10 @@ static void sumcheck_finals_n0(Context &ctx, Fe *const *tables, const uint8_t *k
 
 static void sumcheck_host_tail(Context &ctx, const Gate &gate, int k, int n, int j0, const Fe *eq_point,
                                const Fe *const *dev_tables, const uint8_t *kinds, size_t len, bool fold_first,
-                               const Fe &rho_prev, Fe P, const RoundFn &round_fn, Fe *rounds_out, Fe *point_out,
-                               Fe *finals_out) {
+                               const Fe &rho_prev, const Fe &P_in, const RoundFn &round_fn, Fe *rounds_out,
+                               Fe *point_out, Fe *finals_out) {
     using namespace hfr;
+    W4 P = w4_of(P_in);
     const int D = gate.x_degree(), d = gate.degree();
     std::vector<W4> tab((size_t)k * len);
     std::vector<std::vector<uint8_t>> narrow(k);
@@ -705,17 +786,17 @@ static void sumcheck_host_tail(Context &ctx, const Gate &gate, int k, int n, int
                     for (int t = 0; t < k; t++) cur[t] = w4_add(cur[t], step[t]);
             }
         }
-        const Fe &rj = eq_point[j];
-        Fe T[SC_MAX_NODES], vals[SC_MAX_NODES];
-        for (int x = 0; x <= D; x++) T[x] = f_mul<Fr>(P, fe_of(S[x]));
-        extrapolate(T, D, d);
-        for (int x = 0; x <= d; x++) vals[x] = f_mul<Fr>(eq_at(rj, x), T[x]);
+        const EqLine eqj(eq_point[j]);
+        W4 T[SC_MAX_NODES], cw[SC_MAX_NODES];
+        for (int x = 0; x <= D; x++) T[x] = w4_mul(P, S[x]);
+        round_poly(T, D, d, eqj, false, zero, cw);
         Fe *coeffs = rounds_out + (size_t)j * (d + 1);
-        interpolate_host(vals, d, coeffs);
+        for (int x = 0; x <= d; x++) coeffs[x] = fe_of(cw[x]);
         const Fe rho = round_fn(coeffs, d + 1);
         point_out[j] = rho;
-        P = f_mul<Fr>(P, eq_fe(rj, rho));
-        fold(w4_of(rho));
+        const W4 rw = w4_of(rho);
+        P = w4_mul(P, eqj.at(rw));
+        fold(rw);
     }
     for (int t = 0; t < k; t++) finals_out[t] = fe_of(tab[(size_t)t * len]);
 }
@@ -794,10 +875,12 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
     // Round 0 reads the inputs T^0 (2^n entries). Round j >= 1 folds T^(j-1)
     // with rho_(j-1) into T^j (2^(n-j) entries: odd j into bufA, even j into
     // bufB, so source and destination never alias) and evaluates T^j's pairs.
-    const Fe prod_scale = product ? f_from_u64<Fr>(uint64_t(1) << (5 * k)) : f_one<Fr>();  // k <= 3 there
+    // the product kernel's node sums carry 2^(-5 k), and 2^-5 more when E = lo hi
+    const W4 prod_scale = hfr::w4_of(product ? f_from_u64<Fr>(uint64_t(1) << (5 * k)) : f_one<Fr>());  // k <= 3
+    const W4 prod_scale_lo = hfr::w4_of(product ? f_from_u64<Fr>(uint64_t(1) << (5 * k + 5)) : f_one<Fr>());
     Fe rho = f_zero<Fr>();
-    Fe claim = f_zero<Fr>();
-    Fe P = f_one<Fr>();  // prod_{l<j} eq(r_l, rho_l)
+    W4 claim = smallc().zero;
+    W4 P = smallc().one;  // prod_{l<j} eq(r_l, rho_l)
     for (int j = 0; j < n; j++) {
         if (j > 0) ctx.yield_point();  // between rounds (the last one synchronised the stream)
         if (j > 0 && (size_t)k * (len0 >> j) <= TAIL_ELEMS) {  // T^j is small: the rest on the host
@@ -805,7 +888,7 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
             for (int t = 0; t < k; t++) src[t] = const_cast<Fe *>(cur[t]);
             // T^(j-1) is still the caller's table when j == 1
             sumcheck_host_tail(ctx, gate, k, n, j, eq_point, src.data(), j == 1 ? kinds : nullptr, len0 >> (j - 1),
-                               true, rho, P, round_fn, rounds_out, point_out, finals_out);
+                               true, rho, hfr::fe_of(P), round_fn, rounds_out, point_out, finals_out);
             return;
         }
         const bool fold = j > 0;
@@ -832,23 +915,23 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
             eq.hi = eqB + (size_t)(s - h0) * bstride;
         }
         if (timing) HIP_CHECK(hipEventRecord(ev0, st));
+        // cleared before the launch: the previous round's kernel has set it and writes it no more
+        if (sc.done_host) __atomic_store_n(sc.done_host, 0u, __ATOMIC_RELEASE);
         launch_round(ctx, gate, k, product, full, D, narrow && j <= 1, tp, m, fold, rho, eq, partials, counter,
-                     hvals_dev);
+                     hvals_dev, sc.done_dev);
         HIP_CHECK(hipGetLastError());
         if (timing) HIP_CHECK(hipEventRecord(ev1, st));
         const int nn = full ? D + 1 : D;
         if (!mapped_nodes)
             HIP_CHECK(hipMemcpyAsync(hvals, hvals_dev, nn * sizeof(Fe), hipMemcpyDeviceToHost, st));
         round_fn.begin(d + 1);  // the transcript's message prefix, while the kernel runs
-        ctx.sync();
-        if (product) {  // the product kernel's node sums carry 2^(-5 k), and 2^-5 more for E = lo hi
-            Fe scale = prod_scale;
-            if (eq.lo)
-                for (int q = 0; q < 5; q++) scale = f_add<Fr>(scale, scale);
-            for (int x = 0; x < nn; x++) hvals[x] = f_mul<Fr>(hvals[x], scale);
-        }
+        if (sc.done_host)
+            wait_round(ctx, sc.done_host);
+        else
+            ctx.sync();
         if (timing) {
             float ms = 0;
+            HIP_CHECK(hipEventSynchronize(ev1));
             HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
             ctx.stats.sumcheck_kernel_ms += ms;
             ctx.stats.sumcheck_launches += 1;
@@ -856,25 +939,28 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
             // half-size fold
             ctx.stats.sumcheck_bytes += (uint64_t)k * 32 * (fold ? 6 * m : 2 * m);
         }
-        // T(X) = P S(X) at X = 0..D, then X = D+1..d by extrapolation
-        Fe T[SC_MAX_NODES], vals[SC_MAX_NODES];
+        // T(X) = P S(X) at X = 0..D (the node sums rescaled for the product
+        // kernel), then X = D+1..d by extrapolation
+        using namespace hfr;
+        const EqLine eqj(rj);
+        const W4 Ps = product ? w4_mul(P, eq.lo ? prod_scale_lo : prod_scale) : P;
+        W4 T[SC_MAX_NODES], cw[SC_MAX_NODES];
         if (full) {
-            for (int x = 0; x <= D; x++) T[x] = f_mul<Fr>(P, hvals[x]);
+            for (int x = 0; x <= D; x++) T[x] = w4_mul(Ps, w4_of(hvals[x]));
         } else {
-            T[0] = f_mul<Fr>(P, hvals[0]);
-            for (int x = 2; x <= D; x++) T[x] = f_mul<Fr>(P, hvals[x - 1]);
-            const Fe g0 = f_mul<Fr>(eq_at(rj, 0), T[0]);
-            T[1] = f_mul<Fr>(f_sub<Fr>(claim, g0), inv_r[j]);  // g(1) = r_j T(1)
+            T[0] = w4_mul(Ps, w4_of(hvals[0]));
+            for (int x = 2; x <= D; x++) T[x] = w4_mul(Ps, w4_of(hvals[x - 1]));
+            const W4 g0 = w4_mul(eqj.base, T[0]);
+            T[1] = w4_mul(w4_sub(claim, g0), w4_of(inv_r[j]));  // g(1) = r_j T(1)
         }
-        extrapolate(T, D, d);
-        for (int x = 0; x <= d; x++) vals[x] = f_mul<Fr>(eq_at(rj, x), T[x]);
-        if (!full) vals[1] = f_sub<Fr>(claim, vals[0]);
+        round_poly(T, D, d, eqj, !full, claim, cw);
         Fe *coeffs = rounds_out + (size_t)j * (d + 1);
-        interpolate_host(vals, d, coeffs);
+        for (int x = 0; x <= d; x++) coeffs[x] = fe_of(cw[x]);
         rho = round_fn(coeffs, d + 1);
         point_out[j] = rho;
-        claim = horner(coeffs, d, rho);
-        P = f_mul<Fr>(P, eq_fe(rj, rho));
+        const W4 rw = w4_of(rho);
+        claim = horner_w4(cw, d, rw);
+        P = w4_mul(P, eqj.at(rw));
         if (fold)
             for (int t = 0; t < k; t++) cur[t] = nxt[t];
     }

@@ -51,8 +51,21 @@ __device__ __forceinline__ Fe wave_sum(Fe v) {
 }
 
 // Sum of the first nn of NN per-thread accumulators over the grid into out[].
+// With `done` set (pinned host memory, like out), the block that writes out[]
+// then sets *done = 1 behind a system-scope fence, so the host can wait for
+// the node sums without synchronising the stream.
+__device__ __forceinline__ void signal_done(unsigned *done) {
+    if (!done) return;
+    __syncthreads();  // every out[] store of the block issued
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 template <int NN>
-__device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *counter, Fe *out) {
+__device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *counter, Fe *out,
+                             unsigned *done = nullptr) {
     __shared__ Fe lds[RWAVES][NN];
     __shared__ bool is_last;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -73,7 +86,10 @@ __device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *coun
         else
             partials[(size_t)blockIdx.x * NN + tid] = s;
     }
-    if (single) return;
+    if (single) {
+        signal_done(done);
+        return;
+    }
     __threadfence();
     __syncthreads();
     if (tid == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
@@ -99,6 +115,7 @@ __device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *coun
         out[tid] = s;
     }
     if (tid == 0) *counter = 0;
+    signal_done(done);
 }
 
 // The suffix-eq tables of one proof, as products over the variables they
