@@ -189,6 +189,8 @@ def _declare(L):
     d("apg_engine_stats", [VP, C.POINTER(ApgEngineInfo)])
     d("apg_engine_set_policy", [VP, I32, I32, C.c_double])
     d("apg_engine_set_preemption", [VP, C.c_double, C.POINTER(C.c_uint64)])
+    d("apg_poseidon2_permute_dev", [VP, VP, SZ])
+    d("apg_transcript_run_dev", [VP, C.POINTER(VP), SZ, VP, SZ, VP, VP, SZ, VP, SZ])
     d("apg_engine_set_profiling", [VP, I32])
     d("apg_engine_kernel_stats", [VP, C.POINTER(ApgStats)])
     d("apg_engine_reset_kernel_stats", [VP])
@@ -857,6 +859,92 @@ class Transcript:
         if buf.size != 232:
             raise ValueError("a transcript state is 232 bytes")
         _check(lib().apg_transcript_set_state(self.h, _p(buf)))
+
+
+def poseidon2_permute_device(states, ctx: Optional[Context] = None) -> np.ndarray:
+    """The width-3 Poseidon2 permutation (crates/transcript/src/lib.rs:121-146) of
+    n states (n x 3 Montgomery Fr elements, uint64 (n, 3, 4)) on the device,
+    one state per thread."""
+    ctx = ctx or default_context()
+    a = np.ascontiguousarray(np.asarray(states, dtype=np.uint64).reshape(-1, 3, 4)).copy()
+    if a.shape[0]:
+        _check(lib().apg_poseidon2_permute_dev(ctx.h, _p(a), a.shape[0]))
+    return a
+
+
+class TranscriptProgram:
+    """Transcript operations recorded for one Transcript and run on the device
+    with others (run_transcripts_device): the typed layer (crates/transcript/src/
+    lib.rs:306-346) as raw observe and sample steps. sample() and challenge()
+    return the index of their output in the run's result."""
+
+    OBSERVE, OBSERVE_U64, SAMPLE = 0, 1, 2
+
+    def __init__(self, transcript: "Transcript"):
+        self.transcript = transcript
+        self.ops = []        # (op, a, b)
+        self.operands = []   # Montgomery elements, uint64 x 4
+        self.samples = 0
+
+    def observe(self, x):
+        self.ops.append((self.OBSERVE, len(self.operands), 0))
+        self.operands.append(np.asarray(x, dtype=np.uint64).reshape(4))
+
+    def observe_u64(self, v: int):
+        self.ops.append((self.OBSERVE_U64, v & 0xFFFFFFFF, v >> 32))
+
+    def sample(self) -> int:
+        self.ops.append((self.SAMPLE, self.samples, 0))
+        self.samples += 1
+        return self.samples - 1
+
+    def append_scalars(self, tag: int, xs):
+        xs = list(xs)
+        self.observe_u64(tag)
+        self.observe_u64(len(xs))
+        for x in xs:
+            self.observe(x)
+
+    def append_bytes(self, tag: int, data: bytes):
+        self.observe_u64(tag)
+        self.observe_u64(len(data))
+        for off in range(0, len(data), 31):
+            self.observe(fr_from_bytes(bytes(data[off:off + 31]).ljust(32, b"\0")))
+
+    def challenge(self, tag: int) -> int:
+        self.observe_u64(tag)
+        return self.sample()
+
+
+def run_transcripts_device(programs: Sequence[TranscriptProgram], ctx: Optional[Context] = None) -> list:
+    """Runs every program on its transcript on the device, one transcript per
+    thread (apg_transcript_run_dev); the transcripts take their new states.
+    Returns each program's sampled elements, in order."""
+    ctx = ctx or default_context()
+    if not programs:
+        return []
+    ops, offs, operands, out_base = [], [0], [], []
+    n_out = 0
+    for prog in programs:
+        base_in = len(operands)
+        out_base.append(n_out)
+        for op, a, b in prog.ops:
+            if op == TranscriptProgram.OBSERVE:
+                a += base_in
+            elif op == TranscriptProgram.SAMPLE:
+                a += n_out
+            ops.append((op, a, b, 0))
+        operands.extend(prog.operands)
+        n_out += prog.samples
+        offs.append(len(ops))
+    op_arr = np.array(ops, dtype=np.uint32).reshape(-1, 4) if ops else np.zeros((1, 4), dtype=np.uint32)
+    off_arr = np.array(offs, dtype=np.uint32)
+    in_arr = np.stack(operands).astype(np.uint64) if operands else np.zeros((1, 4), dtype=np.uint64)
+    out_arr = np.zeros((max(n_out, 1), 4), dtype=np.uint64)
+    hs = (VP * len(programs))(*[p.transcript.h for p in programs])
+    _check(lib().apg_transcript_run_dev(ctx.h, hs, len(programs), _p(op_arr), len(ops), _p(off_arr), _p(in_arr),
+                                        len(operands), _p(out_arr), n_out))
+    return [[out_arr[out_base[i] + j].copy() for j in range(p.samples)] for i, p in enumerate(programs)]
 
 
 # ------------------------------------------------------------------ sumcheck

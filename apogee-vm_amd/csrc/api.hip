@@ -18,6 +18,7 @@
 #include "msm.h"
 #include "sumcheck.h"
 #include "transcript.h"
+#include "transcript_dev.h"
 
 using namespace apg;
 
@@ -1289,6 +1290,52 @@ int apg_transcript_set_state(apg_transcript *t, const uint8_t in[232]) {
     t->t.input_len = il;
     t->t.output_len = ol;
     return APG_OK;
+}
+
+int apg_poseidon2_permute_dev(apg_ctx *ctx, uint8_t *states_mont, size_t n) {
+    return guarded([&] {
+        need(ctx);
+        if (n == 0) return;
+        need(states_mont);
+        for (size_t i = 0; i < 3 * n; i++)
+            if (!f_is_canonical<Fr>(load_fe(states_mont + 32 * i)))
+                throw ApgError(APG_ERR_ARG, "poseidon2: state element not canonical");
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        Context &c = ctx->c;
+        Fe *dev = c.d<Fe>("p2.states", 3 * n);
+        HIP_CHECK(hipMemcpyAsync(dev, states_mont, 96 * n, hipMemcpyHostToDevice, c.stream));
+        poseidon2_permute_device(c, dev, n);
+        HIP_CHECK(hipMemcpyAsync(states_mont, dev, 96 * n, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+int apg_transcript_run_dev(apg_ctx *ctx, apg_transcript *const *ts, size_t count, const uint32_t *ops, size_t n_ops,
+                           const uint32_t *offsets, const uint8_t *operands_mont, size_t n_operands,
+                           uint8_t *outputs_mont, size_t n_outputs) {
+    return guarded([&] {
+        need(ctx);
+        if (count == 0) return;
+        need(ts);
+        need(offsets);
+        if (n_ops) need(ops);
+        if (n_operands) need(operands_mont);
+        if (n_outputs) need(outputs_mont);
+        std::vector<Transcript *> tp(count);
+        for (size_t i = 0; i < count; i++) {
+            need(ts[i]);
+            tp[i] = &ts[i]->t;
+        }
+        std::vector<Fe> in(n_operands), out(n_outputs);
+        for (size_t i = 0; i < n_operands; i++) {
+            in[i] = load_fe(operands_mont + 32 * i);
+            if (!f_is_canonical<Fr>(in[i])) throw ApgError(APG_ERR_ARG, "transcript program: operand not canonical");
+        }
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        transcript_run_device(ctx->c, tp.data(), count, ops, n_ops, offsets, in.data(), n_operands, out.data(),
+                              n_outputs);
+        for (size_t i = 0; i < n_outputs; i++) store_fe(outputs_mont + 32 * i, out[i]);
+    });
 }
 
 int apg_fr_to_bytes(const uint8_t mont[32], uint8_t canonical[32]) {

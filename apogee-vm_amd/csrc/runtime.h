@@ -116,6 +116,7 @@ struct Context {
     // written by the thread that owns the context, read by others
     std::atomic<size_t> held_bytes{0};
     int cu_count = 256;
+    bool p2_rc_ready = false;  // the Poseidon2 constants sit in the arena ("p2.rc")
     // Phase-boundary preemption, bound by the shard engine to its scheduler
     // (jobqueue.h); empty for plain contexts, where yield_point does nothing.
     std::function<bool()> yield_on, yield_wanted;

@@ -14,6 +14,10 @@
 namespace apg {
 
 void poseidon2_permute_host(Fe s[3]);
+// The round constants in Montgomery form, in the order the rounds use them:
+// 4 x 3 initial full rounds, 56 internal, 4 x 3 terminal full rounds.
+constexpr int P2_RC_INTERNAL = 12, P2_RC_TERMINAL = 68, P2_RC_COUNT = 80;
+void poseidon2_rc_mont(Fe out[P2_RC_COUNT]);
 // form 0 as poseidon2_permute_host, 1 with the portable multiply, 2 with the
 // mulx/adcx/adox one; false if the form is unknown or the CPU lacks it
 bool poseidon2_permute_form(int form, Fe s[3]);

@@ -121,6 +121,16 @@ void poseidon2_permute_host(Fe st[3]) {
     permute<w4_mul_lazy>(st);
 }
 
+void poseidon2_rc_mont(Fe out[P2_RC_COUNT]) {
+    const RoundConstants &k = rc();
+    for (int r = 0; r < 4; r++)
+        for (int l = 0; l < 3; l++) {
+            out[3 * r + l] = fe_of(k.ini[r][l]);
+            out[P2_RC_TERMINAL + 3 * r + l] = fe_of(k.ter[r][l]);
+        }
+    for (int i = 0; i < 56; i++) out[P2_RC_INTERNAL + i] = fe_of(k.mid[i]);
+}
+
 bool poseidon2_permute_form(int form, Fe st[3]) {
     switch (form) {
         case 0: poseidon2_permute_host(st); return true;

@@ -483,6 +483,24 @@ int apg_transcript_round(void *transcript, const uint8_t *coeffs_mont, size_t n_
 /* sponge state: state[3], input[2], output[2] (Montgomery, 32 B each), then
  * input_len and output_len as u32: 232 bytes */
 int apg_transcript_state(const apg_transcript *t, uint8_t out[232]);
+/* ---- the transcript on the device (one transcript per thread) ----
+ * apg_poseidon2_permute_dev permutes n states of three 32-byte Montgomery Fr
+ * elements in place (host buffer, canonical elements): the reference's
+ * poseidon2_permute (crates/transcript/src/lib.rs:121-146) for many states at
+ * once. apg_transcript_run_dev runs, for each i < count, the program
+ * ops[offsets[i] .. offsets[i + 1]) on transcript ts[i] on the device and
+ * leaves each transcript in its new state. An op is four uint32 words:
+ *   {0, a, 0, 0}   observe operands[a]               (Transcript::observe, lib.rs:241-250)
+ *   {1, lo, hi, 0} observe the element lo + 2^32 hi  (tags and lengths of the typed layer)
+ *   {2, o, 0, 0}   outputs[o] = sample()              (Transcript::sample, lib.rs:253-262)
+ * so append_scalars(tag, xs) is {1, tag}, {1, n}, then {0, .} per scalar, and
+ * challenge(tag) is {1, tag}, {2, .} (lib.rs:306-346). Operand or output
+ * indices out of range, unknown ops, decreasing offsets or non-canonical
+ * operands are APG_ERR_ARG. Results equal the host transcript's. */
+int apg_poseidon2_permute_dev(apg_ctx *ctx, uint8_t *states_mont, size_t n);
+int apg_transcript_run_dev(apg_ctx *ctx, apg_transcript *const *ts, size_t count, const uint32_t *ops, size_t n_ops,
+                           const uint32_t *offsets, const uint8_t *operands_mont, size_t n_operands,
+                           uint8_t *outputs_mont, size_t n_outputs);
 /* The inverse of apg_transcript_state: sets the sponge state, both buffers
  * and their fill counts from the same 232-byte layout, so a caller can hand
  * its own transcript's state across the boundary and take it back
