@@ -19,9 +19,11 @@ engine (12 workers) and by the plain one-shard-per-call path:
     one 2^20 shard, and a random linear combination of every column of one
     2^21 and one 2^22 shard (sum_c g^c cm_c = MSM(bases, sum_c g^c col_c)).
 Config 5, the served stream: the same shards arriving as a seeded Poisson
-process at 0.5x, 1x and 1.5x the batch's measured capacity into a serving
-engine (4 workers, a fast lane on 32 CUs, shortest first): every result equals
-the plain path's, so a shard's bytes are the same alone and in the mix.
+process at 0.5x, 1x and 1.5x the batch's measured capacity into the serving
+engine of the benchmark (6 workers, 2 of them kept for short shards, shortest
+first, jobs pausing at phase boundaries for jobs 2x smaller): every result
+equals the plain path's, so a shard's bytes are the same alone, in the mix and
+when paused and resumed.
 The reference tests determinism under concurrency the same way
 (crates/prover/tests/streaming.rs:54-110).
 """
@@ -239,8 +241,9 @@ def test_served_stream_matches_plain(apg, batch):
     the serving engine; every result is the plain path's."""
     wl = importlib.import_module("apogee-vm_amd.workload")
     specs, bufs, srs, plain = batch["specs"], batch["bufs"], batch["srs"], batch["plain"]
-    eng = apg.Engine(0, workers=4, hbm_budget=200 << 30, lane_workers=1, lane_cus=32, small_scalars=float(1 << 20))
-    eng.set_policy("shortest", 1, float(1 << 20))
+    eng = apg.Engine(0, workers=6, hbm_budget=200 << 30, lane_workers=2, lane_cus=0, small_scalars=float(1 << 20))
+    eng.set_policy("shortest", 2, float(1 << 20))
+    eng.set_preemption(2.0)
     order = np.random.default_rng(SEED).permutation(len(specs))
     for load in (0.5, 1.0, 1.5):
         arrivals = wl.poisson_arrivals(load * batch["capacity"], len(specs), seed=SEED + int(load * 10))
@@ -261,4 +264,6 @@ def test_served_stream_matches_plain(apg, batch):
               flush=True)
     info = eng.stats()
     assert info["completed"] == 3 * len(specs) and info["queued"] == 0 and info["running"] == 0
+    print(f"pauses at phase boundaries: {eng.pauses()}", flush=True)
+    assert eng.pauses() > 0  # the long shards gave way to short ones
     eng.close()
