@@ -123,6 +123,15 @@ def run_single(ctx, srs, args, nv=20, k=8, deg=3, steps=10, warmup=2):
     ctx.set_profiling(was)
     roof = roofline(st, f"k_accumulate (MSM bucket accumulation), one 2^{nv} x {k} shard alone")
     sc_gbs = st["sumcheck_bytes"] / (st["sumcheck_kernel_ms"] * 1e-3) / 1e9 if st["sumcheck_kernel_ms"] else 0.0
+    # the survey's algorithmic multiplies of a sumcheck over eq and deg - 1
+    # tables (t = deg live tables): 136 ((t + 1)(t - 1) + t) 2^n IMUL
+    live = deg
+    sc_imul = 136 * ((live + 1) * (live - 1) + live) * n
+    sc_imul_rate = sc_imul * steps / (st["sumcheck_kernel_ms"] * 1e-3) if st["sumcheck_kernel_ms"] else 0.0
+    # and its bytes, 96 t 2^n + 32 2^n (eq counted as a table; the library
+    # never materialises eq, so its own count above is lower)
+    sc_survey_bytes = (96 * live + 32) * n
+    sc_survey_gbs = sc_survey_bytes * steps / (st["sumcheck_kernel_ms"] * 1e-3) / 1e9 if st["sumcheck_kernel_ms"] else 0.0
     ff = fixed_formula_msm(buf, n, k)
     msm_s = wall["msm_ms"] * 1e-3 / steps
     ff_rate = ff["imul"] / msm_s if msm_s else 0.0
@@ -136,6 +145,9 @@ def run_single(ctx, srs, args, nv=20, k=8, deg=3, steps=10, warmup=2):
         "host_poseidon2_us": host_permutation_us(),
         "sumcheck_kernel_gbs": round(sc_gbs, 1),
         "sumcheck_hbm_frac": round(sc_gbs * 1e9 / PEAK_HBM, 4),
+        "sumcheck_kernel_gbs_survey_bytes": round(sc_survey_gbs, 1),
+        "sumcheck_kernel_timul_per_s": round(sc_imul_rate / 1e12, 3),
+        "sumcheck_imul_frac": round(sc_imul_rate / PEAK_IMUL, 4),
         "latency_ms": {"p50": round(float(np.percentile(np.array(lat) * 1e3, 50)), 3),
                        "p99": round(float(np.percentile(np.array(lat) * 1e3, 99)), 3)},
         # the whole MSM (every kernel and the host tail, wall clock) against
@@ -790,6 +802,21 @@ def main():
     timed = eng.kernel_stats()
     t_eng = float(np.mean(times))
     info = eng.stats()
+    # where the pass's time goes: the engine on each size class alone and on
+    # the large and medium shards together, against the plain path's time per
+    # class (measured in the plain pass above)
+    nvs = np.array([s.num_vars for s in specs])
+    classes = {"rows_2e8_2e14": nvs <= 14, "rows_2e15_2e19": (nvs >= 15) & (nvs <= 19), "rows_2e20_2e22": nvs >= 20}
+    by_class = {}
+    for name, sel in classes.items():
+        idx = np.flatnonzero(sel)
+        dt, _ = run_engine_pass(eng, srs, [specs[i] for i in idx], [bufs[i] for i in idx])
+        by_class[name] = {"shards": int(len(idx)), "engine_ms": round(1e3 * dt, 2),
+                          "plain_ms": round(1e3 * float(iso[idx].sum()), 2)}
+    idx = np.flatnonzero(nvs >= 15)
+    dt, _ = run_engine_pass(eng, srs, [specs[i] for i in idx], [bufs[i] for i in idx])
+    by_class["rows_2e15_2e22"] = {"shards": int(len(idx)), "engine_ms": round(1e3 * dt, 2),
+                                  "plain_ms": round(1e3 * float(iso[idx].sum()), 2)}
     value = len(specs) / t_eng
     log("single large shard (2^20 x 8) with kernel timing")
     single, roof_single, _ = run_single(ctx, srs, args)
@@ -839,6 +866,7 @@ def main():
         },
         "plain_shards_per_s": round(len(specs) / t_plain, 3),
         "batched_over_plain": round(value / (len(specs) / t_plain), 3),
+        "by_class": by_class,
         "bit_exact_vs_plain": bool(exact),
         "hbm": {"srs_gib": round(srs.device_bytes() / 2**30, 3),
                 "columns_gib": round(sum(32 * s.elements for s in specs) / 2**30, 3),
