@@ -71,10 +71,23 @@ def test_largest_shard_properties(apg, orc, srs22):
     cols = buf.download().reshape(k, n, 4)
     buf.free()
 
-    # commitments of the first and last column against the oracle MSM
+    # commitments against the oracle MSM: column 0 directly, and every column
+    # through a random linear combination (sum_c g^c cm_c = MSM(bases,
+    # sum_c g^c col_c) by linearity; the combination's own check fails for a
+    # wrong column with probability 1 - 16/r)
     bases = srs22.g1(0, n)
-    for c in (0, k - 1):
-        assert p.commitments[c] == orc.jac_to_wire(orc.msm(bases, cols[c], threads=8, workers=16))
+    assert p.commitments[0] == orc.jac_to_wire(orc.msm(bases, cols[0], threads=8, workers=16))
+    gam = orc.fill_uniform(4242, 1)[0]
+    comb, lhs, w = orc.fe(n), None, orc.from_u64(orc.FR, 1)
+    for c in range(k):
+        orc.axpy(w, cols[c], comb)
+        a = orc.affine_from_wire(p.commitments[c])
+        assert a is not None
+        term = orc.jac_mul(orc.jac_from_affine(a), w)
+        lhs = term if lhs is None else orc.jac_add(lhs, term)
+        w = orc.mul(orc.FR, w, gam)
+    assert orc.jac_to_wire(lhs) == orc.jac_to_wire(orc.msm(bases, comb, threads=8, workers=16))
+    del comb
 
     # the transcript replayed from the proof's own messages
     t = orc.Transcript()
@@ -87,9 +100,21 @@ def test_largest_shard_properties(apg, orc, srs22):
         chal.append(t.challenge(orc.TAG_SUMCHECK_CHALLENGE))
     chal = np.stack(chal)
 
-    # round chain: g_j(0) + g_j(1) = g_{j-1}(rho_{j-1})
+    # the combined tables t_o = sum_c rho_w^(o k + c) col_c, and round 0
+    # against the claim: g_0(0) + g_0(1) = sum_x eq(r, x) t_0(x) t_1(x) t_2(x)
+    pw = [orc.from_u64(orc.FR, 1)]
+    for _ in range(f * k - 1):
+        pw.append(orc.mul(orc.FR, pw[-1], rho_w))
+    tabs = [orc.fe(n) for _ in range(f)]
+    for c in range(k):
+        for o in range(f):
+            orc.axpy(pw[o * k + c], cols[c], tabs[o])
     one = orc.from_u64(orc.FR, 1)
     zero = orc.fe()
+    g00 = orc.add(orc.FR, _horner(orc, p.rounds[0], zero), _horner(orc, p.rounds[0], one))
+    assert np.array_equal(g00, orc.product_claim(r, tabs)), "round 0 against the claim"
+
+    # round chain: g_j(0) + g_j(1) = g_{j-1}(rho_{j-1})
     for j in range(1, nv):
         g0 = _horner(orc, p.rounds[j], zero)
         g1 = _horner(orc, p.rounds[j], one)
@@ -104,11 +129,5 @@ def test_largest_shard_properties(apg, orc, srs22):
     assert np.array_equal(last, prod)
 
     # final evaluations = the rho_w-combined columns evaluated at rho
-    col_at = [orc.evaluate(cols[c], chal) for c in range(k)]
-    w = orc.from_u64(orc.FR, 1)
     for o in range(f):
-        acc = orc.fe()
-        for c in range(k):
-            acc = orc.add(orc.FR, acc, orc.mul(orc.FR, w, col_at[c]))
-            w = orc.mul(orc.FR, w, rho_w)
-        assert np.array_equal(acc, p.final_evals[o]), f"table {o}"
+        assert np.array_equal(orc.evaluate(tabs[o], chal), p.final_evals[o]), f"table {o}"
