@@ -129,6 +129,8 @@ def _declare(L):
     d("apg_srs_device_bytes", [VP], SZ)
     d("apg_msm_set_window_bits", [I32, I32])
     d("apg_msm_window_bits", [SZ])
+    d("apg_msm_set_slice_rounds", [I32, I32])
+    d("apg_msm_slice_rounds", [SZ])
     d("apg_srs_release", [VP], None)
     d("apg_msm_g1_fr", [VP, VP, VP, SZ, VP])
     d("apg_msm_g1_u32", [VP, VP, VP, SZ, VP])
@@ -420,6 +422,16 @@ def set_window_bits(log2n: int, c: int):
 
 def window_bits(n: int) -> int:
     return int(lib().apg_msm_window_bits(n))
+
+
+def set_slice_rounds(log2n: int, rounds: int):
+    """Accumulation slices per resident thread for the size class floor(log2 n) =
+    log2n (0 restores the built-in table). Results never depend on it."""
+    _check(lib().apg_msm_set_slice_rounds(log2n, rounds))
+
+
+def slice_rounds(n: int) -> int:
+    return int(lib().apg_msm_slice_rounds(n))
 
 
 # ------------------------------------------------------------------ SRS + commit

@@ -585,6 +585,12 @@ int apg_msm_set_window_bits(int log2n, int c) {
 
 int apg_msm_window_bits(size_t n) { return msm_pick_c(n, 1); }
 
+int apg_msm_set_slice_rounds(int log2n, int rounds) {
+    return guarded([&] { msm_set_slice_rounds(log2n, rounds); });
+}
+
+int apg_msm_slice_rounds(size_t n) { return msm_pick_rounds(n); }
+
 int apg_srs_prepare(apg_ctx *ctx, apg_srs *srs, size_t n) {
     return guarded([&] {
         need(ctx);

@@ -46,6 +46,14 @@ constexpr int MSM_MIN_C = 3, MSM_MAX_C = 17;
 int msm_pick_c(size_t n, size_t columns);
 void msm_set_window_bits(int log2n, int c);
 
+// Accumulation slices per resident k_accumulate thread (the bucket tile: the
+// slice length is the pair total over rounds x resident threads, at least
+// 16 and an eighth of the mean bucket), per size class like the window width;
+// msm_set_slice_rounds overrides one class (0 restores the built-in table).
+constexpr int MSM_MIN_ROUNDS = 1, MSM_MAX_ROUNDS = 16;
+int msm_pick_rounds(size_t n);
+void msm_set_slice_rounds(int log2n, int rounds);
+
 // Builds copies = windows(254, c) shifted copies of the first n bases (device
 // pointer `src`, affine Montgomery) into a new table.
 // msm_table_convert puts n affine Montgomery points (canonical coordinates) in

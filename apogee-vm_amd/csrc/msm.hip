@@ -255,9 +255,9 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_rowscan(uint32_t *__restrict_
 // group in column-major, bin-ascending order: the final key order), the plan,
 // the fine pass's extra chunks (chunks 1.. of every row of more than cap
 // pairs, as (row, chunk)), and, when rows are keys, the key ranges.
-// `target` is the slice count wanted (ACC_ROUNDS, or twice that for a job
-// that pauses between accumulation chunks, per thread k_accumulate keeps
-// resident); slices are at least MIN_SLICE long, at most total / target and an
+// `target` is the slice count wanted (the size class's slice rounds, twice
+// that for a job that pauses between accumulation chunks, per thread
+// k_accumulate keeps resident); slices are at least MIN_SLICE long, at most total / target and an
 // eighth of the mean bucket, a multiple of 4, and never more than slice_cap.
 __global__ void __launch_bounds__(SORT_TPB) k_sort_base(const uint32_t *__restrict__ rowtot, uint32_t rows,
                                                       uint32_t nkeys, uint32_t target, uint32_t slice_cap,
@@ -870,6 +870,34 @@ void host_batch_to_affine(const Xyzz *in, Affine *out, size_t k) {
 
 namespace {
 std::atomic<int> g_window_override[64];  // by floor(log2 n); 0 = the default rule
+std::atomic<int> g_rounds_override[64];  // by floor(log2 n); 0 = the built-in table
+
+int floor_log2(size_t n) {
+    int lg = 0;
+    while (lg < 63 && (size_t(1) << (lg + 1)) <= n) lg++;
+    return lg;
+}
+}
+
+void msm_set_slice_rounds(int log2n, int rounds) {
+    if (log2n < 0 || log2n >= 64) throw ApgError(-2, "slice table: log2 n out of range");
+    if (rounds != 0 && (rounds < MSM_MIN_ROUNDS || rounds > MSM_MAX_ROUNDS))
+        throw ApgError(-2, "slice table: rounds out of range");
+    g_rounds_override[log2n] = rounds;
+}
+
+// Built-in slices per resident thread for floor(log2 n) = 8..22. The sweep of
+// {2, 3, 4, 6, 8} for 8 witness-like columns on MI355X
+// (apogee-vm_amd/autotune.py --what rounds) found every class within 2% of
+// ACC_ROUNDS = 4 (the accumulation is bound by VALU issue, not by the tail
+// of its slices), so the table keeps 4.
+static const int kTunedRounds[23] = {0, 0, 0, 0, 0, 0, 0, 0, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+
+int msm_pick_rounds(size_t n) {
+    const int lg = floor_log2(n);
+    if (const int o = g_rounds_override[lg]) return o;
+    if (lg < 23 && kTunedRounds[lg]) return kTunedRounds[lg];
+    return (int)ACC_ROUNDS;
 }
 
 void msm_set_window_bits(int log2n, int c) {
@@ -997,7 +1025,7 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     // MIN_SLICE pairs. A preemptible job launches one resident wave of slices
     // at a time and pauses between launches, so its slices are half as long.
     const size_t resident = (size_t)ctx.cu_count * 4 * 3 * 64;
-    const size_t rounds = ctx.preemptible() ? 2 * ACC_ROUNDS : ACC_ROUNDS;
+    const size_t rounds = (size_t)msm_pick_rounds(n) * (ctx.preemptible() ? 2 : 1);
     const size_t slice_cap = std::min(rounds * resident + 1, max_pairs / MIN_SLICE + 1);
 
     const unsigned sblocks = (unsigned)(K * geo.nbc);

@@ -211,6 +211,13 @@ size_t apg_srs_device_bytes(const apg_srs *srs);
  * the next commit of that class. */
 int apg_msm_set_window_bits(int log2n, int c);
 int apg_msm_window_bits(size_t n);
+/* Accumulation slices per resident thread per size class (the bucket tile:
+ * more rounds, shorter slices); results never depend on it.
+ * apg_msm_set_slice_rounds overrides one class (0 restores the built-in
+ * table; 1 <= rounds <= 16), for the autotuner. No counterpart in the
+ * reference, whose tasks are windows x chunks (crates/curve/src/msm.rs:242-245). */
+int apg_msm_set_slice_rounds(int log2n, int rounds);
+int apg_msm_slice_rounds(size_t n);
 void apg_srs_release(apg_srs *srs);
 
 /* ---- MSM over explicit bases.

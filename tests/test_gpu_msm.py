@@ -155,3 +155,34 @@ def test_commit_2_16_uniform(apg, orc):
     f = orc.fill_uniform(100, n)
     want = orc.jac_to_wire(orc.msm(srs_aff, f, workers=8)).hex()
     assert wire(apg, apg.commit(s, f)) == want
+
+
+@pytest.mark.parametrize("n", [1 << 12, 3000])
+def test_msm_tuning_knobs_keep_commitments(apg, orc, srs12, n):
+    """The autotuner's knobs (window width, slices per resident accumulation
+    thread) change the schedule, never the points: 8 witness-mix columns
+    committed under several settings of both equal the oracle's."""
+    _, s = srs12
+    srs_aff = s.g1()
+    cols = [orc.fill_witness(4000 + n + i, n) for i in range(8)]
+    want = [orc.jac_to_wire(orc.msm(srs_aff[:n], c, workers=8)).hex() for c in cols]
+    lg = n.bit_length() - 1
+    try:
+        for c, rounds in ((0, 1), (0, 3), (0, 8), (0, 16), (8, 2), (11, 6)):
+            apg.set_window_bits(lg, c)
+            apg.set_slice_rounds(lg, rounds)
+            assert apg.slice_rounds(n) == rounds
+            got = apg.commit_batch(s, cols)
+            assert [wire(apg, g) for g in got] == want, (c, rounds)
+    finally:
+        apg.set_window_bits(lg, 0)
+        apg.set_slice_rounds(lg, 0)
+    assert apg.slice_rounds(n) == 4
+
+
+def test_msm_tuning_knob_refusals(apg):
+    L = apg.lib()
+    assert L.apg_msm_set_slice_rounds(10, 17) == -2
+    assert L.apg_msm_set_slice_rounds(10, -1) == -2
+    assert L.apg_msm_set_slice_rounds(64, 4) == -2
+    assert L.apg_msm_set_window_bits(10, 18) == -2
