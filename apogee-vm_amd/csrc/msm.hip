@@ -1077,7 +1077,9 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     Xyzz29 *part = ctx.d<Xyzz29>("msm.part", 2 * slice_cap);
     const bool chunked = ctx.preemptible();
     if (chunked) ctx.yield_point();
-    if (ctx.profiling) {
+    // kernel timing brackets a single launch only: a chunked (preemptible)
+    // accumulation may pause between its launches
+    if (ctx.profiling && !chunked) {
         HIP_CHECK(hipEventCreate(&g.ev0));
         HIP_CHECK(hipEventCreate(&g.ev1));
         HIP_CHECK(hipEventRecord(g.ev0, st));
@@ -1089,7 +1091,7 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
                            skeys, entries, plan, tab.points, buckets, part, (uint32_t)t0);
         HIP_CHECK(hipGetLastError());
     }
-    if (ctx.profiling) HIP_CHECK(hipEventRecord(g.ev1, st));
+    if (g.ev1) HIP_CHECK(hipEventRecord(g.ev1, st));
     if (chunked) ctx.yield_point();
 
     // 4. buckets: empty ones set, crossing ones merged from their partials. A
@@ -1221,6 +1223,7 @@ void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *f
     host_batch_to_affine(res.data(), out_host, K);
     if (ctx.profiling) {
         for (const GroupRun &g : groups) {
+            if (!g.ev0) continue;  // a chunked accumulation, not timed
             float ms = 0;
             HIP_CHECK(hipEventElapsedTime(&ms, g.ev0, g.ev1));
             ctx.stats.accumulate_ms += ms;

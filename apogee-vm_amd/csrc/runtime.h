@@ -75,7 +75,8 @@ struct HostBuf {
 // stream only when profiling is on (apg_ctx_set_profiling), since the events
 // and their reads cost a few microseconds per launch.
 struct Stats {
-    double accumulate_ms = 0;        // k_accumulate (MSM bucket accumulation), profiling
+    double accumulate_ms = 0;        // k_accumulate (MSM bucket accumulation), profiling;
+                                     // not a preemptible job's chunked launches
     uint64_t accumulate_launches = 0;
     uint64_t accumulate_entries = 0;  // nonzero signed digits processed
     double msm_ms = 0;                // whole msm_run, host wall clock
