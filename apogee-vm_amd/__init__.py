@@ -190,7 +190,7 @@ def _declare(L):
     d("apg_engine_wait", [VP, C.c_uint64, VP, VP, VP, VP, C.POINTER(ApgJobStats)])
     d("apg_engine_stats", [VP, C.POINTER(ApgEngineInfo)])
     d("apg_engine_set_policy", [VP, I32, I32, C.c_double])
-    d("apg_engine_set_preemption", [VP, C.c_double, C.POINTER(C.c_uint64)])
+    d("apg_engine_set_preemption", [VP, C.c_double, I32, C.POINTER(C.c_uint64)])
     d("apg_poseidon2_permute_dev", [VP, VP, SZ])
     d("apg_transcript_run_dev", [VP, C.POINTER(VP), SZ, VP, SZ, VP, VP, SZ, VP, SZ])
     d("apg_engine_set_profiling", [VP, I32])
@@ -1206,7 +1206,7 @@ class Engine:
             _check(lib().apg_engine_create(device, workers, hbm_budget, C.byref(h)))
         self.h = h
         self._jobs = {}
-        self._preempt = 0.0
+        self._preempt = (0.0, "size")
         _live["engine"].add(self)
 
     def close(self):
@@ -1266,18 +1266,24 @@ class Engine:
             raise ValueError(f"unknown policy {policy!r}")
         _check(lib().apg_engine_set_policy(self.h, self.POLICIES[policy], reserve, float(small_scalars)))
 
-    def set_preemption(self, ratio: float = 0.0) -> int:
+    PREEMPT = {"size": 0, "urgency": 1}
+
+    def set_preemption(self, ratio: float = 0.0, mode: str = "size") -> int:
         """Phase-boundary preemption: with ratio >= 1 a running job pauses at its next
-        phase boundary while a job ratio times smaller (rows x columns) runs beside it;
-        0 turns it off. Returns the pauses so far."""
+        phase boundary while a job it gives way to runs beside it: one ratio times
+        smaller (rows x columns, mode "size"), or one whose urgency (latency so far
+        plus estimated remaining time, over its estimated time alone) exceeds its own
+        times ratio (mode "urgency"). 0 turns it off. Returns the pauses so far."""
+        if mode not in self.PREEMPT:
+            raise ValueError(f"unknown preemption mode {mode!r}")
         pauses = C.c_uint64(0)
-        _check(lib().apg_engine_set_preemption(self.h, float(ratio), C.byref(pauses)))
-        self._preempt = float(ratio)
+        _check(lib().apg_engine_set_preemption(self.h, float(ratio), self.PREEMPT[mode], C.byref(pauses)))
+        self._preempt = (float(ratio), mode)
         return int(pauses.value)
 
     def pauses(self) -> int:
         """Pauses of jobs at phase boundaries so far (the preemption setting is kept)."""
-        return self.set_preemption(self._preempt)
+        return self.set_preemption(*self._preempt)
 
     def set_profiling(self, on: bool = True):
         _check(lib().apg_engine_set_profiling(self.h, 1 if on else 0))

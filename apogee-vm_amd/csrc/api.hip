@@ -240,6 +240,14 @@ size_t shard_hbm_estimate(size_t num_vars, size_t k, int degree) {
     return msm + sc + (size_t(64) << 20);
 }
 
+// A shard's time alone on an MI355X, for urgency preemption: a fixed part,
+// ~45 us per sumcheck round and ~0.45 ns per scalar, fitted to the plain
+// path's times over the mixed batch (within ~25% from 2^8 x 13 to 2^22 x 24;
+// the scheduler needs only their order and rough scale).
+double shard_time_estimate_ms(size_t num_vars, size_t k) {
+    return 0.5 + 0.045 * (double)num_vars + 4.5e-7 * std::ldexp((double)k, (int)num_vars);
+}
+
 // The shard composition: commit the k columns, absorb the commitments as one
 // COMMITMENT message (pcs::append_g1_list, crates/pcs/src/lib.rs:220), draw
 // rho (MERCURY_BATCH), fold the columns into degree - 1 tables with weights
@@ -250,7 +258,12 @@ void prove_shard_impl(Context &c, apg_srs *srs, const Fe *cols, size_t num_vars,
                       uint8_t *final_evals_out) {
     const size_t n = size_t(1) << num_vars;
     std::vector<uint8_t> cm_mont(64 * k);
-    commit_device(c, srs, cols, nullptr, n, k, cm_mont.data());
+    // the phases' shares of the shard's time (progress for urgency
+    // preemption): the commit MSM ~85%, the combination ~3%, the rounds
+    {
+        ProgressSpan span(c, 0.0, 0.85);
+        commit_device(c, srs, cols, nullptr, n, k, cm_mont.data());
+    }
     for (size_t i = 0; i < k; i++) apg_g1_to_wire(&cm_mont[64 * i], commitments_wire + 64 * i);
     t.append_g1_wire(TAG_COMMITMENT, commitments_wire, k);
     const Fe rho = t.challenge(TAG_MERCURY_BATCH);
@@ -268,7 +281,8 @@ void prove_shard_impl(Context &c, apg_srs *srs, const Fe *cols, size_t num_vars,
     std::vector<Fe *> outs(f);
     for (int o = 0; o < f; o++) outs[o] = comb + (size_t)o * n;
     combine_columns_device(c, cols, n, (int)k, w.data(), scale.data(), f, outs.data());
-    c.yield_point();
+    c.yield_point(0.88);
+    ProgressSpan span(c, 0.88, 1.0);
     std::vector<uint8_t> r(32 * num_vars);
     for (size_t i = 0; i < num_vars; i++) store_fe(&r[32 * i], t.challenge(TAG_SUMCHECK_CHALLENGE));
     Gate g;
@@ -1132,6 +1146,7 @@ int apg_engine_submit_shard(apg_engine *e, apg_srs *srs, const void *columns_dev
         auto job = std::make_unique<Job>();
         job->mem_bytes = shard_hbm_estimate(num_vars, k, degree);
         job->work = std::ldexp((double)k, (int)num_vars);
+        job->iso_ms = shard_time_estimate_ms(num_vars, k);
         const Transcript t0 = seed ? seed->t : Transcript();
         const Fe *cols = static_cast<const Fe *>(columns_dev);
         job->fn = [=](Context &c, Job &j) {
@@ -1203,11 +1218,13 @@ int apg_engine_set_policy(apg_engine *e, int policy, int reserve, double small_s
     });
 }
 
-int apg_engine_set_preemption(apg_engine *e, double ratio, uint64_t *pauses) {
+int apg_engine_set_preemption(apg_engine *e, double ratio, int mode, uint64_t *pauses) {
     return guarded([&] {
         need(e);
         if (!(ratio >= 0) || (ratio > 0 && ratio < 1)) throw ApgError(APG_ERR_ARG, "preemption ratio must be 0 or >= 1");
-        e->e->set_preemption(ratio);
+        if (mode != APG_PREEMPT_SIZE && mode != APG_PREEMPT_URGENCY)
+            throw ApgError(APG_ERR_ARG, "unknown preemption mode");
+        e->e->set_preemption(ratio, static_cast<Preempt>(mode));
         if (pauses) *pauses = e->e->pauses();
     });
 }

@@ -26,7 +26,9 @@ class Engine {
     int workers() const { return (int)ctxs_.size(); }
     void set_policy(Policy p, int reserve, double small_work) { sched_->set_policy(p, reserve, small_work); }
     // fast-lane jobs count for preemption only when the lane shares its CUs
-    void set_preemption(double ratio) { sched_->set_preemption(ratio, !lane_split_); }
+    void set_preemption(double ratio, Preempt mode = Preempt::kSize) {
+        sched_->set_preemption(ratio, !lane_split_, mode);
+    }
     uint64_t pauses() { return sched_->pauses(); }
     // Kernel timing on every worker's stream (see Stats); summed over workers
     // as their jobs finish.

@@ -58,10 +58,11 @@ Engine::Engine(int device, int workers, size_t hbm_budget, int lane_workers, int
         c.stats = Stats();
     };
     hooks.bind_yield = [](Context &c, std::function<bool()> on, std::function<bool()> wanted,
-                          std::function<void()> wait) {
+                          std::function<void()> wait, std::function<void(double)> progress) {
         c.yield_on = std::move(on);
         c.yield_wanted = std::move(wanted);
         c.yield_wait = std::move(wait);
+        c.yield_progress = std::move(progress);
     };
     sched_ = std::make_unique<Scheduler<Context>>(raw, hbm_budget, lane_workers, std::move(hooks));
 }

@@ -53,9 +53,22 @@ namespace apg {
 // stream at each yield point; a smaller job that starts beside it has the
 // CUs to itself once the phase in flight ends. A paused job
 // keeps its worker, so a job only pauses for one that already runs (never
-// for a queued one, which might wait for that very worker): no cycle of
-// waits can form, because each pause is for a strictly smaller job.
+// for a queued one, which might wait for that very worker), and the job
+// that comes first in the order (the smallest, or the most urgent) never
+// pauses: some job always runs, and no cycle of waits can form.
 enum class Policy { kFifo = 0, kShortest = 1, kLongest = 2 };
+
+// What a running job gives way to (set_preemption):
+//   kSize:    a job whose work times the ratio is below its own;
+//   kUrgency: a job whose urgency exceeds its own times the ratio, the
+//             urgency of a job being the latency over its isolated time it
+//             would reach if it ran alone from now on, (elapsed + remaining)
+//             / isolated, with remaining = isolated x (1 - progress). A job
+//             falling behind its isolated time gains priority, so the
+//             latency-over-isolated ratios even out across sizes. A paused
+//             job re-checks every millisecond, since urgencies move with the
+//             clock.
+enum class Preempt { kSize = 0, kUrgency = 1 };
 
 struct EngineStats {
     size_t queued = 0, running = 0, completed = 0;
@@ -67,6 +80,7 @@ struct BasicJob {
     uint64_t id = 0;
     size_t mem_bytes = 0;  // HBM the job needs on its worker while it runs (arena included)
     double work = 0;       // size estimate (scalars), orders the queue
+    double iso_ms = 0;     // estimated time alone on the device (urgency preemption)
     std::function<void(Ctx &, BasicJob &)> fn;
     int status = 1;  // 1 pending, 0 done, < 0 error code
     std::string error;
@@ -89,8 +103,10 @@ struct WorkerHooks {
     std::function<void(Ctx &)> collect;
     // on the worker's thread before its first job: hands the context the
     // worker's preemption calls (whether preemption is on, whether the job
-    // should pause now, and the pause itself)
-    std::function<void(Ctx &, std::function<bool()>, std::function<bool()>, std::function<void()>)> bind_yield;
+    // should pause now, the pause itself, and the job's progress report)
+    std::function<void(Ctx &, std::function<bool()>, std::function<bool()>, std::function<void()>,
+                       std::function<void(double)>)>
+        bind_yield;
 };
 
 inline double sched_now_ms() {
@@ -110,6 +126,9 @@ class Scheduler {
         reserve_ = std::max(0, std::min(reserve, nw - 1));
         running_mem_.assign(nw, 0);
         running_work_.assign(nw, 0.0);
+        progress_.assign(nw, 0.0);
+        t_submit_.assign(nw, 0.0);
+        iso_.assign(nw, 0.0);
         busy_.assign(nw, false);
         paused_.assign(nw, false);
         for (int w = 0; w < nw; w++) threads_.emplace_back([this, w] { run(w); });
@@ -188,10 +207,11 @@ class Scheduler {
     }
 
     // ratio 0 turns preemption off (paused jobs resume); otherwise >= 1
-    void set_preemption(double ratio, bool count_lane = false) {
+    void set_preemption(double ratio, bool count_lane = false, Preempt mode = Preempt::kSize) {
         std::lock_guard<std::mutex> lk(mu_);
         preempt_ratio_ = ratio > 0 ? std::max(1.0, ratio) : 0.0;
         count_lane_ = count_lane;
+        mode_ = mode;
         preempt_on_ = preempt_ratio_ > 0;
         pause_cv_.notify_all();
     }
@@ -204,14 +224,27 @@ class Scheduler {
         return yield_wanted_locked(w);
     }
 
-    // pauses the job on worker w until no smaller job runs beside it
+    // pauses the job on worker w until no job it gives way to runs beside it
     void yield_wait(int w) {
         std::unique_lock<std::mutex> lk(mu_);
         if (!yield_wanted_locked(w)) return;
         pauses_++;
         paused_[w] = true;
-        pause_cv_.wait(lk, [&] { return !yield_wanted_locked(w); });
+        if (mode_ == Preempt::kUrgency) {
+            // a timed wait on the system clock (pthread_cond_timedwait, which
+            // the thread sanitizer models; a steady-clock wait is not)
+            while (yield_wanted_locked(w))
+                pause_cv_.wait_until(lk, std::chrono::system_clock::now() + std::chrono::milliseconds(1));
+        } else {
+            pause_cv_.wait(lk, [&] { return !yield_wanted_locked(w); });
+        }
         paused_[w] = false;
+    }
+
+    // the fraction of its work the job on worker w has done
+    void report_progress(int w, double p) {
+        std::lock_guard<std::mutex> lk(mu_);
+        progress_[w] = std::min(1.0, std::max(0.0, p));
     }
 
     // pauses so far (for tests and statistics)
@@ -230,8 +263,19 @@ class Scheduler {
     int workers() const { return (int)ctxs_.size(); }
 
    private:
+    double urgency_locked(int v, double now) const {
+        const double iso = std::max(iso_[v], 0.05);
+        return (now - t_submit_[v] + iso * (1.0 - progress_[v])) / iso;
+    }
+
     bool yield_wanted_locked(int w) const {
         if (preempt_ratio_ <= 0 || stop_ || w < reserve_ || !busy_[w]) return false;
+        if (mode_ == Preempt::kUrgency) {
+            const double now = sched_now_ms(), uw = urgency_locked(w, now) * preempt_ratio_;
+            for (int v = count_lane_ ? 0 : reserve_; v < (int)ctxs_.size(); v++)
+                if (v != w && busy_[v] && !paused_[v] && urgency_locked(v, now) > uw) return true;
+            return false;
+        }
         for (int v = count_lane_ ? 0 : reserve_; v < (int)ctxs_.size(); v++)
             if (v != w && busy_[v] && !paused_[v] && running_work_[v] * preempt_ratio_ < running_work_[w])
                 return true;
@@ -282,7 +326,7 @@ class Scheduler {
             if (hooks_.bind_yield)
                 hooks_.bind_yield(
                     ctx, [this] { return preemption_on(); }, [this, w] { return yield_wanted(w); },
-                    [this, w] { yield_wait(w); });
+                    [this, w] { yield_wait(w); }, [this, w](double p) { report_progress(w, p); });
             if (hooks_.start) hooks_.start(w, ctx);
         } catch (const ApgError &e) {
             start_code = e.code;
@@ -311,6 +355,9 @@ class Scheduler {
                 busy_[w] = true;
                 running_mem_[w] = job->mem_bytes;
                 running_work_[w] = job->work;
+                progress_[w] = 0.0;
+                t_submit_[w] = job->t_submit;
+                iso_[w] = job->iso_ms;
                 running_++;
                 peak_ = std::max(peak_, in_use_locked());
             }
@@ -360,10 +407,11 @@ class Scheduler {
     double small_work_ = 0;
     int idle_fast_ = 0;  // fast-lane workers waiting for a job
     std::vector<size_t> running_mem_;
-    std::vector<double> running_work_;
+    std::vector<double> running_work_, progress_, t_submit_, iso_;
     std::vector<bool> busy_, paused_;
     double preempt_ratio_ = 0;
     bool count_lane_ = false;
+    Preempt mode_ = Preempt::kSize;
     std::atomic<bool> preempt_on_{false};
     uint64_t pauses_ = 0;
     std::vector<std::thread> threads_;

@@ -951,6 +951,7 @@ namespace {
 // Host-side record of one column group, kept until the MSM's single sync.
 struct GroupRun {
     size_t k0, K;
+    double p0 = 0, p1 = 1;     // this group's share of the MSM (progress reports)
     MsmShape sh;
     size_t w_off;              // first window sum of this group in the pinned array
     uint32_t *total_host;      // the group's pair count (pinned)
@@ -1059,8 +1060,9 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     }
 #undef APG_SORT_C
 #undef APG_SORT
+    const auto at = [&](double f) { return g.p0 + (g.p1 - g.p0) * f; };  // progress within the MSM
     if (fine) {
-        ctx.yield_point();
+        ctx.yield_point(at(0.1));
         const dim3 fg(rows + xgroups);
         hipLaunchKernelGGL(k_sort_fine_count, fg, dim3(SORT_TPB), size_t(4) << geo.fine_bits, st, keys_c, geo, cap,
                            rows, rowbase, rowtot, plan, extra, fcount);
@@ -1076,7 +1078,7 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     Xyzz29 *buckets = ctx.d<Xyzz29>("msm.buckets", nkeys);
     Xyzz29 *part = ctx.d<Xyzz29>("msm.part", 2 * slice_cap);
     const bool chunked = ctx.preemptible();
-    if (chunked) ctx.yield_point();
+    if (chunked) ctx.yield_point(at(0.18));
     // kernel timing brackets a single launch only: a chunked (preemptible)
     // accumulation may pause between its launches
     if (ctx.profiling && !chunked) {
@@ -1086,13 +1088,13 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     }
     const size_t chunk = chunked ? (resident + TPB - 1) / TPB * TPB : slice_cap;
     for (size_t t0 = 0; t0 < slice_cap; t0 += chunk) {
-        if (t0 > 0) ctx.yield_point();
+        if (t0 > 0) ctx.yield_point(at(0.18 + 0.7 * (double)t0 / (double)slice_cap));
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(std::min(chunk, slice_cap - t0), TPB)), dim3(TPB), 0, st,
                            skeys, entries, plan, tab.points, buckets, part, (uint32_t)t0);
         HIP_CHECK(hipGetLastError());
     }
     if (g.ev1) HIP_CHECK(hipEventRecord(g.ev1, st));
-    if (chunked) ctx.yield_point();
+    if (chunked) ctx.yield_point(at(0.88));
 
     // 4. buckets: empty ones set, crossing ones merged from their partials. A
     // heavy bucket spans more than LIGHT_ITEMS / 2 slices, each slice belongs
@@ -1116,7 +1118,7 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
                        buckets);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamWaitEvent(st, ctx.ev_join, 0));
-    ctx.yield_point();
+    ctx.yield_point(at(0.94));
 
     // 5. reduce: per problem (column, group): W = sum_b (b + 1) S_b, written
     // by k_weighted straight into pinned host memory
@@ -1213,7 +1215,9 @@ void msm_run(Context &ctx, const BaseTable &tab, size_t n, size_t K, const Fe *f
     for (size_t gi = 0; gi < groups.size(); gi++) {
         GroupRun &g = groups[gi];
         g.total_host = h_tot + gi;
-        if (gi > 0) ctx.yield_point();
+        g.p0 = (double)gi / (double)groups.size();
+        g.p1 = (double)(gi + 1) / (double)groups.size();
+        if (gi > 0) ctx.yield_point(g.p0);
         msm_enqueue_group(ctx, tab, n, g, fr_dev ? fr_dev + g.k0 * n : nullptr,
                           u32_dev ? u32_dev + g.k0 * n : nullptr, w_dev + g.w_off, tot_dev + gi);
     }

@@ -122,15 +122,19 @@ struct Context {
     // (jobqueue.h); empty for plain contexts, where yield_point does nothing.
     std::function<bool()> yield_on, yield_wanted;
     std::function<void()> yield_wait;
+    std::function<void(double)> yield_progress;
+    double prog_lo = 0, prog_hi = 1;  // the running phase's share of the job (ProgressSpan)
 
     bool preemptible() const { return yield_on && yield_on(); }
 
     // A phase boundary of a job: the host waits for the stream to drain (so a
-    // smaller job that starts now waits for at most the phase in flight),
-    // then pauses while the scheduler asks it to.
-    void yield_point() {
+    // job that starts now waits for at most the phase in flight), reports the
+    // job's progress (`done`: the fraction of the current phase done, or < 0
+    // for none), then pauses while the scheduler asks it to.
+    void yield_point(double done = -1) {
         if (!preemptible()) return;
         sync();
+        if (done >= 0 && yield_progress) yield_progress(prog_lo + (prog_hi - prog_lo) * done);
         if (yield_wanted()) yield_wait();
     }
 
@@ -162,6 +166,21 @@ struct Context {
         return static_cast<T *>(host[name].get(count * sizeof(T), retired_host, hipHostMallocCoherent));
     }
     void sync() { HIP_CHECK(hipStreamSynchronize(stream)); }
+};
+
+// A phase of a job as the fraction [a, b) of the enclosing phase's share,
+// for the progress its yield points report; restores the share on exit.
+struct ProgressSpan {
+    Context &c;
+    double lo, hi;
+    ProgressSpan(Context &ctx, double a, double b) : c(ctx), lo(ctx.prog_lo), hi(ctx.prog_hi) {
+        c.prog_lo = lo + (hi - lo) * a;
+        c.prog_hi = lo + (hi - lo) * b;
+    }
+    ~ProgressSpan() {
+        c.prog_lo = lo;
+        c.prog_hi = hi;
+    }
 };
 
 // Streams (default priority; non-blocking, or restricted to the CUs set in

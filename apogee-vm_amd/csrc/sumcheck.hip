@@ -27,6 +27,7 @@
 //    then (more than one block) through per-block partials summed by the last
 //    block to finish.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -882,7 +883,9 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
     W4 claim = smallc().zero;
     W4 P = smallc().one;  // prod_{l<j} eq(r_l, rho_l)
     for (int j = 0; j < n; j++) {
-        if (j > 0) ctx.yield_point();  // between rounds (the last one synchronised the stream)
+        // between rounds (the last one synchronised the stream); round j
+        // starts after 1 - 2^-j of the rounds' work
+        if (j > 0) ctx.yield_point(1.0 - std::ldexp(1.0, -j));
         if (j > 0 && (size_t)k * (len0 >> j) <= TAIL_ELEMS) {  // T^j is small: the rest on the host
             std::vector<Fe *> src(k);
             for (int t = 0; t < k; t++) src[t] = const_cast<Fe *>(cur[t]);

@@ -70,6 +70,7 @@ SERVE_WORKERS = 6
 SERVE_RESERVE = 2
 SERVE_LANE_CUS = 0
 SERVE_PREEMPT = 2.0
+SERVE_PREEMPT_MODE = "size"
 
 apg = None
 wl = None
@@ -718,6 +719,10 @@ def main():
                     help="fast-lane workers, kept for shards of at most --small-scalars rows x columns")
     ap.add_argument("--lane-cus", type=int, default=None, help="CUs of the fast lane (0: no CU split)")
     ap.add_argument("--small-scalars", type=float, default=float(1 << 20))
+    ap.add_argument("--serve-capacity", type=float, default=0.0,
+                    help="shards/s the serving loads are fractions of (0: measure the serving engine's)")
+    ap.add_argument("--preempt-mode", default=None, choices=["size", "urgency"],
+                    help="what a job gives way to (serving default %s)" % SERVE_PREEMPT_MODE)
     ap.add_argument("--preempt", type=float, default=None,
                     help="phase-boundary preemption ratio (0: off; serving default %g)" % SERVE_PREEMPT)
     ap.add_argument("--seed", type=int, default=20261203)
@@ -730,7 +735,8 @@ def main():
     serving = args.workload == "serve"
     for name, mixed_default, serve_default in (("workers", 12, SERVE_WORKERS), ("policy", "fifo", "shortest"),
                                                ("reserve", 0, SERVE_RESERVE), ("lane_cus", 0, SERVE_LANE_CUS),
-                                               ("preempt", 0.0, SERVE_PREEMPT)):
+                                               ("preempt", 0.0, SERVE_PREEMPT),
+                                               ("preempt_mode", "size", SERVE_PREEMPT_MODE)):
         if getattr(args, name) is None:
             setattr(args, name, serve_default if serving else mixed_default)
     apg = importlib.import_module("apogee-vm_amd")
@@ -781,7 +787,7 @@ def main():
     eng = apg.Engine(0, workers=args.workers, hbm_budget=200 << 30, lane_workers=args.reserve,
                      lane_cus=args.lane_cus, small_scalars=args.small_scalars)
     eng.set_policy(args.policy, args.reserve, args.small_scalars)
-    eng.set_preemption(args.preempt)
+    eng.set_preemption(args.preempt, args.preempt_mode)
     eng.set_profiling(True)
 
     if args.workload == "serve":
@@ -839,9 +845,10 @@ def main():
     seng = apg.Engine(0, workers=SERVE_WORKERS, hbm_budget=200 << 30, lane_workers=SERVE_RESERVE,
                       lane_cus=SERVE_LANE_CUS, small_scalars=args.small_scalars)
     seng.set_policy("shortest", SERVE_RESERVE, args.small_scalars)
-    seng.set_preemption(SERVE_PREEMPT)
+    seng.set_preemption(SERVE_PREEMPT, SERVE_PREEMPT_MODE)
     sargs = argparse.Namespace(**{**vars(args), "workers": SERVE_WORKERS, "policy": "shortest",
                                   "reserve": SERVE_RESERVE, "lane_cus": SERVE_LANE_CUS, "preempt": SERVE_PREEMPT,
+                                  "preempt_mode": SERVE_PREEMPT_MODE,
                                   "warmup": 1})
     served = serve(seng, srs, specs, bufs, plain, iso, sargs)
     seng.close()
@@ -901,6 +908,9 @@ def serve(eng, srs, specs, bufs, plain, iso, args):
         run_engine_pass(eng, srs, specs, bufs)
     dt, _ = run_engine_pass(eng, srs, specs, bufs)  # capacity probe
     capacity = len(specs) / dt
+    measured = capacity
+    if getattr(args, "serve_capacity", 0.0) > 0:
+        capacity = args.serve_capacity
     log(f"batched capacity {capacity:.1f} shards/s")
     iso_ms = iso * 1e3
     rows = []
@@ -957,8 +967,10 @@ def serve(eng, srs, specs, bufs, plain, iso, args):
             "pauses": eng.pauses(),
             "config": {"workload": "served stream of the mixed batch's shards", "parallelism": "1 GPU",
                        "engine_workers": args.workers, "engine_policy": args.policy, "preempt": args.preempt,
+                       "preempt_mode": args.preempt_mode,
                        "reserve": args.reserve, "lane_cus": args.lane_cus, "small_scalars": args.small_scalars, "hbm_budget_gib": 200},
-            "capacity_shards_per_s": round(capacity, 2), "serve": rows, "bit_exact_vs_plain": bool(exact),
+            "capacity_shards_per_s": round(capacity, 2), "measured_capacity_shards_per_s": round(measured, 2),
+            "serve": rows, "bit_exact_vs_plain": bool(exact),
             "hbm": {"srs_gib": round(srs.device_bytes() / 2**30, 3),
                     "engine_arena_gib": round(info["arena_bytes"] / 2**30, 3)}}
 

@@ -457,16 +457,23 @@ int apg_engine_create_lanes(int device, int workers, size_t hbm_budget_bytes, in
 /* Preemption at phase boundaries. With ratio >= 1, a job running on a worker
  * outside the fast lane pauses at its next phase boundary (between the MSM's
  * sort passes, accumulation chunks and bucket phases, after the column
- * combination, between sumcheck rounds) while another worker runs a job whose
- * rows x columns times `ratio` is below its own (fast-lane jobs count when
- * the lane has no CUs of its own), and resumes when none does; the job's
- * host thread drains its stream at each boundary, so the smaller job has the
- * CUs to itself once the phase in flight ends. ratio 0 (the default) turns
- * it off. Results do not change: a job's bytes depend only on its inputs.
- * *pauses (nullable) receives the pauses so far. No counterpart in the
- * reference, whose shard workers run to completion
+ * combination, between sumcheck rounds) while another worker (fast-lane
+ * workers count when the lane has no CUs of its own) runs a job it gives way
+ * to, and resumes when none does:
+ *   APG_PREEMPT_SIZE:    a job whose rows x columns times `ratio` is below its own;
+ *   APG_PREEMPT_URGENCY: a job whose urgency exceeds its own times `ratio`,
+ *     urgency = (time since submission + estimated remaining time) /
+ *     estimated time alone, so a job falling behind its isolated time gains
+ *     priority (the estimates: a linear fit of shard times on MI355X, and the
+ *     fraction of its phases a job has done).
+ * The job's host thread drains its stream at each boundary, so the job given
+ * way to has the CUs to itself once the phase in flight ends. ratio 0 (the
+ * default) turns it off. Results do not change: a job's bytes depend only on
+ * its inputs. *pauses (nullable) receives the pauses so far. No counterpart
+ * in the reference, whose shard workers run to completion
  * (crates/prover/src/streaming.rs:601-680). */
-int apg_engine_set_preemption(apg_engine *e, double ratio, uint64_t *pauses);
+enum { APG_PREEMPT_SIZE = 0, APG_PREEMPT_URGENCY = 1 };
+int apg_engine_set_preemption(apg_engine *e, double ratio, int mode, uint64_t *pauses);
 /* kernel timing (apg_stats) on every worker's stream, summed as jobs finish */
 int apg_engine_set_profiling(apg_engine *e, int on);
 int apg_engine_kernel_stats(apg_engine *e, apg_stats *out);

@@ -34,17 +34,21 @@ struct MockCtx {
     int index = 0;
     std::function<bool()> on, wanted;
     std::function<void()> wait;
-    void yield_point() {
-        if (on && on() && wanted()) wait();
+    std::function<void(double)> progress;
+    void yield_point(double p = -1) {
+        if (!(on && on())) return;
+        if (p >= 0) progress(p);
+        if (wanted()) wait();
     }
 };
 
 void bind(apg::WorkerHooks<MockCtx> &hooks) {
     hooks.bind_yield = [](MockCtx &c, std::function<bool()> on, std::function<bool()> wanted,
-                          std::function<void()> wait) {
+                          std::function<void()> wait, std::function<void(double)> progress) {
         c.on = std::move(on);
         c.wanted = std::move(wanted);
         c.wait = std::move(wait);
+        c.progress = std::move(progress);
     };
 }
 
@@ -82,7 +86,7 @@ std::unique_ptr<Job> make_job(uint64_t input, size_t mem, double work, int reser
         const int spins = (int)(mix(input) % 2000);
         for (int i = 0; i < spins; i++) {
             v = mix(v);
-            if (i % 500 == 499) c.yield_point();
+            if (i % 500 == 499) c.yield_point((double)i / spins);
         }
         std::this_thread::sleep_for(std::chrono::microseconds(mix(input) % 300));
         c.yield_point();
@@ -102,7 +106,7 @@ std::vector<uint8_t> expected(uint64_t input) {
 }
 
 void run_mix(apg::Policy policy, int workers, int reserve, size_t budget, int submitters, int per_thread,
-             double preempt = 0) {
+             double preempt = 0, apg::Preempt mode = apg::Preempt::kSize) {
     std::vector<std::unique_ptr<MockCtx>> ctxs;
     std::vector<MockCtx *> raw;
     for (int w = 0; w < workers; w++) {
@@ -121,7 +125,7 @@ void run_mix(apg::Policy policy, int workers, int reserve, size_t budget, int su
     {
         Sched s(raw, budget, reserve, hooks);
         s.set_policy(policy, reserve, small);
-        s.set_preemption(preempt);
+        s.set_preemption(preempt, false, mode);
         std::thread monitor([&] {
             while (!stop_monitor) {
                 const apg::EngineStats st = s.stats();
@@ -178,8 +182,8 @@ void run_mix(apg::Policy policy, int workers, int reserve, size_t budget, int su
     CHECK(!over_budget, "charged HBM within the budget whenever two or more jobs run");
     CHECK(lane_violations == 0, "fast-lane workers ran short jobs only");
     CHECK(collected == submitters * per_thread, "collect hook ran once per job");
-    std::printf("ok mix policy=%d workers=%d reserve=%d budget=%zu preempt=%g\n", (int)policy, workers, reserve,
-                budget, preempt);
+    std::printf("ok mix policy=%d workers=%d reserve=%d budget=%zu preempt=%g mode=%d\n", (int)policy, workers,
+                reserve, budget, preempt, (int)mode);
 }
 
 void unknown_ids() {
@@ -333,6 +337,80 @@ void preemption_order() {
                 at_end.load());
 }
 
+// Urgency: a long job (isolated estimate 100 ms) that has run a while and a
+// short job (1 ms) submitted now: the short one is more urgent at once (its
+// remaining time over its isolated time), so the long job pauses; once the
+// short one ends, the long one resumes. And a job that has waited long
+// enough becomes the more urgent one: two jobs of equal estimates swap as
+// their urgencies cross, and both finish.
+void urgency_order() {
+    std::vector<std::unique_ptr<MockCtx>> ctxs;
+    std::vector<MockCtx *> raw;
+    for (int w = 0; w < 3; w++) {
+        ctxs.push_back(std::make_unique<MockCtx>());
+        ctxs.back()->index = w;
+        raw.push_back(ctxs.back().get());
+    }
+    apg::WorkerHooks<MockCtx> hooks;
+    hooks.held = [](const MockCtx &x) { return x.held.load(); };
+    bind(hooks);
+    std::atomic<int> steps{0}, at_start{-1}, at_end{-1};
+    std::atomic<bool> started{false};
+    {
+        Sched s(raw, 1000, 0, hooks);
+        s.set_policy(apg::Policy::kShortest, 0, 0);
+        s.set_preemption(1.0, false, apg::Preempt::kUrgency);
+        auto lj = std::make_unique<Job>();
+        lj->mem_bytes = 10;
+        lj->work = 1000;
+        lj->iso_ms = 100;
+        lj->fn = [&](MockCtx &c, Job &) {
+            started = true;
+            for (int i = 0; i < 300; i++) {
+                std::this_thread::sleep_for(std::chrono::microseconds(100));
+                steps++;
+                c.yield_point(i / 300.0);
+            }
+        };
+        const uint64_t lid = s.submit(std::move(lj));
+        while (!started) std::this_thread::yield();
+        std::this_thread::sleep_for(std::chrono::milliseconds(2));
+        auto sj = std::make_unique<Job>();
+        sj->mem_bytes = 10;
+        sj->work = 10;
+        sj->iso_ms = 1;
+        sj->fn = [&](MockCtx &, Job &) {
+            at_start = steps.load();
+            std::this_thread::sleep_for(std::chrono::milliseconds(5));
+            at_end = steps.load();
+        };
+        const uint64_t sid = s.submit(std::move(sj));
+        CHECK(s.wait(sid)->status == 0 && s.wait(lid)->status == 0, "both jobs completed (urgency)");
+        CHECK(steps == 300, "the long job finished after resuming (urgency)");
+        CHECK(at_start >= 0 && at_end - at_start <= 1, "the long job paused for the more urgent short one");
+        // two equal jobs: whichever waits becomes the more urgent; both end
+        std::atomic<int> a_steps{0}, b_steps{0};
+        auto mk = [&](std::atomic<int> &ctr) {
+            auto j = std::make_unique<Job>();
+            j->mem_bytes = 10;
+            j->work = 500;
+            j->iso_ms = 10;
+            j->fn = [&ctr](MockCtx &c, Job &) {
+                for (int i = 0; i < 100; i++) {
+                    std::this_thread::sleep_for(std::chrono::microseconds(100));
+                    ctr++;
+                    c.yield_point(i / 100.0);
+                }
+            };
+            return j;
+        };
+        const uint64_t ia = s.submit(mk(a_steps)), ib = s.submit(mk(b_steps));
+        CHECK(s.wait(ia)->status == 0 && s.wait(ib)->status == 0, "equal jobs completed (urgency)");
+        CHECK(a_steps == 100 && b_steps == 100, "equal jobs ran every step");
+    }
+    std::printf("ok urgency order (long job steps %d..%d while the short one ran)\n", at_start.load(), at_end.load());
+}
+
 }  // namespace
 
 namespace apg {
@@ -354,6 +432,9 @@ int main() {
     run_mix(apg::Policy::kShortest, 5, 1, 1000, 4, 30, 1.0);
     run_mix(apg::Policy::kShortest, 6, 0, 150, 4, 30, 2.0);
     run_mix(apg::Policy::kFifo, 4, 1, 130, 3, 40, 4.0);
+    run_mix(apg::Policy::kShortest, 6, 2, 1000, 4, 30, 1.0, apg::Preempt::kUrgency);
+    run_mix(apg::Policy::kShortest, 5, 0, 150, 4, 30, 1.5, apg::Preempt::kUrgency);
+    urgency_order();
     if (failures) {
         std::printf("%d checks failed\n", failures);
         return 1;

@@ -152,12 +152,13 @@ def test_engine_tables_outgrown_while_in_use(apg, orc):
     srs.release()
 
 
-def test_engine_preemption_matches_plain_path(apg):
+@pytest.mark.parametrize("mode", ["size", "urgency"])
+def test_engine_preemption_matches_plain_path(apg, mode):
     """Phase-boundary preemption: a long shard (2^18 rows x 8) whose
     accumulation runs in chunks with yield points between them and between
     its sumcheck rounds, and short shards submitted while it runs, which make
-    it pause. Every shard's bytes equal the plain path's, the long one's
-    included, and the long job paused at least once."""
+    it pause (by size: at least once; by urgency: as the urgencies fall).
+    Every shard's bytes equal the plain path's, the long one's included."""
     import time
 
     ctx = apg.default_context()
@@ -176,7 +177,7 @@ def test_engine_preemption_matches_plain_path(apg):
         plain.append((apg.prove_shard(srs, b, nv, k, 3, t), t.state()))
     eng = apg.Engine(0, workers=4, hbm_budget=16 << 30)
     eng.set_policy("shortest", 0, 0.0)
-    eng.set_preemption(1.0)
+    eng.set_preemption(1.0, mode)
     ids = [eng.submit_shard(srs, bufs[0], specs[0][0], specs[0][1], 3)]
     t0 = time.perf_counter()
     while eng.stats()["running"] == 0 and time.perf_counter() - t0 < 5:
@@ -187,7 +188,8 @@ def test_engine_preemption_matches_plain_path(apg):
         assert q.commitments == p.commitments
         assert np.array_equal(q.rounds, p.rounds) and np.array_equal(q.final_evals, p.final_evals)
         assert t.state() == st
-    assert eng.pauses() >= 1
+    if mode == "size":
+        assert eng.pauses() >= 1
     # preemption off again: nothing more pauses
     before = eng.set_preemption(0.0)
     ids = [eng.submit_shard(srs, b, nv, k, 3) for (nv, k, _), b in zip(specs, bufs)]
