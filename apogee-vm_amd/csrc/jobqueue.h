@@ -230,14 +230,12 @@ class Scheduler {
         if (!yield_wanted_locked(w)) return;
         pauses_++;
         paused_[w] = true;
-        if (mode_ == Preempt::kUrgency) {
-            // a timed wait on the system clock (pthread_cond_timedwait, which
-            // the thread sanitizer models; a steady-clock wait is not)
-            while (yield_wanted_locked(w))
-                pause_cv_.wait_until(lk, std::chrono::system_clock::now() + std::chrono::milliseconds(1));
-        } else {
-            pause_cv_.wait(lk, [&] { return !yield_wanted_locked(w); });
-        }
+        // woken by every job's end and every policy change, and re-checked
+        // each millisecond, since urgencies move with the clock (a timed wait
+        // on the system clock: pthread_cond_timedwait, which the thread
+        // sanitizer models; a steady-clock wait it does not)
+        while (yield_wanted_locked(w))
+            pause_cv_.wait_until(lk, std::chrono::system_clock::now() + std::chrono::milliseconds(1));
         paused_[w] = false;
     }
 
