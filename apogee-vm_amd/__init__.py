@@ -192,6 +192,7 @@ def _declare(L):
     d("apg_engine_set_policy", [VP, I32, I32, C.c_double])
     d("apg_engine_set_preemption", [VP, C.c_double, I32, C.POINTER(C.c_uint64)])
     d("apg_poseidon2_permute_dev", [VP, VP, SZ])
+    d("apg_sumcheck_prove_graph_dev", [VP, C.POINTER(VP), SZ, SZ, VP, VP, VP, VP, VP])
     d("apg_transcript_run_dev", [VP, C.POINTER(VP), SZ, VP, SZ, VP, VP, SZ, VP, SZ])
     d("apg_engine_set_profiling", [VP, I32])
     d("apg_engine_kernel_stats", [VP, C.POINTER(ApgStats)])
@@ -1106,6 +1107,23 @@ def prove_sumcheck_device(eq_point, gate: Gate, tables, transcript: Transcript, 
     _check(lib().apg_sumcheck_prove_dev_lifted(ctx.h, gate.c_terms(), len(gate.terms), ptrs, _p(kinds), len(tables),
                                                n, _p(eqp), cb, transcript.h, _p(rounds), _p(point), _p(finals)))
     return rounds[:n], point[:n], finals
+
+
+def prove_sumcheck_graph(eq_point, tables, transcript: Transcript, ctx: Optional[Context] = None):
+    """The product sumcheck eq * t_0 * ... * t_(k-1) (k <= 3) over DeviceBuffers
+    with every round and transcript step on the device, one captured graph per
+    proof (apg_sumcheck_prove_graph_dev). Returns (rounds, point, final_evals);
+    the transcript advances as the host-driven path's would."""
+    ctx = ctx or default_context()
+    n, k = len(eq_point), len(tables)
+    ptrs = (C.c_void_p * k)(*[b.ptr.value for b in tables])
+    eqp = _fe_array(eq_point)
+    rounds = np.zeros((n, max(4, k + 2), 4), dtype=np.uint64)
+    point = np.zeros((n, 4), dtype=np.uint64)
+    finals = np.zeros((k, 4), dtype=np.uint64)
+    _check(lib().apg_sumcheck_prove_graph_dev(ctx.h, ptrs, k, n, _p(eqp), transcript.h, _p(rounds), _p(point),
+                                              _p(finals)))
+    return rounds, point, finals
 
 
 def witness_digest(columns) -> np.ndarray:

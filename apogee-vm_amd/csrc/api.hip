@@ -846,6 +846,39 @@ int apg_sumcheck_prove_dev(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
     });
 }
 
+int apg_sumcheck_prove_graph_dev(apg_ctx *ctx, void *const *tables_dev, size_t k, size_t num_vars,
+                                 const uint8_t *eq_point_mont, apg_transcript *t, uint8_t *rounds_out,
+                                 uint8_t *point_out, uint8_t *final_evals_out) {
+    return guarded([&] {
+        need(ctx);
+        need(t);
+        need(tables_dev);
+        if (k < 1 || k > 3) throw ApgError(APG_ERR_ARG, "graph sumcheck: k must be 1, 2 or 3");
+        if (num_vars < 1 || num_vars > 30) throw ApgError(APG_ERR_ARG, "graph sumcheck: num_vars out of range");
+        need(eq_point_mont);
+        need(rounds_out);
+        need(point_out);
+        need(final_evals_out);
+        const size_t nc = std::max<size_t>(4, k + 2);  // coefficients per round
+        std::vector<Fe> eqp(num_vars), rounds(num_vars * nc), point(num_vars), finals(k);
+        for (size_t i = 0; i < num_vars; i++) eqp[i] = load_fe(eq_point_mont + 32 * i);
+        std::vector<Fe *> tabs(k);
+        for (size_t i = 0; i < k; i++) {
+            need(tables_dev[i]);
+            tabs[i] = static_cast<Fe *>(tables_dev[i]);
+        }
+        std::lock_guard<std::mutex> lk(ctx->c.mu);
+        HIP_CHECK(hipSetDevice(ctx->c.device));
+        const double t0 = now_ms();
+        sumcheck_prove_graph(ctx->c, tabs.data(), (int)k, (int)num_vars, eqp.data(), t->t, rounds.data(),
+                             point.data(), finals.data());
+        ctx->c.stats.sumcheck_ms += now_ms() - t0;
+        for (size_t i = 0; i < rounds.size(); i++) store_fe(rounds_out + 32 * i, rounds[i]);
+        for (size_t i = 0; i < num_vars; i++) store_fe(point_out + 32 * i, point[i]);
+        for (size_t i = 0; i < k; i++) store_fe(final_evals_out + 32 * i, finals[i]);
+    });
+}
+
 int apg_sumcheck_prove_dev_lifted(apg_ctx *ctx, const apg_term *terms, size_t n_terms, void *const *tables_dev,
                                   const int *backings, size_t k, size_t num_vars, const uint8_t *eq_point_mont,
                                   apg_round_fn round_fn, void *user, uint8_t *rounds_out, uint8_t *point_out,

@@ -6,6 +6,7 @@
 
 #include "field.cuh"
 #include "runtime.h"
+#include "transcript.h"
 
 namespace apg {
 
@@ -82,6 +83,14 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables_dev
 // device (once, with the device current; the library does it when a context
 // is created). Sizes whose scratch (sumcheck_noalloc_bytes) exceeds the
 // limit are refused. Calls on one device are serialised by a lock.
+// The product summand eq * t_0 * ... * t_(k-1) (1 <= k <= 3) over device
+// tables, every round and its transcript step on the device, captured as one
+// graph and launched once: no host step between rounds. The transcript is
+// the library's (it moves to the device and back); the bytes equal
+// sumcheck_prove_device's with the same transcript. Tables are read only.
+void sumcheck_prove_graph(Context &ctx, Fe *const *tables_dev, int k, int num_vars, const Fe *eq_point_host,
+                          Transcript &t, Fe *rounds_out, Fe *point_out, Fe *finals_out);
+
 void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *tables_dev, int k,
                                    int num_vars, const Fe *eq_point_host, const RoundFn &round_fn,
                                    Fe *rounds_out, Fe *point_out, Fe *finals_out, const uint8_t *kinds = nullptr);

@@ -36,6 +36,8 @@
 #include "field29.cuh"
 #include "host_fr.h"
 #include "sumcheck_dev.cuh"
+#include "transcript_dev.cuh"
+#include "transcript_dev.h"
 
 namespace apg {
 
@@ -103,10 +105,13 @@ __device__ __forceinline__ void pair_of(const TablePtrs &tp, int t, size_t i, bo
 // 32-bit CIOS): the fold takes the challenge in the R' = 2^261 domain and so
 // stays exact, and each node sum carries 2^(-5 KT), which the host removes.
 template <int KT, bool FULL, bool LIFT>
-__global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, int fold, F29 r29,
+// rho_dev (graph-captured rounds): the challenge in device memory, written by
+// the previous round's transcript step, replacing r29.
+__global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, int fold, F29 r29_arg,
                                                       EqArgs eq, Fe *partials, unsigned *counter,
-                                                      Fe *out, unsigned *done) {
+                                                      Fe *out, unsigned *done, const Fe *rho_dev) {
     constexpr int NN = FULL ? KT + 1 : KT;
+    const F29 r29 = rho_dev ? f29_from_mont32<Fr>(*rho_dev) : r29_arg;
     Fe acc[NN];
 #pragma unroll
     for (int x = 0; x < NN; x++) acc[x] = f_zero<Fr>();
@@ -427,10 +432,10 @@ static void launch_product(Context &ctx, bool full, const TablePtrs &tp, size_t 
     const F29 r29 = f29_from_mont32<Fr>(r);  // rho R' mod r
     if (full)
         hipLaunchKernelGGL((k_round_product<KT, true, LIFT>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
-                           partials, counter, out, done);
+                           partials, counter, out, done, (const Fe *)nullptr);
     else
         hipLaunchKernelGGL((k_round_product<KT, false, LIFT>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
-                           partials, counter, out, done);
+                           partials, counter, out, done, (const Fe *)nullptr);
 }
 
 template <int KT>
@@ -968,6 +973,246 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
             for (int t = 0; t < k; t++) cur[t] = nxt[t];
     }
     throw ApgError(-5, "sumcheck: the host tail was not reached");  // k * 2 <= TAIL_ELEMS always holds
+}
+
+
+// ---- the whole sumcheck as one captured graph, with the device transcript ----
+//
+// For callers that want no host in the loop (one graph launch per proof):
+// the product summand eq * t_0 * ... * t_(k-1), every round evaluating all
+// nodes (no node from the claim), and after each round kernel a one-thread
+// kernel that turns the node sums into the round polynomial, runs the
+// round's transcript step on the device (transcript_dev.cuh) and leaves the
+// challenge in device memory, where the next round kernel folds with it. The
+// bytes equal the host-driven path's; the price is the device transcript's
+// latency (a permutation on one thread is ~70x the host's).
+
+namespace {
+
+struct GraphConsts {
+    Fe num[SC_MAX_NODES + 1], inv[SC_MAX_NODES + 1], inv6;
+};
+
+struct GraphState {
+    DevTranscript tr;
+    Fe P;    // prod_{l<j} eq(r_l, rho_l)
+    Fe rho;  // the last challenge
+    GraphConsts K;
+};
+
+// the device forms of interpolate_w4 and extrapolate_w4
+__device__ void dev_interpolate(const Fe *v, int d, Fe *c, const GraphConsts &K) {
+    if (d == 3) {  // crates/sumcheck/src/lib.rs:294-305
+        const Fe d1 = f_sub<Fr>(v[1], v[0]);
+        const Fe t = f_sub<Fr>(v[2], v[1]);
+        const Fe d2 = f_sub<Fr>(t, d1);
+        const Fe d3 = f_sub<Fr>(f_sub<Fr>(f_sub<Fr>(v[3], v[2]), t), d2);
+        c[0] = v[0];
+        c[1] = f_add<Fr>(f_sub<Fr>(d1, f_mul<Fr>(d2, K.inv[2])), f_mul<Fr>(d3, K.inv[3]));
+        c[2] = f_mul<Fr>(f_sub<Fr>(d2, d3), K.inv[2]);
+        c[3] = f_mul<Fr>(d3, K.inv6);
+        return;
+    }
+    Fe diff[SC_MAX_NODES], delta[SC_MAX_NODES];
+    for (int i = 0; i <= d; i++) diff[i] = v[i];
+    for (int k = 0; k <= d; k++) {
+        delta[k] = diff[0];
+        for (int i = 0; i + k < d; i++) diff[i] = f_sub<Fr>(diff[i + 1], diff[i]);
+    }
+    Fe basis[SC_MAX_NODES], acc[SC_MAX_NODES];
+    for (int i = 0; i <= d; i++) basis[i] = acc[i] = f_zero<Fr>();
+    basis[0] = K.num[1];
+    acc[0] = delta[0];
+    for (int k = 1; k <= d; k++) {
+        Fe nb[SC_MAX_NODES];
+        for (int i = 0; i <= d; i++) nb[i] = f_zero<Fr>();
+        for (int i = 0; i < k; i++) {
+            nb[i + 1] = f_add<Fr>(nb[i + 1], basis[i]);
+            nb[i] = f_sub<Fr>(nb[i], f_mul<Fr>(basis[i], K.num[k - 1]));
+        }
+        for (int i = 0; i <= k; i++) basis[i] = f_mul<Fr>(nb[i], K.inv[k]);
+        for (int i = 0; i <= k; i++) acc[i] = f_add<Fr>(acc[i], f_mul<Fr>(basis[i], delta[k]));
+    }
+    for (int i = 0; i <= d; i++) c[i] = acc[i];
+}
+
+__device__ void dev_extrapolate(Fe *v, int D, int d) {
+    Fe last[SC_MAX_NODES], row[SC_MAX_NODES];
+    for (int i = 0; i <= D; i++) row[i] = v[i];
+    for (int k = 0; k <= D; k++) {
+        last[k] = row[D - k];
+        for (int i = 0; i + k < D; i++) row[i] = f_sub<Fr>(row[i + 1], row[i]);
+    }
+    for (int x = D + 1; x <= d; x++) {
+        for (int k = D - 1; k >= 0; k--) last[k] = f_add<Fr>(last[k], last[k + 1]);
+        v[x] = last[0];
+    }
+}
+
+// One thread: round j's polynomial from the node sums S(0..D) (times
+// `scale`, which undoes the product kernel's 2^-5 factors), its transcript
+// step, the challenge and the running eq product.
+__global__ void k_graph_round(GraphState *st, const Fe *__restrict__ nodes, int D, int d, Fe scale,
+                              const Fe *__restrict__ r, int j, Fe *__restrict__ rounds, Fe *__restrict__ point,
+                              const Fe *__restrict__ rc) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const Fe Ps = f_mul<Fr>(st->P, scale);
+    Fe T[SC_MAX_NODES], c[SC_MAX_NODES];
+    for (int x = 0; x <= D; x++) T[x] = f_mul<Fr>(Ps, nodes[x]);
+    dev_extrapolate(T, D, d);
+    const Fe one = st->K.num[1];
+    const Fe base = f_sub<Fr>(one, r[j]), slope = f_sub<Fr>(f_add<Fr>(r[j], r[j]), one);
+    Fe e = base;
+    for (int x = 0; x <= d; x++) {
+        T[x] = f_mul<Fr>(e, T[x]);
+        e = f_add<Fr>(e, slope);
+    }
+    dev_interpolate(T, d, c, st->K);
+    for (int x = 0; x <= d; x++) rounds[(size_t)j * (d + 1) + x] = c[x];
+    DevTranscript t = st->tr;
+    dt_append_scalars(t, TAG_SUMCHECK_ROUND, c, d + 1, rc);
+    const Fe rho = dt_challenge(t, TAG_SUMCHECK_CHALLENGE, rc);
+    st->tr = t;
+    st->rho = rho;
+    point[j] = rho;
+    st->P = f_mul<Fr>(st->P, f_add<Fr>(base, f_mul<Fr>(rho, slope)));
+}
+
+// The final evaluations: each table's last pair folded with the last challenge.
+__global__ void k_graph_finals(const GraphState *st, TablePtrs tp, int k, Fe *__restrict__ finals) {
+    const int t = threadIdx.x;
+    if (blockIdx.x != 0 || t >= k) return;
+    const Fe lo = ldfe(tp.src[t], 0), hi = ldfe(tp.src[t], 1);
+    finals[t] = f_add<Fr>(lo, f_mul<Fr>(st->rho, f_sub<Fr>(hi, lo)));
+}
+
+template <int KT>
+void launch_graph_product(Context &ctx, bool full, const TablePtrs &tp, size_t m, bool fold, const EqArgs &eq,
+                          Fe *partials, unsigned *counter, Fe *out, const Fe *rho_dev) {
+    const dim3 g(round_blocks(ctx, m, 1)), b(RTPB);
+    const F29 none{};
+    if (full)
+        hipLaunchKernelGGL((k_round_product<KT, true, false>), g, b, 0, ctx.stream, tp, m, (int)fold, none, eq,
+                           partials, counter, out, (unsigned *)nullptr, rho_dev);
+    else
+        hipLaunchKernelGGL((k_round_product<KT, false, false>), g, b, 0, ctx.stream, tp, m, (int)fold, none, eq,
+                           partials, counter, out, (unsigned *)nullptr, rho_dev);
+}
+
+}  // namespace
+
+void sumcheck_prove_graph(Context &ctx, Fe *const *tables, int k, int n, const Fe *eq_point, Transcript &tr,
+                          Fe *rounds_out, Fe *point_out, Fe *finals_out) {
+    if (k < 1 || k > 3) throw ApgError(-2, "graph sumcheck: the product of 1 to 3 tables");
+    if (n < 1 || n > 30) throw ApgError(-2, "graph sumcheck: num_vars out of range");
+    // S has degree k in the round variable and eq adds one; rounds are at
+    // least cubic (four coefficients, the reference's [Fr; 4]), as Gate::degree
+    const int D = k, d = std::max(3, k + 1);
+    const size_t len0 = size_t(1) << n;
+    const ScratchLayout l = sumcheck_layout(k, n);
+    SumcheckScratch sc = sumcheck_carve(ctx.d<uint8_t>("sg.scratch", l.total), l);
+    sc.counter = ctx.zeroed<unsigned>("sg.counter", 64);
+    GraphState *st = ctx.d<GraphState>("sg.state", 1);
+    Fe *rounds = ctx.d<Fe>("sg.rounds", (size_t)n * (d + 1));
+    Fe *point = ctx.d<Fe>("sg.point", n);
+    Fe *finals = ctx.d<Fe>("sg.finals", k);
+    const Fe *rc = poseidon2_rc_device(ctx);
+
+    // inputs of the graph: the eq point and the starting state
+    GraphState h{};
+    std::memcpy(&h.tr, &tr, sizeof(DevTranscript));
+    h.P = f_one<Fr>();
+    h.rho = f_zero<Fr>();
+    for (int x = 0; x <= SC_MAX_NODES; x++) {
+        h.K.num[x] = f_from_u64<Fr>((uint64_t)x);
+        h.K.inv[x] = x ? f_inv<Fr>(h.K.num[x]) : f_zero<Fr>();
+    }
+    h.K.inv6 = f_inv<Fr>(f_from_u64<Fr>(6));
+    hipStream_t s = ctx.stream;
+    HIP_CHECK(hipMemcpyAsync(sc.r, eq_point, n * sizeof(Fe), hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    ctx.sync();  // h lives on this frame; nothing else is in flight when the capture starts
+
+    const int h0 = std::max(1, std::min(n, (n + 1) / 2));
+    const size_t astride = size_t(1) << (h0 - 1), bstride = size_t(1) << (n - h0);
+    const size_t asz = (size_t)(h0 - 1) * astride, bsz = (size_t)(n - h0 + 1) * bstride;
+    std::vector<Fe *> bufA(k), bufB(k);
+    for (int t = 0; t < k; t++) {
+        bufA[t] = sc.poolA + (size_t)t * std::max<size_t>(len0 / 2, 1);
+        bufB[t] = sc.poolB + (size_t)t * std::max<size_t>(len0 / 4, 1);
+    }
+    std::vector<const Fe *> cur(k);
+    for (int t = 0; t < k; t++) cur[t] = tables[t];
+    // the product kernel's node sums carry 2^(-5 k), and 2^-5 more with two eq factor tables
+    const Fe scale = f_from_u64<Fr>(uint64_t(1) << (5 * k)), scale_lo = f_from_u64<Fr>(uint64_t(1) << (5 * k + 5));
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    struct GraphGuard {
+        hipGraph_t &g;
+        hipGraphExec_t &e;
+        ~GraphGuard() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } guard{graph, exec};
+    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        hipLaunchKernelGGL(k_eq_suffix, dim3(grid_for(asz + bsz, 256)), dim3(256), 0, s, sc.r, n, h0, sc.eqA, sc.eqB);
+        for (int j = 0; j < n; j++) {
+            const bool fold = j > 0;
+            const size_t m = len0 >> (j + 1);
+            TablePtrs tp;
+            for (int t = 0; t < k; t++) {
+                Fe *nxt = (j % 2 == 1) ? bufA[t] : bufB[t];
+                tp.src[t] = cur[t];
+                tp.kind[t] = (uint8_t)SC_FR;
+                tp.dst[t] = nxt;
+            }
+            const int sidx = j + 1;
+            EqArgs eq;
+            if (sidx < h0) {
+                eq.lo = sc.eqA + (size_t)(sidx - 1) * astride;
+                eq.lo_bits = (uint32_t)(h0 - sidx);
+                eq.hi = sc.eqB;
+            } else {
+                eq.lo = nullptr;
+                eq.lo_bits = 0;
+                eq.hi = sc.eqB + (size_t)(sidx - h0) * bstride;
+            }
+            switch (k) {
+                case 1: launch_graph_product<1>(ctx, true, tp, m, fold, eq, sc.partials, sc.counter, sc.nodes, &st->rho); break;
+                case 2: launch_graph_product<2>(ctx, true, tp, m, fold, eq, sc.partials, sc.counter, sc.nodes, &st->rho); break;
+                default: launch_graph_product<3>(ctx, true, tp, m, fold, eq, sc.partials, sc.counter, sc.nodes, &st->rho); break;
+            }
+            hipLaunchKernelGGL(k_graph_round, dim3(1), dim3(64), 0, s, st, sc.nodes, D, d, eq.lo ? scale_lo : scale,
+                               sc.r, j, rounds, point, rc);
+            if (fold)
+                for (int t = 0; t < k; t++) cur[t] = tp.dst[t];
+        }
+        TablePtrs last;
+        for (int t = 0; t < k; t++) {
+            last.src[t] = cur[t];
+            last.dst[t] = nullptr;
+            last.kind[t] = (uint8_t)SC_FR;
+        }
+        hipLaunchKernelGGL(k_graph_finals, dim3(1), dim3(64), 0, s, st, last, k, finals);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(s, &graph));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIP_CHECK(hipGraphLaunch(exec, s));
+    HIP_CHECK(hipMemcpyAsync(rounds_out, rounds, (size_t)n * (d + 1) * sizeof(Fe), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(point_out, point, n * sizeof(Fe), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(finals_out, finals, k * sizeof(Fe), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
+    ctx.sync();
+    std::memcpy(&tr, &h.tr, sizeof(DevTranscript));
 }
 
 }  // namespace apg

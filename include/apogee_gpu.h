@@ -351,6 +351,22 @@ int apg_sumcheck_prove_dev_lifted(apg_ctx *ctx, const apg_term *terms, size_t n_
                                   apg_round_fn round_fn, void *user, uint8_t *rounds_out, uint8_t *point_out,
                                   uint8_t *final_evals_out);
 
+/* The product summand eq(r, x) t_0(x) ... t_(k-1)(x), 1 <= k <= 3, over device
+ * tables of 2^num_vars entries (read, not modified), with every round and its
+ * transcript step (SUMCHECK_ROUND, SUMCHECK_CHALLENGE) on the device: the
+ * rounds are captured as one HIP graph and launched once, with no host step
+ * between rounds. The transcript is the library's own (its state moves to the
+ * device and back); rounds (num_vars x max(4, k + 2) coefficients), point and
+ * final evaluations equal apg_sumcheck_prove_dev's for the gate with one
+ * term of factors 0..k-1 driven by apg_transcript_round on the same
+ * transcript. Replaces the round loop of gkr::prove_sumcheck
+ * (crates/gkr/src/lib.rs:792-899) for callers that cannot run a host step per
+ * round; a device permutation runs ~70x slower than a host one, so the host-
+ * driven entry is the faster one for a single proof. */
+int apg_sumcheck_prove_graph_dev(apg_ctx *ctx, void *const *tables_dev, size_t k, size_t num_vars,
+                                 const uint8_t *eq_point_mont, apg_transcript *t, uint8_t *rounds_out,
+                                 uint8_t *point_out, uint8_t *final_evals_out);
+
 /* apg_sumcheck_prove_dev with no device allocation in the call. No
  * counterpart in the reference: it is for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
