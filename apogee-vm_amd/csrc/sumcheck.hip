@@ -516,7 +516,9 @@ static void wait_round(Context &ctx, volatile unsigned *done) {
             }
             if (e != hipErrorNotReady) HIP_CHECK(e);
         }
+#if defined(__x86_64__) || defined(__i386__)
         __builtin_ia32_pause();
+#endif
     }
 }
 
