@@ -139,10 +139,12 @@ constexpr uint32_t MAX_COARSE = 256;
 constexpr int MAX_FINE_BITS = 12;
 // LDS: a scatter block stages up to SCATTER_CAP pairs (40 KiB); a fine
 // workgroup holds a chunk of at most FINE_CAP pairs, FINE_PER_THREAD to a
-// thread, and stages it (32 KiB, less when its counters need more room, so
-// that it stays within 64 KiB)
+// thread, and stages it (60 KiB, less when its counters need more room, so
+// that it stays within 64 KiB). Longer fine chunks write longer runs per key:
+// 7680 against 4096 measured the 2^20 x 8 commit 2% faster and the mixed
+// batch 1-2% (3072 / 7168 for the scatter stage measured no gain).
 constexpr uint32_t SCATTER_CAP = 5120;
-constexpr uint32_t FINE_CAP = 4096;
+constexpr uint32_t FINE_CAP = 7680;
 constexpr int FINE_PER_THREAD = FINE_CAP / SORT_TPB;
 constexpr size_t LDS_LIMIT = 65536;
 constexpr size_t MAX_GROUP_COLUMNS = 64;  // keeps k_sort_base's rows <= 64 x 256
