@@ -33,7 +33,7 @@ FLAGS = [
 
 # Per-file additions. The sumcheck round kernels keep their per-table and
 # per-node arrays in registers only when their `#pragma unroll` loops unroll
-# fully; at the default pragma threshold the three-table and lookahead rounds
+# fully; at the default pragma threshold the three-table and lifted rounds
 # stop short and put those arrays in scratch memory (300-500 B per lane).
 FILE_FLAGS = {
     "sumcheck.hip": ["-mllvm", "-pragma-unroll-threshold=200000"],

@@ -23,6 +23,10 @@
 //  * The fold of round j-1 is fused into the evaluation of round j: one pass
 //    reads each table once (4 entries per pair of the folded table), writes
 //    the half-size fold and accumulates the node sums.
+//  * A product of two tables runs its device rounds two per launch
+//    (k_round_pair): round j+1's node sums are degree-2 polynomials in the
+//    challenge of round j, kept at three points and interpolated by the host
+//    once the challenge is drawn; the next launch binds both challenges.
 //  * Node sums reduce by wave shuffles, then across the block's waves in LDS,
 //    then (more than one block) through per-block partials summed by the last
 //    block to finish.
@@ -79,6 +83,18 @@ struct GateArgs {
     int nterms;
     int32_t f[SC_MAX_TERMS][3];
     F29 coef[SC_MAX_TERMS];
+};
+
+// Node sums of a paired-round launch (k_round_pair): two of round j, six of
+// round j+1 (three challenge values y by two nodes X)
+constexpr int PAIR_OUTS = 8;
+// node sums a launch may write: the partials' row and the nodes buffer
+constexpr int NODE_SLOTS = PAIR_OUTS > SC_MAX_NODES ? PAIR_OUTS : SC_MAX_NODES;
+// the bind weights of a paired-round launch (the challenges folded at launch,
+// in the R' domain) and round j's eq factors c_0 = 1 - r_(j+1), c_1 = r_(j+1)
+struct PairWeights {
+    F29 w[3];
+    F29 c0, c1;
 };
 
 // lo/hi of pair i of table t, folding the previous round's table if asked.
@@ -156,6 +172,105 @@ __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, 
         }
     }
     reduce_nodes<NN>(acc, NN, partials, counter, out, done);
+}
+
+// Entry e of a table bound by two more variables, from its four entries
+// a_0..a_3 at 4e.. of the table they were stored in:
+//   a_0 + ra (a_1 - a_0) + rb (a_2 - a_0) + ra rb (a_3 - a_2 - a_1 + a_0),
+// the binds by ra and then rb as one dot product with one reduction.
+__device__ __forceinline__ Fe fold_twice(const Fe *src, size_t e, const F29 (&w)[3]) {
+    const Fe a0 = ldfe(src, 4 * e), a1 = ldfe(src, 4 * e + 1), a2 = ldfe(src, 4 * e + 2), a3 = ldfe(src, 4 * e + 3);
+    const Fe d1 = f_sub<Fr>(a1, a0);
+    const F29 d[3] = {f29_from32(d1), f29_from32(f_sub<Fr>(a2, a0)), f29_from32(f_sub<Fr>(f_sub<Fr>(a3, a2), d1))};
+    return f_add<Fr>(a0, f29_pack_small<Fr>(f29_dot<Fr, 3>(d, w)));
+}
+
+// Rounds j and j+1 of the product summand of two tables in one pass, both
+// with node 1 from the claim (r_j and r_(j+1) nonzero). Thread p takes pair p
+// of T^j: its entries folded from the stored table T^(j-FD) with the FD
+// challenges known at launch (w: rho_(j-1) for FD = 1; rho_(j-2), rho_(j-1)
+// and their product for FD = 2), written back as the next pass's input. It adds
+//  * round j at X = 0, 2: E_j(p) line_0(p, X) line_1(p, X), with E_j(2i + b) =
+//    E_(j+1)(i) c_b, c_0 = 1 - r_(j+1), c_1 = r_(j+1);
+//  * round j+1 as a polynomial in its unknown challenge rho: pair i of T^(j+1)
+//    is (line(2i, rho), line(2i+1, rho)), so E_(j+1)(i) prod_t (lo_t + X (hi_t
+//    - lo_t)) has degree 2 in rho and is kept at rho = y = 0, 1, 2; the even
+//    lane of the pair (lo = its own line) takes X = 0 and the odd lane X = 2
+//    (2 hi - lo, with lo from the even lane).
+// out[0..1]: round j at X = 0, 2; out[2 + 2y + X/2]: round j+1's sums at
+// (rho = y, X). The host takes round j+1's node sums at rho = rho_j by
+// interpolation over y. Products as in k_round_product (each node sum carries
+// 2^-10, and 2^-5 more when E_(j+1) = lo hi).
+template <int FD>
+__global__ void __launch_bounds__(RTPB) k_round_pair(TablePtrs tp, size_t m, PairWeights pw, EqArgs eq,
+                                                   Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
+    constexpr int KT = 2;
+    Fe accj[2], accl[3];
+#pragma unroll
+    for (int x = 0; x < 2; x++) accj[x] = f_zero<Fr>();
+#pragma unroll
+    for (int y = 0; y < 3; y++) accl[y] = f_zero<Fr>();
+    const bool odd = threadIdx.x & 1;
+    F29 cb;  // c_b of this lane's pairs (p has the parity of the thread)
+#pragma unroll
+    for (int q = 0; q < 9; q++) cb.v[q] = odd ? pw.c1.v[q] : pw.c0.v[q];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;  // even: pairs p and p ^ 1 in lanes 2i, 2i+1
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < m; p += stride) {
+        Fe l[3][KT];  // line_t(p, y) at y = 0, 1, 2
+#pragma unroll
+        for (int t = 0; t < KT; t++) {
+            Fe lo, hi;
+            if (FD == 1) {
+                const Fe *s = tp.src[t];
+                const Fe a0 = ldfe(s, 4 * p), a1 = ldfe(s, 4 * p + 1), a2 = ldfe(s, 4 * p + 2),
+                         a3 = ldfe(s, 4 * p + 3);
+                lo = f_add<Fr>(a0, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(f_sub<Fr>(a1, a0)), pw.w[0])));
+                hi = f_add<Fr>(a2, f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(f_sub<Fr>(a3, a2)), pw.w[0])));
+            } else {
+                lo = fold_twice(tp.src[t], 2 * p, pw.w);
+                hi = fold_twice(tp.src[t], 2 * p + 1, pw.w);
+            }
+            stfe(tp.dst[t], 2 * p, lo);
+            stfe(tp.dst[t], 2 * p + 1, hi);
+            l[0][t] = lo;
+            l[1][t] = hi;
+            l[2][t] = f_add<Fr>(hi, f_sub<Fr>(hi, lo));
+        }
+        const size_t i = p >> 1;
+        const Fe e1 = eq.lo ? f_mul_r261<Fr>(ldfe(eq.lo, i & ((size_t(1) << eq.lo_bits) - 1)),
+                                             ldfe(eq.hi, i >> eq.lo_bits))
+                            : ldfe(eq.hi, i);
+        const Fe ej = f29_pack_small<Fr>(f29_mul<Fr>(f29_from32(e1), cb));  // exactly E_(j+1)(i) c_b
+        accj[0] = f_add<Fr>(accj[0], f_mul_r261<Fr>(ej, f_mul_r261<Fr>(l[0][0], l[0][1])));
+        accj[1] = f_add<Fr>(accj[1], f_mul_r261<Fr>(ej, f_mul_r261<Fr>(l[2][0], l[2][1])));
+#pragma unroll
+        for (int y = 0; y < 3; y++) {
+            Fe v[KT];
+#pragma unroll
+            for (int t = 0; t < KT; t++) {
+                Fe o;
+#pragma unroll
+                for (int q = 0; q < 8; q++) o.l[q] = __shfl_xor(l[y][t].l[q], 1, 64);
+                // even lane: lo; odd lane: hi + (hi - lo)
+                v[t] = odd ? f_add<Fr>(l[y][t], f_sub<Fr>(l[y][t], o)) : l[y][t];
+            }
+            accl[y] = f_add<Fr>(accl[y], f_mul_r261<Fr>(e1, f_mul_r261<Fr>(v[0], v[1])));
+        }
+    }
+    __shared__ Fe lds[RWAVES][PAIR_OUTS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+        const Fe s = wave_sum(accj[x]);
+        if (lane == 0) lds[w][x] = s;
+    }
+#pragma unroll
+    for (int y = 0; y < 3; y++) {
+        const Fe s = wave_sum_parity(accl[y]);
+        if (lane < 2) lds[w][2 + 2 * y + lane] = s;
+    }
+    __syncthreads();
+    finish_nodes<PAIR_OUTS>(lds, PAIR_OUTS, partials, counter, out, done);
 }
 
 // Any gate: terms read from the kernel argument, table lines kept in a local
@@ -448,6 +563,18 @@ static void launch_product(Context &ctx, bool lift, bool full, const TablePtrs &
         launch_product<KT, false>(ctx, full, tp, m, fold, r, eq, partials, counter, out, done);
 }
 
+// A paired-round launch (k_round_pair) over m pairs of T^j, folding with the
+// fd challenges in pw.
+static void launch_pair(Context &ctx, int fd, const TablePtrs &tp, size_t m, const PairWeights &pw,
+                        const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
+    const dim3 g(round_blocks(ctx, m, 1)), b(RTPB);
+    if (fd == 1)
+        hipLaunchKernelGGL(k_round_pair<1>, g, b, 0, ctx.stream, tp, m, pw, eq, partials, counter, out, done);
+    else
+        hipLaunchKernelGGL(k_round_pair<2>, g, b, 0, ctx.stream, tp, m, pw, eq, partials, counter, out, done);
+    HIP_CHECK(hipGetLastError());
+}
+
 // lift: some source table is a narrow backing (the first two rounds); done:
 // the completion flag the round sets after its node sums (or null)
 static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bool full, int D, bool lift,
@@ -538,8 +665,8 @@ static ScratchLayout sumcheck_layout(int k, int n) {
         std::max<size_t>(bsz, 1) * sizeof(Fe),
         (size_t)k * std::max<size_t>(len0 / 2, 1) * sizeof(Fe),
         (size_t)k * std::max<size_t>(len0 / 4, 1) * sizeof(Fe),
-        (size_t)MAX_RBLOCKS * SC_MAX_NODES * sizeof(Fe),
-        SC_MAX_NODES * sizeof(Fe),
+        (size_t)MAX_RBLOCKS * NODE_SLOTS * sizeof(Fe),
+        NODE_SLOTS * sizeof(Fe),
         64 * sizeof(unsigned),
     };
     ScratchLayout l;
@@ -602,10 +729,10 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
     sc.counter = ctx.zeroed<unsigned>("sc.counter", 64);
     // node sums and the round's completion flag in coherent pinned memory:
     // the host waits on the flag, not on the stream
-    Fe *hvals = ctx.hc<Fe>("sc.hvals", SC_MAX_NODES + 1);
+    Fe *hvals = ctx.hc<Fe>("sc.hvals", NODE_SLOTS + 1);
     HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&sc.nodes), hvals, 0));
-    sc.done_host = reinterpret_cast<unsigned *>(hvals + SC_MAX_NODES);
-    sc.done_dev = reinterpret_cast<unsigned *>(sc.nodes + SC_MAX_NODES);
+    sc.done_host = reinterpret_cast<unsigned *>(hvals + NODE_SLOTS);
+    sc.done_dev = reinterpret_cast<unsigned *>(sc.nodes + NODE_SLOTS);
     sumcheck_rounds(ctx, sc, hvals, true, ctx.profiling, gate, tables, kinds, k, n, eq_point, round_fn, rounds_out,
                     point_out, finals_out);
 }
@@ -659,7 +786,7 @@ void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *ta
     // the array keeps the last call's contents: clear the counter
     hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx.stream, sc.counter, size_t(64));
     HIP_CHECK(hipGetLastError());
-    Fe nodes_host[SC_MAX_NODES];
+    Fe nodes_host[NODE_SLOTS];
     sumcheck_rounds(ctx, sc, nodes_host, false, false, gate, tables, kinds, k, n, eq_point, round_fn, rounds_out,
                     point_out, finals_out);
 }
@@ -723,8 +850,8 @@ static void sumcheck_finals_n0(Context &ctx, Fe *const *tables, const uint8_t *k
 }
 
 static void sumcheck_host_tail(Context &ctx, const Gate &gate, int k, int n, int j0, const Fe *eq_point,
-                               const Fe *const *dev_tables, const uint8_t *kinds, size_t len, bool fold_first,
-                               const Fe &rho_prev, const Fe &P_in, const RoundFn &round_fn, Fe *rounds_out,
+                               const Fe *const *dev_tables, const uint8_t *kinds, size_t len, const Fe *pending,
+                               int npending, const Fe &P_in, const RoundFn &round_fn, Fe *rounds_out,
                                Fe *point_out, Fe *finals_out) {
     using namespace hfr;
     W4 P = w4_of(P_in);
@@ -754,7 +881,7 @@ static void sumcheck_host_tail(Context &ctx, const Gate &gate, int k, int n, int
         }
         L /= 2;
     };
-    if (fold_first) fold(w4_of(rho_prev));
+    for (int q = 0; q < npending; q++) fold(w4_of(pending[q]));  // challenges drawn since the tables were stored
     std::vector<W4> coef(gate.nterms);
     for (int q = 0; q < gate.nterms; q++) coef[q] = w4_of(gate.terms[q].coef);
     const W4 one = w4_of(f_one<Fr>()), zero = w4_of(f_zero<Fr>());
@@ -822,7 +949,7 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
     const size_t len0 = size_t(1) << n;
 
     if ((size_t)k * len0 <= TAIL_ELEMS) {  // small enough for the host from the start
-        sumcheck_host_tail(ctx, gate, k, n, 0, eq_point, tables, kinds, len0, false, f_zero<Fr>(), f_one<Fr>(),
+        sumcheck_host_tail(ctx, gate, k, n, 0, eq_point, tables, kinds, len0, nullptr, 0, f_one<Fr>(),
                            round_fn, rounds_out, point_out, finals_out);
         return;
     }
@@ -880,40 +1007,21 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
         }
     } evg{ev0, ev1};
 
-    // Round 0 reads the inputs T^0 (2^n entries). Round j >= 1 folds T^(j-1)
-    // with rho_(j-1) into T^j (2^(n-j) entries: odd j into bufA, even j into
-    // bufB, so source and destination never alias) and evaluates T^j's pairs.
+    // Round 0 reads the inputs T^0 (2^n entries). A single-round launch at
+    // round j >= 1 folds T^(j-1) with rho_(j-1) into T^j (2^(n-j) entries:
+    // odd j into bufA, even j into bufB, so source and destination never
+    // alias) and evaluates T^j's pairs. A paired launch (k_round_pair) at
+    // round j folds the stored table with the one or two challenges drawn
+    // since it was written into T^j, in the buffer that does not hold its
+    // source, and evaluates rounds j and j+1.
     // the product kernel's node sums carry 2^(-5 k), and 2^-5 more when E = lo hi
     const W4 prod_scale = hfr::w4_of(product ? f_from_u64<Fr>(uint64_t(1) << (5 * k)) : f_one<Fr>());  // k <= 3
     const W4 prod_scale_lo = hfr::w4_of(product ? f_from_u64<Fr>(uint64_t(1) << (5 * k + 5)) : f_one<Fr>());
-    Fe rho = f_zero<Fr>();
     W4 claim = smallc().zero;
     W4 P = smallc().one;  // prod_{l<j} eq(r_l, rho_l)
-    for (int j = 0; j < n; j++) {
-        // between rounds (the last one synchronised the stream); round j
-        // starts after 1 - 2^-j of the rounds' work
-        if (j > 0) ctx.yield_point(1.0 - std::ldexp(1.0, -j));
-        if (j > 0 && (size_t)k * (len0 >> j) <= TAIL_ELEMS) {  // T^j is small: the rest on the host
-            std::vector<Fe *> src(k);
-            for (int t = 0; t < k; t++) src[t] = const_cast<Fe *>(cur[t]);
-            // T^(j-1) is still the caller's table when j == 1
-            sumcheck_host_tail(ctx, gate, k, n, j, eq_point, src.data(), j == 1 ? kinds : nullptr, len0 >> (j - 1),
-                               true, rho, hfr::fe_of(P), round_fn, rounds_out, point_out, finals_out);
-            return;
-        }
-        const bool fold = j > 0;
-        const size_t m = len0 >> (j + 1);  // pairs of T^j
-        const Fe &rj = eq_point[j];
-        const bool full = j == 0 || f_is_zero(rj);
-        TablePtrs tp;
-        std::vector<Fe *> nxt(k);
-        for (int t = 0; t < k; t++) {
-            nxt[t] = (j % 2 == 1) ? bufA[t] : bufB[t];
-            tp.src[t] = cur[t];
-            tp.kind[t] = (j <= 1 && kinds) ? kinds[t] : (uint8_t)SC_FR;
-            tp.dst[t] = nxt[t];
-        }
-        const int s = j + 1;
+
+    // E_(s-1), the suffix-eq factor of round s-1's pairs (tables at s)
+    auto eq_args = [&](int s) {
         EqArgs eq;
         if (s < h0) {
             eq.lo = eqA + (size_t)(s - 1) * astride;
@@ -924,11 +1032,142 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
             eq.lo_bits = 0;
             eq.hi = eqB + (size_t)(s - h0) * bstride;
         }
+        return eq;
+    };
+
+    // Round j on the host from the node sums S (X = 0..D, or X = 0, 2..D with
+    // node 1 from the claim): T(X) = P S(X) (rescaled for the product kernel)
+    // at X = 0..D, then X = D+1..d by extrapolation; returns rho_j.
+    auto host_round = [&](int j, const W4 *S, bool full, bool eq_lo) -> Fe {
+        using namespace hfr;
+        const EqLine eqj(eq_point[j]);
+        const W4 Ps = product ? w4_mul(P, eq_lo ? prod_scale_lo : prod_scale) : P;
+        W4 T[SC_MAX_NODES], cw[SC_MAX_NODES];
+        if (full) {
+            for (int x = 0; x <= D; x++) T[x] = w4_mul(Ps, S[x]);
+        } else {
+            T[0] = w4_mul(Ps, S[0]);
+            for (int x = 2; x <= D; x++) T[x] = w4_mul(Ps, S[x - 1]);
+            const W4 g0 = w4_mul(eqj.base, T[0]);
+            T[1] = w4_mul(w4_sub(claim, g0), w4_of(inv_r[j]));  // g(1) = r_j T(1)
+        }
+        round_poly(T, D, d, eqj, !full, claim, cw);
+        Fe *coeffs = rounds_out + (size_t)j * (d + 1);
+        for (int x = 0; x <= d; x++) coeffs[x] = fe_of(cw[x]);
+        const Fe rho = round_fn(coeffs, d + 1);
+        point_out[j] = rho;
+        const W4 rw = w4_of(rho);
+        claim = horner_w4(cw, d, rw);
+        P = w4_mul(P, eqj.at(rw));
+        return rho;
+    };
+
+    // G: the first round left to the host. The device rounds from first_pair
+    // to G-1 run in pairs (an even count of them) when the summand is the
+    // product of two tables and every r_j of those rounds is nonzero.
+    int G = 1;
+    while (G < n && (size_t)k * (len0 >> G) > TAIL_ELEMS) G++;
+    int first_pair = G;
+    if (product && k == 2) {
+        bool nonzero = true;
+        for (int j = 1; j < G; j++) nonzero = nonzero && !f_is_zero(eq_point[j]);
+        if (nonzero) {
+            first_pair = (G - 1) % 2 == 0 ? 1 : 2;
+            if (narrow && first_pair < 2) first_pair += 2;  // rounds 0 and 1 lift
+        }
+    }
+
+    Fe pend[2];  // challenges drawn since cur was written, oldest first
+    int npend = 0;
+    int j = 0;
+    while (j < n) {
+        // between launches (the last one's flag was seen); round j starts
+        // after 1 - 2^-j of the rounds' work
+        if (j > 0) ctx.yield_point(1.0 - std::ldexp(1.0, -j));
+        if (j > 0 && (size_t)k * (len0 >> j) <= TAIL_ELEMS) {  // T^j is small: the rest on the host
+            std::vector<Fe *> src(k);
+            for (int t = 0; t < k; t++) src[t] = const_cast<Fe *>(cur[t]);
+            // cur is T^(j - npend), still the caller's table when j == npend
+            sumcheck_host_tail(ctx, gate, k, n, j, eq_point, src.data(), j == npend ? kinds : nullptr,
+                               len0 >> (j - npend), pend, npend, hfr::fe_of(P), round_fn, rounds_out, point_out,
+                               finals_out);
+            return;
+        }
+        const size_t m = len0 >> (j + 1);  // pairs of T^j
+        std::vector<Fe *> nxt(k);
+        TablePtrs tp;
+        if (j >= first_pair) {
+            // rounds j and j+1 in one launch; T^j into the buffer cur is not in
+            // (bufB, len0 / 4 entries per table, only from j = 2 on: at j = 1
+            // cur is still the caller's table)
+            const bool into_a = cur[0] != bufA[0];
+            for (int t = 0; t < k; t++) {
+                nxt[t] = into_a ? bufA[t] : bufB[t];
+                tp.src[t] = cur[t];
+                tp.kind[t] = (uint8_t)SC_FR;
+                tp.dst[t] = nxt[t];
+            }
+            const EqArgs eq = eq_args(j + 2);
+            PairWeights pw;
+            pw.w[0] = f29_from_mont32<Fr>(pend[0]);
+            pw.w[1] = npend == 2 ? f29_from_mont32<Fr>(pend[1]) : F29{};
+            pw.w[2] = npend == 2 ? f29_from_mont32<Fr>(f_mul<Fr>(pend[0], pend[1])) : F29{};
+            pw.c0 = f29_from_mont32<Fr>(f_sub<Fr>(f_one<Fr>(), eq_point[j + 1]));
+            pw.c1 = f29_from_mont32<Fr>(eq_point[j + 1]);
+            if (timing) HIP_CHECK(hipEventRecord(ev0, st));
+            if (sc.done_host) __atomic_store_n(sc.done_host, 0u, __ATOMIC_RELEASE);
+            launch_pair(ctx, npend, tp, m, pw, eq, partials, counter, hvals_dev, sc.done_dev);
+            if (timing) HIP_CHECK(hipEventRecord(ev1, st));
+            if (!mapped_nodes)
+                HIP_CHECK(hipMemcpyAsync(hvals, hvals_dev, PAIR_OUTS * sizeof(Fe), hipMemcpyDeviceToHost, st));
+            round_fn.begin(d + 1);
+            if (sc.done_host)
+                wait_round(ctx, sc.done_host);
+            else
+                ctx.sync();
+            if (timing) {
+                float ms = 0;
+                HIP_CHECK(hipEventSynchronize(ev1));
+                HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+                ctx.stats.sumcheck_kernel_ms += ms;
+                ctx.stats.sumcheck_launches += 1;
+                // reads the stored table once, writes T^j
+                ctx.stats.sumcheck_bytes += (uint64_t)k * 32 * ((npend == 2 ? 8 : 4) * m + 2 * m);
+            }
+            using namespace hfr;
+            W4 h[PAIR_OUTS];
+            for (int x = 0; x < PAIR_OUTS; x++) h[x] = w4_of(hvals[x]);
+            const Fe rj = host_round(j, h, false, eq.lo != nullptr);
+            // round j+1's node sums at rho = rho_j: Lagrange weights over y = 0, 1, 2
+            const SmallConsts &K = smallc();
+            const W4 r = w4_of(rj), r1 = w4_sub(r, K.one), r2 = w4_sub(r, K.num[2]);
+            const W4 L0 = w4_mul(w4_mul(r1, r2), K.inv2), L1 = w4_sub(K.zero, w4_mul(r, r2)),
+                     L2 = w4_mul(w4_mul(r, r1), K.inv2);
+            W4 S[2];
+            for (int x = 0; x < 2; x++)
+                S[x] = w4_add(w4_add(w4_mul(L0, h[2 + x]), w4_mul(L1, h[4 + x])), w4_mul(L2, h[6 + x]));
+            const Fe rj1 = host_round(j + 1, S, false, eq.lo != nullptr);
+            for (int t = 0; t < k; t++) cur[t] = nxt[t];
+            pend[0] = rj;
+            pend[1] = rj1;
+            npend = 2;
+            j += 2;
+            continue;
+        }
+        const bool fold = j > 0;  // before any pair: at most one challenge pending
+        const bool full = j == 0 || f_is_zero(eq_point[j]);
+        for (int t = 0; t < k; t++) {
+            nxt[t] = (j % 2 == 1) ? bufA[t] : bufB[t];
+            tp.src[t] = cur[t];
+            tp.kind[t] = (j <= 1 && kinds) ? kinds[t] : (uint8_t)SC_FR;
+            tp.dst[t] = nxt[t];
+        }
+        const EqArgs eq = eq_args(j + 1);
         if (timing) HIP_CHECK(hipEventRecord(ev0, st));
         // cleared before the launch: the previous round's kernel has set it and writes it no more
         if (sc.done_host) __atomic_store_n(sc.done_host, 0u, __ATOMIC_RELEASE);
-        launch_round(ctx, gate, k, product, full, D, narrow && j <= 1, tp, m, fold, rho, eq, partials, counter,
-                     hvals_dev, sc.done_dev);
+        launch_round(ctx, gate, k, product, full, D, narrow && j <= 1, tp, m, fold, fold ? pend[0] : f_zero<Fr>(),
+                     eq, partials, counter, hvals_dev, sc.done_dev);
         HIP_CHECK(hipGetLastError());
         if (timing) HIP_CHECK(hipEventRecord(ev1, st));
         const int nn = full ? D + 1 : D;
@@ -949,30 +1188,13 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
             // half-size fold
             ctx.stats.sumcheck_bytes += (uint64_t)k * 32 * (fold ? 6 * m : 2 * m);
         }
-        // T(X) = P S(X) at X = 0..D (the node sums rescaled for the product
-        // kernel), then X = D+1..d by extrapolation
-        using namespace hfr;
-        const EqLine eqj(rj);
-        const W4 Ps = product ? w4_mul(P, eq.lo ? prod_scale_lo : prod_scale) : P;
-        W4 T[SC_MAX_NODES], cw[SC_MAX_NODES];
-        if (full) {
-            for (int x = 0; x <= D; x++) T[x] = w4_mul(Ps, w4_of(hvals[x]));
-        } else {
-            T[0] = w4_mul(Ps, w4_of(hvals[0]));
-            for (int x = 2; x <= D; x++) T[x] = w4_mul(Ps, w4_of(hvals[x - 1]));
-            const W4 g0 = w4_mul(eqj.base, T[0]);
-            T[1] = w4_mul(w4_sub(claim, g0), w4_of(inv_r[j]));  // g(1) = r_j T(1)
-        }
-        round_poly(T, D, d, eqj, !full, claim, cw);
-        Fe *coeffs = rounds_out + (size_t)j * (d + 1);
-        for (int x = 0; x <= d; x++) coeffs[x] = fe_of(cw[x]);
-        rho = round_fn(coeffs, d + 1);
-        point_out[j] = rho;
-        const W4 rw = w4_of(rho);
-        claim = horner_w4(cw, d, rw);
-        P = w4_mul(P, eqj.at(rw));
+        W4 S[SC_MAX_NODES];
+        for (int x = 0; x < nn; x++) S[x] = hfr::w4_of(hvals[x]);
+        pend[0] = host_round(j, S, full, eq.lo != nullptr);
+        npend = 1;
         if (fold)
             for (int t = 0; t < k; t++) cur[t] = nxt[t];
+        j += 1;
     }
     throw ApgError(-5, "sumcheck: the host tail was not reached");  // k * 2 <= TAIL_ELEMS always holds
 }

@@ -50,7 +50,19 @@ __device__ __forceinline__ Fe wave_sum(Fe v) {
     return v;
 }
 
-// Sum of the first nn of NN per-thread accumulators over the grid into out[].
+// The sums over the even and over the odd lanes of the wave: lane 0 ends
+// with the even lanes' sum, lane 1 with the odd lanes'.
+__device__ __forceinline__ Fe wave_sum_parity(Fe v) {
+#pragma unroll
+    for (int off = 32; off > 1; off >>= 1) {
+        Fe o;
+#pragma unroll
+        for (int q = 0; q < 8; q++) o.l[q] = __shfl_xor(v.l[q], off, 64);
+        v = f_add<Fr>(v, o);
+    }
+    return v;
+}
+
 // With `done` set (pinned host memory, like out), the block that writes out[]
 // then sets *done = 1 behind a system-scope fence, so the host can wait for
 // the node sums without synchronising the stream.
@@ -63,20 +75,16 @@ __device__ __forceinline__ void signal_done(unsigned *done) {
     }
 }
 
+// The grid's sums of nn values, given each wave's sums in lds[wave][x]
+// (written before a barrier): the block's sums go to out[] (one block) or to
+// its row of partials, and the last block to finish adds the partials into
+// out[]. The release fence before the counter increment and the acquire
+// fence after it make the other blocks' partials visible to plain loads.
 template <int NN>
-__device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *counter, Fe *out,
-                             unsigned *done = nullptr) {
-    __shared__ Fe lds[RWAVES][NN];
+__device__ void finish_nodes(Fe (&lds)[RWAVES][NN], int nn, Fe *partials, unsigned *counter, Fe *out,
+                             unsigned *done) {
     __shared__ bool is_last;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-#pragma unroll
-    for (int x = 0; x < NN; x++) {
-        if (x < nn) {
-            const Fe v = wave_sum(acc[x]);
-            if (lane == 0) lds[w][x] = v;
-        }
-    }
-    __syncthreads();
+    const int tid = threadIdx.x;
     const bool single = gridDim.x == 1;
     if (tid < nn) {
         Fe s = lds[0][tid];
@@ -84,7 +92,7 @@ __device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *coun
         if (single)
             out[tid] = s;
         else
-            partials[(size_t)blockIdx.x * NN + tid] = s;
+            stfe(partials, (size_t)blockIdx.x * NN + tid, s);
     }
     if (single) {
         signal_done(done);
@@ -96,26 +104,40 @@ __device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *coun
     __syncthreads();
     if (!is_last) return;
     __threadfence();
-    for (int x = 0; x < nn; x++) {
-        Fe a = f_zero<Fr>();
-        for (unsigned b = tid; b < gridDim.x; b += RTPB) {
-            // partials written by other blocks: read through the L2
-            const Fe *pp = partials + (size_t)b * NN + x;
-            Fe v;
-            for (int q = 0; q < 8; q++) v.l[q] = __atomic_load_n(&pp->l[q], __ATOMIC_RELAXED);
-            a = f_add<Fr>(a, v);
-        }
-        a = wave_sum(a);
-        if (lane == 0) lds[w][x] = a;
+    // all outputs at once: value x summed by the tpx lanes tid / tpx == x
+    // (an aligned group inside one wave), each over every tpx-th block
+    int tpx = 64;
+    while (tpx * nn > RTPB) tpx >>= 1;
+    const int x = tid / tpx, sub = tid % tpx;
+    Fe a = f_zero<Fr>();
+    if (x < nn)
+        for (unsigned b = sub; b < gridDim.x; b += tpx) a = f_add<Fr>(a, ldfe(partials, (size_t)b * NN + x));
+    for (int off = tpx / 2; off > 0; off >>= 1) {
+        Fe o;
+#pragma unroll
+        for (int q = 0; q < 8; q++) o.l[q] = __shfl_xor(a.l[q], off, 64);
+        a = f_add<Fr>(a, o);
     }
-    __syncthreads();
-    if (tid < nn) {
-        Fe s = lds[0][tid];
-        for (int q = 1; q < RWAVES; q++) s = f_add<Fr>(s, lds[q][tid]);
-        out[tid] = s;
-    }
+    if (x < nn && sub == 0) out[x] = a;
     if (tid == 0) *counter = 0;
     signal_done(done);
+}
+
+// Sum of the first nn of NN per-thread accumulators over the grid into out[].
+template <int NN>
+__device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *counter, Fe *out,
+                             unsigned *done = nullptr) {
+    __shared__ Fe lds[RWAVES][NN];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int x = 0; x < NN; x++) {
+        if (x < nn) {
+            const Fe v = wave_sum(acc[x]);
+            if (lane == 0) lds[w][x] = v;
+        }
+    }
+    __syncthreads();
+    finish_nodes<NN>(lds, nn, partials, counter, out, done);
 }
 
 // The suffix-eq tables of one proof, as products over the variables they

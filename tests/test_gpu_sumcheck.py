@@ -200,3 +200,47 @@ def test_lifted_backings_match_oracle(apg, orc, n, kinds, shape):
     assert np.array_equal(fg, np.stack([x[0] for x in fo]) if isinstance(fo, list) else fo.reshape(fg.shape))
     for b, _ in dev:
         b.free()
+
+
+def _column(orc, rng, kind, m, seed):
+    if kind == "u8":
+        return rng.integers(0, 256, m, dtype=np.uint8)
+    if kind == "u16":
+        return rng.integers(0, 1 << 16, m, dtype=np.uint16)
+    if kind == "u32":
+        return rng.integers(0, 1 << 32, m, dtype=np.uint32)
+    return orc.fill_uniform(seed, m)
+
+
+# Two-table products run their device rounds two per launch after the single
+# ones (round 0, the lifted rounds, and one more when the count of device
+# rounds left is odd); the host takes over once the tables hold at most 256
+# entries in all (round n - 7 here).
+@pytest.mark.parametrize("n,zero_at,kinds", [
+    (9, None, ("fr", "fr")),    # device rounds 0-1: no pair
+    (10, None, ("fr", "fr")),   # round 0, then rounds 1-2 paired
+    (11, None, ("fr", "fr")),   # rounds 0 and 1 alone, then 2-3
+    (13, None, ("fr", "fr")),   # rounds 0 and 1 alone, then 2-3 and 4-5
+    (12, 3, ("fr", "fr")),      # r_3 = 0 on a device round: every round alone, round 3 on all nodes
+    (12, 8, ("fr", "fr")),      # r_8 = 0 on a host round: the device rounds still paired
+    (12, None, ("u8", "u32")),  # lifted rounds 0-1 and round 2 alone, then 3-4
+    (13, None, ("u16", "fr")),  # lifted rounds 0-1 alone, then 2-3 and 4-5
+])
+def test_paired_rounds_match_oracle(apg, orc, n, zero_at, kinds):
+    ctx = apg.default_context()
+    rng = np.random.default_rng(900 + n)
+    cols = [_column(orc, rng, kd, 1 << n, 910 + 10 * n + i) for i, kd in enumerate(kinds)]
+    lifted = [c if c.ndim == 2 else _lift(orc, c) for c in cols]
+    r = orc.fill_uniform(930 + n, n)
+    if zero_at is not None:
+        r[zero_at] = 0
+    one = orc.from_u64(orc.FR, 1)
+    dev = [apg.upload_backed(ctx, c) for c in cols]
+    t_gpu, t_orc = apg.Transcript(), orc.Transcript()
+    rg, pg, fg = apg.prove_sumcheck_device(r, apg.Gate.product(2), dev, t_gpu)
+    ro, po, fo = orc.prove_sumcheck(r, [(one, [0, 1])], lifted, t_orc, workers=8)
+    assert np.array_equal(rg, ro) and np.array_equal(pg, po)
+    assert np.array_equal(fg, np.stack([x[0] for x in fo]))
+    assert np.array_equal(t_gpu.challenge_scalar(5), t_orc.challenge(5))
+    for b, _ in dev:
+        b.free()
