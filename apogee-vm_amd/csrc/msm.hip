@@ -145,7 +145,11 @@ constexpr int MAX_FINE_BITS = 12;
 // batch 1-2% (3072 / 7168 for the scatter stage measured no gain).
 constexpr uint32_t SCATTER_CAP = 5120;
 constexpr uint32_t FINE_CAP = 7680;
-constexpr int FINE_PER_THREAD = FINE_CAP / SORT_TPB;
+// the fine pass's workgroups: 512 threads (two per CU fit the LDS), so that
+// its phases between barriers, each short, have eight waves per workgroup
+constexpr int FINE_TPB = 512;
+constexpr int FINE_PER_THREAD = FINE_CAP / FINE_TPB;
+static_assert(FINE_CAP % FINE_TPB == 0, "fine chunks hold whole rows of the workgroup");
 constexpr size_t LDS_LIMIT = 65536;
 constexpr size_t MAX_GROUP_COLUMNS = 64;  // keeps k_sort_base's rows <= 64 x 256
 
@@ -169,10 +173,11 @@ struct MsmPlan {
     HeavyPlan heavy;
 };
 
-// Exclusive scan of one value per thread over a SORT_TPB-thread block;
-// `total` receives the block's sum. Every thread of the block calls it.
+// Exclusive scan of one value per thread over a TPB-thread block; `total`
+// receives the block's sum. Every thread of the block calls it.
+template <int TPB = SORT_TPB>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &total) {
-    __shared__ uint32_t ws[SORT_TPB / 64];
+    __shared__ uint32_t ws[TPB / 64];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t incl = v;
 #pragma unroll
@@ -184,7 +189,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t &t
     __syncthreads();
     uint32_t before = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < SORT_TPB / 64; w++) {
+    for (int w = 0; w < TPB / 64; w++) {
         if (w < (int)wave) before += ws[w];
         tot += ws[w];
     }
@@ -412,7 +417,7 @@ __device__ __forceinline__ void load_chunk(const uint32_t *keys_in, const uint32
                                            uint32_t m, uint2 (&v)[FINE_PER_THREAD]) {
 #pragma unroll
     for (int q = 0; q < FINE_PER_THREAD; q++) {
-        const uint32_t e = threadIdx.x + (uint32_t)q * SORT_TPB;
+        const uint32_t e = threadIdx.x + (uint32_t)q * FINE_TPB;
         v[q] = make_uint2(0u, 0u);
         if (e < m) v[q] = make_uint2(keys_in[at + e], vals_in[at + e]);
     }
@@ -420,7 +425,7 @@ __device__ __forceinline__ void load_chunk(const uint32_t *keys_in, const uint32
 
 // Fine counts of every row of more than cap pairs: each chunk counts in LDS
 // and adds its counts to the row's (global atomics, one per bucket).
-__global__ void __launch_bounds__(SORT_TPB) k_sort_fine_count(const uint32_t *__restrict__ keys_in, SortGeom g,
+__global__ void __launch_bounds__(FINE_TPB) k_sort_fine_count(const uint32_t *__restrict__ keys_in, SortGeom g,
                                                             uint32_t cap, uint32_t rows,
                                                             const uint32_t *__restrict__ rowbase,
                                                             const uint32_t *__restrict__ rowtot,
@@ -434,27 +439,27 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_fine_count(const uint32_t *__
         if (E <= cap) return;  // uniform per workgroup
         const uint32_t c0 = chunk * cap, m = min(cap, E - c0);
         __syncthreads();
-        for (uint32_t f = threadIdx.x; f < FB; f += SORT_TPB) sm[f] = 0;
+        for (uint32_t f = threadIdx.x; f < FB; f += FINE_TPB) sm[f] = 0;
         __syncthreads();
         uint32_t k[FINE_PER_THREAD];
 #pragma unroll
         for (int q = 0; q < FINE_PER_THREAD; q++) {  // every load in flight before the counts
-            const uint32_t e = threadIdx.x + (uint32_t)q * SORT_TPB;
+            const uint32_t e = threadIdx.x + (uint32_t)q * FINE_TPB;
             k[q] = 0;
             if (e < m) k[q] = keys_in[rowbase[r] + c0 + e];
         }
 #pragma unroll
         for (int q = 0; q < FINE_PER_THREAD; q++)
-            if (threadIdx.x + (uint32_t)q * SORT_TPB < m) atomicAdd(&sm[k[q] & mask], 1u);
+            if (threadIdx.x + (uint32_t)q * FINE_TPB < m) atomicAdd(&sm[k[q] & mask], 1u);
         __syncthreads();
-        for (uint32_t f = threadIdx.x; f < FB; f += SORT_TPB)
+        for (uint32_t f = threadIdx.x; f < FB; f += FINE_TPB)
             if (sm[f]) atomicAdd(&fcount[(size_t)r * FB + f], sm[f]);
     });
 }
 
 // Dynamic LDS: 3 x 2^fine_bits words of counters, then cap <= FINE_CAP pairs
-// (cap a multiple of SORT_TPB).
-__global__ void __launch_bounds__(SORT_TPB) k_sort_fine(const uint32_t *__restrict__ keys_in,
+// (cap a multiple of FINE_TPB).
+__global__ void __launch_bounds__(FINE_TPB) k_sort_fine(const uint32_t *__restrict__ keys_in,
                                                       const uint32_t *__restrict__ vals_in, SortGeom g, uint32_t cap,
                                                       uint32_t rows, const uint32_t *__restrict__ rowbase,
                                                       const uint32_t *__restrict__ rowtot,
@@ -477,23 +482,23 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_fine(const uint32_t *__restri
         const bool single = E <= cap;
         const uint32_t c0 = chunk * cap, m = min(cap, E - c0);
         __syncthreads();
-        for (uint32_t f = tid; f < FB; f += SORT_TPB) lcur[f] = 0;
+        for (uint32_t f = tid; f < FB; f += FINE_TPB) lcur[f] = 0;
         uint2 v[FINE_PER_THREAD];
         load_chunk(keys_in, vals_in, start + c0, m, v);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < FINE_PER_THREAD; q++)
-            if (tid + (uint32_t)q * SORT_TPB < m) atomicAdd(&lcur[v[q].x & mask], 1u);
+            if (tid + (uint32_t)q * FINE_TPB < m) atomicAdd(&lcur[v[q].x & mask], 1u);
         __syncthreads();
         // the keys' ranges: from the chunk's counts (a one-chunk row) or the
         // row's (written by chunk 0); the row's bucket starts in rstart
         if (single || chunk == 0) {
             uint32_t carry = start;
-            for (uint32_t b0 = 0; b0 < FB; b0 += SORT_TPB) {
+            for (uint32_t b0 = 0; b0 < FB; b0 += FINE_TPB) {
                 const uint32_t f = b0 + tid;
                 const uint32_t cnt = f < FB ? (single ? lcur[f] : fcount[(size_t)r * FB + f]) : 0u;
                 uint32_t tot;
-                const uint32_t at = carry + block_exclusive_scan(cnt, tot);
+                const uint32_t at = carry + block_exclusive_scan<FINE_TPB>(cnt, tot);
                 if (f < FB) {
                     rstart[f] = at;
                     if (lk0 + f < g.lkeys) {
@@ -506,11 +511,11 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_fine(const uint32_t *__restri
             }
         } else {
             uint32_t carry = start;
-            for (uint32_t b0 = 0; b0 < FB; b0 += SORT_TPB) {
+            for (uint32_t b0 = 0; b0 < FB; b0 += FINE_TPB) {
                 const uint32_t f = b0 + tid;
                 const uint32_t cnt = f < FB ? fcount[(size_t)r * FB + f] : 0u;
                 uint32_t tot;
-                const uint32_t at = carry + block_exclusive_scan(cnt, tot);
+                const uint32_t at = carry + block_exclusive_scan<FINE_TPB>(cnt, tot);
                 if (f < FB) rstart[f] = at;
                 carry += tot;
             }
@@ -521,11 +526,11 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_fine(const uint32_t *__restri
         // bucket (one chunk: at once; several: claimed per bucket)
         {
             uint32_t carry = 0;
-            for (uint32_t b0 = 0; b0 < FB; b0 += SORT_TPB) {
+            for (uint32_t b0 = 0; b0 < FB; b0 += FINE_TPB) {
                 const uint32_t f = b0 + tid;
                 const uint32_t cnt = f < FB ? lcur[f] : 0u;
                 uint32_t tot;
-                const uint32_t lb = carry + block_exclusive_scan(cnt, tot);
+                const uint32_t lb = carry + block_exclusive_scan<FINE_TPB>(cnt, tot);
                 if (f < FB) {
                     uint32_t at = rstart[f];
                     if (!single && cnt) at += atomicAdd(&fclaim[(size_t)r * FB + f], cnt);
@@ -538,9 +543,9 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_fine(const uint32_t *__restri
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < FINE_PER_THREAD; q++)
-            if (tid + (uint32_t)q * SORT_TPB < m) stage[atomicAdd(&lcur[v[q].x & mask], 1u)] = v[q];
+            if (tid + (uint32_t)q * FINE_TPB < m) stage[atomicAdd(&lcur[v[q].x & mask], 1u)] = v[q];
         __syncthreads();
-        for (uint32_t e = tid; e < m; e += SORT_TPB) {
+        for (uint32_t e = tid; e < m; e += FINE_TPB) {
             const uint2 kv = stage[e];
             const uint32_t dest = e + dl[kv.x & mask];
             skeys[dest] = kv.x;
@@ -1016,7 +1021,7 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     // counts and claims, and the list of extra chunks
     const size_t counters = 3 * (size_t(4) << geo.fine_bits);
     const uint32_t cap =
-        (uint32_t)std::min<size_t>(FINE_CAP, (LDS_LIMIT - 256 - counters) / sizeof(uint2) / SORT_TPB * SORT_TPB);
+        (uint32_t)std::min<size_t>(FINE_CAP, (LDS_LIMIT - 256 - counters) / sizeof(uint2) / FINE_TPB * FINE_TPB);
     const size_t xmax = max_pairs / cap + 1;
     const uint32_t xgroups = (uint32_t)std::min<size_t>(xmax, 4 * (size_t)ctx.cu_count);
     uint32_t *fcount = fine ? ctx.d<uint32_t>("msm.fcount", (size_t)rows << geo.fine_bits) : nullptr;
@@ -1066,9 +1071,9 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     if (fine) {
         ctx.yield_point(at(0.1));
         const dim3 fg(rows + xgroups);
-        hipLaunchKernelGGL(k_sort_fine_count, fg, dim3(SORT_TPB), size_t(4) << geo.fine_bits, st, keys_c, geo, cap,
+        hipLaunchKernelGGL(k_sort_fine_count, fg, dim3(FINE_TPB), size_t(4) << geo.fine_bits, st, keys_c, geo, cap,
                            rows, rowbase, rowtot, plan, extra, fcount);
-        hipLaunchKernelGGL(k_sort_fine, fg, dim3(SORT_TPB), counters + (size_t)cap * sizeof(uint2), st, keys_c,
+        hipLaunchKernelGGL(k_sort_fine, fg, dim3(FINE_TPB), counters + (size_t)cap * sizeof(uint2), st, keys_c,
                            vals_c, geo, cap, rows, rowbase, rowtot, plan, extra, fcount, fclaim, skeys, entries,
                            off);
     }
