@@ -1182,7 +1182,10 @@ int apg_engine_submit_shard(apg_engine *e, apg_srs *srs, const void *columns_dev
         job->iso_ms = shard_time_estimate_ms(num_vars, k);
         const Transcript t0 = seed ? seed->t : Transcript();
         const Fe *cols = static_cast<const Fe *>(columns_dev);
+        Engine *eng = e->e.get();
+        const double work = job->work;
         job->fn = [=](Context &c, Job &j) {
+            c.low_acc_prio = eng->small_work() > 0 && work > eng->small_work();
             Transcript t = t0;
             const int d = degree;
             j.commitments.assign(64 * k, 0);

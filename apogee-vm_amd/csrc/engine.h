@@ -2,6 +2,7 @@
 // arenas, admission against an HBM budget (FIFO or shortest-first, jobqueue.h),
 // and per-job timing.
 #pragma once
+#include <atomic>
 #include <memory>
 #include <vector>
 
@@ -24,7 +25,13 @@ class Engine {
     int poll(uint64_t id) { return sched_->poll(id); }
     EngineStats stats() { return sched_->stats(); }
     int workers() const { return (int)ctxs_.size(); }
-    void set_policy(Policy p, int reserve, double small_work) { sched_->set_policy(p, reserve, small_work); }
+    void set_policy(Policy p, int reserve, double small_work) {
+        sched_->set_policy(p, reserve, small_work);
+        small_work_ = small_work;
+    }
+    // jobs above this size (scalars) are long: their bucket accumulation
+    // yields issue slots to the other kernels (0: no such jobs)
+    double small_work() const { return small_work_; }
     // fast-lane jobs count for preemption only when the lane shares its CUs
     void set_preemption(double ratio, Preempt mode = Preempt::kSize) {
         sched_->set_preemption(ratio, !lane_split_, mode);
@@ -39,6 +46,7 @@ class Engine {
    private:
     int device_;
     bool lane_split_ = false;  // the fast lane runs on CUs of its own
+    std::atomic<double> small_work_{0.0};
     Stats kstats_;  // written under the scheduler's lock
     std::vector<std::unique_ptr<Context>> ctxs_;
     std::unique_ptr<Scheduler<Context>> sched_;

@@ -213,6 +213,7 @@ __device__ __forceinline__ void load_block_scalars(const Fe *fr, const uint32_t 
 template <int CT, int WT>
 __global__ void __launch_bounds__(SORT_TPB) k_sort_count(const Fe *__restrict__ fr, const uint32_t *__restrict__ u32,
                                                        SortGeom g, uint32_t *__restrict__ counts) {
+    raise_wave_priority();
     __shared__ uint32_t hist[MAX_COARSE];
     const uint32_t col = blockIdx.x / g.nbc, b = blockIdx.x - col * g.nbc;
     for (uint32_t i = threadIdx.x; i < g.CB; i += SORT_TPB) hist[i] = 0;
@@ -238,6 +239,7 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_count(const Fe *__restrict__ 
 __global__ void __launch_bounds__(SORT_TPB) k_sort_rowscan(uint32_t *__restrict__ counts, uint32_t nbc,
                                                          uint32_t *__restrict__ rowtot, int fine_bits, uint32_t cap,
                                                          uint32_t *__restrict__ fcount, uint32_t *__restrict__ fclaim) {
+    raise_wave_priority();
     uint32_t *row = counts + (size_t)blockIdx.x * nbc;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nbc; base += SORT_TPB) {
@@ -272,6 +274,7 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_base(const uint32_t *__restri
                                                       uint32_t *__restrict__ rowbase, uint32_t *__restrict__ off,
                                                       uint2 *__restrict__ extra, MsmPlan *__restrict__ plan,
                                                       uint32_t *__restrict__ total_host) {
+    raise_wave_priority();
     uint32_t carry = 0, xcarry = 0;
     for (uint32_t base = 0; base < rows; base += SORT_TPB) {
         const uint32_t r = base + threadIdx.x;
@@ -323,6 +326,7 @@ __global__ void __launch_bounds__(SORT_TPB) k_sort_scatter(const Fe *__restrict_
                                                          const uint32_t *__restrict__ rowtot,
                                                          const uint32_t *__restrict__ rowbase,
                                                          uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    raise_wave_priority();
     __shared__ uint2 stage[SCATTER_CAP];
     __shared__ uint32_t cnt[MAX_COARSE], lbase[MAX_COARSE], gbase[MAX_COARSE], cur[MAX_COARSE];
     __shared__ uint32_t range_count;
@@ -432,6 +436,7 @@ __global__ void __launch_bounds__(FINE_TPB) k_sort_fine_count(const uint32_t *__
                                                             const MsmPlan *__restrict__ plan,
                                                             const uint2 *__restrict__ extra,
                                                             uint32_t *__restrict__ fcount) {
+    raise_wave_priority();
     extern __shared__ uint32_t sm[];
     const uint32_t FB = 1u << g.fine_bits, mask = FB - 1;
     fine_items(rows, plan, extra, [&](uint32_t r, uint32_t chunk) {
@@ -468,6 +473,7 @@ __global__ void __launch_bounds__(FINE_TPB) k_sort_fine(const uint32_t *__restri
                                                       const uint32_t *__restrict__ fcount,
                                                       uint32_t *__restrict__ fclaim, uint32_t *__restrict__ skeys,
                                                       uint32_t *__restrict__ entries, uint32_t *__restrict__ off) {
+    raise_wave_priority();
     extern __shared__ uint32_t sm[];
     const uint32_t FB = 1u << g.fine_bits, mask = FB - 1;
     uint32_t *rstart = sm;                                 // the row's bucket starts (several chunks)
@@ -583,7 +589,8 @@ __global__ void __launch_bounds__(TPB) k_accumulate(const uint32_t *__restrict__
                                                   const uint32_t *__restrict__ entries,
                                                   const MsmPlan *__restrict__ plan, const Affine *__restrict__ bases,
                                                   Xyzz29 *__restrict__ buckets, Xyzz29 *__restrict__ part,
-                                                  uint32_t t0) {
+                                                  uint32_t t0, int raise_prio) {
+    if (raise_prio) raise_wave_priority();
     const uint32_t total = plan->total, L = plan->L;
     const uint32_t t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
     if ((size_t)t * L >= total) return;
@@ -656,6 +663,7 @@ __global__ void __launch_bounds__(TPB, 2) k_bucket_light(const uint32_t *__restr
                                                        const MsmPlan *__restrict__ plan,
                                                        const Xyzz29 *__restrict__ part,
                                                        Xyzz29 *__restrict__ buckets) {
+    raise_wave_priority();
     const uint32_t L = plan->L;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
@@ -695,6 +703,7 @@ constexpr uint32_t PIECE = 4 * TPB;
 // (heavy index, piece within the bucket).
 __global__ void k_classify(const uint32_t *__restrict__ off, uint32_t nkeys, MsmPlan *mplan,
                            uint4 *__restrict__ heavy, uint2 *__restrict__ piece_of) {
+    raise_wave_priority();
     const uint32_t L = mplan->L;
     HeavyPlan *plan = &mplan->heavy;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -727,6 +736,7 @@ __global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restr
                                                        const uint4 *__restrict__ heavy,
                                                        const uint2 *__restrict__ piece_of,
                                                        Xyzz29 *__restrict__ piece_sums) {
+    raise_wave_priority();
     __shared__ Xyzz29 lds[TPB / 64];
     const uint32_t L = mplan->L;
     const uint32_t np = mplan->heavy.npieces, tid = threadIdx.x;
@@ -746,6 +756,7 @@ __global__ void __launch_bounds__(TPB, 2) k_heavy_final(const MsmPlan *mplan,
                                                       const uint4 *__restrict__ heavy,
                                                       const Xyzz29 *__restrict__ piece_sums,
                                                       Xyzz29 *__restrict__ buckets) {
+    raise_wave_priority();
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= mplan->heavy.nheavy) return;
     const uint4 e = heavy[h];
@@ -763,6 +774,7 @@ constexpr int RGS = 32;  // threads per row / column sum
 __global__ void __launch_bounds__(TPB, 2) k_reduce_rows(const Xyzz29 *__restrict__ S, uint32_t R,
                                                       uint32_t nprob, Xyzz29 *__restrict__ Z,
                                                       Xyzz29 *__restrict__ U) {
+    raise_wave_priority();
     const size_t gt = (size_t)blockIdx.x * TPB + threadIdx.x;
     const uint32_t per = R + 256;
     const size_t o = gt / RGS;
@@ -794,6 +806,7 @@ __global__ void __launch_bounds__(TPB, 2) k_reduce_rows(const Xyzz29 *__restrict
 __global__ void __launch_bounds__(TPB) k_weighted(const Xyzz29 *__restrict__ a_src, uint32_t ma,
                                                 uint32_t na, const Xyzz29 *__restrict__ b_src,
                                                 uint32_t mb, Xyzz *__restrict__ out) {
+    raise_wave_priority();
     __shared__ Xyzz29 lds[TPB];
     const uint32_t tid = threadIdx.x;
     const bool first = blockIdx.x < na;
@@ -1097,7 +1110,7 @@ void msm_enqueue_group(Context &ctx, const BaseTable &tab, size_t n, GroupRun &g
     for (size_t t0 = 0; t0 < slice_cap; t0 += chunk) {
         if (t0 > 0) ctx.yield_point(at(0.18 + 0.7 * (double)t0 / (double)slice_cap));
         hipLaunchKernelGGL(k_accumulate, dim3(grid_for(std::min(chunk, slice_cap - t0), TPB)), dim3(TPB), 0, st,
-                           skeys, entries, plan, tab.points, buckets, part, (uint32_t)t0);
+                           skeys, entries, plan, tab.points, buckets, part, (uint32_t)t0, ctx.low_acc_prio ? 0 : 1);
         HIP_CHECK(hipGetLastError());
     }
     if (g.ev1) HIP_CHECK(hipEventRecord(g.ev1, st));

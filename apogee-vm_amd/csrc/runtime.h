@@ -118,6 +118,9 @@ struct Context {
     std::atomic<size_t> held_bytes{0};
     int cu_count = 256;
     bool p2_rc_ready = false;  // the Poseidon2 constants sit in the arena ("p2.rc")
+    // this job's bucket accumulation runs below other kernels' waves (a long
+    // job of a served stream, set per job by the engine)
+    bool low_acc_prio = false;
     // Phase-boundary preemption, bound by the shard engine to its scheduler
     // (jobqueue.h); empty for plain contexts, where yield_point does nothing.
     std::function<bool()> yield_on, yield_wanted;
@@ -227,6 +230,12 @@ inline void context_release(Context &c) {
     c.stream = c.side = nullptr;
     c.ev_fork = c.ev_join = nullptr;
 }
+
+// Wave priority of the shard path's kernels: 1, above the bucket
+// accumulation of a long job in a served stream (0), so that when both have
+// waves ready on a SIMD a short job's kernels issue first instead of getting
+// the long job's issue share.
+__device__ __forceinline__ void raise_wave_priority() { __builtin_amdgcn_s_setprio(1); }
 
 inline unsigned grid_for(size_t n, unsigned block) {
     return (unsigned)((n + block - 1) / block);

@@ -126,6 +126,7 @@ template <int KT, bool FULL, bool LIFT>
 __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, int fold, F29 r29_arg,
                                                       EqArgs eq, Fe *partials, unsigned *counter,
                                                       Fe *out, unsigned *done, const Fe *rho_dev) {
+    raise_wave_priority();
     constexpr int NN = FULL ? KT + 1 : KT;
     const F29 r29 = rho_dev ? f29_from_mont32<Fr>(*rho_dev) : r29_arg;
     Fe acc[NN];
@@ -204,6 +205,7 @@ __device__ __forceinline__ Fe fold_twice(const Fe *src, size_t e, const F29 (&w)
 template <int FD>
 __global__ void __launch_bounds__(RTPB) k_round_pair(TablePtrs tp, size_t m, PairWeights pw, EqArgs eq,
                                                    Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
+    raise_wave_priority();
     constexpr int KT = 2;
     Fe accj[2], accl[3];
 #pragma unroll
@@ -280,6 +282,7 @@ template <int NN, bool LIFT>
 __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateArgs gate, size_t m,
                                                    int fold, Fe r, EqArgs eq, int full, int nn,
                                                    Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
+    raise_wave_priority();
     Fe acc[NN];
 #pragma unroll
     for (int x = 0; x < NN; x++) acc[x] = f_zero<Fr>();
@@ -341,6 +344,7 @@ __device__ __forceinline__ F29 combine_group(const Fe *__restrict__ cols, size_t
 
 template <int OUTS>
 __global__ void __launch_bounds__(256) k_combine(const Fe *__restrict__ cols, size_t n, int K, CombineArgs a) {
+    raise_wave_priority();
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     F29 s{};

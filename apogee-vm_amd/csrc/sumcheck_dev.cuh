@@ -147,6 +147,7 @@ __device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *coun
 //   B_s (h0 <= s <= n): 2^(n-s) entries over variables s..n-1, at B + (s-h0) 2^(n-h0).
 __global__ void k_eq_suffix(const Fe *__restrict__ r, int n, int h0, Fe *__restrict__ A,
                             Fe *__restrict__ B) {
+    raise_wave_priority();
     const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t astride = size_t(1) << (h0 - 1), bstride = size_t(1) << (n - h0);
     const size_t asz = (size_t)(h0 - 1) * astride, bsz = (size_t)(n - h0 + 1) * bstride;
