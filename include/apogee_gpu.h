@@ -458,7 +458,10 @@ int apg_engine_stats(apg_engine *e, apg_engine_info *out);
  * estimate fits) or APG_POLICY_LONGEST (largest first among those: a batch
  * submitted at once starts its long shards early). With reserve > 0 the first `reserve` workers are a fast
  * lane that takes only jobs of at most small_scalars rows x columns; the
- * other workers leave such jobs to an idle fast-lane worker. No counterpart
+ * other workers leave such jobs to an idle fast-lane worker. With
+ * small_scalars > 0, jobs above it run their MSM bucket accumulation at a
+ * lower wave priority than every other kernel of the shard path, so a short
+ * job's kernels issue first beside a long one. No counterpart
  * in the reference, whose workers claim shards in emission order
  * (crates/prover/src/streaming.rs:601-680). */
 enum { APG_POLICY_FIFO = 0, APG_POLICY_SHORTEST = 1, APG_POLICY_LONGEST = 2 };
