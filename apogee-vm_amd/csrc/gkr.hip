@@ -254,7 +254,7 @@ __device__ __forceinline__ void fill_cached(const DProg &p, const Reader &rd) {
 template <bool FOLD>
 __global__ void __launch_bounds__(RTPB) k_gkr_round(DProg p, const Fe *const *src, Fe *const *dst, uint32_t ntab,
                                                   uint32_t nlower, size_t m, F29 r, EqArgs eq, const Fe *rho, int n,
-                                                  int b, Fe *slots, Fe *partials, unsigned *counter, Fe *out) {
+                                                  int b, Fe *slots, Fe *partials, Fe *out) {
     Fe acc[3];
 #pragma unroll
     for (int x = 0; x < 3; x++) acc[x] = f_zero<Fr>();
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round(DProg p, const Fe *const *sr
             else acc[2] = f_add<Fr>(acc[2], es);
         }
     }
-    reduce_nodes<3>(acc, 3, partials, counter, out);
+    reduce_nodes<3>(acc, 3, partials, out);
 }
 
 // The fold of a round's tables alone: work item w folds pair w % m of table
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(RTPB) k_gkr_fold(const Fe *const *src, Fe *con
 // sums are the same, the node sums being linear in the gates.
 __global__ void __launch_bounds__(RTPB) k_gkr_round_split(DProg p, const Fe *const *tab, uint32_t nlower, size_t m,
                                                         EqArgs eq, const Fe *rho, int n, int b, Fe *slots,
-                                                        Fe *partials, unsigned *counter, Fe *out) {
+                                                        Fe *partials, Fe *out) {
     Fe acc[3];
 #pragma unroll
     for (int x = 0; x < 3; x++) acc[x] = f_zero<Fr>();
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(RTPB) k_gkr_round_split(DProg p, const Fe *con
             else acc[2] = f_add<Fr>(acc[2], es);
         }
     }
-    reduce_nodes<3>(acc, 3, partials, counter, out);
+    reduce_nodes<3>(acc, 3, partials, out);
 }
 
 // gkr::produce: the producing gates at every row y
@@ -591,9 +591,9 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
     const size_t nslots = (size_t)s.prog.nvirt + l.ncached;
     Fe *slots = ctx.d<Fe>("gkr.slots", std::max<size_t>(nslots * MAX_RBLOCKS * RTPB, 1));
     Fe *partials = ctx.d<Fe>("gkr.partials", (size_t)MAX_RBLOCKS * 3);
-    unsigned *counter = ctx.zeroed<unsigned>("gkr.counter", 64);
-    // the round kernels' last block writes the node sums straight into
-    // pinned host memory; the stream sync after each round makes them visible
+    // the round kernel (one block) or its finishing pass writes the node sums
+    // straight into pinned host memory; the stream sync after each round
+    // makes them visible
     Fe *hvals = ctx.h<Fe>("gkr.hvals", 3);
     Fe *hvals_dev = nullptr;
     HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void **>(&hvals_dev), hvals, 0));
@@ -648,13 +648,16 @@ void gkr_prove_sumcheck_device(Context &ctx, const GkrList &l, const Fe *weights
             const size_t items = m * s.prog.ngates;
             const dim3 gs((unsigned)std::min(cap, std::max<size_t>(1, (items + RTPB - 1) / RTPB)));
             hipLaunchKernelGGL(k_gkr_round_split, gs, b, 0, st, s.prog, fold ? dst : src, (uint32_t)ncols, m, eq,
-                               rho_d, n, j, slots, partials, counter, hvals_dev);
+                               rho_d, n, j, slots, partials, hvals_dev);
+            launch_finish<3>(st, gs.x, 3, partials, hvals_dev, nullptr);
         } else if (fold) {
             hipLaunchKernelGGL(k_gkr_round<true>, g, b, 0, st, s.prog, src, dst, (uint32_t)nt, (uint32_t)ncols, m,
-                               rho29, eq, rho_d, n, j, slots, partials, counter, hvals_dev);
+                               rho29, eq, rho_d, n, j, slots, partials, hvals_dev);
+            launch_finish<3>(st, g.x, 3, partials, hvals_dev, nullptr);
         } else {
             hipLaunchKernelGGL(k_gkr_round<false>, g, b, 0, st, s.prog, src, dst, (uint32_t)nt, (uint32_t)ncols, m,
-                               rho29, eq, rho_d, n, j, slots, partials, counter, hvals_dev);
+                               rho29, eq, rho_d, n, j, slots, partials, hvals_dev);
+            launch_finish<3>(st, g.x, 3, partials, hvals_dev, nullptr);
         }
         HIP_CHECK(hipGetLastError());
         if (ctx.profiling) HIP_CHECK(hipEventRecord(ev1, st));

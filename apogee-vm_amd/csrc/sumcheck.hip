@@ -28,8 +28,8 @@
 //    challenge of round j, kept at three points and interpolated by the host
 //    once the challenge is drawn; the next launch binds both challenges.
 //  * Node sums reduce by wave shuffles, then across the block's waves in LDS,
-//    then (more than one block) through per-block partials summed by the last
-//    block to finish.
+//    then (more than one block) through per-block rows of partials that a
+//    one-block pass after the kernel adds up (no per-block fences).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -124,8 +124,8 @@ template <int KT, bool FULL, bool LIFT>
 // rho_dev (graph-captured rounds): the challenge in device memory, written by
 // the previous round's transcript step, replacing r29.
 __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, int fold, F29 r29_arg,
-                                                      EqArgs eq, Fe *partials, unsigned *counter,
-                                                      Fe *out, unsigned *done, const Fe *rho_dev) {
+                                                      EqArgs eq, Fe *partials, Fe *out, unsigned *done,
+                                                      const Fe *rho_dev) {
     raise_wave_priority();
     constexpr int NN = FULL ? KT + 1 : KT;
     const F29 r29 = rho_dev ? f29_from_mont32<Fr>(*rho_dev) : r29_arg;
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(RTPB) k_round_product(TablePtrs tp, size_t m, 
             }
         }
     }
-    reduce_nodes<NN>(acc, NN, partials, counter, out, done);
+    reduce_nodes<NN>(acc, NN, partials, out, done);
 }
 
 // Entry e of a table bound by two more variables, from its four entries
@@ -204,7 +204,7 @@ __device__ __forceinline__ Fe fold_twice(const Fe *src, size_t e, const F29 (&w)
 // 2^-10, and 2^-5 more when E_(j+1) = lo hi).
 template <int FD>
 __global__ void __launch_bounds__(RTPB) k_round_pair(TablePtrs tp, size_t m, PairWeights pw, EqArgs eq,
-                                                   Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
+                                                   Fe *partials, Fe *out, unsigned *done) {
     raise_wave_priority();
     constexpr int KT = 2;
     Fe accj[2], accl[3];
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(RTPB) k_round_pair(TablePtrs tp, size_t m, Pai
         if (lane < 2) lds[w][2 + 2 * y + lane] = s;
     }
     __syncthreads();
-    finish_nodes<PAIR_OUTS>(lds, PAIR_OUTS, partials, counter, out, done);
+    finish_nodes<PAIR_OUTS>(lds, PAIR_OUTS, partials, out, done);
 }
 
 // Any gate: terms read from the kernel argument, table lines kept in a local
@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(RTPB) k_round_pair(TablePtrs tp, size_t m, Pai
 template <int NN, bool LIFT>
 __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateArgs gate, size_t m,
                                                    int fold, Fe r, EqArgs eq, int full, int nn,
-                                                   Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
+                                                   Fe *partials, Fe *out, unsigned *done) {
     raise_wave_priority();
     Fe acc[NN];
 #pragma unroll
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(RTPB) k_round_gate(TablePtrs tp, int k, GateAr
             }
         }
     }
-    reduce_nodes<NN>(acc, nn, partials, counter, out, done);
+    reduce_nodes<NN>(acc, nn, partials, out, done);
 }
 
 // out_o = scale_o * sum_c w_c col_c, o < OUTS (the shard's tables are
@@ -534,10 +534,12 @@ void combine_columns_device(Context &ctx, const Fe *cols, size_t n, int K, const
 }
 
 // Blocks for m pairs: one pair per thread up to per_cu blocks per CU, beyond
-// that a grid-stride loop. The product rounds take one block per CU: on the
-// 2^20 x 8 shard's rounds 4, 2 and 1 blocks per CU measured 1.47, 1.39 and
-// 1.33 ms (fewer partial sums for the last block to add). The heavier gate
-// rounds keep 4, as the GKR layer's rounds, which lose 16% at one.
+// that a grid-stride loop. The product rounds take two blocks per CU: with
+// the separate finishing pass (no per-block fences) the 2^22 x 2 sumcheck
+// measured 1.26, 1.16 and 1.18 ms at 1, 2 and 4 blocks per CU, 2^20 x 2
+// alike at all three. The heavier gate rounds keep 4, as the GKR layer's
+// rounds, which lose 16% at one.
+constexpr int PRODUCT_BLOCKS_PER_CU = 2;
 static unsigned round_blocks(const Context &ctx, size_t m, int per_cu) {
     const size_t cap = std::min<size_t>(MAX_RBLOCKS, (size_t)per_cu * ctx.cu_count);
     return (unsigned)std::min(cap, std::max<size_t>(1, (m + RTPB - 1) / RTPB));
@@ -545,37 +547,39 @@ static unsigned round_blocks(const Context &ctx, size_t m, int per_cu) {
 
 template <int KT, bool LIFT>
 static void launch_product(Context &ctx, bool full, const TablePtrs &tp, size_t m, bool fold,
-                           const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out,
-                           unsigned *done) {
-    const dim3 g(round_blocks(ctx, m, 1)), b(RTPB);
+                           const Fe &r, const EqArgs &eq, Fe *partials, Fe *out, unsigned *done) {
+    const dim3 g(round_blocks(ctx, m, PRODUCT_BLOCKS_PER_CU)), b(RTPB);
     const F29 r29 = f29_from_mont32<Fr>(r);  // rho R' mod r
-    if (full)
+    if (full) {
         hipLaunchKernelGGL((k_round_product<KT, true, LIFT>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
-                           partials, counter, out, done, (const Fe *)nullptr);
-    else
+                           partials, out, done, (const Fe *)nullptr);
+        launch_finish<KT + 1>(ctx.stream, g.x, KT + 1, partials, out, done);
+    } else {
         hipLaunchKernelGGL((k_round_product<KT, false, LIFT>), g, b, 0, ctx.stream, tp, m, (int)fold, r29, eq,
-                           partials, counter, out, done, (const Fe *)nullptr);
+                           partials, out, done, (const Fe *)nullptr);
+        launch_finish<KT>(ctx.stream, g.x, KT, partials, out, done);
+    }
 }
 
 template <int KT>
 static void launch_product(Context &ctx, bool lift, bool full, const TablePtrs &tp, size_t m, bool fold,
-                           const Fe &r, const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out,
-                           unsigned *done) {
+                           const Fe &r, const EqArgs &eq, Fe *partials, Fe *out, unsigned *done) {
     if (lift)
-        launch_product<KT, true>(ctx, full, tp, m, fold, r, eq, partials, counter, out, done);
+        launch_product<KT, true>(ctx, full, tp, m, fold, r, eq, partials, out, done);
     else
-        launch_product<KT, false>(ctx, full, tp, m, fold, r, eq, partials, counter, out, done);
+        launch_product<KT, false>(ctx, full, tp, m, fold, r, eq, partials, out, done);
 }
 
 // A paired-round launch (k_round_pair) over m pairs of T^j, folding with the
 // fd challenges in pw.
 static void launch_pair(Context &ctx, int fd, const TablePtrs &tp, size_t m, const PairWeights &pw,
-                        const EqArgs &eq, Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
-    const dim3 g(round_blocks(ctx, m, 1)), b(RTPB);
+                        const EqArgs &eq, Fe *partials, Fe *out, unsigned *done) {
+    const dim3 g(round_blocks(ctx, m, PRODUCT_BLOCKS_PER_CU)), b(RTPB);
     if (fd == 1)
-        hipLaunchKernelGGL(k_round_pair<1>, g, b, 0, ctx.stream, tp, m, pw, eq, partials, counter, out, done);
+        hipLaunchKernelGGL(k_round_pair<1>, g, b, 0, ctx.stream, tp, m, pw, eq, partials, out, done);
     else
-        hipLaunchKernelGGL(k_round_pair<2>, g, b, 0, ctx.stream, tp, m, pw, eq, partials, counter, out, done);
+        hipLaunchKernelGGL(k_round_pair<2>, g, b, 0, ctx.stream, tp, m, pw, eq, partials, out, done);
+    launch_finish<PAIR_OUTS>(ctx.stream, g.x, PAIR_OUTS, partials, out, done);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -583,12 +587,12 @@ static void launch_pair(Context &ctx, int fd, const TablePtrs &tp, size_t m, con
 // the completion flag the round sets after its node sums (or null)
 static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bool full, int D, bool lift,
                          const TablePtrs &tp, size_t m, bool fold, const Fe &r, const EqArgs &eq,
-                         Fe *partials, unsigned *counter, Fe *out, unsigned *done) {
+                         Fe *partials, Fe *out, unsigned *done) {
     if (product) {
         switch (k) {
-            case 1: launch_product<1>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out, done); return;
-            case 2: launch_product<2>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out, done); return;
-            case 3: launch_product<3>(ctx, lift, full, tp, m, fold, r, eq, partials, counter, out, done); return;
+            case 1: launch_product<1>(ctx, lift, full, tp, m, fold, r, eq, partials, out, done); return;
+            case 2: launch_product<2>(ctx, lift, full, tp, m, fold, r, eq, partials, out, done); return;
+            case 3: launch_product<3>(ctx, lift, full, tp, m, fold, r, eq, partials, out, done); return;
             default: break;
         }
     }
@@ -609,10 +613,11 @@ static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bo
     do {                                                                                                      \
         if (lift)                                                                                             \
             hipLaunchKernelGGL((k_round_gate<NN, true>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r, eq, \
-                               (int)full, nn, partials, counter, out, done);                                  \
+                               (int)full, nn, partials, out, done);                                           \
         else                                                                                                  \
             hipLaunchKernelGGL((k_round_gate<NN, false>), g, b, 0, ctx.stream, tp, k, ga, m, (int)fold, r,    \
-                               eq, (int)full, nn, partials, counter, out, done);                              \
+                               eq, (int)full, nn, partials, out, done);                                       \
+        launch_finish<NN>(ctx.stream, g.x, nn, partials, out, done);                                          \
     } while (0)
     switch (D + 1) {
         case 2: APG_GATE(2); break;
@@ -624,11 +629,10 @@ static void launch_round(Context &ctx, const Gate &gate, int k, bool product, bo
 
 // Device scratch of one proof, carved from one region in a fixed order (each
 // piece 256-byte aligned): the eq point, the suffix-eq tables, two ping-pong
-// fold pools, the round kernels' per-block partial sums and their counter,
-// and the node sums of a round (read back after each round).
+// fold pools, the round kernels' per-block partial sums, and the node sums
+// of a round (read back after each round).
 struct SumcheckScratch {
     Fe *r, *eqA, *eqB, *poolA, *poolB, *partials, *nodes;
-    unsigned *counter;
     // completion flag of a round (pinned, coherent; its device mapping), or null
     unsigned *done_host = nullptr, *done_dev = nullptr;
 };
@@ -654,7 +658,7 @@ static void wait_round(Context &ctx, volatile unsigned *done) {
 }
 
 struct ScratchLayout {
-    size_t off[8];
+    size_t off[7];
     size_t total;
 };
 
@@ -663,7 +667,7 @@ static ScratchLayout sumcheck_layout(int k, int n) {
     const int h0 = std::max(1, std::min(n, (n + 1) / 2));
     const size_t astride = size_t(1) << (h0 - 1), bstride = size_t(1) << (n - h0);
     const size_t asz = (size_t)(h0 - 1) * astride, bsz = (size_t)(n - h0 + 1) * bstride;
-    const size_t bytes[8] = {
+    const size_t bytes[7] = {
         (size_t)std::max(n, 1) * sizeof(Fe),
         std::max<size_t>(asz, 1) * sizeof(Fe),
         std::max<size_t>(bsz, 1) * sizeof(Fe),
@@ -671,11 +675,10 @@ static ScratchLayout sumcheck_layout(int k, int n) {
         (size_t)k * std::max<size_t>(len0 / 4, 1) * sizeof(Fe),
         (size_t)MAX_RBLOCKS * NODE_SLOTS * sizeof(Fe),
         NODE_SLOTS * sizeof(Fe),
-        64 * sizeof(unsigned),
     };
     ScratchLayout l;
     size_t at = 0;
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < 7; i++) {
         l.off[i] = at;
         at += (bytes[i] + 255) & ~size_t(255);
     }
@@ -692,7 +695,6 @@ static SumcheckScratch sumcheck_carve(uint8_t *base, const ScratchLayout &l) {
     s.poolB = reinterpret_cast<Fe *>(base + l.off[4]);
     s.partials = reinterpret_cast<Fe *>(base + l.off[5]);
     s.nodes = reinterpret_cast<Fe *>(base + l.off[6]);
-    s.counter = reinterpret_cast<unsigned *>(base + l.off[7]);
     return s;
 }
 
@@ -728,9 +730,6 @@ void sumcheck_prove_device(Context &ctx, const Gate &gate, Fe *const *tables, in
     }
     const ScratchLayout l = sumcheck_layout(k, n);
     SumcheckScratch sc = sumcheck_carve(ctx.d<uint8_t>("sc.scratch", l.total), l);
-    // the counter lives apart from the carved region, zero once and left zero
-    // by the round kernels
-    sc.counter = ctx.zeroed<unsigned>("sc.counter", 64);
     // node sums and the round's completion flag in coherent pinned memory:
     // the host waits on the flag, not on the stream
     Fe *hvals = ctx.hc<Fe>("sc.hvals", NODE_SLOTS + 1);
@@ -749,11 +748,6 @@ static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
 constexpr int NOALLOC_MAX_DEVICES = 64;
 std::mutex g_noalloc_mu[NOALLOC_MAX_DEVICES];
 void *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
-
-__global__ void k_zero_words(unsigned *p, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
 }  // namespace
 
 size_t sumcheck_noalloc_limit() { return NOALLOC_SCRATCH_BYTES; }
@@ -786,10 +780,9 @@ void sumcheck_prove_device_noalloc(Context &ctx, const Gate &gate, Fe *const *ta
                                std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte limit");
     // one call at a time per device: the calls share the device's array
     std::lock_guard<std::mutex> lk(g_noalloc_mu[ctx.device]);
+    // the array keeps the last call's contents; every round writes what it
+    // reads (partial rows, node sums) before reading it
     const SumcheckScratch sc = sumcheck_carve(static_cast<uint8_t *>(g_noalloc_ptr[ctx.device]), l);
-    // the array keeps the last call's contents: clear the counter
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, ctx.stream, sc.counter, size_t(64));
-    HIP_CHECK(hipGetLastError());
     Fe nodes_host[NODE_SLOTS];
     sumcheck_rounds(ctx, sc, nodes_host, false, false, gate, tables, kinds, k, n, eq_point, round_fn, rounds_out,
                     point_out, finals_out);
@@ -995,7 +988,6 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
     for (int t = 0; t < k; t++) cur[t] = tables[t];
 
     Fe *partials = sc.partials;
-    unsigned *counter = sc.counter;
     Fe *hvals_dev = sc.nodes;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1120,7 +1112,7 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
             pw.c1 = f29_from_mont32<Fr>(eq_point[j + 1]);
             if (timing) HIP_CHECK(hipEventRecord(ev0, st));
             if (sc.done_host) __atomic_store_n(sc.done_host, 0u, __ATOMIC_RELEASE);
-            launch_pair(ctx, npend, tp, m, pw, eq, partials, counter, hvals_dev, sc.done_dev);
+            launch_pair(ctx, npend, tp, m, pw, eq, partials, hvals_dev, sc.done_dev);
             if (timing) HIP_CHECK(hipEventRecord(ev1, st));
             if (!mapped_nodes)
                 HIP_CHECK(hipMemcpyAsync(hvals, hvals_dev, PAIR_OUTS * sizeof(Fe), hipMemcpyDeviceToHost, st));
@@ -1171,7 +1163,7 @@ static void sumcheck_rounds(Context &ctx, const SumcheckScratch &sc, Fe *hvals, 
         // cleared before the launch: the previous round's kernel has set it and writes it no more
         if (sc.done_host) __atomic_store_n(sc.done_host, 0u, __ATOMIC_RELEASE);
         launch_round(ctx, gate, k, product, full, D, narrow && j <= 1, tp, m, fold, fold ? pend[0] : f_zero<Fr>(),
-                     eq, partials, counter, hvals_dev, sc.done_dev);
+                     eq, partials, hvals_dev, sc.done_dev);
         HIP_CHECK(hipGetLastError());
         if (timing) HIP_CHECK(hipEventRecord(ev1, st));
         const int nn = full ? D + 1 : D;
@@ -1316,15 +1308,19 @@ __global__ void k_graph_finals(const GraphState *st, TablePtrs tp, int k, Fe *__
 
 template <int KT>
 void launch_graph_product(Context &ctx, bool full, const TablePtrs &tp, size_t m, bool fold, const EqArgs &eq,
-                          Fe *partials, unsigned *counter, Fe *out, const Fe *rho_dev) {
-    const dim3 g(round_blocks(ctx, m, 1)), b(RTPB);
+                          Fe *partials, Fe *out, const Fe *rho_dev) {
+    const dim3 g(round_blocks(ctx, m, PRODUCT_BLOCKS_PER_CU)), b(RTPB);
     const F29 none{};
-    if (full)
+    unsigned *no_flag = nullptr;
+    if (full) {
         hipLaunchKernelGGL((k_round_product<KT, true, false>), g, b, 0, ctx.stream, tp, m, (int)fold, none, eq,
-                           partials, counter, out, (unsigned *)nullptr, rho_dev);
-    else
+                           partials, out, no_flag, rho_dev);
+        launch_finish<KT + 1>(ctx.stream, g.x, KT + 1, partials, out, no_flag);
+    } else {
         hipLaunchKernelGGL((k_round_product<KT, false, false>), g, b, 0, ctx.stream, tp, m, (int)fold, none, eq,
-                           partials, counter, out, (unsigned *)nullptr, rho_dev);
+                           partials, out, no_flag, rho_dev);
+        launch_finish<KT>(ctx.stream, g.x, KT, partials, out, no_flag);
+    }
 }
 
 }  // namespace
@@ -1339,7 +1335,6 @@ void sumcheck_prove_graph(Context &ctx, Fe *const *tables, int k, int n, const F
     const size_t len0 = size_t(1) << n;
     const ScratchLayout l = sumcheck_layout(k, n);
     SumcheckScratch sc = sumcheck_carve(ctx.d<uint8_t>("sg.scratch", l.total), l);
-    sc.counter = ctx.zeroed<unsigned>("sg.counter", 64);
     GraphState *st = ctx.d<GraphState>("sg.state", 1);
     Fe *rounds = ctx.d<Fe>("sg.rounds", (size_t)n * (d + 1));
     Fe *point = ctx.d<Fe>("sg.point", n);
@@ -1409,9 +1404,9 @@ void sumcheck_prove_graph(Context &ctx, Fe *const *tables, int k, int n, const F
                 eq.hi = sc.eqB + (size_t)(sidx - h0) * bstride;
             }
             switch (k) {
-                case 1: launch_graph_product<1>(ctx, true, tp, m, fold, eq, sc.partials, sc.counter, sc.nodes, &st->rho); break;
-                case 2: launch_graph_product<2>(ctx, true, tp, m, fold, eq, sc.partials, sc.counter, sc.nodes, &st->rho); break;
-                default: launch_graph_product<3>(ctx, true, tp, m, fold, eq, sc.partials, sc.counter, sc.nodes, &st->rho); break;
+                case 1: launch_graph_product<1>(ctx, true, tp, m, fold, eq, sc.partials, sc.nodes, &st->rho); break;
+                case 2: launch_graph_product<2>(ctx, true, tp, m, fold, eq, sc.partials, sc.nodes, &st->rho); break;
+                default: launch_graph_product<3>(ctx, true, tp, m, fold, eq, sc.partials, sc.nodes, &st->rho); break;
             }
             hipLaunchKernelGGL(k_graph_round, dim3(1), dim3(64), 0, s, st, sc.nodes, D, d, eq.lo ? scale_lo : scale,
                                sc.r, j, rounds, point, rc);

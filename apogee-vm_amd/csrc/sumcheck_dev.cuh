@@ -3,6 +3,7 @@
 // and the grid-wide reduction of per-thread node sums.
 #pragma once
 #include "field.cuh"
+#include "runtime.h"
 
 namespace apg {
 namespace {
@@ -75,15 +76,16 @@ __device__ __forceinline__ void signal_done(unsigned *done) {
     }
 }
 
-// The grid's sums of nn values, given each wave's sums in lds[wave][x]
-// (written before a barrier): the block's sums go to out[] (one block) or to
-// its row of partials, and the last block to finish adds the partials into
-// out[]. The release fence before the counter increment and the acquire
-// fence after it make the other blocks' partials visible to plain loads.
+// The block's sums of nn values, given each wave's sums in lds[wave][x]
+// (written before a barrier): into out[] (then the completion flag) when the
+// grid is one block, else into the block's row of partials (row stride NN),
+// which k_finish_nodes adds up in a launch of its own after the kernel. The
+// kernel boundary orders every row's stores before that pass's loads, so no
+// block fences its stores or counts itself done: a device-scope fence writes
+// back the XCD's L2 and waits, and with many blocks those fences queue (at 4
+// blocks per CU they made round 0 of 2^22 x 2 tables 300 us against 202).
 template <int NN>
-__device__ void finish_nodes(Fe (&lds)[RWAVES][NN], int nn, Fe *partials, unsigned *counter, Fe *out,
-                             unsigned *done) {
-    __shared__ bool is_last;
+__device__ void finish_nodes(Fe (&lds)[RWAVES][NN], int nn, Fe *partials, Fe *out, unsigned *done) {
     const int tid = threadIdx.x;
     const bool single = gridDim.x == 1;
     if (tid < nn) {
@@ -94,39 +96,13 @@ __device__ void finish_nodes(Fe (&lds)[RWAVES][NN], int nn, Fe *partials, unsign
         else
             stfe(partials, (size_t)blockIdx.x * NN + tid, s);
     }
-    if (single) {
-        signal_done(done);
-        return;
-    }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    // all outputs at once: value x summed by the tpx lanes tid / tpx == x
-    // (an aligned group inside one wave), each over every tpx-th block
-    int tpx = 64;
-    while (tpx * nn > RTPB) tpx >>= 1;
-    const int x = tid / tpx, sub = tid % tpx;
-    Fe a = f_zero<Fr>();
-    if (x < nn)
-        for (unsigned b = sub; b < gridDim.x; b += tpx) a = f_add<Fr>(a, ldfe(partials, (size_t)b * NN + x));
-    for (int off = tpx / 2; off > 0; off >>= 1) {
-        Fe o;
-#pragma unroll
-        for (int q = 0; q < 8; q++) o.l[q] = __shfl_xor(a.l[q], off, 64);
-        a = f_add<Fr>(a, o);
-    }
-    if (x < nn && sub == 0) out[x] = a;
-    if (tid == 0) *counter = 0;
-    signal_done(done);
+    if (single) signal_done(done);
 }
 
-// Sum of the first nn of NN per-thread accumulators over the grid into out[].
+// Sum of the first nn of NN per-thread accumulators over the block into
+// out[] or the block's row of partials (finish_nodes).
 template <int NN>
-__device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *counter, Fe *out,
-                             unsigned *done = nullptr) {
+__device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, Fe *out, unsigned *done = nullptr) {
     __shared__ Fe lds[RWAVES][NN];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -137,7 +113,40 @@ __device__ void reduce_nodes(Fe (&acc)[NN], int nn, Fe *partials, unsigned *coun
         }
     }
     __syncthreads();
-    finish_nodes<NN>(lds, nn, partials, counter, out, done);
+    finish_nodes<NN>(lds, nn, partials, out, done);
+}
+
+// The second pass of a multi-block reduction: the rows of `blocks` blocks'
+// partials (row stride NN) added into out[0..nn), then the completion flag.
+// One block; all outputs at once, output x by the tpx lanes tid / tpx == x
+// (an aligned group inside one wave), each over every tpx-th row.
+template <int NN>
+__global__ void __launch_bounds__(RTPB) k_finish_nodes(const Fe *__restrict__ partials, unsigned blocks, int nn,
+                                                     Fe *out, unsigned *done) {
+    raise_wave_priority();
+    const int tid = threadIdx.x;
+    int tpx = 64;
+    while (tpx * nn > RTPB) tpx >>= 1;
+    const int x = tid / tpx, sub = tid % tpx;
+    Fe a = f_zero<Fr>();
+    if (x < nn)
+        for (unsigned b = sub; b < blocks; b += tpx) a = f_add<Fr>(a, ldfe(partials, (size_t)b * NN + x));
+    for (int off = tpx / 2; off > 0; off >>= 1) {
+        Fe o;
+#pragma unroll
+        for (int q = 0; q < 8; q++) o.l[q] = __shfl_xor(a.l[q], off, 64);
+        a = f_add<Fr>(a, o);
+    }
+    if (x < nn && sub == 0) out[x] = a;
+    signal_done(done);
+}
+
+// After a round kernel of `blocks` blocks whose rows of NN partials are in
+// `partials`: the pass that adds them (a one-block grid wrote out[] itself).
+template <int NN>
+inline void launch_finish(hipStream_t st, unsigned blocks, int nn, const Fe *partials, Fe *out, unsigned *done) {
+    if (blocks <= 1) return;
+    hipLaunchKernelGGL(k_finish_nodes<NN>, dim3(1), dim3(RTPB), 0, st, partials, blocks, nn, out, done);
 }
 
 // The suffix-eq tables of one proof, as products over the variables they
