@@ -736,7 +736,7 @@ __global__ void __launch_bounds__(TPB, 2) k_heavy_pieces(const uint32_t *__restr
                                                        const uint4 *__restrict__ heavy,
                                                        const uint2 *__restrict__ piece_of,
                                                        Xyzz29 *__restrict__ piece_sums) {
-    raise_wave_priority();
+    raise_wave_priority_critical();
     __shared__ Xyzz29 lds[TPB / 64];
     const uint32_t L = mplan->L;
     const uint32_t np = mplan->heavy.npieces, tid = threadIdx.x;
@@ -756,7 +756,7 @@ __global__ void __launch_bounds__(TPB, 2) k_heavy_final(const MsmPlan *mplan,
                                                       const uint4 *__restrict__ heavy,
                                                       const Xyzz29 *__restrict__ piece_sums,
                                                       Xyzz29 *__restrict__ buckets) {
-    raise_wave_priority();
+    raise_wave_priority_critical();
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= mplan->heavy.nheavy) return;
     const uint4 e = heavy[h];

@@ -236,6 +236,9 @@ inline void context_release(Context &c) {
 // waves ready on a SIMD a short job's kernels issue first instead of getting
 // the long job's issue share.
 __device__ __forceinline__ void raise_wave_priority() { __builtin_amdgcn_s_setprio(1); }
+// Priority 2: a few waves of dependent additions on a phase's critical path
+// (the MSM's heavy-bucket trees) ahead of the wide kernel beside them.
+__device__ __forceinline__ void raise_wave_priority_critical() { __builtin_amdgcn_s_setprio(2); }
 
 inline unsigned grid_for(size_t n, unsigned block) {
     return (unsigned)((n + block - 1) / block);
