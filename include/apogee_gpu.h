@@ -376,7 +376,8 @@ int apg_sumcheck_prove_graph_dev(apg_ctx *ctx, void *const *tables_dev, size_t k
  * suffix-eq tables, so k x 2^num_vars up to 2^21 entries fit, and a
  * larger call is APG_ERR_ARG with the sizes in apg_last_error(). Calls on one
  * device share that array and are serialised by a lock; each starts from
- * what the last call left, and clears what must start at zero. Results equal
+ * what the last call left, which needs no clearing: every round writes the
+ * partial sums and node sums it reads before reading them. Results equal
  * apg_sumcheck_prove_dev's. */
 int apg_sumcheck_prove_dev_noalloc(apg_ctx *ctx, const apg_term *terms, size_t n_terms,
                                    void *const *tables_dev, size_t k, size_t num_vars,
