@@ -6,7 +6,7 @@
 //
 // Values are kept below fixed multiples of p instead of below p. With table
 // coordinates below p, every accumulator or bucket point of the phase has
-// X < 7.2p, Y < 3.3p, ZZ < 1.1p and ZZZ < 1.1p (the bounds each formula
+// X < 5.2p, Y < 3.3p, ZZ < 1.1p and ZZZ < 1.1p (the bounds each formula
 // below restates); a product of two values below 12p is below 1.1p, so the
 // subtractions add 2p, 4p or 8p above the subtrahend's bound and nothing is
 // reduced until the phase's output. The identity is ZZ = ZZZ = 0.
@@ -50,7 +50,8 @@ APG_HD bool xyzz29_is_id(const Xyzz29 &p) { return f29_is_zero<Fq, 2>(p.zz); }
 
 // dbl-2008-s-1 with ZZ = ZZZ = 1 for x, y < p (bounds in the comments):
 // U = 2y, V = U^2, W = U V, S = x V, M = 3 x^2, X3 = M^2 - 2S,
-// Y3 = M (S - X3) - W y, ZZ3 = V, ZZZ3 = W.
+// Y3 = M (S - X3) - W y, ZZ3 = V, ZZZ3 = W. X3 is M^2 - (S + S) + 4p, the
+// sum taken word by word and subtracted once (2S < 4p).
 APG_HD Xyzz29 xyzz29_mdbl(const F29 &x, const F29 &y) {
     const F29 u = f29_add(y, y);                 // < 2p
     const F29 v = f29_sqr<Fq>(u);                // < 1.03p
@@ -59,7 +60,7 @@ APG_HD Xyzz29 xyzz29_mdbl(const F29 &x, const F29 &y) {
     const F29 x2 = f29_sqr<Fq>(x);               // < 1.01p
     const F29 m = f29_add(f29_add(x2, x2), x2);  // < 3.03p
     Xyzz29 r;
-    r.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sqr<Fq>(m), s), s);  // < 5.06p
+    r.x = f29_sub<Fq, 4>(f29_sqr<Fq>(m), f29_add_words(s, s));  // < 5.06p
     // M (S - X3) - W y as one reduction: M (S - X3 + 8p) + W (2p - y)
     r.y = f29_mul2<Fq>(m, f29_sub<Fq, 8>(s, r.x), w, f29_sub<Fq, 2>(F29{}, y));  // < 1.18p
     r.zz = v;
@@ -72,7 +73,7 @@ APG_HD Xyzz29 xyzz29_mdbl(const F29 &x, const F29 &y) {
 APG_HD void xyzz29_add_affine(Xyzz29 &acc, bool &id, const F29 &x, const F29 &y) {
     const F29 u2 = f29_mul<Fq>(x, acc.zz);     // < 1.01p
     const F29 s2 = f29_mul<Fq>(y, acc.zzz);    // < 1.01p
-    const F29 pd = f29_sub<Fq, 8>(u2, acc.x);  // < 9.01p (acc.x < 7.2p)
+    const F29 pd = f29_sub<Fq, 8>(u2, acc.x);  // < 9.01p (acc.x < 5.2p)
     const F29 r = f29_sub<Fq, 4>(s2, acc.y);   // < 5.01p (acc.y < 3.3p)
     if (f29_is_zero<Fq, 9>(pd)) {
         if (f29_is_zero<Fq, 5>(r))
@@ -88,7 +89,8 @@ APG_HD void xyzz29_add_affine(Xyzz29 &acc, bool &id, const F29 &x, const F29 &y)
     const F29 ppp = f29_mul<Fq>(pd, pp);     // < 1.08p
     acc.zzz = f29_mul<Fq>(acc.zzz, ppp);     // < 1.01p
     const F29 q = f29_mul<Fq>(acc.x, pp);    // < 1.07p
-    const F29 x3 = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sqr<Fq>(r), ppp), q), q);  // < 7.15p
+    // R^2 - (PPP + 2Q) + 4p, the sum word by word (PPP + 2Q < 3.22p)
+    const F29 x3 = f29_sub<Fq, 4>(f29_sqr<Fq>(r), f29_sum3_words(ppp, q, q));  // < 5.15p
     // R (Q - X3) - Y1 PPP as one reduction: R (Q - X3 + 8p) + Y1 (2p - PPP);
     // (5.01 x 9.07 + 3.3 x 2) p^2 / R' + p with R' / p ~ 169: < 1.31p
     acc.y = f29_mul2<Fq>(r, f29_sub<Fq, 8>(q, x3), acc.y, f29_sub<Fq, 2>(F29{}, ppp));
@@ -120,7 +122,7 @@ APG_HD Xyzz29 xyzz29_dbl(const Xyzz29 &p) {
     const F29 x2 = f29_sqr<Fq>(p.x);             // < 1.31p
     const F29 m = f29_add(f29_add(x2, x2), x2);  // < 3.93p
     Xyzz29 r;
-    r.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sqr<Fq>(m), s), s);  // < 5.1p
+    r.x = f29_sub<Fq, 4>(f29_sqr<Fq>(m), f29_add_words(s, s));  // < 5.1p (2S < 2.12p)
     // M (S - X3) - W Y as one reduction: M (S - X3 + 8p) + W (4p - Y), Y < 3.3p
     r.y = f29_mul2<Fq>(m, f29_sub<Fq, 8>(s, r.x), w, f29_sub<Fq, 4>(F29{}, p.y));  // < 1.24p
     r.zz = f29_mul<Fq>(v, p.zz);                 // < 1.01p
@@ -146,7 +148,8 @@ APG_HD Xyzz29 xyzz29_add(const Xyzz29 &p, const Xyzz29 &q) {
     const F29 ppp = f29_mul<Fq>(pd, pp);     // < 1.02p
     const F29 qq = f29_mul<Fq>(u1, pp);      // < 1.01p
     Xyzz29 o;
-    o.x = f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sub<Fq, 2>(f29_sqr<Fq>(r), ppp), qq), qq);  // < 7.06p
+    // R^2 - (PPP + 2QQ) + 4p, the sum word by word (PPP + 2QQ < 3.04p)
+    o.x = f29_sub<Fq, 4>(f29_sqr<Fq>(r), f29_sum3_words(ppp, qq, qq));  // < 5.06p
     // R (QQ - X3) - S1 PPP as one reduction: R (QQ - X3 + 8p) + S1 (2p - PPP)
     o.y = f29_mul2<Fq>(r, f29_sub<Fq, 8>(qq, o.x), s1, f29_sub<Fq, 2>(F29{}, ppp));  // < 1.18p
     o.zz = f29_mul<Fq>(f29_mul<Fq>(p.zz, q.zz), pp);      // < 1.01p

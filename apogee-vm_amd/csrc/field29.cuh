@@ -292,7 +292,26 @@ APG_HD F29 f29_add(const F29 &a, const F29 &b) {
     return r;
 }
 
-// a - b + K p for b < K p (the bound is a's plus K p)
+// a + b (a + b + c) word by word with no carries: words below 2^30 (3 x
+// 2^29), bound the sum of the bounds. Only for the subtrahend of f29_sub,
+// whose signed word arithmetic takes such words (a multiply's operands need
+// 29-bit words).
+APG_HD F29 f29_add_words(const F29 &a, const F29 &b) {
+    F29 r;
+#pragma unroll
+    for (int j = 0; j < 9; j++) r.v[j] = a.v[j] + b.v[j];
+    return r;
+}
+
+APG_HD F29 f29_sum3_words(const F29 &a, const F29 &b, const F29 &c) {
+    F29 r;
+#pragma unroll
+    for (int j = 0; j < 9; j++) r.v[j] = a.v[j] + b.v[j] + c.v[j];
+    return r;
+}
+
+// a - b + K p for b < K p (the bound is a's plus K p); b's words may be
+// below 3 x 2^29 (f29_add_words, f29_sum3_words), a's below 2^29
 template <class P, uint32_t K>
 APG_HD F29 f29_sub(const F29 &a, const F29 &b) {
     F29 r;
