@@ -316,7 +316,7 @@ def run_engine_pass(eng, srs, specs, bufs):
 
 def host_threads():
     """The host threads this job may use: the CPUs it is allowed to run on, at
-    most 16 (the share of a one-GPU box)."""
+    most 16 (a one-GPU machine's share of its host's cores)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -730,8 +730,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the path would shard across GPUs as described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: multi-GPU runs "
+              f"(one process per GPU, each with a contiguous range of SRS points and scalars, "
+              f"or whole shards dealt round-robin) are not built", file=sys.stderr)
         sys.exit(2)
     serving = args.workload == "serve"
     for name, mixed_default, serve_default in (("workers", 12, SERVE_WORKERS), ("policy", "fifo", "shortest"),
